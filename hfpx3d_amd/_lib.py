@@ -1,0 +1,92 @@
+"""ctypes binding of the C ABI declared in include/hfpx3d.h.
+
+The library is the in-tree ``hfpx3d_amd/libhfpx3d.so`` built by
+``hfpx3d_amd/build.py``.  There is no fallback: if the library is missing or
+cannot be loaded, every entry point raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhfpx3d.so")
+
+HFPX_OK = 0
+HFPX_ERR_INVALID = 1
+HFPX_ERR_NOMEM = 2
+HFPX_ERR_SINGULAR = 3
+HFPX_ERR_NOT_CONVERGED = 4
+HFPX_ERR_HIP = 5
+HFPX_F64 = 0
+HFPX_F32 = 1
+
+_c_void_p = ctypes.c_void_p
+_c_i64 = ctypes.c_int64
+_c_int = ctypes.c_int
+_c_double = ctypes.c_double
+_pp = ctypes.POINTER(ctypes.c_void_p)
+
+
+class IRReport(ctypes.Structure):
+    """Mirror of struct hfpx_ir_report."""
+    _fields_ = [
+        ("iterations", ctypes.c_int),
+        ("inner_iterations", ctypes.c_int),
+        ("backward_error", ctypes.c_double),
+        ("converged", ctypes.c_int),
+        ("fell_back", ctypes.c_int),
+    ]
+
+
+# name -> (restype, argtypes); must match include/hfpx3d.h exactly
+SIGNATURES = {
+    "hfpx_create": (_c_int, [_pp, _c_int, _c_void_p]),
+    "hfpx_destroy": (_c_int, [_c_void_p]),
+    "hfpx_status_string": (ctypes.c_char_p, [_c_int]),
+    "hfpx_last_error": (ctypes.c_char_p, [_c_void_p]),
+    "hfpx_stream": (_c_void_p, [_c_void_p]),
+    "hfpx_synchronize": (_c_int, [_c_void_p]),
+    "hfpx_mesh_create": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_i64, _c_void_p, _c_i64, _c_i64,
+                                  _c_void_p, _c_i64, _c_double, _pp]),
+    "hfpx_mesh_destroy": (_c_int, [_c_void_p]),
+    "hfpx_mesh_n_dof": (_c_i64, [_c_void_p]),
+    "hfpx_assemble": (_c_int, [_c_void_p, _c_void_p, _c_double, _c_double, _c_void_p, _c_i64]),
+    "hfpx_assemble_mixed": (_c_int, [_c_void_p, _c_void_p, _c_double, _c_double, _c_void_p, _c_i64,
+                                     _c_void_p, _c_i64]),
+    "hfpx_rhs_sinf": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_int, _c_void_p, _c_i64]),
+}
+
+_lib = None
+
+
+def load():
+    """Load the HIP library (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"HIP library {LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class HfpxError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"hfpx status {status}: {msg}")
+        self.status = status
+
+
+def check(rc, ctx=None):
+    if rc != HFPX_OK:
+        lib = load()
+        msg = lib.hfpx_status_string(rc).decode()
+        if ctx is not None:
+            detail = lib.hfpx_last_error(ctx).decode()
+            if detail:
+                msg = f"{msg}: {detail}"
+        raise HfpxError(rc, msg)

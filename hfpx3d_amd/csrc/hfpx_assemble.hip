@@ -1,0 +1,232 @@
+// hfpx_assemble.hip -- element geometry, dense influence-matrix assembly and
+// right-hand sides on gfx950.
+//
+// Replaces, on the device:
+//   hfp3d::make_el_r_tensor / make_el_tau_crd / make_el_sfm_uniform /
+//   el_cp_uniform          src/Core/element_utilities.cpp:62-238, 364-395
+//   hfp3d::make_3dbem_matrix_s   src/Solvers/system_assembly.cpp:30-219
+//   hfp3d::add_s_inf_to_3dbem_rhs src/Solvers/system_assembly.cpp:222-274
+//
+// Layout in HBM: A is column-major (il::Array2D), row = target DoF
+// (collocation point x traction component), column = source DoF (node x DD
+// component), both global-frame, 18x18 blocks per element pair.
+//
+// Work decomposition: one lane = one (collocation point, source element)
+// pair, i.e. a 3 x 18 slab of A.  A wavefront holds 64 consecutive
+// collocation points against ONE source element, so the source geometry is
+// wave-uniform (read once through the scalar path) while the target data is
+// per lane; the 64 lanes together write 192 contiguous rows of each of the
+// source's 18 columns.
+#include "hfpx_internal.h"
+
+namespace hfpx {
+
+// ---------------------------------------------------------------------
+// element geometry: one thread per element
+// ---------------------------------------------------------------------
+__global__ void k_geometry(const double* __restrict__ nods, const int32_t* __restrict__ conn,
+                           int64_t n_el, double beta, SrcElem<double>* __restrict__ elem,
+                           double* __restrict__ cp) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_el) return;
+  double v[3][3];
+  for (int j = 0; j < 3; ++j) {
+    const int64_t n = conn[3 * e + j];
+    for (int k = 0; k < 3; ++k) v[j][k] = nods[3 * n + k];
+  }
+  // frame: e1 = (v1-v0)/|.|, e3 = normalize(e1 x (v2-v0)), e2 = e3 x e1
+  double a1[3], a2[3], e1[3], e2[3], e3[3];
+  for (int k = 0; k < 3; ++k) {
+    a1[k] = v[1][k] - v[0][k];
+    a2[k] = v[2][k] - v[0][k];
+  }
+  double n1 = sqrt(a1[0] * a1[0] + a1[1] * a1[1] + a1[2] * a1[2]);
+  for (int k = 0; k < 3; ++k) e1[k] = a1[k] / n1;
+  e3[0] = e1[1] * a2[2] - e1[2] * a2[1];
+  e3[1] = e1[2] * a2[0] - e1[0] * a2[2];
+  e3[2] = e1[0] * a2[1] - e1[1] * a2[0];
+  double n3 = sqrt(e3[0] * e3[0] + e3[1] * e3[1] + e3[2] * e3[2]);
+  for (int k = 0; k < 3; ++k) e3[k] /= n3;
+  e2[0] = e3[1] * e1[2] - e3[2] * e1[1];
+  e2[1] = e3[2] * e1[0] - e3[0] * e1[2];
+  e2[2] = e3[0] * e1[1] - e3[1] * e1[0];
+  double n2 = sqrt(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]);
+  for (int k = 0; k < 3; ++k) e2[k] /= n2;
+
+  SrcElem<double> E;
+  for (int k = 0; k < 3; ++k) {
+    E.R[k] = e1[k];
+    E.R[3 + k] = e2[k];
+    E.R[6 + k] = e3[k];
+    E.v0[k] = v[0][k];
+  }
+  double tr[3], ti[3];
+  for (int j = 0; j < 3; ++j) {
+    double d[3] = {v[j][0] - v[0][0], v[j][1] - v[0][1], v[j][2] - v[0][2]};
+    tr[j] = e1[0] * d[0] + e1[1] * d[1] + e1[2] * d[2];
+    ti[j] = e2[0] * d[0] + e2[1] * d[1] + e2[2] * d[2];
+    E.t[2 * j] = tr[j];
+    E.t[2 * j + 1] = ti[j];
+  }
+  for (int m = 0; m < 3; ++m) {
+    const int q = (m + 1) % 3;
+    const double dx = tr[q] - tr[m], dy = ti[q] - ti[m];
+    const double L = sqrt(dx * dx + dy * dy);
+    E.len[m] = L;
+    E.th[2 * m] = dx / L;
+    E.th[2 * m + 1] = dy / L;
+  }
+  // barycentric L_k(t) = Im(conj(t_a - t)(t_b - t)) / area2, (a, b) = (k+1, k+2)
+  const double area2 = (tr[1] - tr[0]) * (ti[2] - ti[0]) - (ti[1] - ti[0]) * (tr[2] - tr[0]);
+  for (int k = 0; k < 3; ++k) {
+    const int a = (k + 1) % 3, b = (k + 2) % 3;
+    E.l0[k] = (tr[a] * ti[b] - ti[a] * tr[b]) / area2;
+    // l1 = (conj(t_b) - conj(t_a)) / (2 i area2)
+    const double xr = tr[b] - tr[a], xi = -(ti[b] - ti[a]);
+    E.l1[2 * k] = xi / (2.0 * area2);
+    E.l1[2 * k + 1] = -xr / (2.0 * area2);
+  }
+  elem[e] = E;
+
+  // collocation points (vertex-ward and mid-edge, pulled towards the centroid)
+  const int64_t ncp = 6 * n_el;
+  double c[3];
+  for (int k = 0; k < 3; ++k) c[k] = (v[0][k] + v[1][k] + v[2][k]) / 3.0;
+  for (int q = 0; q < 3; ++q) {
+    const int m = (q + 1) % 3, l = (m + 1) % 3;
+    for (int k = 0; k < 3; ++k) {
+      cp[k * ncp + 6 * e + q] = (1.0 - beta) * v[q][k] + beta * c[k];
+      cp[k * ncp + 6 * e + 3 + q] = 0.5 * (1.0 - beta) * (v[m][k] + v[l][k]) + beta * c[k];
+    }
+  }
+  for (int q = 0; q < 6; ++q)
+    for (int k = 0; k < 3; ++k) cp[(3 + k) * ncp + 6 * e + q] = -e3[k];
+}
+
+// ---------------------------------------------------------------------
+// assembly
+// ---------------------------------------------------------------------
+struct StoreBoth {
+  double* a64;
+  int64_t lda64;
+  float* a32;
+  int64_t lda32;
+  __device__ __forceinline__ void operator()(int64_t r, int64_t c, double v) const {
+    if (a64) a64[r + c * lda64] = v;
+    if (a32) a32[r + c * lda32] = (float)v;
+  }
+};
+
+template <typename Store>
+__global__ __launch_bounds__(256) void k_assemble(const SrcElem<double>* __restrict__ elem,
+                                                  const double* __restrict__ cp, int64_t n_el,
+                                                  const int32_t* __restrict__ dof, double shear_m,
+                                                  double nu, Store store) {
+  const int lane = threadIdx.x & 63;
+  const int64_t es = __builtin_amdgcn_readfirstlane((int)(blockIdx.y * 4 + (threadIdx.x >> 6)));
+  if (es >= n_el) return;
+  const int64_t ncp = 6 * n_el;
+  const int64_t icp = (int64_t)blockIdx.x * 64 + lane;
+  if (icp >= ncp) return;
+  const SrcElem<double>& E = elem[es];
+
+  const double xg[3] = {cp[icp], cp[ncp + icp], cp[2 * ncp + icp]};
+  const double ng[3] = {cp[3 * ncp + icp], cp[4 * ncp + icp], cp[5 * ncp + icp]};
+  const double d[3] = {xg[0] - E.v0[0], xg[1] - E.v0[1], xg[2] - E.v0[2]};
+  const double y0 = E.R[0] * d[0] + E.R[1] * d[1] + E.R[2] * d[2];
+  const double y1 = E.R[3] * d[0] + E.R[4] * d[1] + E.R[5] * d[2];
+  const double y2 = E.R[6] * d[0] + E.R[7] * d[1] + E.R[8] * d[2];
+  const double nl[3] = {E.R[0] * ng[0] + E.R[1] * ng[1] + E.R[2] * ng[2],
+                        E.R[3] * ng[0] + E.R[4] * ng[1] + E.R[5] * ng[2],
+                        E.R[6] * ng[0] + E.R[7] * ng[1] + E.R[8] * ng[2]};
+  int64_t row[3];
+  if (dof) {
+    const int64_t et = icp / 6, nt = icp % 6;
+    for (int i = 0; i < 3; ++i) row[i] = dof[18 * et + 3 * nt + i];
+  } else {
+    for (int i = 0; i < 3; ++i) row[i] = 3 * icp + i;
+  }
+  pair_influence<double>(E, y0, y1, y2, shear_m, nu, [&](int j, const double (&S)[6][3]) {
+    // traction in the source frame for each local DD component k
+    double tl[3][3];
+    for (int k = 0; k < 3; ++k) {
+      tl[0][k] = nl[0] * S[0][k] + nl[1] * S[3][k] + nl[2] * S[4][k];
+      tl[1][k] = nl[0] * S[3][k] + nl[1] * S[1][k] + nl[2] * S[5][k];
+      tl[2][k] = nl[0] * S[4][k] + nl[1] * S[5][k] + nl[2] * S[2][k];
+    }
+    // T = R^T tl R  (traction to global, DD component to global)
+    double tg[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int k = 0; k < 3; ++k)
+        tg[i][k] = E.R[i] * tl[0][k] + E.R[3 + i] * tl[1][k] + E.R[6 + i] * tl[2][k];
+    for (int jg = 0; jg < 3; ++jg) {
+      int64_t col;
+      if (dof) col = dof[18 * es + 3 * j + jg];
+      else col = 18 * es + 3 * j + jg;
+      if (col < 0) continue;
+      for (int i = 0; i < 3; ++i) {
+        if (row[i] < 0) continue;
+        const double v = tg[i][0] * E.R[jg] + tg[i][1] * E.R[3 + jg] + tg[i][2] * E.R[6 + jg];
+        store(row[i], col, v);
+      }
+    }
+  });
+}
+
+// ---------------------------------------------------------------------
+// right-hand side: one thread per (element, rhs)
+// ---------------------------------------------------------------------
+__global__ void k_rhs(const SrcElem<double>* __restrict__ elem, int64_t n_el,
+                      const int32_t* __restrict__ dof, const double* __restrict__ sinf, int nrhs,
+                      double* __restrict__ b, int64_t ldb) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_el * nrhs) return;
+  const int64_t e = t % n_el, r = t / n_el;
+  const double* s = sinf + 6 * r;
+  const double n[3] = {-elem[e].R[6], -elem[e].R[7], -elem[e].R[8]};
+  const double tr[3] = {n[0] * s[0] + n[1] * s[3] + n[2] * s[4],
+                        n[0] * s[3] + n[1] * s[1] + n[2] * s[5],
+                        n[0] * s[4] + n[1] * s[5] + n[2] * s[2]};
+  for (int ln = 0; ln < 6; ++ln)
+    for (int l = 0; l < 3; ++l) {
+      const int64_t d = dof ? (int64_t)dof[18 * e + 3 * ln + l] : 18 * e + 3 * ln + l;
+      if (d >= 0) b[d + r * ldb] -= tr[l];
+    }
+}
+
+// ---------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------
+int launch_geometry(hfpx_ctx* ctx, hfpx_mesh* m) {
+  const int bs = 128;
+  const unsigned gs = (unsigned)((m->n_el + bs - 1) / bs);
+  hipLaunchKernelGGL(k_geometry, dim3(gs), dim3(bs), 0, ctx->stream, m->nods, m->conn, m->n_el,
+                     m->beta, m->elem, m->cp);
+  HFPX_TRY_HIP(ctx, hipGetLastError());
+  return HFPX_OK;
+}
+
+int launch_assemble(hfpx_ctx* ctx, const hfpx_mesh* m, double shear_m, double poiss_r,
+                    double* a64, int64_t lda64, float* a32, int64_t lda32) {
+  const int64_t ncp = 6 * m->n_el;
+  const unsigned gx = (unsigned)((ncp + 63) / 64);
+  const unsigned gy = (unsigned)((m->n_el + 3) / 4);
+  if (gy > 65535u) return set_error(ctx, HFPX_ERR_INVALID, "too many elements for one launch");
+  StoreBoth st{a64, lda64, a32, lda32};
+  hipLaunchKernelGGL(k_assemble<StoreBoth>, dim3(gx, gy), dim3(256), 0, ctx->stream, m->elem,
+                     m->cp, m->n_el, m->identity_dof ? nullptr : m->dof, shear_m, poiss_r, st);
+  HFPX_TRY_HIP(ctx, hipGetLastError());
+  return HFPX_OK;
+}
+
+int launch_rhs(hfpx_ctx* ctx, const hfpx_mesh* m, const double* sinf_dev, int nrhs, double* b,
+               int64_t ldb) {
+  const int bs = 128;
+  const int64_t nt = m->n_el * nrhs;
+  hipLaunchKernelGGL(k_rhs, dim3((unsigned)((nt + bs - 1) / bs)), dim3(bs), 0, ctx->stream,
+                     m->elem, m->n_el, m->identity_dof ? nullptr : m->dof, sinf_dev, nrhs, b, ldb);
+  HFPX_TRY_HIP(ctx, hipGetLastError());
+  return HFPX_OK;
+}
+
+}  // namespace hfpx

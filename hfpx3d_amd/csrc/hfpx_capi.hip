@@ -1,0 +1,252 @@
+// hfpx_capi.hip -- extern "C" entry points: context, mesh, assembly, RHS.
+#include <cmath>
+#include <cstdarg>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "hfpx_internal.h"
+
+namespace hfpx {
+
+int set_error(hfpx_ctx* ctx, int code, const char* fmt, ...) {
+  if (ctx) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    ctx->last_error = buf;
+  }
+  return code;
+}
+
+int check_hip(hfpx_ctx* ctx, hipError_t e, const char* what) {
+  if (e == hipSuccess) return HFPX_OK;
+  const int code = (e == hipErrorOutOfMemory) ? HFPX_ERR_NOMEM : HFPX_ERR_HIP;
+  return set_error(ctx, code, "%s: %s", what, hipGetErrorString(e));
+}
+
+int ensure_work(hfpx_ctx* ctx, size_t bytes) {
+  if (ctx->work_bytes >= bytes) return HFPX_OK;
+  if (ctx->work) {
+    HFPX_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    HFPX_TRY_HIP(ctx, hipFree(ctx->work));
+    ctx->work = nullptr;
+    ctx->work_bytes = 0;
+  }
+  HFPX_TRY_HIP(ctx, hipMalloc(&ctx->work, bytes));
+  ctx->work_bytes = bytes;
+  return HFPX_OK;
+}
+
+int ensure_hwork(hfpx_ctx* ctx, size_t bytes) {
+  if (ctx->hwork_bytes >= bytes) return HFPX_OK;
+  if (ctx->hwork) {
+    HFPX_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    HFPX_TRY_HIP(ctx, hipHostFree(ctx->hwork));
+    ctx->hwork = nullptr;
+    ctx->hwork_bytes = 0;
+  }
+  HFPX_TRY_HIP(ctx, hipHostMalloc(&ctx->hwork, bytes, hipHostMallocDefault));
+  ctx->hwork_bytes = bytes;
+  return HFPX_OK;
+}
+
+}  // namespace hfpx
+
+using namespace hfpx;
+
+extern "C" {
+
+const char* hfpx_status_string(int status) {
+  switch (status) {
+    case HFPX_OK: return "ok";
+    case HFPX_ERR_INVALID: return "invalid argument";
+    case HFPX_ERR_NOMEM: return "out of memory";
+    case HFPX_ERR_SINGULAR: return "matrix is singular";
+    case HFPX_ERR_NOT_CONVERGED: return "iterative refinement did not converge";
+    case HFPX_ERR_HIP: return "HIP runtime error";
+    default: return "unknown status";
+  }
+}
+
+const char* hfpx_last_error(const hfpx_ctx* ctx) { return ctx ? ctx->last_error.c_str() : ""; }
+
+int hfpx_create(hfpx_ctx** out, int device, void* stream) {
+  if (!out) return HFPX_ERR_INVALID;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return HFPX_ERR_HIP;
+  if (device < 0 || device >= ndev) return HFPX_ERR_INVALID;
+  hfpx_ctx* ctx = new hfpx_ctx();
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess) {
+    delete ctx;
+    return HFPX_ERR_HIP;
+  }
+  if (stream) {
+    ctx->stream = (hipStream_t)stream;
+  } else {
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+      delete ctx;
+      return HFPX_ERR_HIP;
+    }
+    ctx->own_stream = true;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+  *out = ctx;
+  return HFPX_OK;
+}
+
+int hfpx_destroy(hfpx_ctx* ctx) {
+  if (!ctx) return HFPX_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->work) (void)hipFree(ctx->work);
+  if (ctx->hwork) (void)hipHostFree(ctx->hwork);
+  if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return HFPX_OK;
+}
+
+void* hfpx_stream(const hfpx_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int hfpx_synchronize(hfpx_ctx* ctx) {
+  if (!ctx) return HFPX_ERR_INVALID;
+  HFPX_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return HFPX_OK;
+}
+
+int hfpx_mesh_create(hfpx_ctx* ctx, const double* nods_host, int64_t nods_rows, int64_t n_nodes,
+                     const int64_t* conn_host, int64_t conn_rows, int64_t n_el,
+                     const int64_t* dof_h_host, int64_t n_dof, double beta, hfpx_mesh** out) {
+  if (!ctx || !out) return HFPX_ERR_INVALID;
+  *out = nullptr;
+  if (!nods_host || !conn_host || nods_rows < 3 || conn_rows < 3 || n_nodes < 3 || n_el < 1)
+    return set_error(ctx, HFPX_ERR_INVALID, "mesh needs >= 3 coordinate rows, >= 3 nodes, >= 1 element");
+  if (n_el * 6 > (int64_t)1 << 30)
+    return set_error(ctx, HFPX_ERR_INVALID, "n_el = %lld too large", (long long)n_el);
+  std::vector<double> xyz(3 * n_nodes);
+  for (int64_t j = 0; j < n_nodes; ++j)
+    for (int k = 0; k < 3; ++k) {
+      const double v = nods_host[k + nods_rows * j];
+      if (!std::isfinite(v))
+        return set_error(ctx, HFPX_ERR_INVALID, "node %lld has a non-finite coordinate", (long long)j);
+      xyz[3 * j + k] = v;
+    }
+  std::vector<int32_t> cn(3 * n_el);
+  for (int64_t e = 0; e < n_el; ++e)
+    for (int k = 0; k < 3; ++k) {
+      const int64_t v = conn_host[k + conn_rows * e];
+      if (v < 0 || v >= n_nodes)
+        return set_error(ctx, HFPX_ERR_INVALID, "element %lld references node %lld out of range",
+                         (long long)e, (long long)v);
+      cn[3 * e + k] = (int32_t)v;
+    }
+  std::vector<int32_t> dh;
+  bool identity = (dof_h_host == nullptr);
+  if (identity) {
+    n_dof = 18 * n_el;
+  } else {
+    dh.resize(18 * n_el);
+    std::vector<char> seen(n_dof > 0 ? n_dof : 1, 0);
+    identity = (n_dof == 18 * n_el);
+    for (int64_t e = 0; e < n_el; ++e)
+      for (int c = 0; c < 18; ++c) {
+        const int64_t d = dof_h_host[e + n_el * c];
+        if (d < -1 || d >= n_dof)
+          return set_error(ctx, HFPX_ERR_INVALID, "dof_h(%lld, %d) = %lld outside [-1, n_dof)",
+                           (long long)e, c, (long long)d);
+        if (d >= 0) {
+          if (seen[d]) return set_error(ctx, HFPX_ERR_INVALID, "dof %lld used twice", (long long)d);
+          seen[d] = 1;
+        }
+        if (d != 18 * e + c) identity = false;
+        dh[18 * e + c] = (int32_t)d;
+      }
+    for (int64_t d = 0; d < n_dof; ++d)
+      if (!seen[d]) return set_error(ctx, HFPX_ERR_INVALID, "dof %lld never used", (long long)d);
+  }
+  HFPX_TRY_HIP(ctx, hipSetDevice(ctx->device));
+  hfpx_mesh* m = new hfpx_mesh();
+  m->n_el = n_el;
+  m->n_dof = n_dof;
+  m->beta = beta;
+  m->identity_dof = identity;
+  m->n_nodes = n_nodes;
+  hipError_t e = hipMalloc(&m->nods, sizeof(double) * 3 * n_nodes);
+  if (e == hipSuccess) e = hipMalloc(&m->conn, sizeof(int32_t) * 3 * n_el);
+  if (e == hipSuccess) e = hipMalloc(&m->dof, sizeof(int32_t) * 18 * n_el);
+  if (e == hipSuccess) e = hipMalloc(&m->elem, sizeof(SrcElem<double>) * n_el);
+  if (e == hipSuccess) e = hipMalloc(&m->cp, sizeof(double) * 36 * n_el);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(m->nods, xyz.data(), sizeof(double) * 3 * n_nodes, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(m->conn, cn.data(), sizeof(int32_t) * 3 * n_el, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess && !dh.empty())
+    e = hipMemcpyAsync(m->dof, dh.data(), sizeof(int32_t) * 18 * n_el, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    hfpx_mesh_destroy(m);
+    return check_hip(ctx, e, "mesh upload");
+  }
+  *out = m;
+  return HFPX_OK;
+}
+
+int hfpx_mesh_destroy(hfpx_mesh* m) {
+  if (!m) return HFPX_OK;
+  if (m->nods) (void)hipFree(m->nods);
+  if (m->conn) (void)hipFree(m->conn);
+  if (m->dof) (void)hipFree(m->dof);
+  if (m->elem) (void)hipFree(m->elem);
+  if (m->cp) (void)hipFree(m->cp);
+  delete m;
+  return HFPX_OK;
+}
+
+int64_t hfpx_mesh_n_dof(const hfpx_mesh* m) { return m ? m->n_dof : -1; }
+
+int hfpx_assemble(hfpx_ctx* ctx, hfpx_mesh* m, double shear_m, double poiss_r, double* A,
+                  int64_t lda) {
+  return hfpx_assemble_mixed(ctx, m, shear_m, poiss_r, A, lda, nullptr, 0);
+}
+
+int hfpx_assemble_mixed(hfpx_ctx* ctx, hfpx_mesh* m, double shear_m, double poiss_r, double* A64,
+                        int64_t lda64, float* A32, int64_t lda32) {
+  if (!ctx || !m) return HFPX_ERR_INVALID;
+  if (!A64 && !A32) return set_error(ctx, HFPX_ERR_INVALID, "no output matrix");
+  if ((A64 && lda64 < m->n_dof) || (A32 && lda32 < m->n_dof))
+    return set_error(ctx, HFPX_ERR_INVALID, "lda smaller than n_dof = %lld", (long long)m->n_dof);
+  if (!(poiss_r < 1.0) || !(shear_m > 0.0))
+    return set_error(ctx, HFPX_ERR_INVALID, "need shear modulus > 0 and Poisson ratio < 1");
+  HFPX_TRY_HIP(ctx, hipSetDevice(ctx->device));
+  int rc = launch_geometry(ctx, m);
+  if (rc) return rc;
+  return launch_assemble(ctx, m, shear_m, poiss_r, A64, lda64, A32, lda32);
+}
+
+int hfpx_rhs_sinf(hfpx_ctx* ctx, hfpx_mesh* m, const double* sinf_host, int nrhs, double* B,
+                  int64_t ldb) {
+  if (!ctx || !m || !sinf_host || !B || nrhs < 1) return HFPX_ERR_INVALID;
+  if (ldb < m->n_dof) return set_error(ctx, HFPX_ERR_INVALID, "ldb smaller than n_dof");
+  HFPX_TRY_HIP(ctx, hipSetDevice(ctx->device));
+  int rc = ensure_hwork(ctx, sizeof(double) * 6 * nrhs);
+  if (rc) return rc;
+  rc = ensure_work(ctx, sizeof(double) * 6 * nrhs);
+  if (rc) return rc;
+  memcpy(ctx->hwork, sinf_host, sizeof(double) * 6 * nrhs);
+  HFPX_TRY_HIP(ctx, hipMemcpyAsync(ctx->work, ctx->hwork, sizeof(double) * 6 * nrhs,
+                                   hipMemcpyHostToDevice, ctx->stream));
+  rc = launch_geometry(ctx, m);
+  if (rc) return rc;
+  rc = launch_rhs(ctx, m, (const double*)ctx->work, nrhs, B, ldb);
+  if (rc) return rc;
+  // hwork/work are reused by the next call: finish before returning
+  HFPX_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return HFPX_OK;
+}
+
+}  // extern "C"

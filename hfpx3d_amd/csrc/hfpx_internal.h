@@ -1,0 +1,60 @@
+// hfpx_internal.h -- shared host-side definitions of the HIP library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/hfpx3d.h"
+#include "hfpx_pair.h"
+
+// Per-process context: one device, one stream, reusable device workspace.
+struct hfpx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  void* work = nullptr;      // grow-only device workspace
+  size_t work_bytes = 0;
+  void* hwork = nullptr;     // pinned host workspace
+  size_t hwork_bytes = 0;
+  int num_cus = 0;
+  std::string last_error;
+};
+
+// Device-resident mesh: raw nodes/connectivity/DoF map plus the element
+// table that the geometry kernel derives from them.
+struct hfpx_mesh {
+  int64_t n_el = 0;
+  int64_t n_dof = 0;
+  double beta = 0.125;
+  bool identity_dof = true;
+  double* nods = nullptr;      // 3 x n_nodes (x, y, z rows only)
+  int64_t n_nodes = 0;
+  int32_t* conn = nullptr;     // 3 x n_el (vertex ids, 0-based)
+  int32_t* dof = nullptr;      // n_el x 18 (row = element), -1 = fixed
+  hfpx::SrcElem<double>* elem = nullptr;  // n_el
+  double* cp = nullptr;        // 6 arrays of 6*n_el: cx cy cz nx ny nz
+};
+
+namespace hfpx {
+
+int set_error(hfpx_ctx* ctx, int code, const char* fmt, ...);
+int check_hip(hfpx_ctx* ctx, hipError_t e, const char* what);
+int ensure_work(hfpx_ctx* ctx, size_t bytes);
+int ensure_hwork(hfpx_ctx* ctx, size_t bytes);
+
+// launchers (hfpx_assemble.hip)
+int launch_geometry(hfpx_ctx* ctx, hfpx_mesh* m);
+int launch_assemble(hfpx_ctx* ctx, const hfpx_mesh* m, double shear_m, double poiss_r,
+                    double* a64, int64_t lda64, float* a32, int64_t lda32);
+int launch_rhs(hfpx_ctx* ctx, const hfpx_mesh* m, const double* sinf_dev, int nrhs,
+               double* b, int64_t ldb);
+
+}  // namespace hfpx
+
+#define HFPX_TRY_HIP(ctx, expr)                                   \
+  do {                                                            \
+    hipError_t e__ = (expr);                                      \
+    if (e__ != hipSuccess) return hfpx::check_hip(ctx, e__, #expr); \
+  } while (0)

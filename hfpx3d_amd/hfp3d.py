@@ -1,0 +1,307 @@
+"""Python mirror of the reference's hot-path call surface, backed by the HIP
+library through its C ABI.
+
+Names, argument meaning and error behaviour follow the reference's namespace
+``hfp3d`` so that code written against it reads the same:
+
+  Mesh_Geom_T, DoF_Handle_T, Mesh_Data_T        src/Core/surface_mesh_utilities.h:25-104
+  Properties_T, Load_T, Num_Param_T             src/Core/model_parameters.h:34-94
+  SAE_T                                         src/Solvers/system_assembly.h:26-30
+  make_dof_h_crack                              src/Core/surface_mesh_utilities.cpp:28-112
+  make_3dbem_matrix_s                           src/Solvers/system_assembly.cpp:30-219
+  add_s_inf_to_3dbem_rhs                        src/Solvers/system_assembly.cpp:222-274
+  LU (factor + solve)                           il/linear_algebra/dense/factorization/LU.h:46-138
+  write_dd_vector_to_md                         src/Core/surface_mesh_utilities.cpp:256-328
+  make_3dbem_stress_f_s                         src/Solvers/system_assembly.cpp:279-455
+
+Matrices and right-hand sides live on the GPU as torch tensors in
+column-major layout (a logical n x n matrix is the transpose view of a
+row-major (n, lda) buffer).  Meshes, DoF handles and DD arrays are host
+numpy arrays, as in the reference.  Preconditions that the reference checks
+with IL_EXPECT_FAST raise ValueError; numerical failure (a singular matrix)
+is reported through a Status object like il::Status.
+"""
+from dataclasses import dataclass, field
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+# --------------------------------------------------------------------------
+# containers
+# --------------------------------------------------------------------------
+
+
+@dataclass
+class Mesh_Geom_T:
+    nods: np.ndarray = None   # (3 or 6) x n_nodes, float64
+    conn: np.ndarray = None   # (3 or 5) x n_el, int64, rows 0..2 0-based vertex ids
+
+
+@dataclass
+class DoF_Handle_T:
+    n_dof: int = 0
+    dof_h: np.ndarray = None  # n_el x 18, int64, -1 = fixed
+
+
+@dataclass
+class Num_Param_T:
+    beta: float = 0.125
+    tip_type: int = 1
+    is_dd_local: bool = False
+
+
+@dataclass
+class Properties_T:
+    n_solid: int = 1
+    shear_m: list = field(default_factory=lambda: [1.0])
+    poiss_r: list = field(default_factory=lambda: [0.0])
+
+
+@dataclass
+class Load_T:
+    s_inf: np.ndarray = field(default_factory=lambda: np.zeros(6))
+
+
+@dataclass
+class Mesh_Data_T:
+    mesh: Mesh_Geom_T = None
+    dof_h_dd: DoF_Handle_T = field(default_factory=DoF_Handle_T)
+    dd: np.ndarray = None     # (6 n_el) x 3
+
+
+@dataclass
+class SAE_T:
+    n_dof: int = 0
+    matrix: object = None     # torch tensor, n_dof x n_dof, column-major, device
+    rhs_v: object = None      # torch tensor, n_dof (or n_dof x nrhs), device
+
+
+class Status:
+    """Minimal il::Status: ok() / error / info (rank for a singular matrix)."""
+
+    def __init__(self):
+        self.error = None
+        self.info = 0
+
+    def ok(self):
+        return self.error is None
+
+    def abortOnError(self):
+        if self.error is not None:
+            raise RuntimeError(f"{self.error} (info = {self.info})")
+
+
+# --------------------------------------------------------------------------
+# context (one per device, on a dedicated default-priority torch stream)
+# --------------------------------------------------------------------------
+
+class Context:
+    def __init__(self, device=0):
+        import torch
+        self.torch = torch
+        self.device = torch.device("cuda", device)
+        self.lib = _lib.load()
+        self.stream = torch.cuda.Stream(device=self.device)
+        h = ctypes.c_void_p()
+        _lib.check(self.lib.hfpx_create(ctypes.byref(h), device, ctypes.c_void_p(self.stream.cuda_stream)))
+        self.handle = h
+
+    def enter(self):
+        """Order this context's stream after the caller's current stream."""
+        self.stream.wait_stream(self.torch.cuda.current_stream(self.device))
+
+    def leave(self):
+        """Order the caller's current stream after this context's work."""
+        self.torch.cuda.current_stream(self.device).wait_stream(self.stream)
+
+    def check(self, rc):
+        _lib.check(rc, self.handle)
+
+    def close(self):
+        if self.handle:
+            self.lib.hfpx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_contexts = {}
+
+
+def get_context(device=0):
+    ctx = _contexts.get(device)
+    if ctx is None:
+        ctx = Context(device)
+        _contexts[device] = ctx
+    return ctx
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _np_ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class DeviceMesh:
+    """A mesh (plus DoF map and collocation offset) resident on the GPU."""
+
+    def __init__(self, mesh, dof_hndl=None, beta=0.125, ctx=None):
+        self.ctx = ctx or get_context()
+        nods = np.asfortranarray(mesh.nods, dtype=np.float64)
+        conn = np.asfortranarray(mesh.conn, dtype=np.int64)
+        if conn.shape[0] < 3 or conn.shape[1] < 1 or nods.shape[0] < 3 or nods.shape[1] < 3:
+            raise ValueError("mesh needs >= 3 rows of coordinates/connectivity, >= 3 nodes, >= 1 element")
+        self.n_el = conn.shape[1]
+        dof_ptr, n_dof = None, 18 * self.n_el
+        if dof_hndl is not None and dof_hndl.dof_h is not None and dof_hndl.n_dof > 0:
+            dh = np.asfortranarray(dof_hndl.dof_h, dtype=np.int64)
+            if dh.shape != (self.n_el, 18):
+                raise ValueError("dof_h must be n_el x 18")
+            self._dh = dh
+            dof_ptr, n_dof = _np_ptr(dh), int(dof_hndl.n_dof)
+        h = ctypes.c_void_p()
+        self.ctx.check(self.ctx.lib.hfpx_mesh_create(
+            self.ctx.handle, _np_ptr(nods), nods.shape[0], nods.shape[1], _np_ptr(conn), conn.shape[0],
+            self.n_el, dof_ptr, n_dof, float(beta), ctypes.byref(h)))
+        self.handle = h
+        self.n_dof = int(self.ctx.lib.hfpx_mesh_n_dof(h))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.ctx.lib.hfpx_mesh_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def alloc_colmajor(n_rows, n_cols, dtype, ctx, lda=None):
+    """Device buffer for an n_rows x n_cols column-major matrix (lda >= n_rows);
+    returns (logical view with strides (1, lda), lda)."""
+    torch = ctx.torch
+    lda = lda or n_rows
+    with torch.cuda.stream(ctx.stream):
+        buf = torch.empty((n_cols, lda), dtype=dtype, device=ctx.device)
+    return buf.t()[:n_rows], lda
+
+
+# --------------------------------------------------------------------------
+# DoF handle (host)
+# --------------------------------------------------------------------------
+
+def make_dof_h_crack(mesh, ap_order=2, tip_type=1):
+    """Per-element DoF map with tip DoF fixed (-1) and the rest numbered
+    consecutively.  A vertex is at the tip when node rows 3 and 4 are both 1
+    and the element's surface ids (conn rows 3, 4) are equal; tip_type 2 also
+    fixes mid-edge nodes between two tip vertices.  Unlike the reference
+    (surface_mesh_utilities.cpp:85-101), every free DoF is renumbered
+    consecutively for tip_type 2 as well: the reference leaves mid-edge DoF
+    of non-tip edges un-renumbered there, which puts indices past n_dof."""
+    if ap_order != 2:
+        raise ValueError("only quadratic (ap_order = 2) elements are supported")
+    nods, conn = np.asarray(mesh.nods), np.asarray(mesh.conn)
+    n_el = conn.shape[1]
+    ext = nods.shape[0] >= 5 and conn.shape[0] >= 5
+    dof = np.zeros((n_el, 18), dtype=np.int64)
+    n_dof, dec = 18 * n_el, 0
+    for e in range(n_el):
+        tip = [False, False, False]
+        if ext and conn[3, e] == conn[4, e]:
+            for v in range(3):
+                n = conn[v, e]
+                tip[v] = nods[3, n] == 1 and nods[4, n] == 1
+        for v in range(6):
+            fixed = (tip_type >= 1 and v < 3 and tip[v]) or \
+                    (tip_type == 2 and v >= 3 and tip[(v - 2) % 3] and tip[(v - 1) % 3])
+            for l in range(3):
+                ld = 3 * v + l
+                if fixed:
+                    dof[e, ld] = -1
+                    n_dof -= 1
+                    dec += 1
+                else:
+                    dof[e, ld] = 18 * e + ld - dec
+    return DoF_Handle_T(n_dof=n_dof, dof_h=np.asfortranarray(dof))
+
+
+# --------------------------------------------------------------------------
+# assembly and right-hand side
+# --------------------------------------------------------------------------
+
+def make_3dbem_matrix_s(shear_m, poiss_r, mesh, n_par, dof_hndl=None, ctx=None, dtype=None,
+                        device_mesh=None):
+    """Dense n_dof x n_dof traction influence matrix on the GPU (FP64 unless
+    dtype=torch.float32).  If dof_hndl is empty it is built with
+    make_dof_h_crack(mesh, 2, n_par.tip_type) and filled in, like the
+    reference's io argument."""
+    ctx = ctx or get_context()
+    torch = ctx.torch
+    dtype = dtype or torch.float64
+    if dof_hndl is not None and (dof_hndl.n_dof == 0 or dof_hndl.dof_h is None):
+        d = make_dof_h_crack(mesh, 2, n_par.tip_type)
+        dof_hndl.n_dof, dof_hndl.dof_h = d.n_dof, d.dof_h
+    if n_par.is_dd_local:
+        raise ValueError("is_dd_local = true is not supported (global DD only)")
+    dm = device_mesh or DeviceMesh(mesh, dof_hndl, n_par.beta, ctx)
+    A, lda = alloc_colmajor(dm.n_dof, dm.n_dof, dtype, ctx)
+    ctx.enter()
+    if dtype == torch.float64:
+        ctx.check(ctx.lib.hfpx_assemble(ctx.handle, dm.handle, float(shear_m), float(poiss_r), _ptr(A), lda))
+    elif dtype == torch.float32:
+        ctx.check(ctx.lib.hfpx_assemble_mixed(ctx.handle, dm.handle, float(shear_m), float(poiss_r),
+                                              None, 0, _ptr(A), lda))
+    else:
+        raise ValueError("dtype must be float64 or float32")
+    ctx.leave()
+    return A
+
+
+def add_s_inf_to_3dbem_rhs(mesh_data, load, sae, ctx=None, device_mesh=None):
+    """sae.rhs_v -= traction of the far-field stress load.s_inf on every free
+    DoF.  load.s_inf may be 6 or 6 x nrhs (order xx yy zz xy xz yz)."""
+    ctx = ctx or get_context()
+    torch = ctx.torch
+    s = np.asfortranarray(np.asarray(load.s_inf, dtype=np.float64).reshape(6, -1))
+    nrhs = s.shape[1]
+    dm = device_mesh or DeviceMesh(mesh_data.mesh, mesh_data.dof_h_dd, 0.125, ctx)
+    n = dm.n_dof
+    rhs = sae.rhs_v
+    if rhs is None or rhs.shape[0] != n:
+        if nrhs == 1:
+            with torch.cuda.stream(ctx.stream):
+                rhs = torch.zeros(n, dtype=torch.float64, device=ctx.device)
+        else:
+            rhs, _ = alloc_colmajor(n, nrhs, torch.float64, ctx)
+            with torch.cuda.stream(ctx.stream):
+                rhs.zero_()
+    ldb = n if rhs.dim() == 1 else rhs.stride(1)
+    ctx.enter()
+    ctx.check(ctx.lib.hfpx_rhs_sinf(ctx.handle, dm.handle, _np_ptr(s), nrhs, _ptr(rhs), ldb))
+    ctx.leave()
+    sae.rhs_v = rhs
+    sae.n_dof = n
+    return rhs
+
+
+def write_dd_vector_to_md(sol_v, dof_h_dd, n_el=None):
+    """Solution vector -> nodal DD array ((6 n_el) x 3); fixed DoF stay 0."""
+    sol = np.asarray(sol_v.cpu() if hasattr(sol_v, "cpu") else sol_v, dtype=np.float64)
+    dh = np.asarray(dof_h_dd.dof_h)
+    n_el = dh.shape[0]
+    dd = np.zeros((6 * n_el, 3))
+    d = dh.reshape(n_el, 6, 3)
+    mask = d >= 0
+    vals = np.where(mask, sol[np.where(mask, d, 0)], 0.0)
+    dd[:] = vals.reshape(6 * n_el, 3)
+    return dd

@@ -1,0 +1,87 @@
+/*
+ * hfpx3d.h -- C ABI of the MI355X (gfx950) HFPx3D hot path.
+ *
+ * Plain C types only: device pointers are `void*`/`double*`/`float*`
+ * allocated by the caller on the context's device, host pointers are marked
+ * `_host`.  Every function returns an hfpx status code (HFPX_OK = 0) and never
+ * aborts; hfpx_last_error() gives the message of the last failure on a
+ * context.  Arrays are column-major with explicit leading dimensions, the
+ * layout of il::Array2D (il/container/2d/Array2D.h:233-263 in the
+ * reference).  A context is not thread-safe; distinct contexts are
+ * independent.  All work is enqueued on the context's stream; functions that
+ * return host-visible results (info, reports) synchronise that stream.
+ */
+#ifndef HFPX3D_H
+#define HFPX3D_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------- */
+#define HFPX_OK 0
+#define HFPX_ERR_INVALID 1        /* invalid argument / shape              */
+#define HFPX_ERR_NOMEM 2          /* device or host allocation failed      */
+#define HFPX_ERR_SINGULAR 3       /* exact zero pivot (info = column, 1-based) */
+#define HFPX_ERR_NOT_CONVERGED 4  /* iterative refinement did not converge */
+#define HFPX_ERR_HIP 5            /* HIP runtime error                     */
+
+#define HFPX_F64 0
+#define HFPX_F32 1
+
+typedef struct hfpx_ctx hfpx_ctx;
+typedef struct hfpx_mesh hfpx_mesh;
+
+/* ---- context ---------------------------------------------------------- */
+/* Create a context on `device`; `stream` is a hipStream_t to use (NULL: the
+ * library creates its own default-priority stream). */
+int hfpx_create(hfpx_ctx** ctx, int device, void* stream);
+int hfpx_destroy(hfpx_ctx* ctx);
+const char* hfpx_status_string(int status);
+const char* hfpx_last_error(const hfpx_ctx* ctx);
+/* Stream the context enqueues on (a hipStream_t). */
+void* hfpx_stream(const hfpx_ctx* ctx);
+int hfpx_synchronize(hfpx_ctx* ctx);
+
+/* ---- mesh ------------------------------------------------------------- */
+/* Upload a crack mesh.  Replaces the mesh/DoF containers consumed by
+ * hfp3d::make_3dbem_matrix_s (src/Core/surface_mesh_utilities.h:25-47):
+ *   nods_host  nods_rows x n_nodes, rows 0..2 = x, y, z (extra rows ignored)
+ *   conn_host  conn_rows x n_el, rows 0..2 = 0-based vertex ids
+ *   dof_h_host n_el x 18 (col-major, -1 = fixed DoF) as built by
+ *              hfp3d::make_dof_h_crack (surface_mesh_utilities.cpp:28-112),
+ *              or NULL for the identity map (fix_tip = 0, n_dof = 18 n_el)
+ *   beta       collocation offset (Num_Param_T::beta, model_parameters.h:78)
+ */
+int hfpx_mesh_create(hfpx_ctx* ctx, const double* nods_host, int64_t nods_rows,
+                     int64_t n_nodes, const int64_t* conn_host, int64_t conn_rows,
+                     int64_t n_el, const int64_t* dof_h_host, int64_t n_dof,
+                     double beta, hfpx_mesh** mesh);
+int hfpx_mesh_destroy(hfpx_mesh* mesh);
+int64_t hfpx_mesh_n_dof(const hfpx_mesh* mesh);
+
+/* ---- assembly ---------------------------------------------------------- */
+/* Dense influence matrix A (n_dof x n_dof, FP64, device, lda >= n_dof):
+ * element geometry + every (collocation point, source element) traction
+ * block, written once per entry.  Replaces hfp3d::make_3dbem_matrix_s
+ * (src/Solvers/system_assembly.cpp:30-219, declared system_assembly.h:35-39). */
+int hfpx_assemble(hfpx_ctx* ctx, hfpx_mesh* mesh, double shear_m, double poiss_r,
+                  double* A, int64_t lda);
+/* Same matrix rounded to FP32 (device, lda32), optionally together with the
+ * FP64 copy in the same pass (A64 may be NULL).  No reference counterpart:
+ * the FP32 copy is the factorisation operand of the mixed-precision solve. */
+int hfpx_assemble_mixed(hfpx_ctx* ctx, hfpx_mesh* mesh, double shear_m, double poiss_r,
+                        double* A64, int64_t lda64, float* A32, int64_t lda32);
+
+/* Right-hand sides for nrhs far-field stress tensors sinf_host (6 x nrhs,
+ * order xx yy zz xy xz yz, tension positive): B(:, r) -= traction of
+ * sinf(:, r) on every free DoF (B device, n_dof x nrhs, ldb).  Replaces
+ * hfp3d::add_s_inf_to_3dbem_rhs (src/Solvers/system_assembly.cpp:222-274). */
+int hfpx_rhs_sinf(hfpx_ctx* ctx, hfpx_mesh* mesh, const double* sinf_host, int nrhs,
+                  double* B, int64_t ldb);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HFPX3D_H */

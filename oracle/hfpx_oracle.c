@@ -1,0 +1,661 @@
+/*
+ * hfpx_oracle.c -- CPU restatement of the HFPx3D hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker that the HIP path is compared
+ * against; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load it.  It is never part of the product path.
+ *
+ * What it restates (reference paths relative to the reference root):
+ *   element geometry        src/Core/element_utilities.cpp:62-238, 339-395
+ *   per-pair kernel         src/Elasticity/elasticity_kernel_integration.cpp:34-205
+ *                           (values specified by src/Elasticity/h_potential.cpp;
+ *                            the integrals are re-derived here, see below)
+ *   traction / rotation     src/Core/tensor_utilities.cpp:21-91
+ *   DoF handle              src/Core/surface_mesh_utilities.cpp:28-112
+ *   assembly / rhs / stress src/Solvers/system_assembly.cpp:30-455
+ *   DD write-back           src/Core/surface_mesh_utilities.cpp:256-328
+ *   LU (dgetrf / dgetrs)    il/linear_algebra/dense/factorization/LU.h:79-138
+ *                           (LAPACK right-looking LU with partial pivoting)
+ *   Sneddon opening         src/Solvers/test_radial_crack_static.cpp:316-337
+ *
+ * The per-pair kernel.  The stress at X due to a displacement discontinuity
+ * D_i P(xi) on a flat element (plane x3 = 0 of the element frame) follows
+ * from Kelvin's solution; each stress component is a combination of second
+ * derivatives of the single-layer potential Phi[P](X) = int P/|X - xi| dA and
+ * of x3 times its third derivatives (derived symbolically from Kelvin's
+ * tensor).  Integrating by parts in the element plane turns every derivative
+ * of Phi into line integrals along the three edges of moments
+ * int a^j / r^(2l+1) da (a = arc coordinate, r^2 = a^2 + c^2) plus the
+ * solid-angle term dPhi[1]/dx3 and Phi[1].  The hypersingular in-plane case
+ * (x3 = 0, the point inside the element) is the x3 -> 0 limit of the same
+ * formulas with the solid-angle term taken at its principal value (0).
+ * Parity status: pinned against the reference build (oracle/_ref), the
+ * MATLAB fixtures of the reference tests and brute-force quadrature.
+ */
+#include "hfpx_oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef double complex cplx;
+
+static const double kPi = 3.14159265358979323846;
+
+/* ---------------------------------------------------------------------- */
+/* element geometry (element_utilities.cpp:62-87, 89-118, 170-238, 364-395) */
+/* ---------------------------------------------------------------------- */
+
+static void cross3(const double* a, const double* b, double* c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+static void normalize3(double* a) {
+  double n = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+  a[0] /= n; a[1] /= n; a[2] /= n;
+}
+
+/* rows of rt: e1 = (v1-v0)/|.|, e3 = normalize(e1 x (v2-v0)), e2 = e3 x e1
+ * (element_utilities.cpp:62-87); rt col-major: rt[row + 3*col]. */
+static void el_rt(const double* vert, double* rt) {
+  double a1[3], a2[3], e1[3], e2[3], e3[3];
+  for (int k = 0; k < 3; ++k) {
+    a1[k] = vert[k + 3] - vert[k];
+    a2[k] = vert[k + 6] - vert[k];
+  }
+  memcpy(e1, a1, sizeof e1);
+  normalize3(e1);
+  cross3(e1, a2, e3);
+  normalize3(e3);
+  cross3(e3, e1, e2);
+  normalize3(e2);
+  for (int k = 0; k < 3; ++k) {
+    rt[0 + 3 * k] = e1[k];
+    rt[1 + 3 * k] = e2[k];
+    rt[2 + 3 * k] = e3[k];
+  }
+}
+
+/* y = R (x - o) */
+static void to_local(const double* rt, const double* o, const double* x, double* y) {
+  double d[3] = {x[0] - o[0], x[1] - o[1], x[2] - o[2]};
+  for (int i = 0; i < 3; ++i)
+    y[i] = rt[i] * d[0] + rt[i + 3] * d[1] + rt[i + 6] * d[2];
+}
+
+/* Shape functions of the 6-node triangle in complex monomials
+ * [1, t, tb, t^2, tb^2, t*tb]: vertices N_v = L_v (2 L_v - 1), edge nodes
+ * N_{3+v} = 4 L_{v+1} L_{v+2} (node 3+v sits mid-edge opposite vertex v,
+ * element_utilities.cpp:380-393).  The barycentric coordinates are written
+ * as L_k = l0 + l1 t + conj(l1) tb. */
+static void lin_mul(const cplx* a, const cplx* b, double s, cplx* q) {
+  /* q += s * (a0 + a1 t + a2 tb)(b0 + b1 t + b2 tb) */
+  q[0] += s * a[0] * b[0];
+  q[1] += s * (a[0] * b[1] + a[1] * b[0]);
+  q[2] += s * (a[0] * b[2] + a[2] * b[0]);
+  q[3] += s * a[1] * b[1];
+  q[4] += s * a[2] * b[2];
+  q[5] += s * (a[1] * b[2] + a[2] * b[1]);
+}
+
+static void el_sfm(const cplx* t, cplx* sfm /* 6x6 col-major */) {
+  /* twice the signed area */
+  double area2 = cimag(conj(t[1] - t[0]) * (t[2] - t[0]));
+  cplx L[3][3];
+  for (int k = 0; k < 3; ++k) {
+    cplx ta = t[(k + 1) % 3], tb = t[(k + 2) % 3];
+    /* L_k(t) = Im(conj(ta - t) (tb - t)) / area2 */
+    L[k][0] = cimag(conj(ta) * tb) / area2;
+    L[k][1] = (conj(tb) - conj(ta)) / (2.0 * I) / area2;
+    L[k][2] = (ta - tb) / (2.0 * I) / area2;
+  }
+  for (int j = 0; j < 36; ++j) sfm[j] = 0;
+  for (int v = 0; v < 3; ++v) {
+    cplx q[6] = {0, 0, 0, 0, 0, 0};
+    lin_mul(L[v], L[v], 2.0, q);
+    q[0] -= L[v][0]; q[1] -= L[v][1]; q[2] -= L[v][2];
+    for (int c = 0; c < 6; ++c) sfm[v + 6 * c] = q[c];
+    cplx e[6] = {0, 0, 0, 0, 0, 0};
+    lin_mul(L[(v + 1) % 3], L[(v + 2) % 3], 4.0, e);
+    for (int c = 0; c < 6; ++c) sfm[3 + v + 6 * c] = e[c];
+  }
+}
+
+void hfpo_el_geom(const double* vert, double beta, double* rt, double* tau,
+                  double* sfm, double* cp) {
+  el_rt(vert, rt);
+  cplx t[3];
+  for (int j = 0; j < 3; ++j) {
+    double y[3];
+    to_local(rt, vert, vert + 3 * j, y);
+    t[j] = y[0] + I * y[1];
+    tau[2 * j] = y[0];
+    tau[2 * j + 1] = y[1];
+  }
+  cplx s[36];
+  el_sfm(t, s);
+  for (int j = 0; j < 36; ++j) {
+    sfm[2 * j] = creal(s[j]);
+    sfm[2 * j + 1] = cimag(s[j]);
+  }
+  /* collocation points (element_utilities.cpp:364-395) */
+  double c[3];
+  for (int k = 0; k < 3; ++k) c[k] = (vert[k] + vert[k + 3] + vert[k + 6]) / 3.0;
+  for (int v = 0; v < 3; ++v) {
+    int m = (v + 1) % 3, l = (m + 1) % 3;
+    for (int k = 0; k < 3; ++k) {
+      cp[k + 3 * v] = (1.0 - beta) * vert[k + 3 * v] + beta * c[k];
+      cp[k + 3 * (v + 3)] = 0.5 * (1.0 - beta) * (vert[k + 3 * m] + vert[k + 3 * l]) + beta * c[k];
+    }
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* the per-pair kernel                                                      */
+/* ---------------------------------------------------------------------- */
+
+/* Edge moments M[j][l] = int_{am}^{an} a^j (a^2 + c2)^(-l - 1/2) da,
+ * j <= 4, l <= 2, in forms that keep full relative accuracy when the point
+ * is close to the edge line (small c) and the end coordinates share a sign. */
+typedef struct { double m[5][3]; } Moments;
+
+static void edge_moments(double am, double an, double c2, Moments* M) {
+  double rm = sqrt(am * am + c2), rn = sqrt(an * an + c2);
+  /* Lambda = int da / r */
+  double lam, f01, f02;
+  /* per-end pieces with the 1/c^k constants split off */
+  double gm1, gn1, gm2, gn2;
+  if (am >= 0.0) {
+    lam = log((an + rn) / (am + rm));
+    gm1 = -1.0 / (rm * (rm + am));
+    gn1 = -1.0 / (rn * (rn + an));
+    gm2 = -(2.0 * rm + am) / (3.0 * rm * rm * rm * (rm + am) * (rm + am));
+    gn2 = -(2.0 * rn + an) / (3.0 * rn * rn * rn * (rn + an) * (rn + an));
+    f01 = gn1 - gm1;
+    f02 = gn2 - gm2;
+  } else if (an <= 0.0) {
+    lam = log((rm - am) / (rn - an));
+    gm1 = 1.0 / (rm * (rm - am));
+    gn1 = 1.0 / (rn * (rn - an));
+    gm2 = (2.0 * rm - am) / (3.0 * rm * rm * rm * (rm - am) * (rm - am));
+    gn2 = (2.0 * rn - an) / (3.0 * rn * rn * rn * (rn - an) * (rn - an));
+    f01 = gn1 - gm1;
+    f02 = gn2 - gm2;
+  } else { /* am < 0 < an: the foot of the point lies on the edge span */
+    lam = log((an + rn) * (rm - am) / c2);
+    gm1 = 1.0 / (rm * (rm - am));
+    gn1 = -1.0 / (rn * (rn + an));
+    gm2 = (2.0 * rm - am) / (3.0 * rm * rm * rm * (rm - am) * (rm - am));
+    gn2 = -(2.0 * rn + an) / (3.0 * rn * rn * rn * (rn + an) * (rn + an));
+    f01 = 2.0 / c2 + gn1 - gm1;
+    f02 = 4.0 / (3.0 * c2 * c2) + gn2 - gm2;
+  }
+  /* differences of r, 1/r, 1/r^3 without cancellation */
+  double dr = (an - am) * (an + am) / (rn + rm);        /* rn - rm */
+  double dinv = -dr / (rn * rm);                        /* 1/rn - 1/rm */
+  double dinv3 = -dr * (rn * rn + rn * rm + rm * rm) / (rm * rm * rm * rn * rn * rn);
+  M->m[0][0] = lam;
+  M->m[1][0] = dr;
+  M->m[0][1] = f01;
+  M->m[1][1] = -dinv;
+  M->m[2][1] = lam - c2 * f01;
+  M->m[3][1] = dr + c2 * dinv;
+  M->m[0][2] = f02;
+  M->m[1][2] = -dinv3 / 3.0;
+  M->m[2][2] = f01 - c2 * f02;
+  M->m[3][2] = -dinv - c2 * (-dinv3 / 3.0);
+  M->m[4][2] = lam - 2.0 * c2 * f01 + c2 * c2 * f02;
+  M->m[2][0] = M->m[3][0] = M->m[4][0] = 0.0; /* unused */
+  M->m[4][1] = 0.0;
+}
+
+/* per-end solid-angle function (sign handled by the caller) */
+static double solid_end(double a, double s, double ax3) {
+  if (s == 0.0) return 0.0;
+  double r = sqrt(a * a + s * s + ax3 * ax3);
+  return atan(a * s * (a * a + s * s) / ((r + ax3) * (s * s * r + ax3 * a * a)));
+}
+
+static cplx integ(const cplx* p, int np, int l, const Moments* M) {
+  cplx acc = 0;
+  for (int j = 0; j < np; ++j) acc += p[j] * M->m[j][l];
+  return acc;
+}
+
+void hfpo_local_submatrix(double shear_m, double poiss_r, double h, double z_re,
+                          double z_im, const double* tau, const double* sfm,
+                          double* out) {
+  const double nu = poiss_r;
+  const double x3 = -h; /* h is minus the local x3 coordinate (element_utilities.cpp:163) */
+  const cplx z = z_re + I * z_im, zc = conj(z);
+  cplx t[3];
+  for (int k = 0; k < 3; ++k) t[k] = tau[2 * k] + I * tau[2 * k + 1];
+
+  /* shape functions re-centred at z: coefficients of [1, w, wb, w^2, wb^2, w wb],
+   * w = t - z (the binomial shift of element_utilities.cpp:339-360) */
+  cplx p[6][6];
+  for (int j = 0; j < 6; ++j) {
+    cplx q[6];
+    for (int c = 0; c < 6; ++c) q[c] = sfm[2 * (j + 6 * c)] + I * sfm[2 * (j + 6 * c) + 1];
+    p[j][0] = q[0] + q[1] * z + q[2] * zc + q[3] * z * z + q[4] * zc * zc + q[5] * z * zc;
+    p[j][1] = q[1] + 2.0 * q[3] * z + q[5] * zc;
+    p[j][2] = q[2] + 2.0 * q[4] * zc + q[5] * z;
+    p[j][3] = q[3];
+    p[j][4] = q[4];
+    p[j][5] = q[5];
+  }
+
+  /* edge geometry */
+  double es[3];
+  cplx enu[3];
+  Moments em[3];
+  double w1 = 0.0, phi1 = 0.0;
+  cplx enu_r = 0, enub_r = 0;
+  const double ax3 = fabs(x3);
+  const double sgn3 = (x3 > 0) ? 1.0 : ((x3 < 0) ? -1.0 : 0.0);
+  for (int m = 0; m < 3; ++m) {
+    cplx tm = t[m], tn = t[(m + 1) % 3];
+    double L = cabs(tn - tm);
+    cplx th = (tn - tm) / L;
+    cplx nv = -I * th; /* outward normal of the counter-clockwise boundary */
+    double s = creal((tm - z) * conj(nv));
+    double am = creal((tm - z) * conj(th));
+    double an = am + L;
+    double c2 = s * s + x3 * x3;
+    edge_moments(am, an, c2, &em[m]);
+    es[m] = s;
+    enu[m] = nv;
+    w1 += -sgn3 * (solid_end(an, s, ax3) - solid_end(am, s, ax3));
+    phi1 += s * em[m].m[0][0];
+    enu_r += nv * em[m].m[0][0];
+    enub_r += conj(nv) * em[m].m[0][0];
+  }
+  phi1 += x3 * w1;
+
+  const double K = shear_m / (16.0 * kPi * (1.0 - nu));
+  for (int j = 0; j < 6; ++j) {
+    const cplx p00 = p[j][0], p10 = p[j][1], p01 = p[j][2], p20 = p[j][3],
+               p02 = p[j][4], p11 = p[j][5];
+    cplx A = 0, B = 0, C = 0, D = 0, F = 0, Gv = 0, H = 0;
+    for (int m = 0; m < 3; ++m) {
+      const double s = es[m];
+      const cplx nv = enu[m], nb = conj(nv), n2 = nv * nv;
+      const Moments* M = &em[m];
+      /* density along the edge: w = (s + i a) nu */
+      cplx P[3], Q[2];
+      P[0] = p00 + p10 * s * nv + p01 * s * nb + p20 * s * s * n2 + p02 * s * s * nb * nb + p11 * s * s;
+      P[1] = p10 * I * nv - p01 * I * nb + p20 * 2.0 * I * s * n2 - p02 * 2.0 * I * s * nb * nb;
+      P[2] = -p20 * n2 - p02 * nb * nb + p11;
+      /* d/d(wb) of the density: p01 + 2 p02 wb + p11 w */
+      Q[0] = p01 + 2.0 * p02 * s * nb + p11 * s * nv;
+      Q[1] = -2.0 * p02 * I * nb + p11 * I * nv;
+      cplx smP[4] = {s * P[0], s * P[1] - I * P[0], s * P[2] - I * P[1], -I * P[2]};
+      cplx spP[4] = {s * P[0], s * P[1] + I * P[0], s * P[2] + I * P[1], I * P[2]};
+      cplx r2P[5] = {s * s * P[0], s * s * P[1], s * s * P[2] + P[0], P[1], P[2]};
+      cplx sp2P[5] = {s * s * P[0], s * s * P[1] + 2.0 * I * s * P[0],
+                      s * s * P[2] + 2.0 * I * s * P[1] - P[0], 2.0 * I * s * P[2] - P[1], -P[2]};
+      cplx smQ[3] = {s * Q[0], s * Q[1] - I * Q[0], -I * Q[1]};
+      cplx spQ[3] = {s * Q[0], s * Q[1] + I * Q[0], I * Q[1]};
+      const double m00 = M->m[0][0];
+      A += -integ(smP, 4, 1, M) - 2.0 * nb * integ(Q, 2, 0, M);
+      B += -n2 * integ(spP, 4, 1, M) - 2.0 * nv * integ(Q, 2, 0, M);
+      C += x3 * nv * integ(P, 3, 1, M);
+      Gv += 3.0 * x3 * integ(smP, 4, 2, M) + 2.0 * x3 * nb * integ(Q, 2, 1, M);
+      H += 3.0 * x3 * n2 * integ(spP, 4, 2, M) + 2.0 * x3 * nv * integ(Q, 2, 1, M);
+      D += 2.0 * nv * integ(P, 3, 1, M) - 3.0 * nv * integ(r2P, 5, 2, M) -
+           2.0 * integ(smQ, 3, 1, M) - 8.0 * p02 * nb * m00;
+      F += -3.0 * n2 * nv * integ(sp2P, 5, 2, M) - 2.0 * n2 * integ(spQ, 3, 1, M) -
+           8.0 * p02 * nv * m00;
+    }
+    A += 4.0 * p11 * phi1;
+    B += 8.0 * p02 * phi1;
+    C += 2.0 * (p01 * w1 + x3 * (p11 * enu_r + 2.0 * p02 * enub_r));
+    Gv += 4.0 * p11 * w1;
+    H += 8.0 * p02 * w1;
+    const double a = creal(A), g = creal(Gv);
+    /* stress combos S0 = S11+S22, S2 = S11-S22+2iS12, S33, S1 = S13+iS23 */
+    double S0[3], S33[3];
+    cplx S2[3], S1[3];
+    S0[0] = K * (-8.0 * (1.0 + nu) * creal(C) + 4.0 * x3 * creal(D));
+    S2[0] = K * (-8.0 * (1.0 - nu) * C + 2.0 * x3 * (D + F));
+    S33[0] = -4.0 * K * x3 * creal(D);
+    S1[0] = K * (2.0 * (2.0 - nu) * a + 2.0 * nu * B + 2.0 * x3 * (g + H));
+    S0[1] = K * (-8.0 * (1.0 + nu) * cimag(C) + 4.0 * x3 * cimag(D));
+    S2[1] = K * (-8.0 * I * (1.0 - nu) * C + 2.0 * I * x3 * (D - F));
+    S33[1] = -4.0 * K * x3 * cimag(D);
+    S1[1] = I * K * (2.0 * (2.0 - nu) * a - 2.0 * nu * B + 2.0 * x3 * (g - H));
+    S0[2] = 4.0 * K * ((1.0 + 2.0 * nu) * a + x3 * g);
+    S2[2] = 4.0 * K * ((1.0 - 2.0 * nu) * B + x3 * H);
+    S33[2] = 4.0 * K * (a - x3 * g);
+    S1[2] = -4.0 * K * x3 * D;
+    for (int k = 0; k < 3; ++k) {
+      double* col = out + 6 * (3 * j + k);
+      col[0] = 0.5 * (S0[k] + creal(S2[k]));
+      col[1] = 0.5 * (S0[k] - creal(S2[k]));
+      col[2] = S33[k];
+      col[3] = 0.5 * cimag(S2[k]);
+      col[4] = creal(S1[k]);
+      col[5] = cimag(S1[k]);
+    }
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* DoF handle (surface_mesh_utilities.cpp:28-112)                           */
+/* ---------------------------------------------------------------------- */
+
+int64_t hfpo_dof_h(const double* nods, int64_t nods_rows, int64_t n_nodes,
+                   const int64_t* conn, int64_t conn_rows, int64_t n_el,
+                   int tip_type, int64_t* dof_h) {
+  (void)n_nodes;
+  const int ext = (nods_rows >= 5 && conn_rows >= 5);
+  int64_t n_dof = 18 * n_el, dec = 0;
+  for (int64_t e = 0; e < n_el; ++e) {
+    int tip[3] = {0, 0, 0};
+    if (ext && conn[3 + conn_rows * e] == conn[4 + conn_rows * e]) {
+      for (int v = 0; v < 3; ++v) {
+        int64_t n = conn[v + conn_rows * e];
+        tip[v] = nods[3 + nods_rows * n] == 1.0 && nods[4 + nods_rows * n] == 1.0;
+      }
+    }
+    for (int v = 0; v < 6; ++v) {
+      int fixed = 0;
+      if (tip_type >= 1 && v < 3 && tip[v]) fixed = 1;
+      if (tip_type == 2 && v >= 3 && tip[(v - 3 + 1) % 3] && tip[(v - 3 + 2) % 3]) fixed = 1;
+      for (int l = 0; l < 3; ++l) {
+        int64_t ld = 3 * v + l;
+        if (fixed) {
+          dof_h[e + n_el * ld] = -1;
+          --n_dof;
+          ++dec;
+        } else {
+          dof_h[e + n_el * ld] = 18 * e + ld - dec;
+        }
+      }
+    }
+  }
+  return n_dof;
+}
+
+/* ---------------------------------------------------------------------- */
+/* assembly (system_assembly.cpp:30-219)                                    */
+/* ---------------------------------------------------------------------- */
+
+static void el_vert(const double* nods, int64_t nods_rows, const int64_t* conn,
+                    int64_t conn_rows, int64_t e, double* v) {
+  for (int j = 0; j < 3; ++j) {
+    int64_t n = conn[j + conn_rows * e];
+    for (int k = 0; k < 3; ++k) v[k + 3 * j] = nods[k + nods_rows * n];
+  }
+}
+
+/* 18x18 traction influence of source element es on the 6 collocation
+ * points of target element et, global frame (system_assembly.cpp:88-199). */
+static void el2el_block(const double* vs, const double* vt, double shear_m,
+                        double poiss_r, double beta, double* blk /* 18x18 col-major */) {
+  double rs[9], taus[6], sfms[72], cps[18];
+  hfpo_el_geom(vs, beta, rs, taus, sfms, cps);
+  double rt[9], taut[6], sfmt[72], cpt[18];
+  hfpo_el_geom(vt, beta, rt, taut, sfmt, cpt);
+  double nglob[3] = {-rt[2], -rt[5], -rt[8]};
+  double nloc[3];
+  for (int i = 0; i < 3; ++i) nloc[i] = rs[i] * nglob[0] + rs[i + 3] * nglob[1] + rs[i + 6] * nglob[2];
+  for (int nt = 0; nt < 6; ++nt) {
+    double y[3];
+    to_local(rs, vs, cpt + 3 * nt, y);
+    double S[108];
+    hfpo_local_submatrix(shear_m, poiss_r, -y[2], y[0], y[1], taus, sfms, S);
+    for (int c = 0; c < 18; ++c) {
+      const double* s = S + 6 * c;
+      /* traction = N(n) . S  (tensor_utilities.cpp:21-37) */
+      double tl[3] = {nloc[0] * s[0] + nloc[1] * s[3] + nloc[2] * s[4],
+                      nloc[0] * s[3] + nloc[1] * s[1] + nloc[2] * s[5],
+                      nloc[0] * s[4] + nloc[1] * s[5] + nloc[2] * s[2]};
+      double tg[3];
+      for (int i = 0; i < 3; ++i) tg[i] = rs[0 + 3 * i] * tl[0] + rs[1 + 3 * i] * tl[1] + rs[2 + 3 * i] * tl[2];
+      for (int i = 0; i < 3; ++i) blk[3 * nt + i + 18 * c] = tg[i];
+    }
+    /* DD to global: per source node, T . R_s (system_assembly.cpp:164-185) */
+    for (int ns = 0; ns < 6; ++ns) {
+      double tmp[3][3];
+      for (int i = 0; i < 3; ++i)
+        for (int jj = 0; jj < 3; ++jj) tmp[i][jj] = blk[3 * nt + i + 18 * (3 * ns + jj)];
+      for (int i = 0; i < 3; ++i)
+        for (int jj = 0; jj < 3; ++jj)
+          blk[3 * nt + i + 18 * (3 * ns + jj)] =
+              tmp[i][0] * rs[0 + 3 * jj] + tmp[i][1] * rs[1 + 3 * jj] + tmp[i][2] * rs[2 + 3 * jj];
+    }
+  }
+}
+
+int64_t hfpo_assemble(const double* nods, int64_t nods_rows, int64_t n_nodes,
+                      const int64_t* conn, int64_t conn_rows, int64_t n_el,
+                      const int64_t* dof_h, int64_t n_dof, double shear_m,
+                      double poiss_r, double beta, int64_t t0, int64_t t1,
+                      int nthreads, double* A) {
+  (void)n_nodes;
+  if (t1 <= t0) { t0 = 0; t1 = n_el; }
+  memset(A, 0, sizeof(double) * (size_t)n_dof * (size_t)n_dof);
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic)
+#endif
+  for (int64_t es = 0; es < n_el; ++es) {
+    double vs[9], vt[9], blk[324];
+    el_vert(nods, nods_rows, conn, conn_rows, es, vs);
+    for (int64_t et = t0; et < t1; ++et) {
+      el_vert(nods, nods_rows, conn, conn_rows, et, vt);
+      el2el_block(vs, vt, shear_m, poiss_r, beta, blk);
+      for (int c = 0; c < 18; ++c) {
+        int64_t gc = dof_h[es + n_el * c];
+        if (gc < 0) continue;
+        for (int r = 0; r < 18; ++r) {
+          int64_t gr = dof_h[et + n_el * r];
+          if (gr < 0) continue;
+          A[gr + n_dof * gc] += blk[r + 18 * c];
+        }
+      }
+    }
+  }
+  return n_dof;
+}
+
+/* ---------------------------------------------------------------------- */
+/* right-hand side (system_assembly.cpp:222-274)                            */
+/* ---------------------------------------------------------------------- */
+
+void hfpo_rhs(const double* nods, int64_t nods_rows, int64_t n_nodes,
+              const int64_t* conn, int64_t conn_rows, int64_t n_el,
+              const int64_t* dof_h, const double* s_inf, double* rhs) {
+  (void)n_nodes;
+  for (int64_t e = 0; e < n_el; ++e) {
+    double v[9], rt[9];
+    el_vert(nods, nods_rows, conn, conn_rows, e, v);
+    el_rt(v, rt);
+    double n[3] = {-rt[2], -rt[5], -rt[8]};
+    /* s_inf = (xx, yy, zz, xy, xz, yz) */
+    double tr[3] = {n[0] * s_inf[0] + n[1] * s_inf[3] + n[2] * s_inf[4],
+                    n[0] * s_inf[3] + n[1] * s_inf[1] + n[2] * s_inf[5],
+                    n[0] * s_inf[4] + n[1] * s_inf[5] + n[2] * s_inf[2]};
+    for (int ln = 0; ln < 6; ++ln)
+      for (int l = 0; l < 3; ++l) {
+        int64_t d = dof_h[e + n_el * (3 * ln + l)];
+        if (d >= 0) rhs[d] -= tr[l];
+      }
+  }
+}
+
+void hfpo_write_dd(const double* x, const int64_t* dof_h, int64_t n_el, double* dd) {
+  const int64_t nr = 6 * n_el;
+  for (int64_t e = 0; e < n_el; ++e)
+    for (int en = 0; en < 6; ++en)
+      for (int i = 0; i < 3; ++i) {
+        int64_t d = dof_h[e + n_el * (3 * en + i)];
+        dd[6 * e + en + nr * i] = d >= 0 ? x[d] : 0.0;
+      }
+}
+
+/* ---------------------------------------------------------------------- */
+/* stress postprocessing (system_assembly.cpp:279-455)                      */
+/* ---------------------------------------------------------------------- */
+
+void hfpo_stress(const double* nods, int64_t nods_rows, int64_t n_nodes,
+                 const int64_t* conn, int64_t conn_rows, int64_t n_el,
+                 double shear_m, double poiss_r, const double* dd,
+                 const double* pts, int64_t n_pts, const double* s_inf,
+                 int nthreads, double* out) {
+  (void)n_nodes;
+  const int64_t nr = 6 * n_el;
+  double* geo = (double*)malloc(sizeof(double) * 96 * (size_t)n_el);
+  for (int64_t e = 0; e < n_el; ++e) {
+    double v[9], cp[18];
+    el_vert(nods, nods_rows, conn, conn_rows, e, v);
+    double* g = geo + 96 * e; /* rt 9 | tau 6 | sfm 72 | v0 3 | pad */
+    hfpo_el_geom(v, 0.0, g, g + 9, g + 15, cp);
+    g[87] = v[0]; g[88] = v[1]; g[89] = v[2];
+  }
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 4)
+#endif
+  for (int64_t p = 0; p < n_pts; ++p) {
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int64_t e = 0; e < n_el; ++e) {
+      const double* g = geo + 96 * e;
+      const double* rs = g;
+      double y[3];
+      to_local(rs, g + 87, pts + 3 * p, y);
+      double S[108];
+      hfpo_local_submatrix(shear_m, poiss_r, -y[2], y[0], y[1], g + 9, g + 15, S);
+      /* DD of the element in its local frame: d_loc = R d_glob */
+      for (int ns = 0; ns < 6; ++ns) {
+        double dg[3] = {dd[6 * e + ns], dd[6 * e + ns + nr], dd[6 * e + ns + 2 * nr]};
+        double dl[3];
+        for (int i = 0; i < 3; ++i) dl[i] = rs[i] * dg[0] + rs[i + 3] * dg[1] + rs[i + 6] * dg[2];
+        /* local stress, then R^T S R (tensor_utilities.cpp:57-91) */
+        double sl[6] = {0, 0, 0, 0, 0, 0};
+        for (int k = 0; k < 3; ++k)
+          for (int c = 0; c < 6; ++c) sl[c] += S[c + 6 * (3 * ns + k)] * dl[k];
+        double m[3][3] = {{sl[0], sl[3], sl[4]}, {sl[3], sl[1], sl[5]}, {sl[4], sl[5], sl[2]}};
+        double tg[3][3];
+        for (int i = 0; i < 3; ++i)
+          for (int jj = 0; jj < 3; ++jj) {
+            double v = 0;
+            for (int a = 0; a < 3; ++a)
+              for (int b = 0; b < 3; ++b) v += rs[a + 3 * i] * m[a][b] * rs[b + 3 * jj];
+            tg[i][jj] = v;
+          }
+        acc[0] += tg[0][0]; acc[1] += tg[1][1]; acc[2] += tg[2][2];
+        acc[3] += tg[0][1]; acc[4] += tg[0][2]; acc[5] += tg[1][2];
+      }
+    }
+    for (int c = 0; c < 6; ++c) out[p + n_pts * c] = acc[c] + s_inf[c];
+  }
+  free(geo);
+}
+
+/* ---------------------------------------------------------------------- */
+/* LU with partial pivoting (LAPACK dgetrf / dgetrs semantics)              */
+/* ---------------------------------------------------------------------- */
+
+int64_t hfpo_getrf(int64_t n, double* A, int64_t lda, int32_t* ipiv, int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  const int64_t nb = 64;
+  int64_t info = 0;
+  for (int64_t k = 0; k < n; k += nb) {
+    int64_t kb = (n - k < nb) ? n - k : nb;
+    /* unblocked panel A[k:n, k:k+kb] */
+    for (int64_t j = k; j < k + kb; ++j) {
+      int64_t p = j;
+      double mx = fabs(A[j + lda * j]);
+      for (int64_t i = j + 1; i < n; ++i) {
+        double v = fabs(A[i + lda * j]);
+        if (v > mx) { mx = v; p = i; }
+      }
+      ipiv[j] = (int32_t)(p + 1);
+      if (A[p + lda * j] != 0.0) {
+        if (p != j)
+          for (int64_t c = 0; c < n; ++c) {
+            double t = A[j + lda * c];
+            A[j + lda * c] = A[p + lda * c];
+            A[p + lda * c] = t;
+          }
+        double inv = 1.0 / A[j + lda * j];
+        for (int64_t i = j + 1; i < n; ++i) A[i + lda * j] *= inv;
+      } else if (info == 0) {
+        info = j + 1;
+      }
+      for (int64_t c = j + 1; c < k + kb; ++c) {
+        double u = A[j + lda * c];
+        if (u != 0.0)
+          for (int64_t i = j + 1; i < n; ++i) A[i + lda * c] -= A[i + lda * j] * u;
+      }
+    }
+    if (k + kb >= n) break;
+    /* U12 = L11^-1 A12 */
+#ifdef _OPENMP
+#pragma omp parallel for
+#endif
+    for (int64_t c = k + kb; c < n; ++c)
+      for (int64_t j = k; j < k + kb; ++j) {
+        double u = A[j + lda * c];
+        for (int64_t i = j + 1; i < k + kb; ++i) A[i + lda * c] -= A[i + lda * j] * u;
+      }
+    /* A22 -= L21 U12 */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int64_t c = k + kb; c < n; ++c)
+      for (int64_t j = k; j < k + kb; ++j) {
+        double u = A[j + lda * c];
+        if (u == 0.0) continue;
+        const double* l = A + lda * j;
+        double* col = A + lda * c;
+        for (int64_t i = k + kb; i < n; ++i) col[i] -= l[i] * u;
+      }
+  }
+  return info;
+}
+
+void hfpo_getrs(int64_t n, int64_t nrhs, const double* LU, int64_t lda,
+                const int32_t* ipiv, double* B, int64_t ldb) {
+  for (int64_t r = 0; r < nrhs; ++r) {
+    double* b = B + ldb * r;
+    for (int64_t i = 0; i < n; ++i) {
+      int64_t p = ipiv[i] - 1;
+      if (p != i) { double t = b[i]; b[i] = b[p]; b[p] = t; }
+    }
+    for (int64_t j = 0; j < n; ++j) {
+      double v = b[j];
+      if (v != 0.0)
+        for (int64_t i = j + 1; i < n; ++i) b[i] -= LU[i + lda * j] * v;
+    }
+    for (int64_t j = n - 1; j >= 0; --j) {
+      b[j] /= LU[j + lda * j];
+      double v = b[j];
+      if (v != 0.0)
+        for (int64_t i = 0; i < j; ++i) b[i] -= LU[i + lda * j] * v;
+    }
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Sneddon's penny crack opening (test_radial_crack_static.cpp:316-337)     */
+/* ---------------------------------------------------------------------- */
+
+void hfpo_sneddon_dd(double shear_m, double poiss_r, double a, double p,
+                     const double* xyz, double* dd) {
+  double r2 = xyz[0] * xyz[0] + xyz[1] * xyz[1];
+  dd[0] = dd[1] = dd[2] = 0.0;
+  if (r2 < a * a)
+    dd[2] = 4.0 * (1.0 - poiss_r) * a * p / (kPi * shear_m) * sqrt(1.0 - r2 / (a * a));
+}

@@ -1,0 +1,82 @@
+/*
+ * hfpx_oracle.h -- CPU restatement of the HFPx3D hot path (TEST INFRASTRUCTURE).
+ *
+ * This library is the checker, not the product.  Only tests/, the smoke()
+ * entry point and the cpu_baseline leg of bench.py may load it.  The HIP path
+ * never calls into it and fails loudly when its own extension is missing.
+ *
+ * Conventions (all follow the reference's il::Array2D semantics):
+ *   - arrays are column-major; a "3x6" array has element (i, j) at i + 3*j;
+ *   - complex values are stored as interleaved (re, im) doubles;
+ *   - indices are int64_t (il::int_t), pivots are 1-based int32 (lapack_int).
+ */
+#ifndef HFPX_ORACLE_H
+#define HFPX_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Element geometry.  vert: 3x3 (coordinate k of vertex j at k + 3*j).
+ * Outputs rt (3x3 rotation, rows e1, e2, e3), tau (3 complex),
+ * sfm (6x6 complex, rows = shape functions, cols = monomials
+ * [1, t, tb, t^2, tb^2, t*tb]), cp (3x6 collocation points for beta). */
+void hfpo_el_geom(const double* vert, double beta, double* rt, double* tau,
+                  double* sfm, double* cp);
+
+/* 6x18 stress influence (rows S11 S22 S33 S12 S13 S23, cols 3*node + DD
+ * component, all in the source element's local frame) of one element at the
+ * point (h, z) given in that frame. */
+void hfpo_local_submatrix(double shear_m, double poiss_r, double h, double z_re,
+                          double z_im, const double* tau, const double* sfm,
+                          double* out);
+
+/* DoF handle (n_el x 18, col-major, -1 = fixed).  Returns n_dof. */
+int64_t hfpo_dof_h(const double* nods, int64_t nods_rows, int64_t n_nodes,
+                   const int64_t* conn, int64_t conn_rows, int64_t n_el,
+                   int tip_type, int64_t* dof_h);
+
+/* Dense influence matrix, n_dof x n_dof col-major (lda = n_dof), zero-filled
+ * by the callee.  Target rows [t0, t1) of ELEMENTS only are computed when
+ * t1 > t0 (rows of other targets are left zero); pass t0 = t1 = 0 for all. */
+int64_t hfpo_assemble(const double* nods, int64_t nods_rows, int64_t n_nodes,
+                      const int64_t* conn, int64_t conn_rows, int64_t n_el,
+                      const int64_t* dof_h, int64_t n_dof, double shear_m,
+                      double poiss_r, double beta, int64_t t0, int64_t t1,
+                      int nthreads, double* A);
+
+/* rhs[dof] -= (n . s_inf)_l for every free DoF (caller zero-fills rhs). */
+void hfpo_rhs(const double* nods, int64_t nods_rows, int64_t n_nodes,
+              const int64_t* conn, int64_t conn_rows, int64_t n_el,
+              const int64_t* dof_h, const double* s_inf, double* rhs);
+
+/* Solution vector -> nodal DD array (6*n_el x 3, col-major); fixed DoF -> 0. */
+void hfpo_write_dd(const double* x, const int64_t* dof_h, int64_t n_el,
+                   double* dd);
+
+/* Stress at n_pts points (pts 3 x n_pts), out n_pts x 6 col-major
+ * [Sxx Syy Szz Sxy Sxz Syz], s_inf added, deterministic. */
+void hfpo_stress(const double* nods, int64_t nods_rows, int64_t n_nodes,
+                 const int64_t* conn, int64_t conn_rows, int64_t n_el,
+                 double shear_m, double poiss_r, const double* dd,
+                 const double* pts, int64_t n_pts, const double* s_inf,
+                 int nthreads, double* out);
+
+/* LU with partial pivoting (LAPACK dgetrf semantics): A n x n col-major
+ * (lda), ipiv 1-based.  Returns info (0 ok, k > 0: U(k,k) exactly zero). */
+int64_t hfpo_getrf(int64_t n, double* A, int64_t lda, int32_t* ipiv,
+                   int nthreads);
+/* Solve with the factors (dgetrs 'N'): B n x nrhs col-major (ldb). */
+void hfpo_getrs(int64_t n, int64_t nrhs, const double* LU, int64_t lda,
+                const int32_t* ipiv, double* B, int64_t ldb);
+
+/* Sneddon penny crack (radius a, pressure p): opening DD at a point of the
+ * crack plane and the stress field (including sigma_inf_zz = p). */
+void hfpo_sneddon_dd(double shear_m, double poiss_r, double a, double p,
+                     const double* xyz, double* dd);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,224 @@
+"""ctypes access to the oracle (oracle/liboracle.so, the C restatement) and,
+when it has been built, to the reference build (oracle/_ref/libhfpref.so).
+Test infrastructure only."""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE = os.path.join(ROOT, "oracle", "liboracle.so")
+REF = os.path.join(ROOT, "oracle", "_ref", "libhfpref.so")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+_dp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_d = ctypes.c_double
+_int = ctypes.c_int
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _f(a, dt=np.float64):
+    return np.asfortranarray(a, dtype=dt)
+
+
+class _Lib:
+    def __init__(self, path, prefix):
+        self.lib = ctypes.CDLL(path)
+        self.prefix = prefix
+
+    def fn(self, name, res, args):
+        f = getattr(self.lib, self.prefix + name)
+        f.restype = res
+        f.argtypes = args
+        return f
+
+
+def _geom_unpack(sfm, rt, tau, cp):
+    return ((sfm[0::2] + 1j * sfm[1::2]).reshape(6, 6, order="F"), rt.reshape(3, 3, order="F"),
+            tau[0::2] + 1j * tau[1::2], cp.reshape(3, 6, order="F"))
+
+
+def _sub_pack(tau, sfm):
+    t = np.zeros(6)
+    t[0::2], t[1::2] = tau.real, tau.imag
+    s = np.zeros(72)
+    sf = np.asarray(sfm).reshape(-1, order="F")
+    s[0::2], s[1::2] = sf.real, sf.imag
+    return t, s
+
+
+class Oracle:
+    """The C restatement (hfpo_* functions)."""
+
+    def __init__(self, path=ORACLE):
+        L = _Lib(path, "hfpo_")
+        self._geom = L.fn("el_geom", None, [_dp, _d, _dp, _dp, _dp, _dp])
+        self._sub = L.fn("local_submatrix", None, [_d, _d, _d, _d, _d, _dp, _dp, _dp])
+        self._dof = L.fn("dof_h", _i64, [_dp, _i64, _i64, _dp, _i64, _i64, _int, _dp])
+        self._asm = L.fn("assemble", _i64, [_dp, _i64, _i64, _dp, _i64, _i64, _dp, _i64, _d, _d, _d,
+                                            _i64, _i64, _int, _dp])
+        self._rhs = L.fn("rhs", None, [_dp, _i64, _i64, _dp, _i64, _i64, _dp, _dp, _dp])
+        self._wdd = L.fn("write_dd", None, [_dp, _dp, _i64, _dp])
+        self._stress = L.fn("stress", None, [_dp, _i64, _i64, _dp, _i64, _i64, _d, _d, _dp, _dp, _i64,
+                                             _dp, _int, _dp])
+        self._getrf = L.fn("getrf", _i64, [_i64, _dp, _i64, _dp, _int])
+        self._getrs = L.fn("getrs", None, [_i64, _i64, _dp, _i64, _dp, _dp, _i64])
+        self._sned = L.fn("sneddon_dd", None, [_d, _d, _d, _d, _dp, _dp])
+
+    def el_geom(self, vert, beta=0.125):
+        v = _f(vert)
+        rt, tau, sfm, cp = np.zeros(9), np.zeros(6), np.zeros(72), np.zeros(18)
+        self._geom(_p(v), beta, _p(rt), _p(tau), _p(sfm), _p(cp))
+        return _geom_unpack(sfm, rt, tau, cp)
+
+    def local_submatrix(self, G, nu, h, z, tau, sfm):
+        t, s = _sub_pack(tau, sfm)
+        out = np.zeros(108)
+        self._sub(G, nu, h, z.real, z.imag, _p(t), _p(s), _p(out))
+        return out.reshape(6, 18, order="F")
+
+    def dof_h(self, nods, conn, tip_type):
+        nods, conn = _f(nods), _f(conn, np.int64)
+        n_el = conn.shape[1]
+        dof = np.zeros(18 * n_el, dtype=np.int64)
+        n = self._dof(_p(nods), nods.shape[0], nods.shape[1], _p(conn), conn.shape[0], n_el, tip_type, _p(dof))
+        return int(n), dof.reshape(n_el, 18, order="F")
+
+    def assemble(self, nods, conn, G, nu, beta=0.125, tip_type=0, targets=None, nthreads=8):
+        nods, conn = _f(nods), _f(conn, np.int64)
+        n_el = conn.shape[1]
+        n_dof, dof = self.dof_h(nods, conn, tip_type)
+        dof_f = _f(dof, np.int64)
+        A = np.zeros(n_dof * n_dof)
+        t0, t1 = (0, 0) if targets is None else targets
+        self._asm(_p(nods), nods.shape[0], nods.shape[1], _p(conn), conn.shape[0], n_el, _p(dof_f), n_dof,
+                  G, nu, beta, t0, t1, nthreads, _p(A))
+        return A.reshape(n_dof, n_dof, order="F"), dof
+
+    def rhs(self, nods, conn, dof, s_inf):
+        nods, conn = _f(nods), _f(conn, np.int64)
+        dof_f = _f(dof, np.int64)
+        n_dof = int((dof >= 0).sum())
+        r = np.zeros(n_dof)
+        s = np.ascontiguousarray(s_inf, dtype=np.float64)
+        self._rhs(_p(nods), nods.shape[0], nods.shape[1], _p(conn), conn.shape[0], conn.shape[1],
+                  _p(dof_f), _p(s), _p(r))
+        return r
+
+    def write_dd(self, x, dof):
+        dof_f = _f(dof, np.int64)
+        n_el = dof.shape[0]
+        dd = np.zeros(18 * n_el)
+        xx = np.ascontiguousarray(x, dtype=np.float64)
+        self._wdd(_p(xx), _p(dof_f), n_el, _p(dd))
+        return dd.reshape(6 * n_el, 3, order="F")
+
+    def stress(self, nods, conn, G, nu, dd, pts, s_inf, nthreads=8):
+        nods, conn = _f(nods), _f(conn, np.int64)
+        dd, pts = _f(dd), _f(pts)
+        n_pts = pts.shape[1]
+        out = np.zeros(6 * n_pts)
+        s = np.ascontiguousarray(s_inf, dtype=np.float64)
+        self._stress(_p(nods), nods.shape[0], nods.shape[1], _p(conn), conn.shape[0], conn.shape[1], G, nu,
+                     _p(dd), _p(pts), n_pts, _p(s), nthreads, _p(out))
+        return out.reshape(n_pts, 6, order="F")
+
+    def getrf(self, A, nthreads=8):
+        LU = _f(A).copy(order="F")
+        n = LU.shape[0]
+        ipiv = np.zeros(n, dtype=np.int32)
+        info = self._getrf(n, _p(LU), n, _p(ipiv), nthreads)
+        return LU, ipiv, int(info)
+
+    def getrs(self, LU, ipiv, B):
+        n = LU.shape[0]
+        X = _f(B).copy(order="F")
+        nrhs = 1 if X.ndim == 1 else X.shape[1]
+        self._getrs(n, nrhs, _p(_f(LU)), n, _p(np.ascontiguousarray(ipiv, dtype=np.int32)), _p(X), n)
+        return X
+
+    def sneddon_dd(self, G, nu, a, p, xyz):
+        dd = np.zeros(3)
+        self._sned(G, nu, a, p, _p(np.ascontiguousarray(xyz, dtype=np.float64)), _p(dd))
+        return dd
+
+
+class Reference:
+    """The reference's own kernel/assembly/stress code (oracle/_ref)."""
+
+    def __init__(self, path=REF):
+        L = _Lib(path, "ref_")
+        self._sub = L.fn("local_submatrix", _int, [_d, _d, _d, _d, _d, _dp, _dp, _dp])
+        self._geom = L.fn("el_geom", _int, [_dp, _d, _dp, _dp, _dp, _dp])
+        self._dof = L.fn("dof_h", _i64, [_dp, _i64, _i64, _dp, _i64, _i64, _int, _dp])
+        self._asm = L.fn("assemble", _i64, [_dp, _i64, _i64, _dp, _i64, _i64, _d, _d, _d, _int, _int, _dp])
+        self._rhs = L.fn("rhs", _i64, [_dp, _i64, _i64, _dp, _i64, _i64, _int, _dp, _dp])
+        self._stress = L.fn("stress", _int, [_dp, _i64, _i64, _dp, _i64, _i64, _d, _d, _dp, _dp, _i64, _dp,
+                                             _int, _dp])
+
+    def el_geom(self, vert, beta=0.125):
+        v = _f(vert)
+        sfm, rt, tau, cp = np.zeros(72), np.zeros(9), np.zeros(6), np.zeros(18)
+        self._geom(_p(v), beta, _p(sfm), _p(rt), _p(tau), _p(cp))
+        return _geom_unpack(sfm, rt, tau, cp)
+
+    def local_submatrix(self, G, nu, h, z, tau, sfm):
+        t, s = _sub_pack(tau, sfm)
+        out = np.zeros(108)
+        self._sub(G, nu, h, z.real, z.imag, _p(t), _p(s), _p(out))
+        return out.reshape(6, 18, order="F")
+
+    def dof_h(self, nods, conn, tip_type):
+        nods, conn = _f(nods), _f(conn, np.int64)
+        n_el = conn.shape[1]
+        dof = np.zeros(18 * n_el, dtype=np.int64)
+        n = self._dof(_p(nods), nods.shape[0], nods.shape[1], _p(conn), conn.shape[0], n_el, tip_type, _p(dof))
+        return int(n), dof.reshape(n_el, 18, order="F")
+
+    def assemble(self, nods, conn, G, nu, beta=0.125, tip_type=0, nthreads=8):
+        if tip_type == 2:
+            raise ValueError("the reference's tip_type = 2 numbering writes past n_dof")
+        nods, conn = _f(nods), _f(conn, np.int64)
+        n_dof, dof = self.dof_h(nods, conn, tip_type)
+        A = np.zeros(n_dof * n_dof)
+        self._asm(_p(nods), nods.shape[0], nods.shape[1], _p(conn), conn.shape[0], conn.shape[1], G, nu, beta,
+                  tip_type, nthreads, _p(A))
+        return A.reshape(n_dof, n_dof, order="F"), dof
+
+    def rhs(self, nods, conn, tip_type, s_inf):
+        nods, conn = _f(nods), _f(conn, np.int64)
+        n_dof, _ = self.dof_h(nods, conn, tip_type)
+        r = np.zeros(n_dof)
+        s = np.ascontiguousarray(s_inf, dtype=np.float64)
+        self._rhs(_p(nods), nods.shape[0], nods.shape[1], _p(conn), conn.shape[0], conn.shape[1], tip_type,
+                  _p(s), _p(r))
+        return r
+
+    def stress(self, nods, conn, G, nu, dd, pts, s_inf):
+        nods, conn = _f(nods), _f(conn, np.int64)
+        dd, pts = _f(dd), _f(pts)
+        n_pts = pts.shape[1]
+        out = np.zeros(6 * n_pts)
+        s = np.ascontiguousarray(s_inf, dtype=np.float64)
+        self._stress(_p(nods), nods.shape[0], nods.shape[1], _p(conn), conn.shape[0], conn.shape[1], G, nu,
+                     _p(dd), _p(pts), n_pts, _p(s), 1, _p(out))
+        return out.reshape(n_pts, 6, order="F")
+
+
+def oracle():
+    return Oracle()
+
+
+def reference_or_none():
+    try:
+        return Reference()
+    except OSError:
+        return None
+
+
+def golden(name):
+    return os.path.join(GOLDEN, name)
