@@ -53,6 +53,10 @@ SIGNATURES = {
     "hfpx_assemble_mixed": (_c_int, [_c_void_p, _c_void_p, _c_double, _c_double, _c_void_p, _c_i64,
                                      _c_void_p, _c_i64]),
     "hfpx_rhs_sinf": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_int, _c_void_p, _c_i64]),
+    "hfpx_getrf": (_c_int, [_c_void_p, _c_int, _c_void_p, _c_i64, _c_i64, _c_void_p, ctypes.POINTER(_c_i64)]),
+    "hfpx_getrs": (_c_int, [_c_void_p, _c_int, _c_void_p, _c_i64, _c_void_p, _c_i64, _c_void_p, _c_i64, _c_i64]),
+    "hfpx_gemm_sub": (_c_int, [_c_void_p, _c_int, _c_void_p, _c_i64, _c_void_p, _c_i64, _c_void_p, _c_i64,
+                               _c_i64, _c_i64, _c_i64]),
 }
 
 _lib = None

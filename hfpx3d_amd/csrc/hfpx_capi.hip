@@ -1,4 +1,5 @@
 // hfpx_capi.hip -- extern "C" entry points: context, mesh, assembly, RHS.
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdlib>
@@ -247,6 +248,42 @@ int hfpx_rhs_sinf(hfpx_ctx* ctx, hfpx_mesh* m, const double* sinf_host, int nrhs
   // hwork/work are reused by the next call: finish before returning
   HFPX_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return HFPX_OK;
+}
+
+int hfpx_getrf(hfpx_ctx* ctx, int dtype, void* A, int64_t lda, int64_t n, int32_t* ipiv,
+               int64_t* info_host) {
+  if (!ctx) return HFPX_ERR_INVALID;
+  if (info_host) *info_host = 0;
+  if (!A || !ipiv || n < 1 || lda < n || (dtype != HFPX_F64 && dtype != HFPX_F32))
+    return set_error(ctx, HFPX_ERR_INVALID, "getrf: need A, ipiv, n >= 1, lda >= n, dtype F64/F32");
+  if (n > INT32_MAX - 1) return set_error(ctx, HFPX_ERR_INVALID, "getrf: n too large for int32 pivots");
+  HFPX_TRY_HIP(ctx, hipSetDevice(ctx->device));
+  int64_t info = 0;
+  const int rc = lu_getrf(ctx, dtype, A, lda, n, ipiv, &info);
+  if (info_host) *info_host = info;
+  if (rc == HFPX_ERR_SINGULAR)
+    set_error(ctx, rc, "U(%lld,%lld) is exactly zero", (long long)info, (long long)info);
+  return rc;
+}
+
+int hfpx_getrs(hfpx_ctx* ctx, int dtype, const void* LU, int64_t lda, const int32_t* ipiv, int64_t n,
+               void* B, int64_t ldb, int64_t nrhs) {
+  if (!ctx) return HFPX_ERR_INVALID;
+  if (!LU || !ipiv || !B || n < 1 || lda < n || ldb < n || nrhs < 1 ||
+      (dtype != HFPX_F64 && dtype != HFPX_F32))
+    return set_error(ctx, HFPX_ERR_INVALID, "getrs: invalid arguments");
+  HFPX_TRY_HIP(ctx, hipSetDevice(ctx->device));
+  return lu_getrs(ctx, dtype, LU, lda, ipiv, n, B, ldb, nrhs);
+}
+
+int hfpx_gemm_sub(hfpx_ctx* ctx, int dtype, void* C, int64_t ldc, const void* A, int64_t lda,
+                  const void* B, int64_t ldb, int64_t M, int64_t N, int64_t K) {
+  if (!ctx) return HFPX_ERR_INVALID;
+  if (!C || !A || !B || M < 0 || N < 0 || K < 0 || ldc < M || lda < M || ldb < K ||
+      (dtype != HFPX_F64 && dtype != HFPX_F32))
+    return set_error(ctx, HFPX_ERR_INVALID, "gemm_sub: invalid arguments");
+  HFPX_TRY_HIP(ctx, hipSetDevice(ctx->device));
+  return gemm_sub_any(ctx, dtype, C, ldc, A, lda, B, ldb, M, N, K);
 }
 
 }  // extern "C"

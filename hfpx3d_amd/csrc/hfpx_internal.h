@@ -51,6 +51,13 @@ int launch_assemble(hfpx_ctx* ctx, const hfpx_mesh* m, double shear_m, double po
 int launch_rhs(hfpx_ctx* ctx, const hfpx_mesh* m, const double* sinf_dev, int nrhs,
                double* b, int64_t ldb);
 
+// dense linear algebra (hfpx_lu.hip)
+int lu_getrf(hfpx_ctx* ctx, int dtype, void* A, int64_t lda, int64_t n, int32_t* ipiv, int64_t* info);
+int lu_getrs(hfpx_ctx* ctx, int dtype, const void* LU, int64_t lda, const int32_t* ipiv, int64_t n,
+             void* B, int64_t ldb, int64_t nrhs);
+int gemm_sub_any(hfpx_ctx* ctx, int dtype, void* C, int64_t ldc, const void* A, int64_t lda,
+                 const void* B, int64_t ldb, int64_t M, int64_t N, int64_t K);
+
 }  // namespace hfpx
 
 #define HFPX_TRY_HIP(ctx, expr)                                   \

@@ -305,3 +305,89 @@ def write_dd_vector_to_md(sol_v, dof_h_dd, n_el=None):
     vals = np.where(mask, sol[np.where(mask, d, 0)], 0.0)
     dd[:] = vals.reshape(6 * n_el, 3)
     return dd
+
+
+# --------------------------------------------------------------------------
+# LU (il::LU semantics: the matrix is taken over, factorised in place)
+# --------------------------------------------------------------------------
+
+def _dtype_code(torch, t):
+    if t.dtype == torch.float64:
+        return _lib.HFPX_F64
+    if t.dtype == torch.float32:
+        return _lib.HFPX_F32
+    raise ValueError("LU supports float64 and float32 matrices")
+
+
+def _check_colmajor(A):
+    n = A.shape[0]
+    if A.dim() != 2 or A.shape[1] != n or A.stride(0) != 1 or A.stride(1) < n:
+        raise ValueError("expected a square column-major device matrix (strides (1, lda))")
+    return n, A.stride(1)
+
+
+class LU:
+    """LU factorisation with partial pivoting on the GPU.
+
+    Mirrors il::LU<il::Array2D<double>> (il/linear_algebra/dense/factorization/
+    LU.h:46-138): the constructor factorises the given matrix (which it takes
+    over: the storage is overwritten with the factors), reports an exactly
+    singular matrix through `status` (error 'MatrixSingular', info = rank,
+    LU.h:94-98), and solve() applies the factors to one or many right-hand
+    sides (dgetrs)."""
+
+    def __init__(self, A, status=None, ctx=None):
+        self.ctx = ctx or get_context()
+        torch = self.ctx.torch
+        self.n, self.lda = _check_colmajor(A)
+        self.A = A
+        self.code = _dtype_code(torch, A)
+        with torch.cuda.stream(self.ctx.stream):
+            self.ipiv = torch.empty(self.n, dtype=torch.int32, device=self.ctx.device)
+        info = ctypes.c_int64(0)
+        self.ctx.enter()
+        rc = self.ctx.lib.hfpx_getrf(self.ctx.handle, self.code, _ptr(A), self.lda, self.n, _ptr(self.ipiv),
+                                     ctypes.byref(info))
+        self.ctx.leave()
+        self.info = int(info.value)
+        if rc == _lib.HFPX_ERR_SINGULAR:
+            if status is None:
+                raise _lib.HfpxError(rc, f"matrix is singular (U({self.info},{self.info}) = 0)")
+            status.error = "MatrixSingular"
+            status.info = self.info - 1
+        else:
+            self.ctx.check(rc)
+
+    def solve(self, y):
+        """Solve A x = y for a device vector (n) or column-major matrix
+        (n x nrhs); returns a new tensor, y is left unchanged."""
+        torch = self.ctx.torch
+        with torch.cuda.stream(self.ctx.stream):
+            if y.dim() == 1:
+                x = y.to(dtype=self.A.dtype, device=self.ctx.device).clone()
+                nrhs, ldb = 1, self.n
+            else:
+                x, ldb = alloc_colmajor(self.n, y.shape[1], self.A.dtype, self.ctx)
+                x.copy_(y)
+                nrhs = y.shape[1]
+        self.ctx.enter()
+        self.ctx.check(self.ctx.lib.hfpx_getrs(self.ctx.handle, self.code, _ptr(self.A), self.lda,
+                                               _ptr(self.ipiv), self.n, _ptr(x), ldb, nrhs))
+        self.ctx.leave()
+        return x
+
+
+def gemm_sub(C, A, B, ctx=None):
+    """C -= A @ B on MFMA for column-major device matrices (testing aid)."""
+    ctx = ctx or get_context()
+    torch = ctx.torch
+    M, N = C.shape
+    K = A.shape[1]
+    for t in (C, A, B):
+        if t.stride(0) != 1:
+            raise ValueError("column-major operands expected")
+    ctx.enter()
+    ctx.check(ctx.lib.hfpx_gemm_sub(ctx.handle, _dtype_code(torch, C), _ptr(C), C.stride(1), _ptr(A), A.stride(1),
+                                    _ptr(B), B.stride(1), M, N, K))
+    ctx.leave()
+    return C

@@ -81,6 +81,26 @@ int hfpx_assemble_mixed(hfpx_ctx* ctx, hfpx_mesh* mesh, double shear_m, double p
 int hfpx_rhs_sinf(hfpx_ctx* ctx, hfpx_mesh* mesh, const double* sinf_host, int nrhs,
                   double* B, int64_t ldb);
 
+/* ---- dense solve -------------------------------------------------------- */
+/* In-place LU with partial pivoting of the n x n matrix A (device, lda),
+ * dtype HFPX_F64 or HFPX_F32: P A = L U, ipiv (device, n int32) 1-based
+ * sequential row interchanges, *info_host = 0 or the first column (1-based)
+ * with an exactly zero pivot, in which case HFPX_ERR_SINGULAR is returned and
+ * the factorisation is still completed, as LAPACK does.  Replaces the
+ * il::LU<il::Array2D<double>> constructor
+ * (il/linear_algebra/dense/factorization/LU.h:79-99, LAPACKE_dgetrf). */
+int hfpx_getrf(hfpx_ctx* ctx, int dtype, void* A, int64_t lda, int64_t n, int32_t* ipiv,
+               int64_t* info_host);
+/* B (n x nrhs, device, ldb) := A^{-1} B from hfpx_getrf's factors.  Replaces
+ * il::LU::solve (LU.h:106-138, LAPACKE_dgetrs 'N'). */
+int hfpx_getrs(hfpx_ctx* ctx, int dtype, const void* LU, int64_t lda, const int32_t* ipiv,
+               int64_t n, void* B, int64_t ldb, int64_t nrhs);
+/* C (M x N) -= A (M x K) B (K x N), column-major, on MFMA: the
+ * Schur-complement update of hfpx_getrf, exposed for testing and reuse.
+ * No reference counterpart (LAPACK-internal). */
+int hfpx_gemm_sub(hfpx_ctx* ctx, int dtype, void* C, int64_t ldc, const void* A, int64_t lda,
+                  const void* B, int64_t ldb, int64_t M, int64_t N, int64_t K);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,135 @@
+"""MFMA GEMM, LU with partial pivoting and triangular solves (GPU) against
+LAPACK (scipy) and the oracle, through the C ABI."""
+import numpy as np
+import pytest
+
+import oracle_lib
+from oracle_lib import golden
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def env():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a device"
+    from hfpx3d_amd import hfp3d
+    return torch, hfp3d
+
+
+def to_dev(torch, hx, a, dtype):
+    """numpy (n x m) -> column-major device tensor."""
+    ctx = hx.get_context()
+    t, _ = hx.alloc_colmajor(a.shape[0], a.shape[1], dtype, ctx)
+    t.copy_(torch.from_numpy(np.ascontiguousarray(a)))
+    return t
+
+
+@pytest.mark.parametrize("dt", ["f64", "f32"])
+@pytest.mark.parametrize("M,N,K", [(128, 128, 16), (300, 257, 33), (1000, 64, 256), (77, 5, 3), (513, 700, 100)])
+def test_gemm_sub(env, dt, M, N, K):
+    torch, hx = env
+    print(f"case: gemm {dt} {M}x{N}x{K}", flush=True)
+    dtype = torch.float64 if dt == "f64" else torch.float32
+    rng = np.random.default_rng(M + N + K)
+    A, B, C = rng.normal(size=(M, K)), rng.normal(size=(K, N)), rng.normal(size=(M, N))
+    Cd = to_dev(torch, hx, C, dtype)
+    hx.gemm_sub(Cd, to_dev(torch, hx, A, dtype), to_dev(torch, hx, B, dtype))
+    got = Cd.cpu().numpy()
+    ref = C - A @ B
+    tol = 1e-13 if dt == "f64" else 2e-5
+    assert np.abs(got - ref).max() / (np.abs(A).max() * np.abs(B).max() * K) < tol
+
+
+@pytest.mark.parametrize("n", [1, 7, 64, 300, 1000, 2500])
+def test_getrf_f64_matches_lapack(env, n):
+    torch, hx = env
+    sl = pytest.importorskip("scipy.linalg")
+    print(f"case: getrf f64 n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    A = rng.normal(size=(n, n))
+    Ad = to_dev(torch, hx, A, torch.float64)
+    lu = hx.LU(Ad)
+    LU = lu.A.cpu().numpy()
+    piv = lu.ipiv.cpu().numpy()
+    lu2, piv2 = sl.lu_factor(A)
+    assert lu.info == 0
+    # identical partial pivoting (no near-ties in a random matrix)
+    assert np.array_equal(piv - 1, piv2)
+    assert np.abs(LU - lu2).max() < 1e-12 * max(1.0, np.abs(lu2).max()) * max(1, n / 100)
+    b = rng.normal(size=(n, 3))
+    x = lu.solve(to_dev(torch, hx, b, torch.float64)).cpu().numpy()
+    assert np.abs(A @ x - b).max() / (np.abs(A).max() * np.abs(x).max() * n) < 1e-15 * 10
+
+
+@pytest.mark.parametrize("n", [300, 2049])
+def test_getrf_f32_backward_error(env, n):
+    torch, hx = env
+    print(f"case: getrf f32 n={n}", flush=True)
+    rng = np.random.default_rng(n + 1)
+    A = rng.normal(size=(n, n)).astype(np.float32)
+    lu = hx.LU(to_dev(torch, hx, A, torch.float32))
+    LU = lu.A.cpu().numpy().astype(np.float64)
+    piv = lu.ipiv.cpu().numpy() - 1
+    L = np.tril(LU, -1) + np.eye(n)
+    U = np.triu(LU)
+    PA = A.astype(np.float64).copy()
+    for j, p in enumerate(piv):
+        PA[[j, p]] = PA[[p, j]]
+    assert np.abs(PA - L @ U).max() / np.abs(A).max() < n * 1.2e-7
+    assert np.abs(L).max() <= 1.0 + 1e-6  # partial pivoting bound
+
+
+def test_getrf_reports_singular(env):
+    torch, hx = env
+    print("case: singular", flush=True)
+    A = np.random.default_rng(3).normal(size=(50, 50))
+    A[:, 17] = A[:, 4]
+    st = hx.Status()
+    lu = hx.LU(to_dev(torch, hx, A, torch.float64), status=st)
+    assert not st.ok() and st.error == "MatrixSingular"
+    o = oracle_lib.oracle()
+    _, _, info = o.getrf(A)
+    # same first zero pivot as the oracle's LAPACK-semantics LU
+    assert lu.info == info
+
+
+def test_bem_penny121_solution_matches_oracle(env):
+    """Assembly + LU + solve on the 121-element penny (tip_type 0, the C1
+    configuration E = 1, nu = 0.3, p = 1): DD agree with the oracle's FP64
+    solve and with Sneddon's opening away from the tip."""
+    torch, hx = env
+    print("case: penny121 solve", flush=True)
+    nods = np.load(golden("Nodes_pennymesh121el_32.npy"))
+    conn = np.load(golden("Elems_pennymesh121el_32.npy")).astype(np.int64)
+    conn[:3] -= 1
+    G, nu = 1 / 2.6, 0.3
+    mesh = hx.Mesh_Geom_T(nods=nods, conn=conn)
+    npar = hx.Num_Param_T(beta=0.125, tip_type=0)
+    md = hx.Mesh_Data_T(mesh=mesh)
+    sae = hx.SAE_T()
+    sae.matrix = hx.make_3dbem_matrix_s(G, nu, mesh, npar, md.dof_h_dd)
+    hx.add_s_inf_to_3dbem_rhs(md, hx.Load_T(s_inf=np.array([1.0, 1, 1, 0, 0, 0])), sae)
+    st = hx.Status()
+    lu = hx.LU(sae.matrix, status=st)
+    st.abortOnError()
+    x = lu.solve(sae.rhs_v).cpu().numpy()
+    o = oracle_lib.oracle()
+    Ao, dof = o.assemble(nods, conn, G, nu)
+    r = o.rhs(nods, conn, dof, np.array([1.0, 1, 1, 0, 0, 0]))
+    LUo, piv, info = o.getrf(Ao)
+    xo = o.getrs(LUo, piv, r)
+    assert np.abs(x - xo).max() / np.abs(xo).max() < 1e-12
+    dd = hx.write_dd_vector_to_md(x, md.dof_h_dd)
+    # opening against Sneddon 4 (1 - nu) p sqrt(1 - r^2) / (pi G) away from the tip
+    err = 0.0
+    for e in range(conn.shape[1]):
+        v = nods[:3, conn[:3, e]]
+        _, _, _, cp = o.el_geom(v, 0.0)
+        for k in range(6):
+            rr = np.hypot(cp[0, k], cp[1, k])
+            if rr > 0.8:
+                continue
+            s = o.sneddon_dd(G, nu, 1.0, 1.0, cp[:, k])
+            err = max(err, abs(dd[6 * e + k, 2] - s[2]) / s[2])
+    assert err < 0.03
