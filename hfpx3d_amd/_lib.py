@@ -18,6 +18,7 @@ HFPX_ERR_NOT_CONVERGED = 4
 HFPX_ERR_HIP = 5
 HFPX_F64 = 0
 HFPX_F32 = 1
+KERNEL_CLASSES = {"assemble": 0, "gemm_f64": 1, "gemm_f32": 2, "panel": 3, "laswp": 4, "trsm": 5}
 
 _c_void_p = ctypes.c_void_p
 _c_i64 = ctypes.c_int64
@@ -55,6 +56,12 @@ SIGNATURES = {
     "hfpx_rhs_sinf": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_int, _c_void_p, _c_i64]),
     "hfpx_getrf": (_c_int, [_c_void_p, _c_int, _c_void_p, _c_i64, _c_i64, _c_void_p, ctypes.POINTER(_c_i64)]),
     "hfpx_getrs": (_c_int, [_c_void_p, _c_int, _c_void_p, _c_i64, _c_void_p, _c_i64, _c_void_p, _c_i64, _c_i64]),
+    "hfpx_solve_ir": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_i64, _c_void_p, _c_i64, _c_void_p,
+                               _c_i64, _c_void_p, _c_i64, _c_i64, _c_double, _c_int, _c_int,
+                               ctypes.POINTER(IRReport)]),
+    "hfpx_profile": (_c_int, [_c_void_p, _c_int]),
+    "hfpx_profile_read": (_c_int, [_c_void_p, _c_int, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_double),
+                                   ctypes.POINTER(_c_double), ctypes.POINTER(_c_double)]),
     "hfpx_gemm_sub": (_c_int, [_c_void_p, _c_int, _c_void_p, _c_i64, _c_void_p, _c_i64, _c_void_p, _c_i64,
                                _c_i64, _c_i64, _c_i64]),
 }

@@ -54,6 +54,43 @@ int ensure_hwork(hfpx_ctx* ctx, size_t bytes) {
   return HFPX_OK;
 }
 
+int prof_begin(hfpx_ctx* ctx) {
+  if (!ctx->prof) return -1;
+  hipEvent_t a, b;
+  if (ctx->prof_pool.size() >= 2) {
+    a = ctx->prof_pool.back();
+    ctx->prof_pool.pop_back();
+    b = ctx->prof_pool.back();
+    ctx->prof_pool.pop_back();
+  } else {
+    if (hipEventCreate(&a) != hipSuccess) return -1;
+    if (hipEventCreate(&b) != hipSuccess) {
+      (void)hipEventDestroy(a);
+      return -1;
+    }
+  }
+  (void)hipEventRecord(a, ctx->stream);
+  ctx->prof_recs.push_back({-1, a, b, 0.0, 0.0});
+  return (int)ctx->prof_recs.size() - 1;
+}
+
+void prof_end(hfpx_ctx* ctx, int token, int cls, double flops, double bytes) {
+  if (token < 0 || token >= (int)ctx->prof_recs.size()) return;
+  auto& r = ctx->prof_recs[token];
+  (void)hipEventRecord(r.b, ctx->stream);
+  r.cls = cls;
+  r.flops = flops;
+  r.bytes = bytes;
+}
+
+static void prof_clear(hfpx_ctx* ctx) {
+  for (auto& r : ctx->prof_recs) {
+    ctx->prof_pool.push_back(r.a);
+    ctx->prof_pool.push_back(r.b);
+  }
+  ctx->prof_recs.clear();
+}
+
 }  // namespace hfpx
 
 using namespace hfpx;
@@ -107,6 +144,8 @@ int hfpx_destroy(hfpx_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->work) (void)hipFree(ctx->work);
   if (ctx->hwork) (void)hipHostFree(ctx->hwork);
+  prof_clear(ctx);
+  for (auto e : ctx->prof_pool) (void)hipEventDestroy(e);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return HFPX_OK;
@@ -284,6 +323,47 @@ int hfpx_gemm_sub(hfpx_ctx* ctx, int dtype, void* C, int64_t ldc, const void* A,
     return set_error(ctx, HFPX_ERR_INVALID, "gemm_sub: invalid arguments");
   HFPX_TRY_HIP(ctx, hipSetDevice(ctx->device));
   return gemm_sub_any(ctx, dtype, C, ldc, A, lda, B, ldb, M, N, K);
+}
+
+int hfpx_solve_ir(hfpx_ctx* ctx, const double* A64, int64_t lda64, const float* LU32, int64_t ldlu32,
+                  const int32_t* ipiv, int64_t n, const double* B, int64_t ldb, double* X, int64_t ldx,
+                  int64_t nrhs, double tol, int max_outer, int gmres_m, hfpx_ir_report* rep) {
+  if (!ctx) return HFPX_ERR_INVALID;
+  if (rep) *rep = hfpx_ir_report{0, 0, 0.0, 0, 0};
+  if (!A64 || !LU32 || !ipiv || !B || !X || n < 1 || lda64 < n || ldlu32 < n || ldb < n || ldx < n ||
+      nrhs < 1 || max_outer < 0 || !(tol > 0))
+    return set_error(ctx, HFPX_ERR_INVALID, "solve_ir: invalid arguments");
+  HFPX_TRY_HIP(ctx, hipSetDevice(ctx->device));
+  return solve_ir(ctx, A64, lda64, LU32, ldlu32, ipiv, n, B, ldb, X, ldx, nrhs, tol, max_outer, gmres_m, rep);
+}
+
+int hfpx_profile(hfpx_ctx* ctx, int enable) {
+  if (!ctx) return HFPX_ERR_INVALID;
+  HFPX_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (enable) prof_clear(ctx);
+  ctx->prof = enable != 0;
+  return HFPX_OK;
+}
+
+int hfpx_profile_read(hfpx_ctx* ctx, int kc, int64_t* launches, double* ms, double* flops, double* bytes) {
+  if (!ctx || kc < 0 || kc >= HFPX_KC_COUNT) return HFPX_ERR_INVALID;
+  HFPX_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  int64_t n = 0;
+  double t = 0, f = 0, b = 0;
+  for (auto& r : ctx->prof_recs) {
+    if (r.cls != kc) continue;
+    float e = 0.f;
+    HFPX_TRY_HIP(ctx, hipEventElapsedTime(&e, r.a, r.b));
+    ++n;
+    t += e;
+    f += r.flops;
+    b += r.bytes;
+  }
+  if (launches) *launches = n;
+  if (ms) *ms = t;
+  if (flops) *flops = f;
+  if (bytes) *bytes = b;
+  return HFPX_OK;
 }
 
 }  // extern "C"

@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "../../include/hfpx3d.h"
 #include "hfpx_pair.h"
@@ -20,6 +21,15 @@ struct hfpx_ctx {
   size_t hwork_bytes = 0;
   int num_cus = 0;
   std::string last_error;
+  // opt-in per-launch timing (HIP events on the context's stream)
+  bool prof = false;
+  struct ProfRec {
+    int cls;
+    hipEvent_t a, b;
+    double flops, bytes;
+  };
+  std::vector<ProfRec> prof_recs;
+  std::vector<hipEvent_t> prof_pool;
 };
 
 // Device-resident mesh: raw nodes/connectivity/DoF map plus the element
@@ -39,6 +49,11 @@ struct hfpx_mesh {
 
 namespace hfpx {
 
+// Profiling bracket around one launch: prof_begin before the launch,
+// prof_end after it (no-ops unless profiling is enabled on the context).
+int prof_begin(hfpx_ctx* ctx);
+void prof_end(hfpx_ctx* ctx, int token, int cls, double flops, double bytes);
+
 int set_error(hfpx_ctx* ctx, int code, const char* fmt, ...);
 int check_hip(hfpx_ctx* ctx, hipError_t e, const char* what);
 int ensure_work(hfpx_ctx* ctx, size_t bytes);
@@ -57,6 +72,11 @@ int lu_getrs(hfpx_ctx* ctx, int dtype, const void* LU, int64_t lda, const int32_
              void* B, int64_t ldb, int64_t nrhs);
 int gemm_sub_any(hfpx_ctx* ctx, int dtype, void* C, int64_t ldc, const void* A, int64_t lda,
                  const void* B, int64_t ldb, int64_t M, int64_t N, int64_t K);
+
+// mixed-precision solve (hfpx_solve.hip)
+int solve_ir(hfpx_ctx* ctx, const double* A, int64_t lda, const float* LU, int64_t ldlu,
+             const int32_t* ipiv, int64_t n, const double* B, int64_t ldb, double* X, int64_t ldx,
+             int64_t k, double tol, int max_outer, int m, hfpx_ir_report* rep);
 
 }  // namespace hfpx
 

@@ -36,6 +36,8 @@ struct MfmaVec<float> {
   static __device__ __forceinline__ floatx4 mfma(float a, float b, floatx4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  // result element v of lane l is D[4 (l / 16) + v][l % 16]
+  static __device__ __forceinline__ int drow(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 template <>
 struct MfmaVec<double> {
@@ -43,6 +45,8 @@ struct MfmaVec<double> {
   static __device__ __forceinline__ doublex4 mfma(double a, double b, doublex4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  // result element v of lane l is D[4 v + l / 16][l % 16] (measured on gfx950)
+  static __device__ __forceinline__ int drow(int lane, int v) { return 4 * v + (lane >> 4); }
 };
 
 // ---------------------------------------------------------------------
@@ -52,8 +56,8 @@ struct MfmaVec<double> {
 // through LDS.  The MFMA computes the transposed product (U^T L^T) so that the
 // 16 lanes of a group map to 16 consecutive rows of one column of C: the
 // epilogue then writes contiguous column segments.
-// MFMA 16x16x4 operand layout: lane l supplies A[l%16][l/16] and B[l/16][l%16]
-// and receives D[4*(l/16)+v][l%16], v = 0..3.
+// MFMA 16x16x4 operand layout: lane l supplies A[l%16][l/16] and B[l/16][l%16];
+// the row of result element v differs between the f32 and f64 forms (drow).
 // ---------------------------------------------------------------------
 constexpr int GT = 128;  // tile edge
 constexpr int KC = 16;   // K chunk
@@ -127,7 +131,7 @@ __global__ __launch_bounds__(256) void k_gemm_sub(T* __restrict__ C, int64_t ldc
     }
     __syncthreads();
   }
-  // acc[tc][tr][v] = (A B)(row = wr + 16 tr + lane%16, col = wc + 16 tc + 4 (lane/16) + v)
+  // acc[tc][tr][v] = (A B)(row = wr + 16 tr + lane%16, col = wc + 16 tc + drow(lane, v))
 #pragma unroll
   for (int tc = 0; tc < 4; ++tc)
 #pragma unroll
@@ -135,7 +139,7 @@ __global__ __launch_bounds__(256) void k_gemm_sub(T* __restrict__ C, int64_t ldc
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int64_t r = row0 + wr + 16 * tr + (lane & 15);
-        const int64_t cc = col0 + wc + 16 * tc + 4 * (lane >> 4) + v;
+        const int64_t cc = col0 + wc + 16 * tc + MfmaVec<T>::drow(lane, v);
         if (r < M && cc < N) C[r + cc * ldc] -= acc[tc][tr][v];
       }
 }
@@ -403,7 +407,11 @@ static int gemm_sub(hfpx_ctx* ctx, T* C, int64_t ldc, const T* A, int64_t lda, c
   if (M <= 0 || N <= 0 || K <= 0) return HFPX_OK;
   dim3 grid((unsigned)((M + GT - 1) / GT), (unsigned)((N + GT - 1) / GT));
   if (grid.y > 65535u) return set_error(ctx, HFPX_ERR_INVALID, "gemm: N too large");
+  const int tok = prof_begin(ctx);
   hipLaunchKernelGGL(k_gemm_sub<T>, grid, dim3(256), 0, ctx->stream, C, ldc, A, lda, B, ldb, M, N, K);
+  // algorithmic: 2MNK flops; bytes: C read + written, A and B read once
+  prof_end(ctx, tok, sizeof(T) == 8 ? HFPX_KC_GEMM_F64 : HFPX_KC_GEMM_F32, 2.0 * M * N * K,
+           (double)sizeof(T) * (2.0 * M * N + (double)M * K + (double)K * N));
   HFPX_TRY_HIP(ctx, hipGetLastError());
   return HFPX_OK;
 }
@@ -413,8 +421,10 @@ static int laswp(hfpx_ctx* ctx, T* A, int64_t lda, const int32_t* ipiv, int64_t 
                  int64_t c0, int64_t c1) {
   if (c1 <= c0 || k1 <= k0) return HFPX_OK;
   const int bs = 256;
+  const int tok = prof_begin(ctx);
   hipLaunchKernelGGL(k_laswp<T>, dim3((unsigned)((c1 - c0 + bs - 1) / bs)), dim3(bs), 0, ctx->stream,
                      A, lda, ipiv, k0, k1, c0, c1);
+  prof_end(ctx, tok, HFPX_KC_LASWP, 0.0, 0.0);
   HFPX_TRY_HIP(ctx, hipGetLastError());
   return HFPX_OK;
 }
@@ -423,8 +433,10 @@ template <typename T>
 static int trsm_lunit(hfpx_ctx* ctx, const T* L, int64_t ldl, T* B, int64_t ldb, int kb,
                       int64_t ncols) {
   if (ncols <= 0 || kb <= 0) return HFPX_OK;
+  const int tok = prof_begin(ctx);
   hipLaunchKernelGGL(k_trsm_lunit<T>, dim3((unsigned)((ncols + TS_COLS - 1) / TS_COLS)), dim3(256), 0,
                      ctx->stream, L, ldl, B, ldb, kb, ncols);
+  prof_end(ctx, tok, HFPX_KC_TRSM, (double)kb * kb * ncols, 0.0);
   HFPX_TRY_HIP(ctx, hipGetLastError());
   return HFPX_OK;
 }
@@ -433,8 +445,10 @@ template <typename T>
 static int trsm_upper(hfpx_ctx* ctx, const T* U, int64_t ldu, T* B, int64_t ldb, int kb,
                       int64_t ncols) {
   if (ncols <= 0 || kb <= 0) return HFPX_OK;
+  const int tok = prof_begin(ctx);
   hipLaunchKernelGGL(k_trsm_upper<T>, dim3((unsigned)((ncols + TS_COLS - 1) / TS_COLS)), dim3(256), 0,
                      ctx->stream, U, ldu, B, ldb, kb, ncols);
+  prof_end(ctx, tok, HFPX_KC_TRSM, (double)kb * kb * ncols, 0.0);
   HFPX_TRY_HIP(ctx, hipGetLastError());
   return HFPX_OK;
 }
@@ -443,8 +457,11 @@ template <typename T, int IB, int RPT>
 static int panel_launch(hfpx_ctx* ctx, T* A, int64_t lda, int64_t n, int64_t j0, int jb,
                         int32_t* ipiv, int32_t* info, PanelScratch<T>* ps, T* rows, int nwg) {
   void* args[] = {&A, &lda, &n, &j0, &jb, &ipiv, &info, &ps, &rows};
-  HFPX_TRY_HIP(ctx, hipLaunchCooperativeKernel((const void*)k_panel<T, IB, RPT>, dim3(nwg), dim3(PWG),
-                                               args, 0, ctx->stream));
+  const int tok = prof_begin(ctx);
+  const hipError_t e = hipLaunchCooperativeKernel((const void*)k_panel<T, IB, RPT>, dim3(nwg), dim3(PWG),
+                                                  args, 0, ctx->stream);
+  prof_end(ctx, tok, HFPX_KC_PANEL, (double)(n - j0) * jb * jb, 0.0);
+  HFPX_TRY_HIP(ctx, e);
   return HFPX_OK;
 }
 

@@ -391,3 +391,31 @@ def gemm_sub(C, A, B, ctx=None):
                                     _ptr(B), B.stride(1), M, N, K))
     ctx.leave()
     return C
+
+
+def solve_mixed(A64, B, tol=1e-14, max_outer=10, gmres_m=20, ctx=None, LU32=None):
+    """Solve A64 X = B: FP32 LU of A64 (or the given FP32 LU object) as the
+    preconditioner of FP64 GMRES iterative refinement against A64.
+    Returns (X, report dict)."""
+    ctx = ctx or get_context()
+    torch = ctx.torch
+    n, lda = _check_colmajor(A64)
+    if LU32 is None:
+        A32, _ = alloc_colmajor(n, n, torch.float32, ctx)
+        with torch.cuda.stream(ctx.stream):
+            A32.copy_(A64)
+        LU32 = LU(A32, ctx=ctx)
+    B2 = B if B.dim() == 2 else B.reshape(n, 1)
+    nrhs = B2.shape[1]
+    ldb = B2.stride(1) if nrhs > 1 else n
+    X, ldx = alloc_colmajor(n, nrhs, torch.float64, ctx)
+    rep = _lib.IRReport()
+    ctx.enter()
+    rc = ctx.lib.hfpx_solve_ir(ctx.handle, _ptr(A64), lda, _ptr(LU32.A), LU32.lda, _ptr(LU32.ipiv), n, _ptr(B2), ldb,
+                               _ptr(X), ldx, nrhs, float(tol), int(max_outer), int(gmres_m), ctypes.byref(rep))
+    ctx.leave()
+    report = dict(iterations=rep.iterations, inner_iterations=rep.inner_iterations,
+                  backward_error=rep.backward_error, converged=bool(rep.converged))
+    if rc not in (_lib.HFPX_OK, _lib.HFPX_ERR_NOT_CONVERGED):
+        ctx.check(rc)
+    return (X if B.dim() == 2 else X[:, 0]), report

@@ -33,6 +33,15 @@ extern "C" {
 typedef struct hfpx_ctx hfpx_ctx;
 typedef struct hfpx_mesh hfpx_mesh;
 
+/* Outcome of the mixed-precision solve. */
+typedef struct hfpx_ir_report {
+  int iterations;        /* outer refinement steps taken                 */
+  int inner_iterations;  /* GMRES steps summed over the outer steps       */
+  double backward_error; /* max over columns of |b-Ax|/(|A||x|+|b|), inf-norms */
+  int converged;         /* 1 if backward_error <= tol                     */
+  int fell_back;         /* reserved (0)                                  */
+} hfpx_ir_report;
+
 /* ---- context ---------------------------------------------------------- */
 /* Create a context on `device`; `stream` is a hipStream_t to use (NULL: the
  * library creates its own default-priority stream). */
@@ -95,11 +104,42 @@ int hfpx_getrf(hfpx_ctx* ctx, int dtype, void* A, int64_t lda, int64_t n, int32_
  * il::LU::solve (LU.h:106-138, LAPACKE_dgetrs 'N'). */
 int hfpx_getrs(hfpx_ctx* ctx, int dtype, const void* LU, int64_t lda, const int32_t* ipiv,
                int64_t n, void* B, int64_t ldb, int64_t nrhs);
+/* Mixed-precision solve of A X = B for nrhs columns: FP32 LU factors of A
+ * (LU32, ipiv from hfpx_getrf with HFPX_F32) precondition FP64 GMRES(m)
+ * iterative refinement whose residuals use the FP64 matrix A64 (left
+ * untouched).  X (device, ldx) receives the solution; refinement stops when
+ * every column's normwise backward error is <= tol or after max_outer steps
+ * (then HFPX_ERR_NOT_CONVERGED, X holding the last iterate and rep the
+ * achieved error).  No reference counterpart: it stands in for il::LU::solve
+ * (LU.h:106-138) on the FP64 matrix. */
+int hfpx_solve_ir(hfpx_ctx* ctx, const double* A64, int64_t lda64, const float* LU32,
+                  int64_t ldlu32, const int32_t* ipiv, int64_t n, const double* B, int64_t ldb,
+                  double* X, int64_t ldx, int64_t nrhs, double tol, int max_outer, int gmres_m,
+                  hfpx_ir_report* rep);
 /* C (M x N) -= A (M x K) B (K x N), column-major, on MFMA: the
  * Schur-complement update of hfpx_getrf, exposed for testing and reuse.
  * No reference counterpart (LAPACK-internal). */
 int hfpx_gemm_sub(hfpx_ctx* ctx, int dtype, void* C, int64_t ldc, const void* A, int64_t lda,
                   const void* B, int64_t ldb, int64_t M, int64_t N, int64_t K);
+
+/* ---- measurement --------------------------------------------------------- */
+/* Kernel classes timed by the opt-in profiler. */
+#define HFPX_KC_ASSEMBLE 0
+#define HFPX_KC_GEMM_F64 1
+#define HFPX_KC_GEMM_F32 2
+#define HFPX_KC_PANEL 3
+#define HFPX_KC_LASWP 4
+#define HFPX_KC_TRSM 5
+#define HFPX_KC_COUNT 6
+/* Start (enable = 1) or stop (0) recording a HIP event pair around every
+ * launch of the classes above on the context's stream; starting clears the
+ * previous records.  No reference counterpart (the reference times whole
+ * stages with il::Timer, main.cpp:89-175). */
+int hfpx_profile(hfpx_ctx* ctx, int enable);
+/* Totals for one kernel class since the last start: launches, summed kernel
+ * time (ms), algorithmic flops and algorithmic HBM bytes.  Synchronises. */
+int hfpx_profile_read(hfpx_ctx* ctx, int kernel_class, int64_t* launches, double* ms, double* flops,
+                      double* bytes);
 
 #ifdef __cplusplus
 }

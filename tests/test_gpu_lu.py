@@ -83,15 +83,19 @@ def test_getrf_f32_backward_error(env, n):
 def test_getrf_reports_singular(env):
     torch, hx = env
     print("case: singular", flush=True)
+    # an exactly zero column stays exactly zero under elimination, so the
+    # first zero pivot is that column (a duplicated column is not exactly
+    # singular after rounded elimination, in LAPACK either)
     A = np.random.default_rng(3).normal(size=(50, 50))
-    A[:, 17] = A[:, 4]
+    A[:, 17] = 0.0
     st = hx.Status()
     lu = hx.LU(to_dev(torch, hx, A, torch.float64), status=st)
     assert not st.ok() and st.error == "MatrixSingular"
     o = oracle_lib.oracle()
     _, _, info = o.getrf(A)
     # same first zero pivot as the oracle's LAPACK-semantics LU
-    assert lu.info == info
+    assert lu.info == info == 18
+    assert st.info == 17
 
 
 def test_bem_penny121_solution_matches_oracle(env):
