@@ -18,7 +18,7 @@ HFPX_ERR_NOT_CONVERGED = 4
 HFPX_ERR_HIP = 5
 HFPX_F64 = 0
 HFPX_F32 = 1
-KERNEL_CLASSES = {"assemble": 0, "gemm_f64": 1, "gemm_f32": 2, "panel": 3, "laswp": 4, "trsm": 5}
+KERNEL_CLASSES = {"assemble": 0, "gemm_f64": 1, "gemm_f32": 2, "panel": 3, "laswp": 4, "trsm": 5, "stress": 6}
 
 _c_void_p = ctypes.c_void_p
 _c_i64 = ctypes.c_int64
@@ -54,6 +54,8 @@ SIGNATURES = {
     "hfpx_assemble_mixed": (_c_int, [_c_void_p, _c_void_p, _c_double, _c_double, _c_void_p, _c_i64,
                                      _c_void_p, _c_i64]),
     "hfpx_rhs_sinf": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_int, _c_void_p, _c_i64]),
+    "hfpx_stress": (_c_int, [_c_void_p, _c_void_p, _c_double, _c_double, _c_void_p, _c_i64, _c_void_p, _c_i64,
+                             _c_void_p, _c_void_p, _c_i64]),
     "hfpx_getrf": (_c_int, [_c_void_p, _c_int, _c_void_p, _c_i64, _c_i64, _c_void_p, ctypes.POINTER(_c_i64)]),
     "hfpx_getrs": (_c_int, [_c_void_p, _c_int, _c_void_p, _c_i64, _c_void_p, _c_i64, _c_void_p, _c_i64, _c_i64]),
     "hfpx_solve_ir": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_void_p, _c_i64, _c_void_p, _c_i64, _c_void_p,

@@ -366,4 +366,15 @@ int hfpx_profile_read(hfpx_ctx* ctx, int kc, int64_t* launches, double* ms, doub
   return HFPX_OK;
 }
 
+int hfpx_stress(hfpx_ctx* ctx, hfpx_mesh* m, double shear_m, double poiss_r, const double* dd, int64_t ldd,
+                const double* pts, int64_t n_pts, const double* sinf_host, double* S, int64_t ldS) {
+  if (!ctx || !m) return HFPX_ERR_INVALID;
+  if (!dd || !pts || !sinf_host || !S || n_pts < 1 || ldd < 6 * m->n_el || ldS < n_pts)
+    return set_error(ctx, HFPX_ERR_INVALID, "stress: invalid arguments");
+  if (!(poiss_r < 1.0) || !(shear_m > 0.0))
+    return set_error(ctx, HFPX_ERR_INVALID, "need shear modulus > 0 and Poisson ratio < 1");
+  HFPX_TRY_HIP(ctx, hipSetDevice(ctx->device));
+  return launch_stress(ctx, m, shear_m, poiss_r, dd, ldd, pts, n_pts, sinf_host, S, ldS);
+}
+
 }  // extern "C"

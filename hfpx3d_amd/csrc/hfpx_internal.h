@@ -3,7 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <algorithm>
 #include <cstdio>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -72,6 +74,10 @@ int lu_getrs(hfpx_ctx* ctx, int dtype, const void* LU, int64_t lda, const int32_
              void* B, int64_t ldb, int64_t nrhs);
 int gemm_sub_any(hfpx_ctx* ctx, int dtype, void* C, int64_t ldc, const void* A, int64_t lda,
                  const void* B, int64_t ldb, int64_t M, int64_t N, int64_t K);
+
+// stress postprocessing (hfpx_stress.hip)
+int launch_stress(hfpx_ctx* ctx, hfpx_mesh* m, double shear_m, double nu, const double* dd, int64_t ldd,
+                  const double* pts, int64_t n_pts, const double* sinf_host, double* out, int64_t ldo);
 
 // mixed-precision solve (hfpx_solve.hip)
 int solve_ir(hfpx_ctx* ctx, const double* A, int64_t lda, const float* LU, int64_t ldlu,

@@ -419,3 +419,37 @@ def solve_mixed(A64, B, tol=1e-14, max_outer=10, gmres_m=20, ctx=None, LU32=None
     if rc not in (_lib.HFPX_OK, _lib.HFPX_ERR_NOT_CONVERGED):
         ctx.check(rc)
     return (X if B.dim() == 2 else X[:, 0]), report
+
+
+# --------------------------------------------------------------------------
+# stress postprocessing
+# --------------------------------------------------------------------------
+
+def make_3dbem_stress_f_s(shear_m, poiss_r, m_data, load, n_par, m_pts_crd, ctx=None, device_mesh=None):
+    """Stresses (n_pts x 6: Sxx Syy Szz Sxy Sxz Syz, tension positive) at the
+    points m_pts_crd (3 x n_pts) due to the nodal DD m_data.dd ((6 n_el) x 3,
+    global components) plus load.s_inf, like the reference (the far-field
+    stress is added to every point).  Returns a numpy array."""
+    ctx = ctx or get_context()
+    torch = ctx.torch
+    if n_par is not None and n_par.is_dd_local:
+        raise ValueError("is_dd_local = true is not supported (global DD only)")
+    pts = np.asarray(m_pts_crd, dtype=np.float64)
+    if pts.ndim != 2 or pts.shape[0] != 3:
+        raise ValueError("m_pts_crd must be 3 x n_pts")
+    n_pts = pts.shape[1]
+    dd = np.asarray(m_data.dd, dtype=np.float64)
+    n_el = np.asarray(m_data.mesh.conn).shape[1]
+    if dd.shape != (6 * n_el, 3):
+        raise ValueError("m_data.dd must be (6 n_el) x 3")
+    dm = device_mesh or DeviceMesh(m_data.mesh, None, 0.0, ctx)
+    with torch.cuda.stream(ctx.stream):
+        d_dd = torch.from_numpy(np.ascontiguousarray(dd.T)).to(ctx.device)      # (3, 6 n_el): col-major dd
+        d_pts = torch.from_numpy(np.ascontiguousarray(pts.T)).to(ctx.device)    # (n_pts, 3): col-major pts
+        out = torch.empty((6, n_pts), dtype=torch.float64, device=ctx.device)   # col-major n_pts x 6
+    s_inf = np.ascontiguousarray(np.asarray(load.s_inf, dtype=np.float64).reshape(6))
+    ctx.enter()
+    ctx.check(ctx.lib.hfpx_stress(ctx.handle, dm.handle, float(shear_m), float(poiss_r), _ptr(d_dd), 6 * n_el,
+                                  _ptr(d_pts), n_pts, _np_ptr(s_inf), _ptr(out), n_pts))
+    ctx.leave()
+    return out.t().cpu().numpy()

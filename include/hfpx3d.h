@@ -90,6 +90,18 @@ int hfpx_assemble_mixed(hfpx_ctx* ctx, hfpx_mesh* mesh, double shear_m, double p
 int hfpx_rhs_sinf(hfpx_ctx* ctx, hfpx_mesh* mesh, const double* sinf_host, int nrhs,
                   double* B, int64_t ldb);
 
+/* ---- postprocessing ------------------------------------------------------ */
+/* Stress at n_pts observation points: S (device, n_pts x 6, col-major, ldS,
+ * order Sxx Syy Szz Sxy Sxz Syz, tension positive) = sinf_host (6 values) +
+ * the stress of the nodal DD dd (device, 6 n_el x 3 col-major, ldd, global
+ * components, row 6 e + node) at pts (device, 3 x n_pts col-major).
+ * Deterministic (fixed summation order).  Replaces
+ * hfp3d::make_3dbem_stress_f_s (src/Solvers/system_assembly.cpp:279-455,
+ * declared system_assembly.h:49-54). */
+int hfpx_stress(hfpx_ctx* ctx, hfpx_mesh* mesh, double shear_m, double poiss_r, const double* dd,
+                int64_t ldd, const double* pts, int64_t n_pts, const double* sinf_host, double* S,
+                int64_t ldS);
+
 /* ---- dense solve -------------------------------------------------------- */
 /* In-place LU with partial pivoting of the n x n matrix A (device, lda),
  * dtype HFPX_F64 or HFPX_F32: P A = L U, ipiv (device, n int32) 1-based
@@ -130,7 +142,8 @@ int hfpx_gemm_sub(hfpx_ctx* ctx, int dtype, void* C, int64_t ldc, const void* A,
 #define HFPX_KC_PANEL 3
 #define HFPX_KC_LASWP 4
 #define HFPX_KC_TRSM 5
-#define HFPX_KC_COUNT 6
+#define HFPX_KC_STRESS 6
+#define HFPX_KC_COUNT 7
 /* Start (enable = 1) or stop (0) recording a HIP event pair around every
  * launch of the classes above on the context's stream; starting clears the
  * previous records.  No reference counterpart (the reference times whole
