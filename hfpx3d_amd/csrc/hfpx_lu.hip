@@ -59,38 +59,44 @@ struct MfmaVec<double> {
 // MFMA 16x16x4 operand layout: lane l supplies A[l%16][l/16] and B[l/16][l%16];
 // the row of result element v differs between the f32 and f64 forms (drow).
 // ---------------------------------------------------------------------
-constexpr int GT = 128;  // tile edge
+constexpr int GT = 128;  // tile rows (M)
 constexpr int KC = 16;   // K chunk
 
-template <typename T>
+// GTN = tile columns (N): 128, or 64 for skinny products (many right-hand
+// sides), where a 128-wide tile would leave half of every MFMA idle.
+template <typename T, int GTN>
 __global__ __launch_bounds__(256) void k_gemm_sub(T* __restrict__ C, int64_t ldc,
                                                   const T* __restrict__ A, int64_t lda,
                                                   const T* __restrict__ B, int64_t ldb, int64_t M,
                                                   int64_t N, int64_t K) {
   typedef typename MfmaVec<T>::type vec4;
-  __shared__ T sA[2][KC][GT];  // sA[k][i] = A(row0 + i, k0 + k)
-  __shared__ T sB[2][KC][GT];  // sB[k][j] = B(k0 + k, col0 + j)
+  constexpr int WN = GTN / 2;      // wave tile columns
+  constexpr int TC = WN / 16;      // 16x16 MFMA tiles across a wave's columns
+  constexpr int BQ = KC * GTN / 256;  // B elements loaded per thread per chunk
+  __shared__ T sA[2][KC][GT];      // sA[k][i] = A(row0 + i, k0 + k)
+  __shared__ T sB[2][KC][GTN];     // sB[k][j] = B(k0 + k, col0 + j)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int64_t row0 = (int64_t)blockIdx.x * GT, col0 = (int64_t)blockIdx.y * GT;
-  const int wr = (wave & 1) * 64, wc = (wave >> 1) * 64;  // wave's 64x64 sub-tile
+  const int64_t row0 = (int64_t)blockIdx.x * GT, col0 = (int64_t)blockIdx.y * GTN;
+  const int wr = (wave & 1) * 64, wc = (wave >> 1) * WN;  // wave's 64 x WN sub-tile
 
-  vec4 acc[4][4];
+  vec4 acc[TC][4];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < TC; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = vec4{0, 0, 0, 0};
 
-  // global -> register staging: each thread loads 8 elements of A and 8 of B
-  T ra[8], rb[8];
+  T ra[8], rb[BQ];
   auto load_global = [&](int64_t k0) {
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-      const int e = tid + 256 * q;  // 0 .. 2047 = KC * GT
-      // A: i fastest (coalesced down a column)
+      const int e = tid + 256 * q;  // 0 .. KC * GT - 1
       const int ai = e & (GT - 1), ak = e >> 7;
       const int64_t gr = row0 + ai, gk = k0 + ak;
       ra[q] = (gr < M && gk < K) ? A[gr + gk * lda] : T(0);
-      // B: k fastest (KC contiguous rows of one column)
+    }
+#pragma unroll
+    for (int q = 0; q < BQ; ++q) {
+      const int e = tid + 256 * q;  // 0 .. KC * GTN - 1
       const int bk = e & (KC - 1), bj = e >> 4;
       const int64_t gc = col0 + bj, gk2 = k0 + bk;
       rb[q] = (gc < N && gk2 < K) ? B[gk2 + gc * ldb] : T(0);
@@ -101,6 +107,10 @@ __global__ __launch_bounds__(256) void k_gemm_sub(T* __restrict__ C, int64_t ldc
     for (int q = 0; q < 8; ++q) {
       const int e = tid + 256 * q;
       sA[buf][e >> 7][e & (GT - 1)] = ra[q];
+    }
+#pragma unroll
+    for (int q = 0; q < BQ; ++q) {
+      const int e = tid + 256 * q;
       sB[buf][e & (KC - 1)][e >> 4] = rb[q];
     }
   };
@@ -115,25 +125,22 @@ __global__ __launch_bounds__(256) void k_gemm_sub(T* __restrict__ C, int64_t ldc
 #pragma unroll
     for (int k4 = 0; k4 < KC; k4 += 4) {
       const int kk = k4 + (lane >> 4);
-      T fu[4], fl[4];
+      T fu[TC], fl[4];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        fu[t] = sB[buf][kk][wc + 16 * t + (lane & 15)];  // U^T rows = C columns
-        fl[t] = sA[buf][kk][wr + 16 * t + (lane & 15)];  // L^T cols = C rows
-      }
+      for (int t = 0; t < TC; ++t) fu[t] = sB[buf][kk][wc + 16 * t + (lane & 15)];  // U^T rows = C columns
 #pragma unroll
-      for (int tc = 0; tc < 4; ++tc)
+      for (int t = 0; t < 4; ++t) fl[t] = sA[buf][kk][wr + 16 * t + (lane & 15)];   // L^T cols = C rows
+#pragma unroll
+      for (int tc = 0; tc < TC; ++tc)
 #pragma unroll
         for (int tr = 0; tr < 4; ++tr) acc[tc][tr] = MfmaVec<T>::mfma(fu[tc], fl[tr], acc[tc][tr]);
     }
-    if (c + 1 < nchunks) {
-      store_lds(buf ^ 1);
-    }
+    if (c + 1 < nchunks) store_lds(buf ^ 1);
     __syncthreads();
   }
   // acc[tc][tr][v] = (A B)(row = wr + 16 tr + lane%16, col = wc + 16 tc + drow(lane, v))
 #pragma unroll
-  for (int tc = 0; tc < 4; ++tc)
+  for (int tc = 0; tc < TC; ++tc)
 #pragma unroll
     for (int tr = 0; tr < 4; ++tr)
 #pragma unroll
@@ -168,6 +175,7 @@ struct PanelCtx {
   int32_t* info;
   T* red_v;
   int64_t* red_i;
+  int* red_w;
   T* s_piv;
 };
 
@@ -237,25 +245,53 @@ __device__ __forceinline__ void panel_step(T (&a)[RPT][IB], const int64_t (&ri)[
       }
       __threadfence();
       grid.sync();
-      // 2. global pivot (every thread read red_i[0] before the grid barrier)
-      if (t == 0) {
+      // 2. global pivot: every thread reads candidates q = t, t + PWG, ...,
+      //    then a workgroup arg-max (every thread read red_i[0] before the
+      //    grid barrier, so the LDS slots can be reused)
+      {
         T gb = T(-1);
         int64_t gi = -1;
-        int gw = 0;
-        for (int q = 0; q < P.nwg; ++q) {
+        int gw = -1;
+        for (int q = t; q < P.nwg; q += PWG) {
           const T ov = P.ps->val[par][q];
           const int64_t oi = P.ps->idx[par][q];
-          if (oi < 0) continue;
-          if (ov > gb || (ov == gb && oi < gi)) {
+          if (oi >= 0 && (ov > gb || (ov == gb && oi < gi))) {
             gb = ov;
             gi = oi;
             gw = q;
           }
         }
-        P.red_i[0] = gi;
-        P.red_i[1] = gw;
+        for (int off = 32; off > 0; off >>= 1) {
+          const T ov = __shfl_xor(gb, off);
+          const int64_t oi = __shfl_xor(gi, off);
+          const int ow = __shfl_xor(gw, off);
+          if (oi >= 0 && (ov > gb || (ov == gb && (gi < 0 || oi < gi)))) {
+            gb = ov;
+            gi = oi;
+            gw = ow;
+          }
+        }
+        if ((t & 63) == 0) {
+          P.red_v[t >> 6] = gb;
+          P.red_i[t >> 6] = gi;
+          P.red_w[t >> 6] = gw;
+        }
+        __syncthreads();
+        if (t == 0) {
+          for (int q = 1; q < PWG / 64; ++q) {
+            const T ov = P.red_v[q];
+            const int64_t oi = P.red_i[q];
+            if (oi >= 0 && (ov > gb || (ov == gb && (gi < 0 || oi < gi)))) {
+              gb = ov;
+              gi = oi;
+              gw = P.red_w[q];
+            }
+          }
+          P.red_i[0] = gi;
+          P.red_i[1] = gw;
+        }
+        __syncthreads();
       }
-      __syncthreads();
       const int64_t p = P.red_i[0];
       const int pw = (int)P.red_i[1];
       const T* prow = P.rows + ((size_t)par * (P.nwg + 1) + pw) * IB;
@@ -304,6 +340,7 @@ __global__ __launch_bounds__(PWG) void k_panel(T* __restrict__ A, int64_t lda, i
   const int64_t base = (int64_t)w * chunk;
   __shared__ T red_v[PWG / 64];
   __shared__ int64_t red_i[PWG / 64];
+  __shared__ int red_w[PWG / 64];
   __shared__ T s_piv[IB];
 
   T a[RPT][IB];
@@ -317,7 +354,7 @@ __global__ __launch_bounds__(PWG) void k_panel(T* __restrict__ A, int64_t lda, i
     for (int c = 0; c < IB; ++c)
       a[r][c] = (ok && c < jb) ? A[(j0 + ri[r]) + (j0 + c) * lda] : T(0);
   }
-  PanelCtx<T> P{nwg, w, t, jb, j0, ps, rows, ipiv, info, red_v, red_i, s_piv};
+  PanelCtx<T> P{nwg, w, t, jb, j0, ps, rows, ipiv, info, red_v, red_i, red_w, s_piv};
   panel_step<T, IB, RPT, 0>(a, ri, P, grid);
 #pragma unroll
   for (int r = 0; r < RPT; ++r)
@@ -348,54 +385,53 @@ __global__ void k_laswp(T* __restrict__ A, int64_t lda, const int32_t* __restric
 }
 
 // ---------------------------------------------------------------------
-// B[k0:k0+kb, c0:c1] := L^{-1} B with L the unit lower triangle of
-// A[k0:k0+kb, k0:k0+kb].  Workgroup = 64 columns x 4 row-groups; the block
-// of B lives in LDS.
+// Leaf triangular solves for kb <= LB rows: one thread per column of B,
+// the column held in registers, the kb x kb triangle in LDS (read as
+// wave-wide broadcasts).  B := L^{-1} B (unit lower) or U^{-1} B (upper).
 // ---------------------------------------------------------------------
-constexpr int TS_COLS = 16;
-constexpr int TS_MAXK = 256;
+constexpr int LB = 64;
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_trsm_lunit(const T* __restrict__ L, int64_t ldl,
-                                                    T* __restrict__ B, int64_t ldb, int kb,
-                                                    int64_t ncols) {
-  __shared__ T x[TS_MAXK][TS_COLS + 1];
-  const int tid = threadIdx.x;
-  const int64_t cb = (int64_t)blockIdx.x * TS_COLS;
-  const int lc = tid & (TS_COLS - 1), lr = tid / TS_COLS;  // 8 row-lanes
-  for (int i = lr; i < kb; i += 256 / TS_COLS)
-    x[i][lc] = (cb + lc < ncols) ? B[i + (cb + lc) * ldb] : T(0);
-  __syncthreads();
-  for (int j = 0; j < kb - 1; ++j) {
-    const T xj = x[j][lc];
-    for (int i = j + 1 + lr; i < kb; i += 256 / TS_COLS) x[i][lc] -= L[i + (int64_t)j * ldl] * xj;
-    __syncthreads();
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void k_trsm_leaf(const T* __restrict__ Lm, int64_t ldl, T* __restrict__ B,
+                                                   int64_t ldb, int kb, int64_t ncols) {
+  __shared__ T sL[LB][LB + 1];
+  for (int e = threadIdx.x; e < LB * LB; e += 256) {
+    const int i = e % LB, j = e / LB;
+    sL[i][j] = (i < kb && j < kb) ? Lm[i + (int64_t)j * ldl] : T(0);
   }
-  for (int i = lr; i < kb; i += 256 / TS_COLS)
-    if (cb + lc < ncols) B[i + (cb + lc) * ldb] = x[i][lc];
-}
-
-// B := U^{-1} B for the upper triangle U of A[k0:k0+kb, k0:k0+kb] (non-unit).
-template <typename T>
-__global__ __launch_bounds__(256) void k_trsm_upper(const T* __restrict__ U, int64_t ldu,
-                                                    T* __restrict__ B, int64_t ldb, int kb,
-                                                    int64_t ncols) {
-  __shared__ T x[TS_MAXK][TS_COLS + 1];
-  const int tid = threadIdx.x;
-  const int64_t cb = (int64_t)blockIdx.x * TS_COLS;
-  const int lc = tid & (TS_COLS - 1), lr = tid / TS_COLS;
-  for (int i = lr; i < kb; i += 256 / TS_COLS)
-    x[i][lc] = (cb + lc < ncols) ? B[i + (cb + lc) * ldb] : T(0);
   __syncthreads();
-  for (int j = kb - 1; j >= 0; --j) {
-    if (lr == 0) x[j][lc] /= U[j + (int64_t)j * ldu];
-    __syncthreads();
-    const T xj = x[j][lc];
-    for (int i = lr; i < j; i += 256 / TS_COLS) x[i][lc] -= U[i + (int64_t)j * ldu] * xj;
-    __syncthreads();
+  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= ncols) return;
+  T* col = B + c * ldb;
+  T x[LB];
+#pragma unroll
+  for (int i = 0; i < LB; ++i) x[i] = (i < kb) ? col[i] : T(0);
+  if (LOWER) {
+#pragma unroll
+    for (int i = 1; i < LB; ++i) {
+      // the guard keeps the scheduler from hoisting every LDS read of the
+      // unrolled loop nest at once (which spills)
+      if (i < kb) {
+        T acc = x[i];
+#pragma unroll
+        for (int j = 0; j < i; ++j) acc -= sL[i][j] * x[j];
+        x[i] = acc;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int i = LB - 1; i >= 0; --i) {
+      if (i < kb) {
+        T acc = x[i];
+#pragma unroll
+        for (int j = i + 1; j < LB; ++j) acc -= sL[i][j] * x[j];
+        x[i] = acc / sL[i][i];
+      }
+    }
   }
-  for (int i = lr; i < kb; i += 256 / TS_COLS)
-    if (cb + lc < ncols) B[i + (cb + lc) * ldb] = x[i][lc];
+#pragma unroll
+  for (int i = 0; i < LB; ++i)
+    if (i < kb) col[i] = x[i];
 }
 
 // ---------------------------------------------------------------------
@@ -405,10 +441,15 @@ template <typename T>
 static int gemm_sub(hfpx_ctx* ctx, T* C, int64_t ldc, const T* A, int64_t lda, const T* B,
                     int64_t ldb, int64_t M, int64_t N, int64_t K) {
   if (M <= 0 || N <= 0 || K <= 0) return HFPX_OK;
-  dim3 grid((unsigned)((M + GT - 1) / GT), (unsigned)((N + GT - 1) / GT));
+  const bool skinny = N <= 64;
+  const int gtn = skinny ? 64 : 128;
+  dim3 grid((unsigned)((M + GT - 1) / GT), (unsigned)((N + gtn - 1) / gtn));
   if (grid.y > 65535u) return set_error(ctx, HFPX_ERR_INVALID, "gemm: N too large");
   const int tok = prof_begin(ctx);
-  hipLaunchKernelGGL(k_gemm_sub<T>, grid, dim3(256), 0, ctx->stream, C, ldc, A, lda, B, ldb, M, N, K);
+  if (skinny)
+    hipLaunchKernelGGL((k_gemm_sub<T, 64>), grid, dim3(256), 0, ctx->stream, C, ldc, A, lda, B, ldb, M, N, K);
+  else
+    hipLaunchKernelGGL((k_gemm_sub<T, 128>), grid, dim3(256), 0, ctx->stream, C, ldc, A, lda, B, ldb, M, N, K);
   // algorithmic: 2MNK flops; bytes: C read + written, A and B read once
   prof_end(ctx, tok, sizeof(T) == 8 ? HFPX_KC_GEMM_F64 : HFPX_KC_GEMM_F32, 2.0 * M * N * K,
            (double)sizeof(T) * (2.0 * M * N + (double)M * K + (double)K * N));
@@ -429,27 +470,44 @@ static int laswp(hfpx_ctx* ctx, T* A, int64_t lda, const int32_t* ipiv, int64_t 
   return HFPX_OK;
 }
 
-template <typename T>
-static int trsm_lunit(hfpx_ctx* ctx, const T* L, int64_t ldl, T* B, int64_t ldb, int kb,
-                      int64_t ncols) {
-  if (ncols <= 0 || kb <= 0) return HFPX_OK;
+template <typename T, bool LOWER>
+static int trsm_leaf(hfpx_ctx* ctx, const T* L, int64_t ldl, T* B, int64_t ldb, int kb, int64_t ncols) {
   const int tok = prof_begin(ctx);
-  hipLaunchKernelGGL(k_trsm_lunit<T>, dim3((unsigned)((ncols + TS_COLS - 1) / TS_COLS)), dim3(256), 0,
-                     ctx->stream, L, ldl, B, ldb, kb, ncols);
+  hipLaunchKernelGGL((k_trsm_leaf<T, LOWER>), dim3((unsigned)((ncols + 255) / 256)), dim3(256), 0, ctx->stream,
+                     L, ldl, B, ldb, kb, ncols);
   prof_end(ctx, tok, HFPX_KC_TRSM, (double)kb * kb * ncols, 0.0);
   HFPX_TRY_HIP(ctx, hipGetLastError());
   return HFPX_OK;
 }
 
+// B (kb x ncols) := L^{-1} B, L unit lower kb x kb: leaves of LB rows with
+// MFMA updates from the rows already solved (left-looking).
 template <typename T>
-static int trsm_upper(hfpx_ctx* ctx, const T* U, int64_t ldu, T* B, int64_t ldb, int kb,
-                      int64_t ncols) {
+static int trsm_lunit(hfpx_ctx* ctx, const T* L, int64_t ldl, T* B, int64_t ldb, int kb, int64_t ncols) {
   if (ncols <= 0 || kb <= 0) return HFPX_OK;
-  const int tok = prof_begin(ctx);
-  hipLaunchKernelGGL(k_trsm_upper<T>, dim3((unsigned)((ncols + TS_COLS - 1) / TS_COLS)), dim3(256), 0,
-                     ctx->stream, U, ldu, B, ldb, kb, ncols);
-  prof_end(ctx, tok, HFPX_KC_TRSM, (double)kb * kb * ncols, 0.0);
-  HFPX_TRY_HIP(ctx, hipGetLastError());
+  int rc;
+  for (int i0 = 0; i0 < kb; i0 += LB) {
+    const int ib = std::min(LB, kb - i0);
+    if (i0 > 0 && (rc = gemm_sub<T>(ctx, B + i0, ldb, L + i0, ldl, B, ldb, ib, ncols, i0))) return rc;
+    if ((rc = trsm_leaf<T, true>(ctx, L + i0 + (int64_t)i0 * ldl, ldl, B + i0, ldb, ib, ncols))) return rc;
+  }
+  return HFPX_OK;
+}
+
+// B (kb x ncols) := U^{-1} B, U upper (non-unit) kb x kb, bottom-up leaves.
+template <typename T>
+static int trsm_upper(hfpx_ctx* ctx, const T* U, int64_t ldu, T* B, int64_t ldb, int kb, int64_t ncols) {
+  if (ncols <= 0 || kb <= 0) return HFPX_OK;
+  int rc;
+  const int nl = (kb + LB - 1) / LB;
+  for (int b = nl - 1; b >= 0; --b) {
+    const int i0 = b * LB, ib = std::min(LB, kb - i0);
+    const int done = kb - (i0 + ib);
+    if (done > 0 && (rc = gemm_sub<T>(ctx, B + i0, ldb, U + i0 + (int64_t)(i0 + ib) * ldu, ldu, B + i0 + ib, ldb,
+                                      ib, ncols, done)))
+      return rc;
+    if ((rc = trsm_leaf<T, false>(ctx, U + i0 + (int64_t)i0 * ldu, ldu, B + i0, ldb, ib, ncols))) return rc;
+  }
   return HFPX_OK;
 }
 
@@ -556,7 +614,7 @@ static int getrf_impl(hfpx_ctx* ctx, T* A, int64_t lda, int64_t n, int32_t* ipiv
 template <typename T>
 static int getrs_impl(hfpx_ctx* ctx, const T* LU, int64_t lda, const int32_t* ipiv, int64_t n,
                       T* B, int64_t ldb, int64_t nrhs) {
-  constexpr int NB = TS_MAXK;
+  constexpr int NB = 256;
   int rc;
   // row interchanges on B (one thread per right-hand side)
   if ((rc = laswp<T>(ctx, B, ldb, ipiv, 0, n, 0, nrhs))) return rc;
