@@ -12,6 +12,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
+HOST = os.path.join(HERE, "host")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhfpx3d.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
@@ -27,13 +28,19 @@ CFLAGS = [
 ]
 
 
+EXES = {"hfpx3d_main": "hfpx3d_main.cpp", "hfpx3d_host_check": "hfpx3d_host_check.cpp"}
+
+
 def _sources():
-    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+    srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+    srcs.append(os.path.join(HOST, "hfp3d_host.cpp"))
+    return srcs
 
 
 def _compile(src):
-    obj = os.path.join(BUILD, os.path.basename(src)[:-4] + ".o")
-    cmd = [HIPCC, *CFLAGS, "-c", src, "-o", obj]
+    obj = os.path.join(BUILD, os.path.splitext(os.path.basename(src))[0] + ".o")
+    lang = ["-x", "hip"] if src.endswith(".hip") else []
+    cmd = [HIPCC, *CFLAGS, *lang, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")
@@ -57,6 +64,15 @@ def build(verbose=False, jobs=None):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # host executables linked against the library (rpath: the package dir)
+    for exe, src in EXES.items():
+        out = os.path.join(HERE, exe)
+        cmd = [HIPCC, "-O2", "-std=c++17", "-I", INCLUDE, os.path.join(HOST, src), "-o", out + ".tmp",
+               "-L", HERE, "-lhfpx3d", "-Wl,-rpath,$ORIGIN"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"building {exe} failed:\n{r.stdout}\n{r.stderr}")
+        os.replace(out + ".tmp", out)
     return LIB
 
 

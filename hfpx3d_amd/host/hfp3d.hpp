@@ -1,0 +1,178 @@
+// hfp3d.hpp -- C++ host mirror of the HFPx3D hot-path call surface, backed by
+// the HIP library through its C ABI (include/hfpx3d.h).
+//
+// Same names, argument meaning and stage semantics as the reference's
+// namespace hfp3d:
+//   Mesh_Geom_T, DoF_Handle_T, Mesh_Data_T      src/Core/surface_mesh_utilities.h:25-104
+//   Properties_T, Load_T, Num_Param_T, Sim_Param_T  src/Core/model_parameters.h:34-104
+//   SAE_T                                       src/Solvers/system_assembly.h:26-30
+//   IO_param_T, read_config                     src/IO/config_file_io.h:22-53
+//   make_dof_h_crack, write_dd_vector_to_md     src/Core/surface_mesh_utilities.cpp:28-112, 256-328
+//   make_3dbem_matrix_s, add_s_inf_to_3dbem_rhs,
+//   make_3dbem_stress_f_s                       src/Solvers/system_assembly.cpp:30-455
+//   load_mesh_from_numpy_32/_64, load_mesh_from_csv,
+//   load_crd_from_csv/_numpy                    src/IO/mesh_file_io.cpp:28-471
+//   save_data_to_csv                            src/IO/data_file_io.h:171-204
+//   LU (factor + solve)                         il/linear_algebra/dense/factorization/LU.h:46-138
+//
+// Differences, by design: the dense matrix lives in device memory
+// (DeviceMatrix, moved into LU like il::LU takes its matrix by value);
+// failures that the reference handles with abort() are reported through
+// Status instead.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/hfpx3d.h"
+
+namespace hfp3d {
+
+// Column-major 2D array (ld = size(0)), the layout of il::Array2D.
+template <typename T>
+class Array2D {
+ public:
+  Array2D() = default;
+  Array2D(int64_t n0, int64_t n1, T v = T()) : n0_(n0), n1_(n1), d_((size_t)(n0 * n1), v) {}
+  int64_t size(int k) const { return k == 0 ? n0_ : n1_; }
+  T& operator()(int64_t i, int64_t j) { return d_[(size_t)(i + n0_ * j)]; }
+  const T& operator()(int64_t i, int64_t j) const { return d_[(size_t)(i + n0_ * j)]; }
+  T* data() { return d_.data(); }
+  const T* data() const { return d_.data(); }
+  void resize(int64_t n0, int64_t n1) {
+    n0_ = n0;
+    n1_ = n1;
+    d_.assign((size_t)(n0 * n1), T());
+  }
+
+ private:
+  int64_t n0_ = 0, n1_ = 0;
+  std::vector<T> d_;
+};
+
+struct Status {
+  std::string error;  // empty: ok
+  int64_t info = 0;   // e.g. rank for MatrixSingular (LU.h:94-98)
+  bool ok() const { return error.empty(); }
+  void abortOnError() const;
+};
+
+struct Mesh_Geom_T {
+  Array2D<double> nods;   // (3 or 6) x n_nodes
+  Array2D<int64_t> conn;  // (3 or 5) x n_el, 0-based vertex ids in rows 0..2
+};
+
+struct DoF_Handle_T {
+  int64_t n_dof = 0;
+  Array2D<int64_t> dof_h;  // n_el x 18, -1 = fixed
+};
+
+struct Mesh_Data_T {
+  Mesh_Geom_T mesh;
+  DoF_Handle_T dof_h_dd;
+  DoF_Handle_T dof_h_pp;
+  Array2D<double> dd;  // (6 n_el) x 3, global components
+};
+
+struct Properties_T {
+  int64_t n_solid = 1;
+  std::vector<double> shear_m{1.0};
+  std::vector<double> poiss_r{0.0};
+};
+
+struct Load_T {
+  double s_inf[6] = {0, 0, 0, 0, 0, 0};  // xx yy zz xy xz yz, tension positive
+};
+
+struct Num_Param_T {
+  double beta = 0.125;
+  int tip_type = 1;
+  bool is_dd_local = false;
+};
+
+struct Sim_Param_T {
+  bool do_save_matrix = false;
+  bool do_save_solution = false;
+  bool do_postprocess = false;
+};
+
+struct IO_param_T {
+  std::string src_dir, cf_name, input_dir, conn_f_name, node_f_name, in_f_format;
+  std::string obs_p_dir, obs_p_f_name, obs_p_f_format;
+  std::string output_dir, out_f_name, out_f_format, matr_f_name, strs_f_name;
+};
+
+// One GPU context per process (device 0 unless set_device is called first).
+hfpx_ctx* context();
+void set_device(int device);
+
+// Dense n x n FP64 matrix in device memory (move-only).
+class DeviceMatrix {
+ public:
+  DeviceMatrix() = default;
+  explicit DeviceMatrix(int64_t n);
+  ~DeviceMatrix();
+  DeviceMatrix(DeviceMatrix&& o) noexcept { *this = std::move(o); }
+  DeviceMatrix& operator=(DeviceMatrix&& o) noexcept;
+  DeviceMatrix(const DeviceMatrix&) = delete;
+  DeviceMatrix& operator=(const DeviceMatrix&) = delete;
+  int64_t size(int) const { return n_; }
+  double* data() { return d_; }
+  const double* data() const { return d_; }
+  Array2D<double> to_host() const;
+
+ private:
+  double* d_ = nullptr;
+  int64_t n_ = 0;
+};
+
+struct SAE_T {
+  int64_t n_dof = 0;
+  DeviceMatrix matrix;
+  std::vector<double> rhs_v;
+};
+
+// ---- mesh / DoF -----------------------------------------------------------
+DoF_Handle_T make_dof_h_crack(const Mesh_Geom_T& mesh, int ap_order, int tip_type);
+void write_dd_vector_to_md(const std::vector<double>& sol_v, const DoF_Handle_T& dof_h_dd, Mesh_Data_T& m_data);
+
+// ---- the GPU stages --------------------------------------------------------
+DeviceMatrix make_3dbem_matrix_s(double shear_m, double poiss_r, const Mesh_Geom_T& mesh,
+                                 const Num_Param_T& n_par, DoF_Handle_T& dof_hndl);
+void add_s_inf_to_3dbem_rhs(const Mesh_Data_T& mesh_data, const Load_T& load, SAE_T& sae);
+Array2D<double> make_3dbem_stress_f_s(double shear_m, double poiss_r, const Mesh_Data_T& m_data,
+                                      const Load_T& load, const Num_Param_T& n_par,
+                                      const Array2D<double>& m_pts_crd);
+
+class LU {
+ public:
+  LU(DeviceMatrix A, Status& status);
+  ~LU();
+  LU(const LU&) = delete;
+  LU& operator=(const LU&) = delete;
+  std::vector<double> solve(const std::vector<double>& y) const;
+
+ private:
+  DeviceMatrix A_;
+  int32_t* ipiv_ = nullptr;
+};
+
+// ---- IO ---------------------------------------------------------------------
+void load_mesh_from_numpy_32(const std::string& src_dir, const std::string& conn_f_name,
+                             const std::string& node_f_name, int64_t origin, Mesh_Geom_T& mesh,
+                             Status& status);
+void load_mesh_from_numpy_64(const std::string& src_dir, const std::string& conn_f_name,
+                             const std::string& node_f_name, int64_t origin, Mesh_Geom_T& mesh,
+                             Status& status);
+void load_mesh_from_csv(const std::string& src_dir, const std::string& conn_f_name,
+                        const std::string& node_f_name, int64_t origin, Mesh_Geom_T& mesh, Status& status);
+Array2D<double> load_crd_from_csv(const std::string& src_dir, const std::string& f_name, Status& status);
+Array2D<double> load_crd_from_numpy(const std::string& src_dir, const std::string& f_name, Status& status);
+bool save_data_to_csv(const Array2D<double>& a, const std::string& trg_dir, const std::string& of_name);
+
+IO_param_T set_default_paths(const std::string& src_path);
+void read_config(const std::string& f_path, Mesh_Data_T& mesh_data, Properties_T& props, Load_T& load,
+                 Num_Param_T& num_param, Sim_Param_T& sim_param, IO_param_T& io_param, Status& status);
+
+}  // namespace hfp3d

@@ -1,0 +1,608 @@
+// hfp3d_host.cpp -- implementation of the C++ host mirror (hfp3d.hpp).
+#include <hip/hip_runtime.h>
+
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <sstream>
+
+#include "hfp3d.hpp"
+
+namespace hfp3d {
+
+// ------------------------------------------------------------------ basics
+void Status::abortOnError() const {
+  if (!ok()) {
+    std::cerr << "hfp3d error: " << error << " (info " << info << ")" << std::endl;
+    std::abort();
+  }
+}
+
+static int g_device = 0;
+static hfpx_ctx* g_ctx = nullptr;
+
+void set_device(int device) { g_device = device; }
+
+hfpx_ctx* context() {
+  if (!g_ctx) {
+    if (hfpx_create(&g_ctx, g_device, nullptr) != HFPX_OK) {
+      std::cerr << "hfp3d: no usable GPU (device " << g_device << ")" << std::endl;
+      std::abort();
+    }
+  }
+  return g_ctx;
+}
+
+static void check(int rc, const char* what) {
+  if (rc != HFPX_OK) {
+    std::cerr << "hfp3d: " << what << ": " << hfpx_status_string(rc) << ": " << hfpx_last_error(context())
+              << std::endl;
+    std::abort();
+  }
+}
+
+template <typename T>
+static T* dalloc(size_t n) {
+  void* p = nullptr;
+  if (hipMalloc(&p, sizeof(T) * (n ? n : 1)) != hipSuccess) {
+    std::cerr << "hfp3d: device allocation of " << sizeof(T) * n << " bytes failed" << std::endl;
+    std::abort();
+  }
+  return (T*)p;
+}
+
+static void h2d(void* d, const void* h, size_t bytes) {
+  if (hipMemcpy(d, h, bytes, hipMemcpyHostToDevice) != hipSuccess) std::abort();
+}
+static void d2h(void* h, const void* d, size_t bytes) {
+  if (hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost) != hipSuccess) std::abort();
+}
+
+DeviceMatrix::DeviceMatrix(int64_t n) : d_(dalloc<double>((size_t)n * n)), n_(n) {}
+DeviceMatrix::~DeviceMatrix() {
+  if (d_) (void)hipFree(d_);
+}
+DeviceMatrix& DeviceMatrix::operator=(DeviceMatrix&& o) noexcept {
+  if (this != &o) {
+    if (d_) (void)hipFree(d_);
+    d_ = o.d_;
+    n_ = o.n_;
+    o.d_ = nullptr;
+    o.n_ = 0;
+  }
+  return *this;
+}
+Array2D<double> DeviceMatrix::to_host() const {
+  Array2D<double> a(n_, n_);
+  (void)hipDeviceSynchronize();
+  d2h(a.data(), d_, sizeof(double) * n_ * n_);
+  return a;
+}
+
+// RAII device mesh
+struct MeshHandle {
+  hfpx_mesh* m = nullptr;
+  MeshHandle(const Mesh_Geom_T& mesh, const DoF_Handle_T* dof, double beta) {
+    const int64_t* dh = (dof && dof->n_dof > 0 && dof->dof_h.size(0) == mesh.conn.size(1)) ? dof->dof_h.data()
+                                                                                            : nullptr;
+    check(hfpx_mesh_create(context(), mesh.nods.data(), mesh.nods.size(0), mesh.nods.size(1), mesh.conn.data(),
+                           mesh.conn.size(0), mesh.conn.size(1), dh, dh ? dof->n_dof : 0, beta, &m),
+          "mesh upload");
+  }
+  ~MeshHandle() { hfpx_mesh_destroy(m); }
+};
+
+// ------------------------------------------------------------------ DoF
+DoF_Handle_T make_dof_h_crack(const Mesh_Geom_T& mesh, int ap_order, int tip_type) {
+  DoF_Handle_T d;
+  if (ap_order != 2) return d;
+  const int64_t n_el = mesh.conn.size(1);
+  const bool ext = mesh.nods.size(0) >= 5 && mesh.conn.size(0) >= 5;
+  d.n_dof = 18 * n_el;
+  d.dof_h = Array2D<int64_t>(n_el, 18, 0);
+  int64_t dec = 0;
+  for (int64_t e = 0; e < n_el; ++e) {
+    bool tip[3] = {false, false, false};
+    if (ext && mesh.conn(3, e) == mesh.conn(4, e))
+      for (int v = 0; v < 3; ++v) {
+        const int64_t n = mesh.conn(v, e);
+        tip[v] = mesh.nods(3, n) == 1.0 && mesh.nods(4, n) == 1.0;
+      }
+    for (int v = 0; v < 6; ++v) {
+      const bool fixed = (tip_type >= 1 && v < 3 && tip[v]) ||
+                         (tip_type == 2 && v >= 3 && tip[(v - 2) % 3] && tip[(v - 1) % 3]);
+      for (int l = 0; l < 3; ++l) {
+        const int ld = 3 * v + l;
+        if (fixed) {
+          d.dof_h(e, ld) = -1;
+          --d.n_dof;
+          ++dec;
+        } else {
+          d.dof_h(e, ld) = 18 * e + ld - dec;
+        }
+      }
+    }
+  }
+  return d;
+}
+
+void write_dd_vector_to_md(const std::vector<double>& sol_v, const DoF_Handle_T& dof_h_dd, Mesh_Data_T& m_data) {
+  const int64_t n_el = dof_h_dd.dof_h.size(0);
+  m_data.dd = Array2D<double>(6 * n_el, 3, 0.0);
+  for (int64_t e = 0; e < n_el; ++e)
+    for (int en = 0; en < 6; ++en)
+      for (int i = 0; i < 3; ++i) {
+        const int64_t j = dof_h_dd.dof_h(e, 3 * en + i);
+        if (j >= 0) m_data.dd(6 * e + en, i) = sol_v[(size_t)j];
+      }
+}
+
+// ------------------------------------------------------------------ GPU stages
+DeviceMatrix make_3dbem_matrix_s(double shear_m, double poiss_r, const Mesh_Geom_T& mesh,
+                                 const Num_Param_T& n_par, DoF_Handle_T& dof_hndl) {
+  if (dof_hndl.n_dof == 0 || dof_hndl.dof_h.size(0) == 0) dof_hndl = make_dof_h_crack(mesh, 2, n_par.tip_type);
+  MeshHandle mh(mesh, &dof_hndl, n_par.beta);
+  DeviceMatrix A(dof_hndl.n_dof);
+  check(hfpx_assemble(context(), mh.m, shear_m, poiss_r, A.data(), dof_hndl.n_dof), "assembly");
+  check(hfpx_synchronize(context()), "assembly");
+  return A;
+}
+
+void add_s_inf_to_3dbem_rhs(const Mesh_Data_T& mesh_data, const Load_T& load, SAE_T& sae) {
+  const int64_t n = mesh_data.dof_h_dd.n_dof;
+  if ((int64_t)sae.rhs_v.size() != n) sae.rhs_v.assign((size_t)n, 0.0);
+  MeshHandle mh(mesh_data.mesh, &mesh_data.dof_h_dd, 0.125);
+  double* b = dalloc<double>((size_t)n);
+  h2d(b, sae.rhs_v.data(), sizeof(double) * n);
+  check(hfpx_rhs_sinf(context(), mh.m, load.s_inf, 1, b, n), "rhs");
+  d2h(sae.rhs_v.data(), b, sizeof(double) * n);
+  (void)hipFree(b);
+  sae.n_dof = n;
+}
+
+Array2D<double> make_3dbem_stress_f_s(double shear_m, double poiss_r, const Mesh_Data_T& m_data,
+                                      const Load_T& load, const Num_Param_T& n_par,
+                                      const Array2D<double>& m_pts_crd) {
+  (void)n_par;
+  const int64_t n_el = m_data.mesh.conn.size(1), np = m_pts_crd.size(1);
+  Array2D<double> out(np, 6, 0.0);
+  if (np == 0) return out;
+  MeshHandle mh(m_data.mesh, nullptr, 0.0);
+  double* dd = dalloc<double>(18 * (size_t)n_el);
+  double* pts = dalloc<double>(3 * (size_t)np);
+  double* s = dalloc<double>(6 * (size_t)np);
+  h2d(dd, m_data.dd.data(), sizeof(double) * 18 * n_el);
+  h2d(pts, m_pts_crd.data(), sizeof(double) * 3 * np);
+  check(hfpx_stress(context(), mh.m, shear_m, poiss_r, dd, 6 * n_el, pts, np, load.s_inf, s, np), "stress");
+  d2h(out.data(), s, sizeof(double) * 6 * np);
+  (void)hipFree(dd);
+  (void)hipFree(pts);
+  (void)hipFree(s);
+  return out;
+}
+
+LU::LU(DeviceMatrix A, Status& status) : A_(std::move(A)) {
+  const int64_t n = A_.size(0);
+  ipiv_ = dalloc<int32_t>((size_t)n);
+  int64_t info = 0;
+  const int rc = hfpx_getrf(context(), HFPX_F64, A_.data(), n, n, ipiv_, &info);
+  if (rc == HFPX_ERR_SINGULAR) {
+    status.error = "MatrixSingular";
+    status.info = info - 1;
+  } else {
+    check(rc, "LU factorisation");
+  }
+}
+LU::~LU() {
+  if (ipiv_) (void)hipFree(ipiv_);
+}
+std::vector<double> LU::solve(const std::vector<double>& y) const {
+  const int64_t n = A_.size(0);
+  std::vector<double> x(y);
+  double* b = dalloc<double>((size_t)n);
+  h2d(b, x.data(), sizeof(double) * n);
+  check(hfpx_getrs(context(), HFPX_F64, A_.data(), n, ipiv_, n, b, n, 1), "LU solve");
+  check(hfpx_synchronize(context()), "LU solve");
+  d2h(x.data(), b, sizeof(double) * n);
+  (void)hipFree(b);
+  return x;
+}
+
+// ------------------------------------------------------------------ npy
+// Reads a 2-D .npy array (little-endian <i4, <i8 or <f8, either order) into a
+// column-major array of T.  Read-only: unlike the reference's il::load, the
+// file is opened "rb" (il opens "r+b", which fails on read-only files).
+template <typename T>
+static bool load_npy(const std::string& path, Array2D<T>& out, std::string& err) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) {
+    err = "cannot open " + path;
+    return false;
+  }
+  char magic[6];
+  f.read(magic, 6);
+  if (!f || std::memcmp(magic, "\x93NUMPY", 6) != 0) {
+    err = path + " is not an npy file";
+    return false;
+  }
+  unsigned char ver[2];
+  f.read((char*)ver, 2);
+  uint32_t hlen = 0;
+  if (ver[0] == 1) {
+    unsigned char b[2];
+    f.read((char*)b, 2);
+    hlen = b[0] | (b[1] << 8);
+  } else {
+    unsigned char b[4];
+    f.read((char*)b, 4);
+    hlen = b[0] | (b[1] << 8) | (b[2] << 16) | ((uint32_t)b[3] << 24);
+  }
+  std::string hdr(hlen, '\0');
+  f.read(&hdr[0], hlen);
+  auto field = [&](const std::string& key) {
+    size_t p = hdr.find("'" + key + "'");
+    if (p == std::string::npos) return std::string();
+    p = hdr.find(':', p);
+    size_t q = p + 1;
+    while (q < hdr.size() && std::isspace((unsigned char)hdr[q])) ++q;
+    if (hdr[q] == '\'') {
+      size_t e = hdr.find('\'', q + 1);
+      return hdr.substr(q + 1, e - q - 1);
+    }
+    if (hdr[q] == '(') {
+      size_t e = hdr.find(')', q);
+      return hdr.substr(q + 1, e - q - 1);
+    }
+    size_t e = hdr.find_first_of(",}", q);
+    return hdr.substr(q, e - q);
+  };
+  const std::string descr = field("descr"), fo = field("fortran_order"), shape = field("shape");
+  const bool fortran = fo.find("True") != std::string::npos;
+  std::vector<int64_t> dims;
+  {
+    std::stringstream ss(shape);
+    std::string tok;
+    while (std::getline(ss, tok, ','))
+      if (tok.find_first_not_of(" ") != std::string::npos) dims.push_back(std::stoll(tok));
+  }
+  if (dims.size() == 1) dims.push_back(1);
+  if (dims.size() != 2) {
+    err = path + ": only 1-D and 2-D arrays are supported";
+    return false;
+  }
+  const int64_t n0 = dims[0], n1 = dims[1], cnt = n0 * n1;
+  out.resize(n0, n1);
+  auto fill = [&](auto dummy) {
+    using S = decltype(dummy);
+    std::vector<S> buf((size_t)cnt);
+    f.read((char*)buf.data(), sizeof(S) * cnt);
+    if (!f) return false;
+    for (int64_t i = 0; i < n0; ++i)
+      for (int64_t j = 0; j < n1; ++j) out(i, j) = (T)buf[(size_t)(fortran ? i + n0 * j : i * n1 + j)];
+    return true;
+  };
+  bool ok;
+  if (descr == "<f8") ok = fill(double());
+  else if (descr == "<i4") ok = fill(int32_t());
+  else if (descr == "<i8") ok = fill(int64_t());
+  else {
+    err = path + ": unsupported dtype " + descr;
+    return false;
+  }
+  if (!ok) err = path + ": truncated data";
+  return ok;
+}
+
+static std::string join(const std::string& dir, const std::string& f) {
+  if (dir.empty()) return f;
+  return dir.back() == '/' ? dir + f : dir + "/" + f;
+}
+
+static void shift_origin(Mesh_Geom_T& mesh, int64_t origin) {
+  if (origin == 0) return;
+  const int64_t nr = mesh.conn.size(0) >= 3 ? 3 : mesh.conn.size(0);
+  for (int64_t k = 0; k < mesh.conn.size(1); ++k)
+    for (int64_t j = 0; j < nr; ++j) mesh.conn(j, k) -= origin;
+}
+
+static void load_npy_mesh(const std::string& src_dir, const std::string& cf, const std::string& nf, int64_t origin,
+                          Mesh_Geom_T& mesh, Status& status) {
+  std::string err;
+  if (!load_npy(join(src_dir, cf), mesh.conn, err) || !load_npy(join(src_dir, nf), mesh.nods, err)) {
+    status.error = err;
+    return;
+  }
+  shift_origin(mesh, origin);
+}
+
+void load_mesh_from_numpy_32(const std::string& d, const std::string& cf, const std::string& nf, int64_t origin,
+                             Mesh_Geom_T& mesh, Status& status) {
+  load_npy_mesh(d, cf, nf, origin, mesh, status);
+}
+void load_mesh_from_numpy_64(const std::string& d, const std::string& cf, const std::string& nf, int64_t origin,
+                             Mesh_Geom_T& mesh, Status& status) {
+  load_npy_mesh(d, cf, nf, origin, mesh, status);
+}
+
+// CSV: one row per node / element / point; the array is stored transposed
+// (rows of the file become columns), as in mesh_file_io.cpp:56-116.
+template <typename T>
+static bool load_csv(const std::string& path, Array2D<T>& out, std::string& err) {
+  std::ifstream f(path);
+  if (!f) {
+    err = "cannot open " + path;
+    return false;
+  }
+  std::vector<std::vector<T>> rows;
+  std::string line;
+  size_t ncol = 0;
+  while (std::getline(f, line)) {
+    if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
+    std::vector<T> r;
+    std::stringstream ls(line);
+    std::string cell;
+    while (std::getline(ls, cell, ',')) {
+      if (cell.find_first_not_of(" \t\r") == std::string::npos) continue;
+      std::stringstream cs(cell);
+      double v = 0;
+      cs >> v;
+      r.push_back((T)v);
+    }
+    if (ncol && r.size() != ncol) {
+      err = path + ": row size is not constant";
+      return false;
+    }
+    ncol = r.size();
+    rows.push_back(r);
+  }
+  out.resize((int64_t)ncol, (int64_t)rows.size());
+  for (size_t k = 0; k < rows.size(); ++k)
+    for (size_t j = 0; j < ncol; ++j) out((int64_t)j, (int64_t)k) = rows[k][j];
+  return true;
+}
+
+void load_mesh_from_csv(const std::string& d, const std::string& cf, const std::string& nf, int64_t origin,
+                        Mesh_Geom_T& mesh, Status& status) {
+  std::string err;
+  if (!load_csv(join(d, cf), mesh.conn, err) || !load_csv(join(d, nf), mesh.nods, err)) {
+    status.error = err;
+    return;
+  }
+  shift_origin(mesh, origin);
+}
+
+Array2D<double> load_crd_from_csv(const std::string& d, const std::string& f, Status& status) {
+  Array2D<double> a;
+  std::string err;
+  if (!load_csv(join(d, f), a, err)) status.error = err;
+  return a;
+}
+
+Array2D<double> load_crd_from_numpy(const std::string& d, const std::string& f, Status& status) {
+  Array2D<double> a;
+  std::string err;
+  if (!load_npy(join(d, f), a, err)) status.error = err;
+  return a;
+}
+
+bool save_data_to_csv(const Array2D<double>& a, const std::string& trg_dir, const std::string& of_name) {
+  std::ofstream o(join(trg_dir, of_name));
+  if (!o) return false;
+  for (int64_t j = 0; j < a.size(0); ++j) {
+    for (int64_t k = 0; k < a.size(1); ++k) {
+      o << a(j, k);  // default stream precision (6 significant digits), like the reference
+      if (k < a.size(1) - 1) o << ",";
+    }
+    o << "\n";
+  }
+  return (bool)o;
+}
+
+// ------------------------------------------------------------------ config
+// A TOML subset: key = value pairs (strings, integers, floats, booleans),
+// '#' comments and [table] headers.  Values keep their type: like the
+// reference's reader, a key read as a float is ignored when written as an
+// integer literal and vice versa (config_file_io.cpp:147-275).
+namespace {
+struct TVal {
+  enum Kind { Str, Int, Dbl, Bool } kind = Str;
+  std::string s;
+  int64_t i = 0;
+  double d = 0;
+  bool b = false;
+};
+using Table = std::map<std::string, TVal>;
+
+std::string trim(const std::string& x) {
+  size_t a = x.find_first_not_of(" \t\r"), b = x.find_last_not_of(" \t\r");
+  return a == std::string::npos ? std::string() : x.substr(a, b - a + 1);
+}
+
+bool parse_toml(const std::string& path, std::map<std::string, Table>& out, std::string& err) {
+  std::ifstream f(path);
+  if (!f) {
+    err = "cannot open config file " + path;
+    return false;
+  }
+  std::string line, table;
+  int ln = 0;
+  while (std::getline(f, line)) {
+    ++ln;
+    // strip comments outside strings
+    bool in_s = false;
+    for (size_t k = 0; k < line.size(); ++k) {
+      if (line[k] == '"') in_s = !in_s;
+      if (line[k] == '#' && !in_s) {
+        line = line.substr(0, k);
+        break;
+      }
+    }
+    line = trim(line);
+    if (line.empty()) continue;
+    if (line.front() == '[') {
+      table = trim(line.substr(1, line.find(']') - 1));
+      continue;
+    }
+    size_t eq = line.find('=');
+    if (eq == std::string::npos) {
+      err = path + ":" + std::to_string(ln) + ": expected key = value";
+      return false;
+    }
+    std::string key = trim(line.substr(0, eq)), v = trim(line.substr(eq + 1));
+    TVal t;
+    if (!v.empty() && v.front() == '"') {
+      t.kind = TVal::Str;
+      t.s = v.substr(1, v.rfind('"') - 1);
+    } else if (v == "true" || v == "false") {
+      t.kind = TVal::Bool;
+      t.b = v == "true";
+    } else {
+      bool is_int = !v.empty();
+      for (size_t k = 0; k < v.size(); ++k)
+        if (!(std::isdigit((unsigned char)v[k]) || ((v[k] == '-' || v[k] == '+') && k == 0) || v[k] == '_'))
+          is_int = false;
+      char* end = nullptr;
+      if (is_int) {
+        std::string w;
+        for (char c : v)
+          if (c != '_') w += c;
+        t.kind = TVal::Int;
+        t.i = std::strtoll(w.c_str(), &end, 10);
+      } else {
+        t.kind = TVal::Dbl;
+        t.d = std::strtod(v.c_str(), &end);
+        if (!end || *end != '\0') {
+          err = path + ":" + std::to_string(ln) + ": cannot parse value of " + key;
+          return false;
+        }
+      }
+    }
+    out[table][key] = t;
+  }
+  return true;
+}
+
+const TVal* get(const Table& t, const std::string& k, TVal::Kind kind) {
+  auto it = t.find(k);
+  if (it == t.end() || it->second.kind != kind) return nullptr;
+  return &it->second;
+}
+
+std::string dir_of(const std::string& base, const std::string& d) {
+  std::string r = (!d.empty() && d.front() == '/') ? d : join(base, d);
+  if (r.empty() || r.back() != '/') r += '/';
+  return r;
+}
+}  // namespace
+
+IO_param_T set_default_paths(const std::string& src_path) {
+  IO_param_T p;
+  p.src_dir = src_path.empty() ? std::string(".") : src_path;
+  p.cf_name = p.src_dir + "/config.toml";
+  p.input_dir = p.src_dir + "/Mesh_Files/";
+  p.output_dir = p.src_dir + "/Test_Output/";
+  p.matr_f_name = "test_assembly";
+  p.out_f_name = "test_solution";
+  p.strs_f_name = "test_stresses";
+  return p;
+}
+
+void read_config(const std::string& f_path, Mesh_Data_T& mesh_data, Properties_T& props, Load_T& load,
+                 Num_Param_T& num_param, Sim_Param_T& sim_param, IO_param_T& io_param, Status& status) {
+  std::string p = f_path;
+  for (auto& c : p)
+    if (c == '\\') c = '/';
+  const size_t slash = p.rfind('/');
+  const std::string src_dir = slash == std::string::npos ? std::string(".") : p.substr(0, slash);
+  io_param = set_default_paths(src_dir);
+  std::map<std::string, Table> doc;
+  std::string err;
+  if (!parse_toml(p, doc, err)) {
+    status.error = err;
+    return;
+  }
+  const Table& c = doc[""];
+  if (auto v = get(c, "mesh_input_directory", TVal::Str)) io_param.input_dir = dir_of(src_dir, v->s);
+  if (auto v = get(c, "mesh_conn_fname", TVal::Str)) io_param.conn_f_name = v->s;
+  else {
+    status.error = "config: mesh_conn_fname missing";
+    return;
+  }
+  if (auto v = get(c, "nodes_crd_fname", TVal::Str)) io_param.node_f_name = v->s;
+  else {
+    status.error = "config: nodes_crd_fname missing";
+    return;
+  }
+  io_param.in_f_format = "npy32";
+  if (auto v = get(c, "mesh_input_format", TVal::Str)) io_param.in_f_format = v->s;
+  int64_t origin = 0;
+  if (auto v = get(c, "array_origin", TVal::Int)) origin = v->i;
+  sim_param = Sim_Param_T();
+  if (auto v = get(c, "do_save_matrix", TVal::Bool)) sim_param.do_save_matrix = v->b;
+  if (auto v = get(c, "do_save_solution", TVal::Bool)) sim_param.do_save_solution = v->b;
+  if (auto v = get(c, "do_postprocess", TVal::Bool)) sim_param.do_postprocess = v->b;
+  // material (config_file_io.cpp:171-219): optional [solid] table first
+  props = Properties_T();
+  if (auto v = get(c, "number_solids", TVal::Int)) props.n_solid = v->i ? v->i : 1;
+  props.shear_m.assign((size_t)props.n_solid, 1.0);
+  props.poiss_r.assign((size_t)props.n_solid, 0.0);
+  const bool solid_table = doc.count("solid") > 0;
+  const Table& mat = solid_table ? doc["solid"] : c;
+  if (auto v = get(mat, "Poisson_ratio", TVal::Dbl)) props.poiss_r[0] = v->d;
+  // in [solid] a non-float Shear_modulus falls back to Young_modulus; at the
+  // top level any Shear_modulus entry suppresses Young_modulus
+  const TVal* g = get(mat, "Shear_modulus", TVal::Dbl);
+  if (g) {
+    props.shear_m[0] = g->d;
+  } else if (solid_table || !mat.count("Shear_modulus")) {
+    if (auto v = get(mat, "Young_modulus", TVal::Dbl)) props.shear_m[0] = v->d / 2.0 / (1.0 + props.poiss_r[0]);
+  }
+  num_param = Num_Param_T();
+  if (auto v = get(c, "cp_offset", TVal::Dbl)) num_param.beta = v->d;
+  if (auto v = get(c, "fix_tip", TVal::Int)) num_param.tip_type = (int)v->i;
+  if (auto v = get(c, "is_dd_local", TVal::Bool)) num_param.is_dd_local = v->b;
+  // far-field stress: compression-positive input, normal components flipped
+  const char* keys[6] = {"S_xx", "S_yy", "S_zz", "S_xy", "S_xz", "S_yz"};
+  for (int k = 0; k < 6; ++k) {
+    load.s_inf[k] = 0.0;
+    if (auto v = get(c, keys[k], TVal::Dbl)) load.s_inf[k] = k < 3 ? -v->d : v->d;
+  }
+  // mesh
+  if (io_param.in_f_format == "csv")
+    load_mesh_from_csv(io_param.input_dir, io_param.conn_f_name, io_param.node_f_name, origin, mesh_data.mesh, status);
+  else if (io_param.in_f_format == "npy64")
+    load_mesh_from_numpy_64(io_param.input_dir, io_param.conn_f_name, io_param.node_f_name, origin, mesh_data.mesh,
+                            status);
+  else
+    load_mesh_from_numpy_32(io_param.input_dir, io_param.conn_f_name, io_param.node_f_name, origin, mesh_data.mesh,
+                            status);
+  if (!status.ok()) return;
+  // observation points and outputs
+  io_param.obs_p_dir = io_param.input_dir;
+  if (auto v = get(c, "observ_crd_directory", TVal::Str)) io_param.obs_p_dir = dir_of(src_dir, v->s);
+  if (auto v = get(c, "observ_crd_fname", TVal::Str)) io_param.obs_p_f_name = v->s;
+  else if (sim_param.do_postprocess) {
+    status.error = "config: observ_crd_fname missing";
+    return;
+  }
+  io_param.obs_p_f_format = "npy32";
+  if (auto v = get(c, "observ_input_format", TVal::Str)) io_param.obs_p_f_format = v->s;
+  if (auto v = get(c, "output_directory", TVal::Str)) io_param.output_dir = dir_of(src_dir, v->s);
+  if (auto v = get(c, "output_signature", TVal::Str)) {
+    io_param.matr_f_name += v->s;
+    io_param.out_f_name += v->s;
+    io_param.strs_f_name += v->s;
+  }
+  io_param.out_f_format = "csv";
+  if (auto v = get(c, "output_format", TVal::Str)) io_param.out_f_format = v->s;
+  io_param.matr_f_name += ".csv";
+  io_param.out_f_name += ".csv";
+  io_param.strs_f_name += ".csv";
+}
+
+}  // namespace hfp3d

@@ -1,0 +1,46 @@
+"""The main()-equivalent driver (hfpx3d_amd/hfpx3d_main) end to end on the
+GPU against the reference's own example outputs (Test_Output/): the 24- and
+121-element penny cracks with the parameters of the reference config
+(G = 1, nu = 0.35, S_zz = -1, fix_tip = 1, beta = 0.125), solution (nodes +
+DD) and stresses at the 11 observation points, written at 6 significant
+digits by the reference."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle_lib import golden
+from test_host import REF_CONFIG
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+MAIN = os.path.join(ROOT, "hfpx3d_amd", "hfpx3d_main")
+
+
+@pytest.mark.parametrize("stem,tag", [("pennymesh24el_32", "psc_24"), ("pennymesh121el_32", "psc_121")])
+def test_driver_reproduces_reference_outputs(tmp_path, stem, tag):
+    print(f"case: driver {tag}", flush=True)
+    (tmp_path / "mesh").mkdir()
+    (tmp_path / "obs").mkdir()
+    (tmp_path / "out").mkdir()
+    np.save(tmp_path / "mesh" / "Elems.npy", np.load(golden(f"Elems_{stem}.npy")))
+    np.save(tmp_path / "mesh" / "Nodes.npy", np.load(golden(f"Nodes_{stem}.npy")))
+    np.savetxt(tmp_path / "obs" / "observation_points.csv", np.load(golden("testoutput_observation_points.npy")),
+               delimiter=",", fmt="%.17g")
+    cfg = REF_CONFIG.format(conn="Elems.npy", nods="Nodes.npy", fmt="npy32", fix_tip=1)
+    cfg = cfg.replace("S_xz = 0.25", "S_xz = 0").replace("_psc_24_ele", f"_{tag}_ele")
+    (tmp_path / "config.toml").write_text(cfg)
+    r = subprocess.run([MAIN, str(tmp_path / "config.toml")], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    assert "Assembly:" in r.stdout and "Solution:" in r.stdout and "Fixed DoF" in r.stdout
+    sol = np.loadtxt(tmp_path / "out" / f"test_solution_{tag}_ele.csv", delimiter=",")
+    ref = np.load(golden(f"testoutput_test_solution_{tag}_ele.npy"))
+    assert sol.shape == ref.shape
+    assert np.abs(sol[:, :3] - ref[:, :3]).max() < 1e-5          # node coordinates
+    assert np.abs(sol[:, 4:] - ref[:, 4:]).max() < 2e-6          # DD (6 digits)
+    st = np.loadtxt(tmp_path / "out" / f"test_stresses_{tag}_ele.csv", delimiter=",")
+    sref = np.load(golden(f"testoutput_test_stresses_{tag}_ele.npy"))
+    assert np.abs(st - sref).max() < 2e-5 * np.abs(sref).max()
