@@ -18,6 +18,9 @@ HFPX_ERR_NOT_CONVERGED = 4
 HFPX_ERR_HIP = 5
 HFPX_F64 = 0
 HFPX_F32 = 1
+KNOB_GEMM_F32 = 0
+GEMM_F32_VARIANTS = 7
+
 KERNEL_CLASSES = {"assemble": 0, "gemm_f64": 1, "gemm_f32": 2, "panel": 3, "laswp": 4, "trsm": 5, "stress": 6}
 
 _c_void_p = ctypes.c_void_p
@@ -66,6 +69,8 @@ SIGNATURES = {
                                    ctypes.POINTER(_c_double), ctypes.POINTER(_c_double)]),
     "hfpx_gemm_sub": (_c_int, [_c_void_p, _c_int, _c_void_p, _c_i64, _c_void_p, _c_i64, _c_void_p, _c_i64,
                                _c_i64, _c_i64, _c_i64]),
+    "hfpx_tune_set": (_c_int, [_c_void_p, _c_int, _c_int]),
+    "hfpx_tune_get": (_c_int, [_c_void_p, _c_int, ctypes.POINTER(_c_int)]),
 }
 
 _lib = None

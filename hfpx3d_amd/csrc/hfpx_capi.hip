@@ -134,6 +134,12 @@ int hfpx_create(hfpx_ctx** out, int device, void* stream) {
   }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+  if (hipMalloc((void**)&ctx->sync, 256) != hipSuccess || hipMemset(ctx->sync, 0, 256) != hipSuccess) {
+    if (ctx->sync) (void)hipFree(ctx->sync);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return HFPX_ERR_HIP;
+  }
   *out = ctx;
   return HFPX_OK;
 }
@@ -143,6 +149,8 @@ int hfpx_destroy(hfpx_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->work) (void)hipFree(ctx->work);
+  if (ctx->lu_work) (void)hipFree(ctx->lu_work);
+  if (ctx->sync) (void)hipFree(ctx->sync);
   if (ctx->hwork) (void)hipHostFree(ctx->hwork);
   prof_clear(ctx);
   for (auto e : ctx->prof_pool) (void)hipEventDestroy(e);
@@ -363,6 +371,24 @@ int hfpx_profile_read(hfpx_ctx* ctx, int kc, int64_t* launches, double* ms, doub
   if (ms) *ms = t;
   if (flops) *flops = f;
   if (bytes) *bytes = b;
+  return HFPX_OK;
+}
+
+int hfpx_tune_set(hfpx_ctx* ctx, int knob, int value) {
+  if (!ctx) return HFPX_ERR_INVALID;
+  if (knob == HFPX_KNOB_GEMM_F32) {
+    if (value < 0 || value >= gemm_f32_variants())
+      return set_error(ctx, HFPX_ERR_INVALID, "tune: GEMM_F32 variant %d out of range [0, %d)", value,
+                       gemm_f32_variants());
+    ctx->knob[knob] = value;
+    return HFPX_OK;
+  }
+  return set_error(ctx, HFPX_ERR_INVALID, "tune: unknown knob %d", knob);
+}
+
+int hfpx_tune_get(const hfpx_ctx* ctx, int knob, int* value) {
+  if (!ctx || !value || knob < 0 || knob >= HFPX_KNOB_COUNT) return HFPX_ERR_INVALID;
+  *value = ctx->knob[knob];
   return HFPX_OK;
 }
 

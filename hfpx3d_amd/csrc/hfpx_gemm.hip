@@ -1,0 +1,316 @@
+// hfpx_gemm.hip -- C -= A B on the MFMA units of gfx950, FP64 and FP32:
+// the Schur-complement updates of the LU and the blocked triangular solves,
+// and the FP64 residuals of the mixed-precision solve.
+//
+// No direct reference counterpart: inside LAPACKE_dgetrf/dgetrs
+// (il/linear_algebra/dense/factorization/LU.h:79-138) this is dgemm.
+#include <algorithm>
+
+#include "hfpx_internal.h"
+
+namespace hfpx {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct MfmaVec;
+template <>
+struct MfmaVec<float> {
+  typedef floatx4 type;
+  static __device__ __forceinline__ floatx4 mfma(float a, float b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  // result element v of lane l is D[4 (l / 16) + v][l % 16]
+  static __device__ __forceinline__ int drow(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+template <>
+struct MfmaVec<double> {
+  typedef doublex4 type;
+  static __device__ __forceinline__ doublex4 mfma(double a, double b, doublex4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // result element v of lane l is D[4 v + l / 16][l % 16] (measured on gfx950)
+  static __device__ __forceinline__ int drow(int lane, int v) { return 4 * v + (lane >> 4); }
+};
+
+// ---------------------------------------------------------------------
+// C (M x N) -= A (M x K) * B (K x N), all column-major.
+//
+// Workgroup tile 128 x 128 (4 waves of 64 x 64), K in chunks of KC staged
+// through LDS.  The MFMA computes the transposed product (U^T L^T) so that the
+// 16 lanes of a group map to 16 consecutive rows of one column of C: the
+// epilogue then writes contiguous column segments.
+// MFMA 16x16x4 operand layout: lane l supplies A[l%16][l/16] and B[l/16][l%16];
+// the row of result element v differs between the f32 and f64 forms (drow).
+// ---------------------------------------------------------------------
+constexpr int GT = 128;  // tile rows (M)
+constexpr int KC = 16;   // K chunk
+
+// GTN = tile columns (N): 128, or 64 for skinny products (many right-hand
+// sides), where a 128-wide tile would leave half of every MFMA idle.
+template <typename T, int GTN>
+__global__ __launch_bounds__(256) void k_gemm_sub(T* __restrict__ C, int64_t ldc,
+                                                  const T* __restrict__ A, int64_t lda,
+                                                  const T* __restrict__ B, int64_t ldb, int64_t M,
+                                                  int64_t N, int64_t K) {
+  typedef typename MfmaVec<T>::type vec4;
+  constexpr int WN = GTN / 2;      // wave tile columns
+  constexpr int TC = WN / 16;      // 16x16 MFMA tiles across a wave's columns
+  constexpr int BQ = KC * GTN / 256;  // B elements loaded per thread per chunk
+  __shared__ T sA[2][KC][GT];      // sA[k][i] = A(row0 + i, k0 + k)
+  __shared__ T sB[2][KC][GTN];     // sB[k][j] = B(k0 + k, col0 + j)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t row0 = (int64_t)blockIdx.x * GT, col0 = (int64_t)blockIdx.y * GTN;
+  const int wr = (wave & 1) * 64, wc = (wave >> 1) * WN;  // wave's 64 x WN sub-tile
+
+  vec4 acc[TC][4];
+#pragma unroll
+  for (int a = 0; a < TC; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = vec4{0, 0, 0, 0};
+
+  T ra[8], rb[BQ];
+  auto load_global = [&](int64_t k0) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = tid + 256 * q;  // 0 .. KC * GT - 1
+      const int ai = e & (GT - 1), ak = e >> 7;
+      const int64_t gr = row0 + ai, gk = k0 + ak;
+      ra[q] = (gr < M && gk < K) ? A[gr + gk * lda] : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < BQ; ++q) {
+      const int e = tid + 256 * q;  // 0 .. KC * GTN - 1
+      const int bk = e & (KC - 1), bj = e >> 4;
+      const int64_t gc = col0 + bj, gk2 = k0 + bk;
+      rb[q] = (gc < N && gk2 < K) ? B[gk2 + gc * ldb] : T(0);
+    }
+  };
+  auto store_lds = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = tid + 256 * q;
+      sA[buf][e >> 7][e & (GT - 1)] = ra[q];
+    }
+#pragma unroll
+    for (int q = 0; q < BQ; ++q) {
+      const int e = tid + 256 * q;
+      sB[buf][e & (KC - 1)][e >> 4] = rb[q];
+    }
+  };
+
+  const int nchunks = (int)((K + KC - 1) / KC);
+  load_global(0);
+  store_lds(0);
+  __syncthreads();
+  for (int c = 0; c < nchunks; ++c) {
+    const int buf = c & 1;
+    if (c + 1 < nchunks) load_global((int64_t)(c + 1) * KC);
+#pragma unroll
+    for (int k4 = 0; k4 < KC; k4 += 4) {
+      const int kk = k4 + (lane >> 4);
+      T fu[TC], fl[4];
+#pragma unroll
+      for (int t = 0; t < TC; ++t) fu[t] = sB[buf][kk][wc + 16 * t + (lane & 15)];  // U^T rows = C columns
+#pragma unroll
+      for (int t = 0; t < 4; ++t) fl[t] = sA[buf][kk][wr + 16 * t + (lane & 15)];   // L^T cols = C rows
+#pragma unroll
+      for (int tc = 0; tc < TC; ++tc)
+#pragma unroll
+        for (int tr = 0; tr < 4; ++tr) acc[tc][tr] = MfmaVec<T>::mfma(fu[tc], fl[tr], acc[tc][tr]);
+    }
+    if (c + 1 < nchunks) store_lds(buf ^ 1);
+    __syncthreads();
+  }
+  // acc[tc][tr][v] = (A B)(row = wr + 16 tr + lane%16, col = wc + 16 tc + drow(lane, v))
+#pragma unroll
+  for (int tc = 0; tc < TC; ++tc)
+#pragma unroll
+    for (int tr = 0; tr < 4; ++tr)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t r = row0 + wr + 16 * tr + (lane & 15);
+        const int64_t cc = col0 + wc + 16 * tc + MfmaVec<T>::drow(lane, v);
+        if (r < M && cc < N) C[r + cc * ldc] -= acc[tc][tr][v];
+      }
+}
+
+// ---------------------------------------------------------------------
+// FP32 C (M x N) -= A (M x K) * B (K x N) on v_mfma_f32_32x32x2f32, for the
+// large trailing updates of the LU.
+//
+// Workgroup = 4 waves in a 2 x 2 arrangement, wave tile (32 WR) x (32 WC),
+// workgroup tile GM = 64 WR rows by GN = 64 WC columns; K in chunks of KC
+// through double-buffered LDS (one barrier per chunk).  As above, the MFMA
+// forms the transposed product: operand a (32 x 2) is a block of B^T, operand
+// b (2 x 32) a block of A^T, so that lane l % 32 indexes 32 consecutive rows
+// of C and every result element of a lane group lands in one 128-byte column
+// segment.  32x32x2 operand layout: lane l supplies a[l%32][l/32] and
+// b[l/32][l%32]; result element v of lane l is D[8(v/4) + 4(l/32) + v%4][l%32].
+// LDS rows are padded (A by 32 floats, B by 33) so that the transposing B
+// stores and the two half-waves' operand reads fall in distinct banks.
+// ---------------------------------------------------------------------
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+template <int WR, int WC, int KC>
+__global__ __launch_bounds__(256) void k_sgemm32(float* __restrict__ C, int64_t ldc,
+                                                 const float* __restrict__ A, int64_t lda,
+                                                 const float* __restrict__ B, int64_t ldb, int64_t M,
+                                                 int64_t N, int64_t K) {
+  constexpr int GM = 64 * WR, GN = 64 * WC;
+  constexpr int PA = GM + 32, PB = GN + 33;
+  constexpr int QA = GM * KC / 256, QB = GN * KC / 256;
+  static_assert(QA >= 1 && QB >= 1, "tile too small for the loaders");
+  __shared__ float sA[2][KC * PA];  // sA[k * PA + i] = A(row0 + i, k0 + k)
+  __shared__ float sB[2][KC * PB];  // sB[k * PB + j] = B(k0 + k, col0 + j)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t row0 = (int64_t)blockIdx.x * GM, col0 = (int64_t)blockIdx.y * GN;
+  const int wr = (wave & 1) * 32 * WR, wc = (wave >> 1) * 32 * WC;
+
+  floatx16 acc[WC][WR];
+#pragma unroll
+  for (int a = 0; a < WC; ++a)
+#pragma unroll
+    for (int b = 0; b < WR; ++b)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
+
+  float ra[QA], rb[QB];
+  auto load_global = [&](int64_t k0) {
+#pragma unroll
+    for (int q = 0; q < QA; ++q) {
+      const int e = tid + 256 * q;  // consecutive lanes: consecutive rows of one column
+      const int i = e % GM, k = e / GM;
+      const int64_t gr = row0 + i, gk = k0 + k;
+      ra[q] = (gr < M && gk < K) ? A[gr + gk * lda] : 0.f;
+    }
+#pragma unroll
+    for (int q = 0; q < QB; ++q) {
+      const int e = tid + 256 * q;  // consecutive lanes: consecutive k of one column
+      const int k = e % KC, j = e / KC;
+      const int64_t gc = col0 + j, gk = k0 + k;
+      rb[q] = (gc < N && gk < K) ? B[gk + gc * ldb] : 0.f;
+    }
+  };
+  auto store_lds = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < QA; ++q) {
+      const int e = tid + 256 * q;
+      sA[buf][(e / GM) * PA + (e % GM)] = ra[q];
+    }
+#pragma unroll
+    for (int q = 0; q < QB; ++q) {
+      const int e = tid + 256 * q;
+      sB[buf][(e % KC) * PB + (e / KC)] = rb[q];
+    }
+  };
+
+  const int nchunks = (int)((K + KC - 1) / KC);
+  load_global(0);
+  store_lds(0);
+  __syncthreads();
+  const int l32 = lane & 31, hk = lane >> 5;
+  for (int c = 0; c < nchunks; ++c) {
+    const int buf = c & 1;
+    if (c + 1 < nchunks) load_global((int64_t)(c + 1) * KC);
+    const float* pa = sA[buf] + wr + l32;
+    const float* pb = sB[buf] + wc + l32;
+#pragma unroll
+    for (int s = 0; s < KC / 2; ++s) {
+      const int k = 2 * s + hk;
+      float fa[WC], fb[WR];
+#pragma unroll
+      for (int t = 0; t < WC; ++t) fa[t] = pb[k * PB + 32 * t];  // C columns
+#pragma unroll
+      for (int t = 0; t < WR; ++t) fb[t] = pa[k * PA + 32 * t];  // C rows
+#pragma unroll
+      for (int tc = 0; tc < WC; ++tc)
+#pragma unroll
+        for (int tr = 0; tr < WR; ++tr)
+          acc[tc][tr] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[tc], fb[tr], acc[tc][tr], 0, 0, 0);
+    }
+    if (c + 1 < nchunks) store_lds(buf ^ 1);
+    __syncthreads();
+  }
+  // acc[tc][tr][v] = (A B)(row0 + wr + 32 tr + l32, col0 + wc + 32 tc + 8 (v/4) + 4 hk + v%4)
+#pragma unroll
+  for (int tc = 0; tc < WC; ++tc)
+#pragma unroll
+    for (int tr = 0; tr < WR; ++tr) {
+      const int64_t r = row0 + wr + 32 * tr + l32;
+      if (r >= M) continue;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int64_t cc = col0 + wc + 32 * tc + 8 * (v >> 2) + 4 * hk + (v & 3);
+        if (cc < N) C[r + cc * ldc] -= acc[tc][tr][v];
+      }
+    }
+}
+
+// ---------------------------------------------------------------------
+// host dispatch
+// ---------------------------------------------------------------------
+// FP32 trailing-update kernels selectable through HFPX_KNOB_GEMM_F32
+// (0 = 16x16x4 128 x 128 tile; 1.. = 32x32x2 family).
+template <int WR, int WC, int KC>
+static void launch_sgemm32(hfpx_ctx* ctx, float* C, int64_t ldc, const float* A, int64_t lda,
+                           const float* B, int64_t ldb, int64_t M, int64_t N, int64_t K) {
+  dim3 grid((unsigned)((M + 64 * WR - 1) / (64 * WR)), (unsigned)((N + 64 * WC - 1) / (64 * WC)));
+  hipLaunchKernelGGL((k_sgemm32<WR, WC, KC>), grid, dim3(256), 0, ctx->stream, C, ldc, A, lda, B, ldb, M,
+                     N, K);
+}
+
+int gemm_f32_variants() { return 7; }
+
+template <typename T>
+int gemm_sub(hfpx_ctx* ctx, T* C, int64_t ldc, const T* A, int64_t lda, const T* B,
+                    int64_t ldb, int64_t M, int64_t N, int64_t K) {
+  if (M <= 0 || N <= 0 || K <= 0) return HFPX_OK;
+  const bool skinny = N <= 64;
+  const int gtn = skinny ? 64 : 128;
+  dim3 grid((unsigned)((M + GT - 1) / GT), (unsigned)((N + gtn - 1) / gtn));
+  if ((N + 63) / 64 > 65535) return set_error(ctx, HFPX_ERR_INVALID, "gemm: N too large");
+  const int tok = prof_begin(ctx);
+  const int var = ctx->knob[HFPX_KNOB_GEMM_F32];
+  bool done = false;
+  if constexpr (sizeof(T) == 4) {
+    if (!skinny && var > 0) {
+      switch (var) {
+        case 1: launch_sgemm32<2, 2, 16>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
+        case 2: launch_sgemm32<2, 2, 32>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
+        case 3: launch_sgemm32<4, 2, 16>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
+        case 4: launch_sgemm32<4, 2, 32>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
+        case 5: launch_sgemm32<2, 4, 16>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
+        default: launch_sgemm32<4, 4, 16>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
+      }
+      done = true;
+    }
+  }
+  if (!done) {
+    if (skinny)
+      hipLaunchKernelGGL((k_gemm_sub<T, 64>), grid, dim3(256), 0, ctx->stream, C, ldc, A, lda, B, ldb, M, N, K);
+    else
+      hipLaunchKernelGGL((k_gemm_sub<T, 128>), grid, dim3(256), 0, ctx->stream, C, ldc, A, lda, B, ldb, M, N, K);
+  }
+  // algorithmic: 2MNK flops; bytes: C read + written, A and B read once
+  prof_end(ctx, tok, sizeof(T) == 8 ? HFPX_KC_GEMM_F64 : HFPX_KC_GEMM_F32, 2.0 * M * N * K,
+           (double)sizeof(T) * (2.0 * M * N + (double)M * K + (double)K * N));
+  HFPX_TRY_HIP(ctx, hipGetLastError());
+  return HFPX_OK;
+}
+
+
+template int gemm_sub<double>(hfpx_ctx*, double*, int64_t, const double*, int64_t, const double*, int64_t,
+                              int64_t, int64_t, int64_t);
+template int gemm_sub<float>(hfpx_ctx*, float*, int64_t, const float*, int64_t, const float*, int64_t,
+                             int64_t, int64_t, int64_t);
+
+int gemm_sub_any(hfpx_ctx* ctx, int dtype, void* C, int64_t ldc, const void* A, int64_t lda,
+                 const void* B, int64_t ldb, int64_t M, int64_t N, int64_t K) {
+  if (dtype == HFPX_F64)
+    return gemm_sub<double>(ctx, (double*)C, ldc, (const double*)A, lda, (const double*)B, ldb, M, N, K);
+  return gemm_sub<float>(ctx, (float*)C, ldc, (const float*)A, lda, (const float*)B, ldb, M, N, K);
+}
+
+}  // namespace hfpx

@@ -21,7 +21,12 @@ struct hfpx_ctx {
   size_t work_bytes = 0;
   void* hwork = nullptr;     // pinned host workspace
   size_t hwork_bytes = 0;
+  void* lu_work = nullptr;   // grow-only device workspace of getrf / getrs
+  size_t lu_work_bytes = 0;
+  uint64_t* sync = nullptr;  // grid-barrier arrival counter (device)
+  uint64_t sync_base = 0;    // its value once every launched barrier completes
   int num_cus = 0;
+  int knob[HFPX_KNOB_COUNT] = {0};  // kernel-variant choices (hfpx_tune_set)
   std::string last_error;
   // opt-in per-launch timing (HIP events on the context's stream)
   bool prof = false;
@@ -72,6 +77,12 @@ int launch_rhs(hfpx_ctx* ctx, const hfpx_mesh* m, const double* sinf_dev, int nr
 int lu_getrf(hfpx_ctx* ctx, int dtype, void* A, int64_t lda, int64_t n, int32_t* ipiv, int64_t* info);
 int lu_getrs(hfpx_ctx* ctx, int dtype, const void* LU, int64_t lda, const int32_t* ipiv, int64_t n,
              void* B, int64_t ldb, int64_t nrhs);
+int gemm_f32_variants();
+// C (M x N) -= A (M x K) B (K x N), column-major, on the context stream
+// (hfpx_gemm.hip; instantiated for float and double).
+template <typename T>
+int gemm_sub(hfpx_ctx* ctx, T* C, int64_t ldc, const T* A, int64_t lda, const T* B, int64_t ldb,
+             int64_t M, int64_t N, int64_t K);
 int gemm_sub_any(hfpx_ctx* ctx, int dtype, void* C, int64_t ldc, const void* A, int64_t lda,
                  const void* B, int64_t ldb, int64_t M, int64_t N, int64_t K);
 

@@ -15,372 +15,335 @@
 // outer panel, and then the trailing matrix, get the row interchanges, a
 // unit-lower triangular solve for the U rows, and the Schur-complement update
 // C -= L21 U12 on MFMA (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32).
-#include <hip/hip_cooperative_groups.h>
-
 #include <algorithm>
 
 #include "hfpx_internal.h"
 
-namespace cg = cooperative_groups;
-
 namespace hfpx {
 
-typedef float floatx4 __attribute__((ext_vector_type(4)));
-typedef double doublex4 __attribute__((ext_vector_type(4)));
-
-template <typename T>
-struct MfmaVec;
-template <>
-struct MfmaVec<float> {
-  typedef floatx4 type;
-  static __device__ __forceinline__ floatx4 mfma(float a, float b, floatx4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  // result element v of lane l is D[4 (l / 16) + v][l % 16]
-  static __device__ __forceinline__ int drow(int lane, int v) { return 4 * (lane >> 4) + v; }
-};
-template <>
-struct MfmaVec<double> {
-  typedef doublex4 type;
-  static __device__ __forceinline__ doublex4 mfma(double a, double b, doublex4 c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  // result element v of lane l is D[4 v + l / 16][l % 16] (measured on gfx950)
-  static __device__ __forceinline__ int drow(int lane, int v) { return 4 * v + (lane >> 4); }
-};
+__device__ __forceinline__ void fence_agent_release() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+}
+__device__ __forceinline__ void fence_agent_acquire() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
 
 // ---------------------------------------------------------------------
-// C (M x N) -= A (M x K) * B (K x N), all column-major.
-//
-// Workgroup tile 128 x 128 (4 waves of 64 x 64), K in chunks of KC staged
-// through LDS.  The MFMA computes the transposed product (U^T L^T) so that the
-// 16 lanes of a group map to 16 consecutive rows of one column of C: the
-// epilogue then writes contiguous column segments.
-// MFMA 16x16x4 operand layout: lane l supplies A[l%16][l/16] and B[l/16][l%16];
-// the row of result element v differs between the f32 and f64 forms (drow).
+// Grid barrier for co-resident (cooperatively launched) workgroups.  The
+// arrival counter only grows: the host passes its value before the launch
+// (base), so barrier b of a launch with nwg workgroups completes when the
+// counter reaches base + (b + 1) nwg.
+// Memory ordering is paid once per workgroup, not per wave: the workgroup
+// barrier makes every wave's stores happen-before thread 0, whose agent-scope
+// release fence publishes them (one L2 write-back per workgroup); thread 0
+// then arrives and polls with relaxed atomics and ends with an agent-scope
+// acquire fence, which the closing workgroup barrier extends to the other
+// waves.  The wait is bounded: after ~2^24 polls thread 0 raises *err and
+// leaves (co-residency, guaranteed by the launch, makes this unreachable).
 // ---------------------------------------------------------------------
-constexpr int GT = 128;  // tile rows (M)
-constexpr int KC = 16;   // K chunk
-
-// GTN = tile columns (N): 128, or 64 for skinny products (many right-hand
-// sides), where a 128-wide tile would leave half of every MFMA idle.
-template <typename T, int GTN>
-__global__ __launch_bounds__(256) void k_gemm_sub(T* __restrict__ C, int64_t ldc,
-                                                  const T* __restrict__ A, int64_t lda,
-                                                  const T* __restrict__ B, int64_t ldb, int64_t M,
-                                                  int64_t N, int64_t K) {
-  typedef typename MfmaVec<T>::type vec4;
-  constexpr int WN = GTN / 2;      // wave tile columns
-  constexpr int TC = WN / 16;      // 16x16 MFMA tiles across a wave's columns
-  constexpr int BQ = KC * GTN / 256;  // B elements loaded per thread per chunk
-  __shared__ T sA[2][KC][GT];      // sA[k][i] = A(row0 + i, k0 + k)
-  __shared__ T sB[2][KC][GTN];     // sB[k][j] = B(k0 + k, col0 + j)
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int64_t row0 = (int64_t)blockIdx.x * GT, col0 = (int64_t)blockIdx.y * GTN;
-  const int wr = (wave & 1) * 64, wc = (wave >> 1) * WN;  // wave's 64 x WN sub-tile
-
-  vec4 acc[TC][4];
-#pragma unroll
-  for (int a = 0; a < TC; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = vec4{0, 0, 0, 0};
-
-  T ra[8], rb[BQ];
-  auto load_global = [&](int64_t k0) {
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int e = tid + 256 * q;  // 0 .. KC * GT - 1
-      const int ai = e & (GT - 1), ak = e >> 7;
-      const int64_t gr = row0 + ai, gk = k0 + ak;
-      ra[q] = (gr < M && gk < K) ? A[gr + gk * lda] : T(0);
-    }
-#pragma unroll
-    for (int q = 0; q < BQ; ++q) {
-      const int e = tid + 256 * q;  // 0 .. KC * GTN - 1
-      const int bk = e & (KC - 1), bj = e >> 4;
-      const int64_t gc = col0 + bj, gk2 = k0 + bk;
-      rb[q] = (gc < N && gk2 < K) ? B[gk2 + gc * ldb] : T(0);
-    }
-  };
-  auto store_lds = [&](int buf) {
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int e = tid + 256 * q;
-      sA[buf][e >> 7][e & (GT - 1)] = ra[q];
-    }
-#pragma unroll
-    for (int q = 0; q < BQ; ++q) {
-      const int e = tid + 256 * q;
-      sB[buf][e & (KC - 1)][e >> 4] = rb[q];
-    }
-  };
-
-  const int nchunks = (int)((K + KC - 1) / KC);
-  load_global(0);
-  store_lds(0);
+__device__ __forceinline__ void grid_barrier(uint64_t* ctr, uint64_t target, int32_t* err) {
   __syncthreads();
-  for (int c = 0; c < nchunks; ++c) {
-    const int buf = c & 1;
-    if (c + 1 < nchunks) load_global((int64_t)(c + 1) * KC);
-#pragma unroll
-    for (int k4 = 0; k4 < KC; k4 += 4) {
-      const int kk = k4 + (lane >> 4);
-      T fu[TC], fl[4];
-#pragma unroll
-      for (int t = 0; t < TC; ++t) fu[t] = sB[buf][kk][wc + 16 * t + (lane & 15)];  // U^T rows = C columns
-#pragma unroll
-      for (int t = 0; t < 4; ++t) fl[t] = sA[buf][kk][wr + 16 * t + (lane & 15)];   // L^T cols = C rows
-#pragma unroll
-      for (int tc = 0; tc < TC; ++tc)
-#pragma unroll
-        for (int tr = 0; tr < 4; ++tr) acc[tc][tr] = MfmaVec<T>::mfma(fu[tc], fl[tr], acc[tc][tr]);
-    }
-    if (c + 1 < nchunks) store_lds(buf ^ 1);
-    __syncthreads();
-  }
-  // acc[tc][tr][v] = (A B)(row = wr + 16 tr + lane%16, col = wc + 16 tc + drow(lane, v))
-#pragma unroll
-  for (int tc = 0; tc < TC; ++tc)
-#pragma unroll
-    for (int tr = 0; tr < 4; ++tr)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int64_t r = row0 + wr + 16 * tr + (lane & 15);
-        const int64_t cc = col0 + wc + 16 * tc + MfmaVec<T>::drow(lane, v);
-        if (r < M && cc < N) C[r + cc * ldc] -= acc[tc][tr][v];
+  if (threadIdx.x == 0) {
+    fence_agent_release();
+    __hip_atomic_fetch_add(ctr, (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t polls = 0;
+    while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++polls > (1u << 24)) {
+        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
       }
+    }
+    fence_agent_acquire();
+  }
+  __syncthreads();
 }
 
 // ---------------------------------------------------------------------
 // Cooperative panel factorisation of A[j0:n, j0:j0+jb] (jb <= IB).
-// Workgroup w owns rows [j0 + w*chunk, j0 + (w+1)*chunk); thread t holds
-// rows base + t + WG*r, r < RPT, of all jb columns in registers.
+// Thread t of workgroup w holds row w*PWG + t (relative to j0) of all jb
+// columns in registers.  Per column: workgroup arg-max -> candidate (value,
+// row, and the row itself) published -> grid barrier -> every wave reduces
+// the candidates redundantly -> interchange + rank-1 update in registers.
+// Pivot choice is LAPACK's: the largest |a|, the first such row on ties.
 // ---------------------------------------------------------------------
-constexpr int PWG = 512;  // threads per panel workgroup
+constexpr int PWG = 1024;          // threads (rows) per panel workgroup
+constexpr int PANEL_MAX_WG = 1024;  // candidate slots
 
 template <typename T>
 struct PanelScratch {
-  T val[2][1024];          // per-workgroup candidate |pivot| (double-buffered)
-  int64_t idx[2][1024];    // per-workgroup candidate row (relative)
+  T val[2][PANEL_MAX_WG];        // per-workgroup candidate |pivot| (by column parity)
+  int32_t idx[2][PANEL_MAX_WG];  // per-workgroup candidate row (relative to j0)
 };
 
 template <typename T>
 struct PanelCtx {
-  int nwg, w, t;
-  int jb;
+  int nwg, w, t, jb;
   int64_t j0;
+  int32_t row;  // row (relative to j0) held by this thread, -1 if none
   PanelScratch<T>* ps;
   T* rows;
   int32_t* ipiv;
   int32_t* info;
+  int32_t* err;
+  uint64_t* ctr;
+  uint64_t base;
   T* red_v;
-  int64_t* red_i;
-  int* red_w;
+  int32_t* red_i;
   T* s_piv;
 };
 
-// One column step of the panel (column J known at compile time, so the
-// register-resident rows are indexed statically).
-template <typename T, int IB, int RPT, int J>
-__device__ __forceinline__ void panel_step(T (&a)[RPT][IB], const int64_t (&ri)[RPT],
-                                           const PanelCtx<T>& P, cg::grid_group& grid) {
+// (bv, bi) := better of itself and (ov, oi): larger value, then smaller row;
+// a row of -1 marks "no candidate".
+template <typename T>
+__device__ __forceinline__ bool pick(T& bv, int32_t& bi, T ov, int32_t oi) {
+  if (oi >= 0 && (bi < 0 || ov > bv || (ov == bv && oi < bi))) {
+    bv = ov;
+    bi = oi;
+    return true;
+  }
+  return false;
+}
+
+template <typename T>
+__device__ __forceinline__ void wave_argmax(T& bv, int32_t& bi, int32_t& bw) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const T ov = __shfl_xor(bv, off);
+    const int32_t oi = __shfl_xor(bi, off);
+    const int32_t ow = __shfl_xor(bw, off);
+    if (pick(bv, bi, ov, oi)) bw = ow;
+  }
+}
+
+template <typename T, int IB, int J>
+__device__ __forceinline__ void panel_step(T (&a)[IB], const PanelCtx<T>& P) {
   if constexpr (J < IB) {
     if (J < P.jb) {
       const int par = J & 1;
-      const int t = P.t;
-      // 1. local pivot candidate among rows >= J
-      T best = T(-1);
-      int64_t bidx = -1;
-#pragma unroll
-      for (int r = 0; r < RPT; ++r) {
-        if (ri[r] >= J) {
-          const T v = fabs(a[r][J]);
-          if (v > best || (v == best && ri[r] < bidx)) {
-            best = v;
-            bidx = ri[r];
-          }
-        }
+      const int t = P.t, lane = t & 63;
+      // 1. candidate of this workgroup
+      T bv = T(-1);
+      int32_t bi = -1, bw = 0;
+      if (P.row >= J) {
+        bv = fabs(a[J]);
+        bi = P.row;
       }
-      for (int off = 32; off > 0; off >>= 1) {
-        const T ov = __shfl_xor(best, off);
-        const int64_t oi = __shfl_xor(bidx, off);
-        if (ov > best || (ov == best && oi >= 0 && (bidx < 0 || oi < bidx))) {
-          best = ov;
-          bidx = oi;
-        }
-      }
-      if ((t & 63) == 0) {
-        P.red_v[t >> 6] = best;
-        P.red_i[t >> 6] = bidx;
+      wave_argmax(bv, bi, bw);
+      if (lane == 0) {
+        P.red_v[t >> 6] = bv;
+        P.red_i[t >> 6] = bi;
       }
       __syncthreads();
-      if (t == 0) {
-        for (int q = 1; q < PWG / 64; ++q) {
-          const T ov = P.red_v[q];
-          const int64_t oi = P.red_i[q];
-          if (ov > best || (ov == best && oi >= 0 && (bidx < 0 || oi < bidx))) {
-            best = ov;
-            bidx = oi;
-          }
-        }
-        P.ps->val[par][P.w] = best;
-        P.ps->idx[par][P.w] = bidx;
-        P.red_i[0] = bidx;
+      bv = T(-1);
+      bi = -1;
+      if (lane < PWG / 64) {
+        bv = P.red_v[lane];
+        bi = P.red_i[lane];
       }
-      // publish the candidate row and (owner of row J) row J itself
+      wave_argmax(bv, bi, bw);  // now uniform over the workgroup
       T* cand = P.rows + ((size_t)par * (P.nwg + 1) + P.w) * IB;
       T* rowj = P.rows + ((size_t)par * (P.nwg + 1) + P.nwg) * IB;
-      __syncthreads();
-      const int64_t wg_best = P.red_i[0];
-#pragma unroll
-      for (int r = 0; r < RPT; ++r) {
-        if (ri[r] >= 0 && ri[r] == wg_best) {
-#pragma unroll
-          for (int c = 0; c < IB; ++c) cand[c] = a[r][c];
-        }
-        if (ri[r] == J) {
-#pragma unroll
-          for (int c = 0; c < IB; ++c) rowj[c] = a[r][c];
-        }
+      if (t == 0) {
+        P.ps->val[par][P.w] = bv;
+        P.ps->idx[par][P.w] = bi;
       }
-      __threadfence();
-      grid.sync();
-      // 2. global pivot: every thread reads candidates q = t, t + PWG, ...,
-      //    then a workgroup arg-max (every thread read red_i[0] before the
-      //    grid barrier, so the LDS slots can be reused)
-      {
-        T gb = T(-1);
-        int64_t gi = -1;
-        int gw = -1;
-        for (int q = t; q < P.nwg; q += PWG) {
-          const T ov = P.ps->val[par][q];
-          const int64_t oi = P.ps->idx[par][q];
-          if (oi >= 0 && (ov > gb || (ov == gb && oi < gi))) {
-            gb = ov;
-            gi = oi;
-            gw = q;
-          }
-        }
-        for (int off = 32; off > 0; off >>= 1) {
-          const T ov = __shfl_xor(gb, off);
-          const int64_t oi = __shfl_xor(gi, off);
-          const int ow = __shfl_xor(gw, off);
-          if (oi >= 0 && (ov > gb || (ov == gb && (gi < 0 || oi < gi)))) {
-            gb = ov;
-            gi = oi;
-            gw = ow;
-          }
-        }
-        if ((t & 63) == 0) {
-          P.red_v[t >> 6] = gb;
-          P.red_i[t >> 6] = gi;
-          P.red_w[t >> 6] = gw;
-        }
-        __syncthreads();
-        if (t == 0) {
-          for (int q = 1; q < PWG / 64; ++q) {
-            const T ov = P.red_v[q];
-            const int64_t oi = P.red_i[q];
-            if (oi >= 0 && (ov > gb || (ov == gb && (gi < 0 || oi < gi)))) {
-              gb = ov;
-              gi = oi;
-              gw = P.red_w[q];
-            }
-          }
-          P.red_i[0] = gi;
-          P.red_i[1] = gw;
-        }
-        __syncthreads();
+      if (P.row >= 0 && P.row == bi) {
+#pragma unroll
+        for (int c = 0; c < IB; ++c) cand[c] = a[c];
       }
-      const int64_t p = P.red_i[0];
-      const int pw = (int)P.red_i[1];
-      const T* prow = P.rows + ((size_t)par * (P.nwg + 1) + pw) * IB;
+      if (P.row == J) {
+#pragma unroll
+        for (int c = 0; c < IB; ++c) rowj[c] = a[c];
+      }
+      grid_barrier(P.ctr, P.base + (uint64_t)(J + 1) * P.nwg, P.err);
+      // 2. global pivot, reduced redundantly by every wave
+      T gv = T(-1);
+      int32_t gi = -1, gw = 0;
+      for (int q = lane; q < P.nwg; q += 64)
+        if (pick(gv, gi, P.ps->val[par][q], P.ps->idx[par][q])) gw = q;
+      wave_argmax(gv, gi, gw);
+      const T* prow = P.rows + ((size_t)par * (P.nwg + 1) + gw) * IB;
       if (t < IB) P.s_piv[t] = prow[t];
       __syncthreads();
       const T pv = P.s_piv[J];
       const bool zero = (pv == T(0));
       if (P.w == 0 && t == 0) {
-        P.ipiv[P.j0 + J] = (int32_t)(P.j0 + (zero ? J : p) + 1);
+        P.ipiv[P.j0 + J] = (int32_t)(P.j0 + (zero ? J : gi) + 1);
         if (zero && *P.info == 0) *P.info = (int32_t)(P.j0 + J + 1);
       }
       const T inv = zero ? T(0) : T(1) / pv;
-      // 3. interchange + rank-1 update of the rows this thread holds
+      // 3. interchange + rank-1 update of this thread's row
+      if (P.row >= J) {
+        if (!zero && P.row == J) {
 #pragma unroll
-      for (int r = 0; r < RPT; ++r) {
-        if (ri[r] < J) continue;
-        if (!zero && ri[r] == J) {
+          for (int c = 0; c < IB; ++c) a[c] = P.s_piv[c];
+        } else {
+          if (!zero && P.row == gi) {
 #pragma unroll
-          for (int c = 0; c < IB; ++c) a[r][c] = P.s_piv[c];
-          continue;
+            for (int c = 0; c < IB; ++c) a[c] = rowj[c];
+          }
+          const T l = a[J] * inv;
+          a[J] = l;
+#pragma unroll
+          for (int c = J + 1; c < IB; ++c) a[c] -= l * P.s_piv[c];
         }
-        if (!zero && ri[r] == p) {
-#pragma unroll
-          for (int c = 0; c < IB; ++c) a[r][c] = rowj[c];
-        }
-        const T l = a[r][J] * inv;
-        a[r][J] = l;
-#pragma unroll
-        for (int c = J + 1; c < IB; ++c) a[r][c] -= l * P.s_piv[c];
       }
     }
-    panel_step<T, IB, RPT, J + 1>(a, ri, P, grid);
+    panel_step<T, IB, J + 1>(a, P);
   }
 }
 
-template <typename T, int IB, int RPT>
-__global__ __launch_bounds__(PWG) void k_panel(T* __restrict__ A, int64_t lda, int64_t n,
-                                               int64_t j0, int jb, int32_t* __restrict__ ipiv,
-                                               int32_t* __restrict__ info,
-                                               PanelScratch<T>* __restrict__ ps,
-                                               T* __restrict__ rows /* [2][nwg+1][IB] */) {
-  cg::grid_group grid = cg::this_grid();
+template <typename T, int IB>
+__global__ __launch_bounds__(PWG) void k_panel(T* __restrict__ A, int64_t lda, int64_t n, int64_t j0, int jb,
+                                               int32_t* __restrict__ ipiv, int32_t* __restrict__ info,
+                                               int32_t* __restrict__ err, PanelScratch<T>* __restrict__ ps,
+                                               T* __restrict__ rows /* [2][nwg+1][IB] */, uint64_t* ctr,
+                                               uint64_t base) {
   const int nwg = gridDim.x, w = blockIdx.x, t = threadIdx.x;
   const int64_t m = n - j0;
-  const int64_t chunk = (m + nwg - 1) / nwg;
-  const int64_t base = (int64_t)w * chunk;
+  const int64_t r = (int64_t)w * PWG + t;
   __shared__ T red_v[PWG / 64];
-  __shared__ int64_t red_i[PWG / 64];
-  __shared__ int red_w[PWG / 64];
+  __shared__ int32_t red_i[PWG / 64];
   __shared__ T s_piv[IB];
-
-  T a[RPT][IB];
-  int64_t ri[RPT];
+  T a[IB];
+  const bool ok = r < m;
 #pragma unroll
-  for (int r = 0; r < RPT; ++r) {
-    ri[r] = base + t + PWG * r;
-    const bool ok = (t + PWG * r) < chunk && ri[r] < m;
-    if (!ok) ri[r] = -1;
+  for (int c = 0; c < IB; ++c) a[c] = (ok && c < jb) ? A[(j0 + r) + (j0 + c) * lda] : T(0);
+  PanelCtx<T> P{nwg, w, t, jb, j0, ok ? (int32_t)r : -1, ps, rows, ipiv, info, err, ctr, base,
+                red_v, red_i, s_piv};
+  panel_step<T, IB, 0>(a, P);
+  if (ok)
 #pragma unroll
     for (int c = 0; c < IB; ++c)
-      a[r][c] = (ok && c < jb) ? A[(j0 + ri[r]) + (j0 + c) * lda] : T(0);
-  }
-  PanelCtx<T> P{nwg, w, t, jb, j0, ps, rows, ipiv, info, red_v, red_i, red_w, s_piv};
-  panel_step<T, IB, RPT, 0>(a, ri, P, grid);
-#pragma unroll
-  for (int r = 0; r < RPT; ++r)
-    if (ri[r] >= 0)
-#pragma unroll
-      for (int c = 0; c < IB; ++c)
-        if (c < jb) A[(j0 + ri[r]) + (j0 + c) * lda] = a[r][c];
+      if (c < jb) A[(j0 + r) + (j0 + c) * lda] = a[c];
 }
 
 // ---------------------------------------------------------------------
-// Row interchanges ipiv[k0:k1) (1-based, sequential) on columns [c0, c1).
-// One thread per column.
+// Row interchanges.  A block of up to SWAP_BLK sequential interchanges
+// ipiv[s0 .. s0+nb) is one permutation of at most 2 SWAP_BLK rows; it is
+// computed once (k_perm_pairs, one workgroup per block) as pairs
+// new A[dst] = old A[src], then applied to many columns in parallel, one
+// wave per column, all loads before any store (k_apply_pairs), or block
+// after block within one workgroup per column (k_apply_pairs_seq).
 // ---------------------------------------------------------------------
-template <typename T>
-__global__ void k_laswp(T* __restrict__ A, int64_t lda, const int32_t* __restrict__ ipiv,
-                        int64_t k0, int64_t k1, int64_t c0, int64_t c1) {
-  const int64_t c = c0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= c1) return;
-  T* col = A + c * lda;
-  for (int64_t j = k0; j < k1; ++j) {
-    const int64_t p = ipiv[j] - 1;
-    if (p != j) {
-      const T tmp = col[j];
-      col[j] = col[p];
-      col[p] = tmp;
+constexpr int SWAP_BLK = 256;
+constexpr int PAIR_MAX = 2 * SWAP_BLK;
+
+struct PermPairs {
+  int32_t count;
+  int32_t pad[63];
+  int32_t dst[PAIR_MAX];
+  int32_t src[PAIR_MAX];
+};
+
+__global__ __launch_bounds__(256) void k_perm_pairs(const int32_t* __restrict__ ipiv, int64_t k0, int64_t k1,
+                                                    int64_t n, PermPairs* __restrict__ out) {
+  __shared__ int32_t piv[SWAP_BLK], borig[SWAP_BLK];
+  __shared__ int32_t hkey[1024], horig[1024];  // rows outside the block (open addressing)
+  __shared__ int32_t olist[SWAP_BLK];
+  const int t = threadIdx.x;
+  const int64_t s0 = k0 + (int64_t)blockIdx.x * SWAP_BLK;
+  const int nb = (int)std::min<int64_t>(SWAP_BLK, k1 - s0);
+  PermPairs* pp = out + blockIdx.x;
+  if (t < nb) {
+    piv[t] = ipiv[s0 + t] - 1;
+    borig[t] = (int32_t)(s0 + t);
+  }
+  for (int i = t; i < 1024; i += 256) hkey[i] = -1;
+  __syncthreads();
+  if (t != 0) return;
+  // borig[i] / horig[h]: original row of the value now held by that row
+  int nout = 0;
+  for (int i = 0; i < nb; ++i) {
+    const int32_t r1 = (int32_t)(s0 + i), r2 = piv[i];
+    if (r2 == r1 || r2 < 0 || r2 >= n) continue;  // no-op or invalid entry
+    int32_t* slot;
+    if (r2 >= s0 && r2 < s0 + nb) {
+      slot = &borig[r2 - s0];
+    } else {
+      uint32_t h = ((uint32_t)r2 * 2654435761u) >> 22;
+      while (hkey[h] != -1 && hkey[h] != r2) h = (h + 1) & 1023;
+      if (hkey[h] == -1) {
+        hkey[h] = r2;
+        horig[h] = r2;
+        olist[nout++] = (int32_t)h;
+      }
+      slot = &horig[h];
     }
+    const int32_t tmp = borig[i];
+    borig[i] = *slot;
+    *slot = tmp;
+  }
+  int cnt = 0;
+  for (int i = 0; i < nb; ++i)
+    if (borig[i] != (int32_t)(s0 + i)) {
+      pp->dst[cnt] = (int32_t)(s0 + i);
+      pp->src[cnt] = borig[i];
+      ++cnt;
+    }
+  for (int q = 0; q < nout; ++q) {
+    const int h = olist[q];
+    if (horig[h] != hkey[h]) {
+      pp->dst[cnt] = hkey[h];
+      pp->src[cnt] = horig[h];
+      ++cnt;
+    }
+  }
+  pp->count = cnt;
+}
+
+// Columns [c0, c1) and [c2, c3), one wave each.
+template <typename T>
+__global__ __launch_bounds__(256) void k_apply_pairs(T* __restrict__ A, int64_t lda,
+                                                     const PermPairs* __restrict__ pp, int64_t c0, int64_t c1,
+                                                     int64_t c2, int64_t c3) {
+  const int lane = threadIdx.x & 63;
+  const int64_t ci = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t n1 = c1 - c0;
+  const bool active = ci < n1 + (c3 - c2);
+  const int64_t col = ci < n1 ? c0 + ci : c2 + (ci - n1);
+  const int cnt = pp->count;
+  T v[PAIR_MAX / 64];
+  int32_t d[PAIR_MAX / 64];
+#pragma unroll
+  for (int q = 0; q < PAIR_MAX / 64; ++q) {
+    const int i = lane + 64 * q;
+    if (active && i < cnt) {
+      d[q] = pp->dst[i];
+      v[q] = A[pp->src[i] + col * lda];
+    }
+  }
+  __syncthreads();  // every load of this wave has returned before any store
+#pragma unroll
+  for (int q = 0; q < PAIR_MAX / 64; ++q) {
+    const int i = lane + 64 * q;
+    if (active && i < cnt) A[d[q] + col * lda] = v[q];
+  }
+}
+
+// All nblk pair blocks in order on column blockIdx.x of B (one workgroup).
+template <typename T>
+__global__ __launch_bounds__(256) void k_apply_pairs_seq(T* __restrict__ B, int64_t ldb,
+                                                         const PermPairs* __restrict__ pp, int nblk) {
+  T* col = B + (int64_t)blockIdx.x * ldb;
+  const int t = threadIdx.x;
+  for (int b = 0; b < nblk; ++b) {
+    const int cnt = pp[b].count;
+    T v0 = T(0), v1 = T(0);
+    int32_t d0 = -1, d1 = -1;
+    if (t < cnt) {
+      d0 = pp[b].dst[t];
+      v0 = col[pp[b].src[t]];
+    }
+    if (t + 256 < cnt) {
+      d1 = pp[b].dst[t + 256];
+      v1 = col[pp[b].src[t + 256]];
+    }
+    __syncthreads();
+    if (d0 >= 0) col[d0] = v0;
+    if (d1 >= 0) col[d1] = v1;
+    __syncthreads();
   }
 }
 
@@ -437,35 +400,29 @@ __global__ __launch_bounds__(256) void k_trsm_leaf(const T* __restrict__ Lm, int
 // ---------------------------------------------------------------------
 // host drivers
 // ---------------------------------------------------------------------
-template <typename T>
-static int gemm_sub(hfpx_ctx* ctx, T* C, int64_t ldc, const T* A, int64_t lda, const T* B,
-                    int64_t ldb, int64_t M, int64_t N, int64_t K) {
-  if (M <= 0 || N <= 0 || K <= 0) return HFPX_OK;
-  const bool skinny = N <= 64;
-  const int gtn = skinny ? 64 : 128;
-  dim3 grid((unsigned)((M + GT - 1) / GT), (unsigned)((N + gtn - 1) / gtn));
-  if (grid.y > 65535u) return set_error(ctx, HFPX_ERR_INVALID, "gemm: N too large");
+// Pairs for the interchanges ipiv[k0, k1), one PermPairs per SWAP_BLK.
+static int perm_pairs(hfpx_ctx* ctx, const int32_t* ipiv, int64_t k0, int64_t k1, int64_t n, PermPairs* pp) {
+  if (k1 <= k0) return HFPX_OK;
+  const unsigned nblk = (unsigned)((k1 - k0 + SWAP_BLK - 1) / SWAP_BLK);
   const int tok = prof_begin(ctx);
-  if (skinny)
-    hipLaunchKernelGGL((k_gemm_sub<T, 64>), grid, dim3(256), 0, ctx->stream, C, ldc, A, lda, B, ldb, M, N, K);
-  else
-    hipLaunchKernelGGL((k_gemm_sub<T, 128>), grid, dim3(256), 0, ctx->stream, C, ldc, A, lda, B, ldb, M, N, K);
-  // algorithmic: 2MNK flops; bytes: C read + written, A and B read once
-  prof_end(ctx, tok, sizeof(T) == 8 ? HFPX_KC_GEMM_F64 : HFPX_KC_GEMM_F32, 2.0 * M * N * K,
-           (double)sizeof(T) * (2.0 * M * N + (double)M * K + (double)K * N));
+  hipLaunchKernelGGL(k_perm_pairs, dim3(nblk), dim3(256), 0, ctx->stream, ipiv, k0, k1, n, pp);
+  prof_end(ctx, tok, HFPX_KC_LASWP, 0.0, 0.0);
   HFPX_TRY_HIP(ctx, hipGetLastError());
   return HFPX_OK;
 }
 
+// Apply one PermPairs block to columns [c0, c1) and [c2, c3) of A.
 template <typename T>
-static int laswp(hfpx_ctx* ctx, T* A, int64_t lda, const int32_t* ipiv, int64_t k0, int64_t k1,
-                 int64_t c0, int64_t c1) {
-  if (c1 <= c0 || k1 <= k0) return HFPX_OK;
-  const int bs = 256;
+static int apply_pairs(hfpx_ctx* ctx, T* A, int64_t lda, const PermPairs* pp, int64_t c0, int64_t c1, int64_t c2,
+                       int64_t c3) {
+  const int64_t ncol = std::max<int64_t>(0, c1 - c0) + std::max<int64_t>(0, c3 - c2);
+  if (ncol <= 0) return HFPX_OK;
+  if (c1 < c0) c1 = c0;
+  if (c3 < c2) c3 = c2;
   const int tok = prof_begin(ctx);
-  hipLaunchKernelGGL(k_laswp<T>, dim3((unsigned)((c1 - c0 + bs - 1) / bs)), dim3(bs), 0, ctx->stream,
-                     A, lda, ipiv, k0, k1, c0, c1);
-  prof_end(ctx, tok, HFPX_KC_LASWP, 0.0, 0.0);
+  hipLaunchKernelGGL(k_apply_pairs<T>, dim3((unsigned)((ncol + 3) / 4)), dim3(256), 0, ctx->stream, A, lda, pp, c0,
+                     c1, c2, c3);
+  prof_end(ctx, tok, HFPX_KC_LASWP, 0.0, 2.0 * sizeof(T) * ncol * 2 * SWAP_BLK);
   HFPX_TRY_HIP(ctx, hipGetLastError());
   return HFPX_OK;
 }
@@ -511,32 +468,42 @@ static int trsm_upper(hfpx_ctx* ctx, const T* U, int64_t ldu, T* B, int64_t ldb,
   return HFPX_OK;
 }
 
-template <typename T, int IB, int RPT>
-static int panel_launch(hfpx_ctx* ctx, T* A, int64_t lda, int64_t n, int64_t j0, int jb,
-                        int32_t* ipiv, int32_t* info, PanelScratch<T>* ps, T* rows, int nwg) {
-  void* args[] = {&A, &lda, &n, &j0, &jb, &ipiv, &info, &ps, &rows};
-  const int tok = prof_begin(ctx);
-  const hipError_t e = hipLaunchCooperativeKernel((const void*)k_panel<T, IB, RPT>, dim3(nwg), dim3(PWG),
-                                                  args, 0, ctx->stream);
-  prof_end(ctx, tok, HFPX_KC_PANEL, (double)(n - j0) * jb * jb, 0.0);
-  HFPX_TRY_HIP(ctx, e);
+// Grow-only workspace of the LU routines (separate from ctx->work, which
+// the mixed-precision solve holds while it calls getrs).
+static int ensure_lu_work(hfpx_ctx* ctx, size_t bytes) {
+  if (ctx->lu_work_bytes >= bytes) return HFPX_OK;
+  if (ctx->lu_work) {
+    HFPX_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    HFPX_TRY_HIP(ctx, hipFree(ctx->lu_work));
+    ctx->lu_work = nullptr;
+    ctx->lu_work_bytes = 0;
+  }
+  HFPX_TRY_HIP(ctx, hipMalloc(&ctx->lu_work, bytes));
+  ctx->lu_work_bytes = bytes;
   return HFPX_OK;
 }
 
 template <typename T, int IB>
-static int panel(hfpx_ctx* ctx, T* A, int64_t lda, int64_t n, int64_t j0, int jb, int32_t* ipiv,
-                 int32_t* info, PanelScratch<T>* ps, T* rows, int max_wg) {
+static int panel(hfpx_ctx* ctx, T* A, int64_t lda, int64_t n, int64_t j0, int jb, int32_t* ipiv, int32_t* dflags,
+                 PanelScratch<T>* ps, T* rows, int max_wg) {
   const int64_t m = n - j0;
-  // workgroups: enough that each thread holds at most RPT rows, at most max_wg
-  int64_t nwg = (m + PWG - 1) / PWG;
-  if (nwg > max_wg) nwg = max_wg;
-  if (nwg < 1) nwg = 1;
-  const int64_t chunk = (m + nwg - 1) / nwg;
-  const int64_t rpt = (chunk + PWG - 1) / PWG;
-  if (rpt <= 1) return panel_launch<T, IB, 1>(ctx, A, lda, n, j0, jb, ipiv, info, ps, rows, (int)nwg);
-  if (rpt <= 2) return panel_launch<T, IB, 2>(ctx, A, lda, n, j0, jb, ipiv, info, ps, rows, (int)nwg);
-  if (rpt <= 4) return panel_launch<T, IB, 4>(ctx, A, lda, n, j0, jb, ipiv, info, ps, rows, (int)nwg);
-  return set_error(ctx, HFPX_ERR_INVALID, "getrf: n = %lld too large for the panel kernel", (long long)n);
+  const int64_t nwg64 = (m + PWG - 1) / PWG;
+  if (nwg64 > max_wg)
+    return set_error(ctx, HFPX_ERR_INVALID, "getrf: n = %lld too large for the panel kernel (at most %lld rows)",
+                     (long long)n, (long long)max_wg * PWG);
+  int nwg = (int)nwg64;
+  int32_t* info = dflags;
+  int32_t* err = dflags + 1;
+  uint64_t* ctr = ctx->sync;
+  uint64_t base = ctx->sync_base;
+  void* args[] = {&A, &lda, &n, &j0, &jb, &ipiv, &info, &err, &ps, &rows, &ctr, &base};
+  const int tok = prof_begin(ctx);
+  const hipError_t e = hipLaunchCooperativeKernel((const void*)k_panel<T, IB>, dim3(nwg), dim3(PWG), args, 0,
+                                                  ctx->stream);
+  prof_end(ctx, tok, HFPX_KC_PANEL, (double)m * jb * jb, 0.0);
+  HFPX_TRY_HIP(ctx, e);
+  ctx->sync_base += (uint64_t)jb * nwg;
+  return HFPX_OK;
 }
 
 template <typename T>
@@ -555,33 +522,36 @@ struct LuCfg<float> {
 template <typename T>
 static int getrf_impl(hfpx_ctx* ctx, T* A, int64_t lda, int64_t n, int32_t* ipiv, int64_t* info_out) {
   constexpr int IB = LuCfg<T>::IB, NB = LuCfg<T>::NB;
+  static_assert(NB <= SWAP_BLK, "one pair block per outer panel");
   // co-resident panel workgroups
-  int occ1 = 0, occ2 = 0, occ4 = 0;
-  HFPX_TRY_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ1, k_panel<T, IB, 1>, PWG, 0));
-  HFPX_TRY_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ2, k_panel<T, IB, 2>, PWG, 0));
-  HFPX_TRY_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ4, k_panel<T, IB, 4>, PWG, 0));
-  const int occ = std::min(occ1, std::min(occ2, occ4));
+  int occ = 0;
+  HFPX_TRY_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_panel<T, IB>, PWG, 0));
   if (occ < 1) return set_error(ctx, HFPX_ERR_HIP, "getrf: panel kernel cannot be resident");
-  int max_wg = std::min(1024, occ * std::max(1, ctx->num_cus));
+  const int max_wg = std::min(PANEL_MAX_WG, occ * std::max(1, ctx->num_cus));
   const size_t rows_bytes = sizeof(T) * 2 * (size_t)(max_wg + 1) * IB;
-  const size_t need = sizeof(PanelScratch<T>) + rows_bytes + 256;
-  int rc = ensure_work(ctx, need + 64);
+  const size_t off_ps = 256, off_rows = off_ps + ((sizeof(PanelScratch<T>) + 255) & ~(size_t)255);
+  const size_t off_pp = off_rows + ((rows_bytes + 255) & ~(size_t)255);
+  int rc = ensure_lu_work(ctx, off_pp + 2 * sizeof(PermPairs));
   if (rc) return rc;
-  char* w = (char*)ctx->work;
-  int32_t* dinfo = (int32_t*)w;
-  PanelScratch<T>* ps = (PanelScratch<T>*)(w + 256);
-  T* rows = (T*)(w + 256 + sizeof(PanelScratch<T>));
-  HFPX_TRY_HIP(ctx, hipMemsetAsync(dinfo, 0, sizeof(int32_t), ctx->stream));
+  char* w = (char*)ctx->lu_work;
+  int32_t* dflags = (int32_t*)w;  // [0] info, [1] barrier timeout
+  PanelScratch<T>* ps = (PanelScratch<T>*)(w + off_ps);
+  T* rows = (T*)(w + off_rows);
+  PermPairs* pp_in = (PermPairs*)(w + off_pp);
+  PermPairs* pp_out = pp_in + 1;
+  HFPX_TRY_HIP(ctx, hipMemsetAsync(dflags, 0, 2 * sizeof(int32_t), ctx->stream));
+  HFPX_TRY_HIP(ctx, hipMemsetAsync(ctx->sync, 0, sizeof(uint64_t), ctx->stream));
+  ctx->sync_base = 0;
 
   for (int64_t k = 0; k < n; k += NB) {
     const int64_t kb = std::min<int64_t>(NB, n - k);
     for (int64_t j = k; j < k + kb; j += IB) {
       const int jb = (int)std::min<int64_t>(IB, k + kb - j);
-      rc = panel<T, IB>(ctx, A, lda, n, j, jb, ipiv, dinfo, ps, rows, max_wg);
+      rc = panel<T, IB>(ctx, A, lda, n, j, jb, ipiv, dflags, ps, rows, max_wg);
       if (rc) return rc;
       // interchanges on the outer panel's other columns
-      if ((rc = laswp<T>(ctx, A, lda, ipiv, j, j + jb, k, j))) return rc;
-      if ((rc = laswp<T>(ctx, A, lda, ipiv, j, j + jb, j + jb, k + kb))) return rc;
+      if ((rc = perm_pairs(ctx, ipiv, j, j + jb, n, pp_in))) return rc;
+      if ((rc = apply_pairs<T>(ctx, A, lda, pp_in, k, j, j + jb, k + kb))) return rc;
       const int64_t rest = k + kb - (j + jb);
       if (rest > 0) {
         T* a12 = A + j + (j + jb) * lda;
@@ -592,8 +562,8 @@ static int getrf_impl(hfpx_ctx* ctx, T* A, int64_t lda, int64_t n, int32_t* ipiv
       }
     }
     // interchanges on the columns left and right of the outer panel
-    if ((rc = laswp<T>(ctx, A, lda, ipiv, k, k + kb, 0, k))) return rc;
-    if ((rc = laswp<T>(ctx, A, lda, ipiv, k, k + kb, k + kb, n))) return rc;
+    if ((rc = perm_pairs(ctx, ipiv, k, k + kb, n, pp_out))) return rc;
+    if ((rc = apply_pairs<T>(ctx, A, lda, pp_out, 0, k, k + kb, n))) return rc;
     const int64_t rest = n - (k + kb);
     if (rest > 0) {
       T* a12 = A + k + (k + kb) * lda;
@@ -603,11 +573,12 @@ static int getrf_impl(hfpx_ctx* ctx, T* A, int64_t lda, int64_t n, int32_t* ipiv
         return rc;
     }
   }
-  int32_t hinfo = 0;
-  HFPX_TRY_HIP(ctx, hipMemcpyAsync(&hinfo, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  int32_t hflags[2] = {0, 0};
+  HFPX_TRY_HIP(ctx, hipMemcpyAsync(hflags, dflags, sizeof(hflags), hipMemcpyDeviceToHost, ctx->stream));
   HFPX_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  if (info_out) *info_out = hinfo;
-  return hinfo ? HFPX_ERR_SINGULAR : HFPX_OK;
+  if (hflags[1]) return set_error(ctx, HFPX_ERR_HIP, "getrf: panel grid barrier timed out");
+  if (info_out) *info_out = hflags[0];
+  return hflags[0] ? HFPX_ERR_SINGULAR : HFPX_OK;
 }
 
 // B (n x nrhs) := A^{-1} B using the factors (dgetrs 'N').
@@ -616,8 +587,19 @@ static int getrs_impl(hfpx_ctx* ctx, const T* LU, int64_t lda, const int32_t* ip
                       T* B, int64_t ldb, int64_t nrhs) {
   constexpr int NB = 256;
   int rc;
-  // row interchanges on B (one thread per right-hand side)
-  if ((rc = laswp<T>(ctx, B, ldb, ipiv, 0, n, 0, nrhs))) return rc;
+  // row interchanges on B: pair blocks for all of ipiv, then one workgroup
+  // per right-hand side applies them in order
+  const int64_t npb = (n + SWAP_BLK - 1) / SWAP_BLK;
+  if ((rc = ensure_lu_work(ctx, sizeof(PermPairs) * (size_t)npb))) return rc;
+  PermPairs* pp = (PermPairs*)ctx->lu_work;
+  if ((rc = perm_pairs(ctx, ipiv, 0, n, n, pp))) return rc;
+  {
+    const int tok = prof_begin(ctx);
+    hipLaunchKernelGGL(k_apply_pairs_seq<T>, dim3((unsigned)nrhs), dim3(256), 0, ctx->stream, B, ldb, pp,
+                       (int)npb);
+    prof_end(ctx, tok, HFPX_KC_LASWP, 0.0, 0.0);
+    HFPX_TRY_HIP(ctx, hipGetLastError());
+  }
   // forward substitution with unit L, blocked
   for (int64_t k = 0; k < n; k += NB) {
     const int64_t kb = std::min<int64_t>(NB, n - k);
@@ -650,13 +632,6 @@ int lu_getrs(hfpx_ctx* ctx, int dtype, const void* LU, int64_t lda, const int32_
              void* B, int64_t ldb, int64_t nrhs) {
   if (dtype == HFPX_F64) return getrs_impl<double>(ctx, (const double*)LU, lda, ipiv, n, (double*)B, ldb, nrhs);
   return getrs_impl<float>(ctx, (const float*)LU, lda, ipiv, n, (float*)B, ldb, nrhs);
-}
-
-int gemm_sub_any(hfpx_ctx* ctx, int dtype, void* C, int64_t ldc, const void* A, int64_t lda,
-                 const void* B, int64_t ldb, int64_t M, int64_t N, int64_t K) {
-  if (dtype == HFPX_F64)
-    return gemm_sub<double>(ctx, (double*)C, ldc, (const double*)A, lda, (const double*)B, ldb, M, N, K);
-  return gemm_sub<float>(ctx, (float*)C, ldc, (const float*)A, lda, (const float*)B, ldb, M, N, K);
 }
 
 }  // namespace hfpx

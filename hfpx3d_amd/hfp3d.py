@@ -393,6 +393,19 @@ def gemm_sub(C, A, B, ctx=None):
     return C
 
 
+def tune_set(knob, value, ctx=None):
+    """Select a kernel variant on the context (hfpx_tune_set)."""
+    ctx = ctx or get_context()
+    ctx.check(ctx.lib.hfpx_tune_set(ctx.handle, int(knob), int(value)))
+
+
+def tune_get(knob, ctx=None):
+    ctx = ctx or get_context()
+    v = ctypes.c_int(0)
+    ctx.check(ctx.lib.hfpx_tune_get(ctx.handle, int(knob), ctypes.byref(v)))
+    return v.value
+
+
 def solve_mixed(A64, B, tol=1e-14, max_outer=10, gmres_m=20, ctx=None, LU32=None):
     """Solve A64 X = B: FP32 LU of A64 (or the given FP32 LU object) as the
     preconditioner of FP64 GMRES iterative refinement against A64.

@@ -154,6 +154,18 @@ int hfpx_profile(hfpx_ctx* ctx, int enable);
 int hfpx_profile_read(hfpx_ctx* ctx, int kernel_class, int64_t* launches, double* ms, double* flops,
                       double* bytes);
 
+/* ---- tuning -------------------------------------------------------------- */
+/* Kernel-variant knobs of a context (no reference counterpart).
+ * HFPX_KNOB_GEMM_F32: FP32 trailing-update GEMM (products with more than 64
+ * columns): 0 = 16x16x4 MFMA, 128 x 128 tile; 1..6 = 32x32x2 MFMA with tiles
+ * 128x128 (K chunk 16), 128x128 (32), 256x128 (16), 256x128 (32),
+ * 128x256 (16), 256x256 (16). */
+#define HFPX_KNOB_GEMM_F32 0
+#define HFPX_KNOB_COUNT 1
+/* Set / read one knob; HFPX_ERR_INVALID for an unknown knob or value. */
+int hfpx_tune_set(hfpx_ctx* ctx, int knob, int value);
+int hfpx_tune_get(const hfpx_ctx* ctx, int knob, int* value);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,6 +41,38 @@ def test_gemm_sub(env, dt, M, N, K):
     assert np.abs(got - ref).max() / (np.abs(A).max() * np.abs(B).max() * K) < tol
 
 
+@pytest.mark.parametrize("variant", range(1, 7))
+@pytest.mark.parametrize("M,N,K", [(300, 257, 33), (513, 700, 100), (1000, 300, 256), (130, 129, 17)])
+def test_sgemm_variants(env, variant, M, N, K):
+    """Every FP32 trailing-update kernel variant, on shapes that cut through
+    its tiles and K chunks."""
+    torch, hx = env
+    from hfpx3d_amd import _lib
+    print(f"case: sgemm variant {variant} {M}x{N}x{K}", flush=True)
+    rng = np.random.default_rng(7 * M + N + K + variant)
+    A, B, C = rng.normal(size=(M, K)), rng.normal(size=(K, N)), rng.normal(size=(M, N))
+    Cd = to_dev(torch, hx, C, torch.float32)
+    old = hx.tune_get(_lib.KNOB_GEMM_F32)
+    hx.tune_set(_lib.KNOB_GEMM_F32, variant)
+    try:
+        hx.gemm_sub(Cd, to_dev(torch, hx, A, torch.float32), to_dev(torch, hx, B, torch.float32))
+    finally:
+        hx.tune_set(_lib.KNOB_GEMM_F32, old)
+    got = Cd.cpu().numpy()
+    ref = C.astype(np.float32).astype(np.float64) - A.astype(np.float32).astype(np.float64) @ \
+        B.astype(np.float32).astype(np.float64)
+    assert np.abs(got - ref).max() / (np.abs(A).max() * np.abs(B).max() * K) < 2e-6
+
+
+def test_tune_rejects_unknown_variant(env):
+    torch, hx = env
+    from hfpx3d_amd import _lib
+    with pytest.raises(_lib.HfpxError):
+        hx.tune_set(_lib.KNOB_GEMM_F32, _lib.GEMM_F32_VARIANTS)
+    with pytest.raises(_lib.HfpxError):
+        hx.tune_set(99, 0)
+
+
 @pytest.mark.parametrize("n", [1, 7, 64, 300, 1000, 2500])
 def test_getrf_f64_matches_lapack(env, n):
     torch, hx = env
