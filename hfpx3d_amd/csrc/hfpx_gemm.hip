@@ -153,7 +153,20 @@ __global__ __launch_bounds__(256) void k_gemm_sub(T* __restrict__ C, int64_t ldc
 // ---------------------------------------------------------------------
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
-template <int WR, int WC, int KC>
+// Options of k_sgemm32 (bit flags):
+//  SG_INT   interior launch: every tile lies wholly inside M x N (the host
+//           launches the edge bands separately), so loads and stores carry
+//           no row/column guards; only a K chunk that crosses K checks k;
+//  SG_PIPE  the LDS operands of k-step s+1 are read while the MFMAs of step s
+//           run (register double buffer);
+//  SG_SWZ   1-D grid remapped so that each of the 8 XCDs (workgroups are
+//           dealt to XCDs round-robin) owns one contiguous eighth of the
+//           tiles, walked in bands of 8 row tiles, so the A and B panels that
+//           concurrent workgroups share stay in that XCD's L2.
+constexpr int SG_INT = 1, SG_PIPE = 2, SG_SWZ = 4;
+constexpr int SG_GROUP = 8;
+
+template <int WR, int WC, int KC, int OPT>
 __global__ __launch_bounds__(256) void k_sgemm32(float* __restrict__ C, int64_t ldc,
                                                  const float* __restrict__ A, int64_t lda,
                                                  const float* __restrict__ B, int64_t ldb, int64_t M,
@@ -162,10 +175,26 @@ __global__ __launch_bounds__(256) void k_sgemm32(float* __restrict__ C, int64_t 
   constexpr int PA = GM + 32, PB = GN + 33;
   constexpr int QA = GM * KC / 256, QB = GN * KC / 256;
   static_assert(QA >= 1 && QB >= 1, "tile too small for the loaders");
+  static_assert(256 % GM == 0 && 256 % KC == 0, "loader strides");
   __shared__ float sA[2][KC * PA];  // sA[k * PA + i] = A(row0 + i, k0 + k)
   __shared__ float sB[2][KC * PB];  // sB[k * PB + j] = B(k0 + k, col0 + j)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int64_t row0 = (int64_t)blockIdx.x * GM, col0 = (int64_t)blockIdx.y * GN;
+  int64_t tile_r, tile_c;
+  if constexpr ((OPT & SG_SWZ) != 0) {
+    const int64_t tm = (M + GM - 1) / GM, tn = (N + GN - 1) / GN, nt = tm * tn;
+    const int64_t id = blockIdx.x, xcd = id % 8, local = id / 8;
+    const int64_t fl = nt / 8, rem = nt % 8;
+    const int64_t nid = xcd * fl + (xcd < rem ? xcd : rem) + local;
+    const int64_t per = (int64_t)SG_GROUP * tn, first = (nid / per) * SG_GROUP;
+    const int64_t gs = (tm - first) < SG_GROUP ? (tm - first) : SG_GROUP;
+    const int64_t r = nid % per;
+    tile_r = first + r % gs;
+    tile_c = r / gs;
+  } else {
+    tile_r = blockIdx.x;
+    tile_c = blockIdx.y;
+  }
+  const int64_t row0 = tile_r * GM, col0 = tile_c * GN;
   const int wr = (wave & 1) * 32 * WR, wc = (wave >> 1) * 32 * WC;
 
   floatx16 acc[WC][WR];
@@ -178,6 +207,26 @@ __global__ __launch_bounds__(256) void k_sgemm32(float* __restrict__ C, int64_t 
 
   float ra[QA], rb[QB];
   auto load_global = [&](int64_t k0) {
+    if constexpr ((OPT & SG_INT) != 0) {
+      // e = tid + 256 q: A row tid % GM, k = tid / GM + q (256 / GM);
+      // B k = tid % KC, column tid / KC + q (256 / KC)
+      const float* pa = A + (row0 + tid % GM) + (k0 + tid / GM) * lda;
+      const float* pb = B + (k0 + tid % KC) + (col0 + tid / KC) * ldb;
+      if (k0 + KC <= K) {
+#pragma unroll
+        for (int q = 0; q < QA; ++q) ra[q] = pa[(int64_t)q * (256 / GM) * lda];
+#pragma unroll
+        for (int q = 0; q < QB; ++q) rb[q] = pb[(int64_t)q * (256 / KC) * ldb];
+      } else {
+#pragma unroll
+        for (int q = 0; q < QA; ++q)
+          ra[q] = (k0 + tid / GM + q * (256 / GM) < K) ? pa[(int64_t)q * (256 / GM) * lda] : 0.f;
+        const bool kin = k0 + tid % KC < K;
+#pragma unroll
+        for (int q = 0; q < QB; ++q) rb[q] = kin ? pb[(int64_t)q * (256 / KC) * ldb] : 0.f;
+      }
+      return;
+    }
 #pragma unroll
     for (int q = 0; q < QA; ++q) {
       const int e = tid + 256 * q;  // consecutive lanes: consecutive rows of one column
@@ -216,19 +265,43 @@ __global__ __launch_bounds__(256) void k_sgemm32(float* __restrict__ C, int64_t 
     if (c + 1 < nchunks) load_global((int64_t)(c + 1) * KC);
     const float* pa = sA[buf] + wr + l32;
     const float* pb = sB[buf] + wc + l32;
+    if constexpr ((OPT & SG_PIPE) != 0) {
+      float fa[2][WC], fb[2][WR];
 #pragma unroll
-    for (int s = 0; s < KC / 2; ++s) {
-      const int k = 2 * s + hk;
-      float fa[WC], fb[WR];
+      for (int t = 0; t < WC; ++t) fa[0][t] = pb[hk * PB + 32 * t];
 #pragma unroll
-      for (int t = 0; t < WC; ++t) fa[t] = pb[k * PB + 32 * t];  // C columns
+      for (int t = 0; t < WR; ++t) fb[0][t] = pa[hk * PA + 32 * t];
 #pragma unroll
-      for (int t = 0; t < WR; ++t) fb[t] = pa[k * PA + 32 * t];  // C rows
+      for (int s = 0; s < KC / 2; ++s) {
+        const int cur = s & 1;
+        if (s + 1 < KC / 2) {
+          const int k = 2 * (s + 1) + hk;
 #pragma unroll
-      for (int tc = 0; tc < WC; ++tc)
+          for (int t = 0; t < WC; ++t) fa[cur ^ 1][t] = pb[k * PB + 32 * t];
 #pragma unroll
-        for (int tr = 0; tr < WR; ++tr)
-          acc[tc][tr] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[tc], fb[tr], acc[tc][tr], 0, 0, 0);
+          for (int t = 0; t < WR; ++t) fb[cur ^ 1][t] = pa[k * PA + 32 * t];
+        }
+#pragma unroll
+        for (int tc = 0; tc < WC; ++tc)
+#pragma unroll
+          for (int tr = 0; tr < WR; ++tr)
+            acc[tc][tr] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[cur][tc], fb[cur][tr], acc[tc][tr], 0, 0, 0);
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < KC / 2; ++s) {
+        const int k = 2 * s + hk;
+        float fa[WC], fb[WR];
+#pragma unroll
+        for (int t = 0; t < WC; ++t) fa[t] = pb[k * PB + 32 * t];  // C columns
+#pragma unroll
+        for (int t = 0; t < WR; ++t) fb[t] = pa[k * PA + 32 * t];  // C rows
+#pragma unroll
+        for (int tc = 0; tc < WC; ++tc)
+#pragma unroll
+          for (int tr = 0; tr < WR; ++tr)
+            acc[tc][tr] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[tc], fb[tr], acc[tc][tr], 0, 0, 0);
+      }
     }
     if (c + 1 < nchunks) store_lds(buf ^ 1);
     __syncthreads();
@@ -239,11 +312,11 @@ __global__ __launch_bounds__(256) void k_sgemm32(float* __restrict__ C, int64_t 
 #pragma unroll
     for (int tr = 0; tr < WR; ++tr) {
       const int64_t r = row0 + wr + 32 * tr + l32;
-      if (r >= M) continue;
+      if ((OPT & SG_INT) == 0 && r >= M) continue;
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const int64_t cc = col0 + wc + 32 * tc + 8 * (v >> 2) + 4 * hk + (v & 3);
-        if (cc < N) C[r + cc * ldc] -= acc[tc][tr][v];
+        if ((OPT & SG_INT) != 0 || cc < N) C[r + cc * ldc] -= acc[tc][tr][v];
       }
     }
 }
@@ -253,12 +326,30 @@ __global__ __launch_bounds__(256) void k_sgemm32(float* __restrict__ C, int64_t 
 // ---------------------------------------------------------------------
 // FP32 trailing-update kernels selectable through HFPX_KNOB_GEMM_F32
 // (0 = 16x16x4 128 x 128 tile; 1.. = 32x32x2 family).
-template <int WR, int WC, int KC>
+template <int WR, int WC, int KC, int OPT>
+static void launch_sgemm32_grid(hfpx_ctx* ctx, float* C, int64_t ldc, const float* A, int64_t lda,
+                                const float* B, int64_t ldb, int64_t M, int64_t N, int64_t K) {
+  if (M <= 0 || N <= 0) return;
+  const int64_t tm = (M + 64 * WR - 1) / (64 * WR), tn = (N + 64 * WC - 1) / (64 * WC);
+  const dim3 grid = (OPT & SG_SWZ) ? dim3((unsigned)(tm * tn)) : dim3((unsigned)tm, (unsigned)tn);
+  hipLaunchKernelGGL((k_sgemm32<WR, WC, KC, OPT>), grid, dim3(256), 0, ctx->stream, C, ldc, A, lda, B, ldb, M,
+                     N, K);
+}
+
+// With SG_INT: the whole tiles by the interior kernel, then the bottom band
+// (all columns) and the right band (interior rows) by the guarded kernel.
+template <int WR, int WC, int KC, int OPT>
 static void launch_sgemm32(hfpx_ctx* ctx, float* C, int64_t ldc, const float* A, int64_t lda,
                            const float* B, int64_t ldb, int64_t M, int64_t N, int64_t K) {
-  dim3 grid((unsigned)((M + 64 * WR - 1) / (64 * WR)), (unsigned)((N + 64 * WC - 1) / (64 * WC)));
-  hipLaunchKernelGGL((k_sgemm32<WR, WC, KC>), grid, dim3(256), 0, ctx->stream, C, ldc, A, lda, B, ldb, M,
-                     N, K);
+  if constexpr ((OPT & SG_INT) == 0) {
+    launch_sgemm32_grid<WR, WC, KC, OPT>(ctx, C, ldc, A, lda, B, ldb, M, N, K);
+  } else {
+    constexpr int EDGE = OPT & ~SG_INT & ~SG_SWZ;
+    const int64_t Mi = M / (64 * WR) * (64 * WR), Ni = N / (64 * WC) * (64 * WC);
+    launch_sgemm32_grid<WR, WC, KC, OPT>(ctx, C, ldc, A, lda, B, ldb, Mi, Ni, K);
+    launch_sgemm32_grid<WR, WC, KC, EDGE>(ctx, C + Mi, ldc, A + Mi, lda, B, ldb, M - Mi, N, K);
+    launch_sgemm32_grid<WR, WC, KC, EDGE>(ctx, C + Ni * ldc, ldc, A, lda, B + Ni * ldb, ldb, Mi, N - Ni, K);
+  }
 }
 
 int gemm_f32_variants() { return 7; }
@@ -272,17 +363,17 @@ int gemm_sub(hfpx_ctx* ctx, T* C, int64_t ldc, const T* A, int64_t lda, const T*
   dim3 grid((unsigned)((M + GT - 1) / GT), (unsigned)((N + gtn - 1) / gtn));
   if ((N + 63) / 64 > 65535) return set_error(ctx, HFPX_ERR_INVALID, "gemm: N too large");
   const int tok = prof_begin(ctx);
-  const int var = ctx->knob[HFPX_KNOB_GEMM_F32];
+  int var = ctx->knob[HFPX_KNOB_GEMM_F32];
+  if (var == 0) var = (M >= 2048 && N >= 512) ? 3 : 2;  // automatic choice
   bool done = false;
   if constexpr (sizeof(T) == 4) {
-    if (!skinny && var > 0) {
+    if (!skinny && var > 1) {
       switch (var) {
-        case 1: launch_sgemm32<2, 2, 16>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
-        case 2: launch_sgemm32<2, 2, 32>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
-        case 3: launch_sgemm32<4, 2, 16>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
-        case 4: launch_sgemm32<4, 2, 32>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
-        case 5: launch_sgemm32<2, 4, 16>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
-        default: launch_sgemm32<4, 4, 16>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
+        case 2: launch_sgemm32<2, 2, 16, 0>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
+        case 3: launch_sgemm32<2, 2, 16, SG_INT>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
+        case 4: launch_sgemm32<2, 2, 16, SG_INT | SG_PIPE>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
+        case 5: launch_sgemm32<2, 2, 16, SG_INT | SG_PIPE | SG_SWZ>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
+        default: launch_sgemm32<2, 2, 16, SG_INT | SG_SWZ>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
       }
       done = true;
     }

@@ -7,14 +7,15 @@
 // ipiv, info = first exactly-zero pivot column), FP32 variants for the
 // mixed-precision solve.
 //
-// Algorithm: right-looking blocked LU, two levels.  An outer panel of NB
-// columns is factorised as a sequence of inner panels of IB columns; each
-// inner panel is factorised by one cooperative kernel whose workgroups keep
-// their rows of the panel in registers and meet once per column at a grid
-// barrier (pivot search -> interchange -> rank-1 update).  The rest of the
-// outer panel, and then the trailing matrix, get the row interchanges, a
-// unit-lower triangular solve for the U rows, and the Schur-complement update
-// C -= L21 U12 on MFMA (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32).
+// Algorithm: right-looking blocked LU.  Each outer block of NB columns is
+// factorised recursively: left half, row interchanges + unit-lower
+// triangular solve + Schur-complement update of the right half, right half,
+// down to panels of IB columns.  A panel is factorised by one cooperative
+// kernel whose workgroups keep their rows in registers and meet once per
+// column at a grid barrier (pivot search -> interchange -> rank-1 update).
+// After each outer block the trailing matrix gets the interchanges, the
+// triangular solve for its U rows and one update C -= L21 U12 with K = NB on
+// MFMA (hfpx_gemm.hip).
 #include <algorithm>
 
 #include "hfpx_internal.h"
@@ -437,35 +438,31 @@ static int trsm_leaf(hfpx_ctx* ctx, const T* L, int64_t ldl, T* B, int64_t ldb, 
   return HFPX_OK;
 }
 
-// B (kb x ncols) := L^{-1} B, L unit lower kb x kb: leaves of LB rows with
-// MFMA updates from the rows already solved (left-looking).
+// B (kb x ncols) := L^{-1} B, L unit lower kb x kb.  Recursive halving down
+// to leaves of LB rows, so that the MFMA updates in between have M = kb/2,
+// kb/4, ... rather than LB.
 template <typename T>
 static int trsm_lunit(hfpx_ctx* ctx, const T* L, int64_t ldl, T* B, int64_t ldb, int kb, int64_t ncols) {
   if (ncols <= 0 || kb <= 0) return HFPX_OK;
+  if (kb <= LB) return trsm_leaf<T, true>(ctx, L, ldl, B, ldb, kb, ncols);
+  const int k1 = ((kb / 2 + LB - 1) / LB) * LB;
   int rc;
-  for (int i0 = 0; i0 < kb; i0 += LB) {
-    const int ib = std::min(LB, kb - i0);
-    if (i0 > 0 && (rc = gemm_sub<T>(ctx, B + i0, ldb, L + i0, ldl, B, ldb, ib, ncols, i0))) return rc;
-    if ((rc = trsm_leaf<T, true>(ctx, L + i0 + (int64_t)i0 * ldl, ldl, B + i0, ldb, ib, ncols))) return rc;
-  }
-  return HFPX_OK;
+  if ((rc = trsm_lunit<T>(ctx, L, ldl, B, ldb, k1, ncols))) return rc;
+  if ((rc = gemm_sub<T>(ctx, B + k1, ldb, L + k1, ldl, B, ldb, kb - k1, ncols, k1))) return rc;
+  return trsm_lunit<T>(ctx, L + k1 + (int64_t)k1 * ldl, ldl, B + k1, ldb, kb - k1, ncols);
 }
 
-// B (kb x ncols) := U^{-1} B, U upper (non-unit) kb x kb, bottom-up leaves.
+// B (kb x ncols) := U^{-1} B, U upper (non-unit) kb x kb, recursive halving
+// (bottom half first).
 template <typename T>
 static int trsm_upper(hfpx_ctx* ctx, const T* U, int64_t ldu, T* B, int64_t ldb, int kb, int64_t ncols) {
   if (ncols <= 0 || kb <= 0) return HFPX_OK;
+  if (kb <= LB) return trsm_leaf<T, false>(ctx, U, ldu, B, ldb, kb, ncols);
+  const int k1 = ((kb / 2 + LB - 1) / LB) * LB;
   int rc;
-  const int nl = (kb + LB - 1) / LB;
-  for (int b = nl - 1; b >= 0; --b) {
-    const int i0 = b * LB, ib = std::min(LB, kb - i0);
-    const int done = kb - (i0 + ib);
-    if (done > 0 && (rc = gemm_sub<T>(ctx, B + i0, ldb, U + i0 + (int64_t)(i0 + ib) * ldu, ldu, B + i0 + ib, ldb,
-                                      ib, ncols, done)))
-      return rc;
-    if ((rc = trsm_leaf<T, false>(ctx, U + i0 + (int64_t)i0 * ldu, ldu, B + i0, ldb, ib, ncols))) return rc;
-  }
-  return HFPX_OK;
+  if ((rc = trsm_upper<T>(ctx, U + k1 + (int64_t)k1 * ldu, ldu, B + k1, ldb, kb - k1, ncols))) return rc;
+  if ((rc = gemm_sub<T>(ctx, B, ldb, U + (int64_t)k1 * ldu, ldu, B + k1, ldb, k1, ncols, kb - k1))) return rc;
+  return trsm_upper<T>(ctx, U, ldu, B, ldb, k1, ncols);
 }
 
 // Grow-only workspace of the LU routines (separate from ctx->work, which
@@ -506,75 +503,93 @@ static int panel(hfpx_ctx* ctx, T* A, int64_t lda, int64_t n, int64_t j0, int jb
   return HFPX_OK;
 }
 
+// Blocking: outer blocks of NB columns are factorised recursively (halves,
+// down to IB-column panels); the trailing matrix is updated once per outer
+// block with K = NB, so it is read and written n / NB times in all.
+constexpr int LU_IB = 32;
+constexpr int LU_NB = 1024;
+constexpr int LU_MAX_PAIR_BLOCKS = LU_NB / SWAP_BLK;
+
 template <typename T>
-struct LuCfg;
-template <>
-struct LuCfg<double> {
-  static constexpr int IB = 32;
-  static constexpr int NB = 256;
+struct LuWork {
+  int32_t* dflags;  // [0] info, [1] barrier timeout
+  PanelScratch<T>* ps;
+  T* rows;
+  PermPairs* pp;  // LU_MAX_PAIR_BLOCKS blocks
+  int max_wg;
 };
-template <>
-struct LuCfg<float> {
-  static constexpr int IB = 32;
-  static constexpr int NB = 256;
-};
+
+// Interchanges ipiv[k0, k1) on columns [c0, c1) and [c2, c3).
+template <typename T>
+static int swap_cols(hfpx_ctx* ctx, const LuWork<T>& W, T* A, int64_t lda, const int32_t* ipiv, int64_t n,
+                     int64_t k0, int64_t k1, int64_t c0, int64_t c1, int64_t c2, int64_t c3) {
+  if (k1 <= k0 || (c1 <= c0 && c3 <= c2)) return HFPX_OK;
+  int rc;
+  if ((rc = perm_pairs(ctx, ipiv, k0, k1, n, W.pp))) return rc;
+  const int64_t nblk = (k1 - k0 + SWAP_BLK - 1) / SWAP_BLK;
+  for (int64_t b = 0; b < nblk; ++b)
+    if ((rc = apply_pairs<T>(ctx, A, lda, W.pp + b, c0, c1, c2, c3))) return rc;
+  return HFPX_OK;
+}
+
+// Factorise columns [j, j + w) over rows [j, n): interchanges are applied to
+// these columns only; the caller applies them elsewhere.
+template <typename T>
+static int factor_cols(hfpx_ctx* ctx, const LuWork<T>& W, T* A, int64_t lda, int64_t n, int64_t j, int64_t w,
+                       int32_t* ipiv) {
+  if (w <= LU_IB) return panel<T, LU_IB>(ctx, A, lda, n, j, (int)w, ipiv, W.dflags, W.ps, W.rows, W.max_wg);
+  const int64_t w1 = ((w / 2 + LU_IB - 1) / LU_IB) * LU_IB;
+  int rc;
+  if ((rc = factor_cols<T>(ctx, W, A, lda, n, j, w1, ipiv))) return rc;
+  if ((rc = swap_cols<T>(ctx, W, A, lda, ipiv, n, j, j + w1, j + w1, j + w, 0, 0))) return rc;
+  T* a12 = A + j + (j + w1) * lda;
+  if ((rc = trsm_lunit<T>(ctx, A + j + j * lda, lda, a12, lda, (int)w1, w - w1))) return rc;
+  if ((rc = gemm_sub<T>(ctx, A + (j + w1) + (j + w1) * lda, lda, A + (j + w1) + j * lda, lda, a12, lda,
+                        n - (j + w1), w - w1, w1)))
+    return rc;
+  if ((rc = factor_cols<T>(ctx, W, A, lda, n, j + w1, w - w1, ipiv))) return rc;
+  return swap_cols<T>(ctx, W, A, lda, ipiv, n, j + w1, j + w, j, j + w1, 0, 0);
+}
 
 template <typename T>
 static int getrf_impl(hfpx_ctx* ctx, T* A, int64_t lda, int64_t n, int32_t* ipiv, int64_t* info_out) {
-  constexpr int IB = LuCfg<T>::IB, NB = LuCfg<T>::NB;
-  static_assert(NB <= SWAP_BLK, "one pair block per outer panel");
+  static_assert(LU_NB % SWAP_BLK == 0, "pair blocks per outer block");
   // co-resident panel workgroups
   int occ = 0;
-  HFPX_TRY_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_panel<T, IB>, PWG, 0));
+  HFPX_TRY_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_panel<T, LU_IB>, PWG, 0));
   if (occ < 1) return set_error(ctx, HFPX_ERR_HIP, "getrf: panel kernel cannot be resident");
-  const int max_wg = std::min(PANEL_MAX_WG, occ * std::max(1, ctx->num_cus));
-  const size_t rows_bytes = sizeof(T) * 2 * (size_t)(max_wg + 1) * IB;
+  LuWork<T> W;
+  W.max_wg = std::min(PANEL_MAX_WG, occ * std::max(1, ctx->num_cus));
+  const size_t rows_bytes = sizeof(T) * 2 * (size_t)(W.max_wg + 1) * LU_IB;
   const size_t off_ps = 256, off_rows = off_ps + ((sizeof(PanelScratch<T>) + 255) & ~(size_t)255);
   const size_t off_pp = off_rows + ((rows_bytes + 255) & ~(size_t)255);
-  int rc = ensure_lu_work(ctx, off_pp + 2 * sizeof(PermPairs));
+  int rc = ensure_lu_work(ctx, off_pp + LU_MAX_PAIR_BLOCKS * sizeof(PermPairs));
   if (rc) return rc;
   char* w = (char*)ctx->lu_work;
-  int32_t* dflags = (int32_t*)w;  // [0] info, [1] barrier timeout
-  PanelScratch<T>* ps = (PanelScratch<T>*)(w + off_ps);
-  T* rows = (T*)(w + off_rows);
-  PermPairs* pp_in = (PermPairs*)(w + off_pp);
-  PermPairs* pp_out = pp_in + 1;
-  HFPX_TRY_HIP(ctx, hipMemsetAsync(dflags, 0, 2 * sizeof(int32_t), ctx->stream));
+  W.dflags = (int32_t*)w;
+  W.ps = (PanelScratch<T>*)(w + off_ps);
+  W.rows = (T*)(w + off_rows);
+  W.pp = (PermPairs*)(w + off_pp);
+  HFPX_TRY_HIP(ctx, hipMemsetAsync(W.dflags, 0, 2 * sizeof(int32_t), ctx->stream));
   HFPX_TRY_HIP(ctx, hipMemsetAsync(ctx->sync, 0, sizeof(uint64_t), ctx->stream));
   ctx->sync_base = 0;
 
-  for (int64_t k = 0; k < n; k += NB) {
-    const int64_t kb = std::min<int64_t>(NB, n - k);
-    for (int64_t j = k; j < k + kb; j += IB) {
-      const int jb = (int)std::min<int64_t>(IB, k + kb - j);
-      rc = panel<T, IB>(ctx, A, lda, n, j, jb, ipiv, dflags, ps, rows, max_wg);
-      if (rc) return rc;
-      // interchanges on the outer panel's other columns
-      if ((rc = perm_pairs(ctx, ipiv, j, j + jb, n, pp_in))) return rc;
-      if ((rc = apply_pairs<T>(ctx, A, lda, pp_in, k, j, j + jb, k + kb))) return rc;
-      const int64_t rest = k + kb - (j + jb);
-      if (rest > 0) {
-        T* a12 = A + j + (j + jb) * lda;
-        if ((rc = trsm_lunit<T>(ctx, A + j + j * lda, lda, a12, lda, jb, rest))) return rc;
-        if ((rc = gemm_sub<T>(ctx, A + (j + jb) + (j + jb) * lda, lda, A + (j + jb) + j * lda, lda, a12,
-                              lda, n - (j + jb), rest, jb)))
-          return rc;
-      }
-    }
-    // interchanges on the columns left and right of the outer panel
-    if ((rc = perm_pairs(ctx, ipiv, k, k + kb, n, pp_out))) return rc;
-    if ((rc = apply_pairs<T>(ctx, A, lda, pp_out, 0, k, k + kb, n))) return rc;
+  for (int64_t k = 0; k < n; k += LU_NB) {
+    const int64_t kb = std::min<int64_t>(LU_NB, n - k);
+    if ((rc = factor_cols<T>(ctx, W, A, lda, n, k, kb, ipiv))) return rc;
+    // interchanges on the columns left and right of the outer block
+    if ((rc = swap_cols<T>(ctx, W, A, lda, ipiv, n, k, k + kb, 0, k, k + kb, n))) return rc;
     const int64_t rest = n - (k + kb);
     if (rest > 0) {
       T* a12 = A + k + (k + kb) * lda;
       if ((rc = trsm_lunit<T>(ctx, A + k + k * lda, lda, a12, lda, (int)kb, rest))) return rc;
-      if ((rc = gemm_sub<T>(ctx, A + (k + kb) + (k + kb) * lda, lda, A + (k + kb) + k * lda, lda, a12,
-                            lda, rest, rest, kb)))
+      if ((rc = gemm_sub<T>(ctx, A + (k + kb) + (k + kb) * lda, lda, A + (k + kb) + k * lda, lda, a12, lda,
+                            rest, rest, kb)))
         return rc;
     }
   }
   int32_t hflags[2] = {0, 0};
-  HFPX_TRY_HIP(ctx, hipMemcpyAsync(hflags, dflags, sizeof(hflags), hipMemcpyDeviceToHost, ctx->stream));
+  HFPX_TRY_HIP(ctx, hipMemcpyAsync(hflags, W.dflags, sizeof(hflags), hipMemcpyDeviceToHost, ctx->stream));
   HFPX_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (hflags[1]) return set_error(ctx, HFPX_ERR_HIP, "getrf: panel grid barrier timed out");
   if (info_out) *info_out = hflags[0];

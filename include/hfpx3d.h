@@ -157,9 +157,11 @@ int hfpx_profile_read(hfpx_ctx* ctx, int kernel_class, int64_t* launches, double
 /* ---- tuning -------------------------------------------------------------- */
 /* Kernel-variant knobs of a context (no reference counterpart).
  * HFPX_KNOB_GEMM_F32: FP32 trailing-update GEMM (products with more than 64
- * columns): 0 = 16x16x4 MFMA, 128 x 128 tile; 1..6 = 32x32x2 MFMA with tiles
- * 128x128 (K chunk 16), 128x128 (32), 256x128 (16), 256x128 (32),
- * 128x256 (16), 256x256 (16). */
+ * columns): 0 = automatic (default: 3 when M >= 2048 and N >= 512, else 2);
+ * 1 = 16x16x4 MFMA, 128 x 128 tile; 2..6 = 32x32x2 MFMA, 128 x 128 tile,
+ * K chunk 16, with 2 no options, 3 unguarded interior tiles (edge bands in
+ * separate launches), 4 = 3 + pipelined LDS operands, 5 = 4 + XCD-grouped
+ * tile order, 6 = 3 + XCD-grouped tile order. */
 #define HFPX_KNOB_GEMM_F32 0
 #define HFPX_KNOB_COUNT 1
 /* Set / read one knob; HFPX_ERR_INVALID for an unknown knob or value. */
