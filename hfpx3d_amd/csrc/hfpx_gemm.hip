@@ -53,8 +53,17 @@ template <typename T, int GTN>
 __global__ __launch_bounds__(256) void k_gemm_sub(T* __restrict__ C, int64_t ldc,
                                                   const T* __restrict__ A, int64_t lda,
                                                   const T* __restrict__ B, int64_t ldb, int64_t M,
-                                                  int64_t N, int64_t K) {
+                                                  int64_t N, int64_t K, int64_t kchunk, int64_t cz) {
   typedef typename MfmaVec<T>::type vec4;
+  // split K: slice z = blockIdx.z covers k in [z kchunk, (z+1) kchunk) and
+  // writes C + z cz (cz = 0 and one slice: the plain product)
+  {
+    const int64_t z = blockIdx.z, kz = z * kchunk;
+    A += kz * lda;
+    B += kz;
+    C += z * cz;
+    K = (K - kz) < kchunk ? (K - kz) : kchunk;
+  }
   constexpr int WN = GTN / 2;      // wave tile columns
   constexpr int TC = WN / 16;      // 16x16 MFMA tiles across a wave's columns
   constexpr int BQ = KC * GTN / 256;  // B elements loaded per thread per chunk
@@ -321,6 +330,37 @@ __global__ __launch_bounds__(256) void k_sgemm32(float* __restrict__ C, int64_t 
     }
 }
 
+// C (M x N, ldc) += sum over slices z = 0 .. ks-1 of P_z (M x N, ld M),
+// in slice order (deterministic).
+template <typename T>
+__global__ __launch_bounds__(256) void k_splitk_sum(T* __restrict__ C, int64_t ldc, const T* __restrict__ P,
+                                                    int64_t M, int64_t N, int ks) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= M * N) return;
+  const int64_t i = e % M, j = e / M;
+  T s = C[i + j * ldc];
+  for (int z = 0; z < ks; ++z) s += P[e + (int64_t)z * M * N];
+  C[i + j * ldc] = s;
+}
+
+constexpr int64_t SPLITK_MIN_K = 8192;
+constexpr int64_t SPLITK_MAX = 32;
+
+// Grow-only workspace of the split-K products (not ctx->work, which callers
+// of gemm_sub may be holding).
+static int ensure_gemm_work(hfpx_ctx* ctx, size_t bytes) {
+  if (ctx->gemm_work_bytes >= bytes) return HFPX_OK;
+  if (ctx->gemm_work) {
+    HFPX_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    HFPX_TRY_HIP(ctx, hipFree(ctx->gemm_work));
+    ctx->gemm_work = nullptr;
+    ctx->gemm_work_bytes = 0;
+  }
+  HFPX_TRY_HIP(ctx, hipMalloc(&ctx->gemm_work, bytes));
+  ctx->gemm_work_bytes = bytes;
+  return HFPX_OK;
+}
+
 // ---------------------------------------------------------------------
 // host dispatch
 // ---------------------------------------------------------------------
@@ -378,11 +418,32 @@ int gemm_sub(hfpx_ctx* ctx, T* C, int64_t ldc, const T* A, int64_t lda, const T*
       done = true;
     }
   }
+  if (!done && skinny && K >= SPLITK_MIN_K) {
+    // few columns, long K (the residuals of the mixed solve): slices of K
+    // write partial products -A_z B_z to a workspace, summed in slice order
+    const int64_t nrt = (M + GT - 1) / GT;
+    int64_t ks = std::min<int64_t>(SPLITK_MAX, std::max<int64_t>(1, (4 * ctx->num_cus + nrt - 1) / nrt));
+    const int64_t kchunk = ((K + ks - 1) / ks + KC - 1) / KC * KC;
+    ks = (K + kchunk - 1) / kchunk;
+    const size_t slice = (size_t)M * N;
+    int rc = ensure_gemm_work(ctx, sizeof(T) * slice * ks);
+    if (rc) return rc;
+    T* P = (T*)ctx->gemm_work;
+    HFPX_TRY_HIP(ctx, hipMemsetAsync(P, 0, sizeof(T) * slice * ks, ctx->stream));
+    hipLaunchKernelGGL((k_gemm_sub<T, 64>), dim3(grid.x, grid.y, (unsigned)ks), dim3(256), 0, ctx->stream, P, M, A,
+                       lda, B, ldb, M, N, K, kchunk, (int64_t)slice);
+    const int64_t tot = M * N;
+    hipLaunchKernelGGL(k_splitk_sum<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream, C, ldc, P,
+                       M, N, (int)ks);
+    done = true;
+  }
   if (!done) {
     if (skinny)
-      hipLaunchKernelGGL((k_gemm_sub<T, 64>), grid, dim3(256), 0, ctx->stream, C, ldc, A, lda, B, ldb, M, N, K);
+      hipLaunchKernelGGL((k_gemm_sub<T, 64>), grid, dim3(256), 0, ctx->stream, C, ldc, A, lda, B, ldb, M, N, K, K,
+                         (int64_t)0);
     else
-      hipLaunchKernelGGL((k_gemm_sub<T, 128>), grid, dim3(256), 0, ctx->stream, C, ldc, A, lda, B, ldb, M, N, K);
+      hipLaunchKernelGGL((k_gemm_sub<T, 128>), grid, dim3(256), 0, ctx->stream, C, ldc, A, lda, B, ldb, M, N, K, K,
+                         (int64_t)0);
   }
   // algorithmic: 2MNK flops; bytes: C read + written, A and B read once
   prof_end(ctx, tok, sizeof(T) == 8 ? HFPX_KC_GEMM_F64 : HFPX_KC_GEMM_F32, 2.0 * M * N * K,

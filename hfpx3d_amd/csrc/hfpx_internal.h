@@ -23,6 +23,8 @@ struct hfpx_ctx {
   size_t hwork_bytes = 0;
   void* lu_work = nullptr;   // grow-only device workspace of getrf / getrs
   size_t lu_work_bytes = 0;
+  void* gemm_work = nullptr;  // grow-only device workspace of split-K products
+  size_t gemm_work_bytes = 0;
   uint64_t* sync = nullptr;  // grid-barrier arrival counter (device)
   uint64_t sync_base = 0;    // its value once every launched barrier completes
   int num_cus = 0;
@@ -85,6 +87,11 @@ int gemm_sub(hfpx_ctx* ctx, T* C, int64_t ldc, const T* A, int64_t lda, const T*
              int64_t M, int64_t N, int64_t K);
 int gemm_sub_any(hfpx_ctx* ctx, int dtype, void* C, int64_t ldc, const void* A, int64_t lda,
                  const void* B, int64_t ldb, int64_t M, int64_t N, int64_t K);
+
+// FP32 triangular sweeps with block inverses (hfpx_sweep.hip)
+size_t sweep_f32_work_bytes(int64_t n);
+int sweep_f32(hfpx_ctx* ctx, const float* LU, int64_t lda, int64_t n, float* B, int64_t ldb, int64_t nrhs,
+              void* work);
 
 // stress postprocessing (hfpx_stress.hip)
 int launch_stress(hfpx_ctx* ctx, hfpx_mesh* m, double shear_m, double nu, const double* dd, int64_t ldd,

@@ -17,6 +17,7 @@
 // triangular solve for its U rows and one update C -= L21 U12 with K = NB on
 // MFMA (hfpx_gemm.hip).
 #include <algorithm>
+#include <type_traits>
 
 #include "hfpx_internal.h"
 
@@ -605,7 +606,9 @@ static int getrs_impl(hfpx_ctx* ctx, const T* LU, int64_t lda, const int32_t* ip
   // row interchanges on B: pair blocks for all of ipiv, then one workgroup
   // per right-hand side applies them in order
   const int64_t npb = (n + SWAP_BLK - 1) / SWAP_BLK;
-  if ((rc = ensure_lu_work(ctx, sizeof(PermPairs) * (size_t)npb))) return rc;
+  const size_t pairs_bytes = (sizeof(PermPairs) * (size_t)npb + 255) & ~(size_t)255;
+  const bool sweep = std::is_same<T, float>::value;  // FP32: block-inverse sweeps
+  if ((rc = ensure_lu_work(ctx, pairs_bytes + (sweep ? sweep_f32_work_bytes(n) : 0)))) return rc;
   PermPairs* pp = (PermPairs*)ctx->lu_work;
   if ((rc = perm_pairs(ctx, ipiv, 0, n, n, pp))) return rc;
   {
@@ -615,6 +618,8 @@ static int getrs_impl(hfpx_ctx* ctx, const T* LU, int64_t lda, const int32_t* ip
     prof_end(ctx, tok, HFPX_KC_LASWP, 0.0, 0.0);
     HFPX_TRY_HIP(ctx, hipGetLastError());
   }
+  if constexpr (std::is_same<T, float>::value)
+    return sweep_f32(ctx, LU, lda, n, B, ldb, nrhs, (char*)ctx->lu_work + pairs_bytes);
   // forward substitution with unit L, blocked
   for (int64_t k = 0; k < n; k += NB) {
     const int64_t kb = std::min<int64_t>(NB, n - k);

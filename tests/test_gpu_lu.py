@@ -26,7 +26,8 @@ def to_dev(torch, hx, a, dtype):
 
 
 @pytest.mark.parametrize("dt", ["f64", "f32"])
-@pytest.mark.parametrize("M,N,K", [(128, 128, 16), (300, 257, 33), (1000, 64, 256), (77, 5, 3), (513, 700, 100)])
+@pytest.mark.parametrize("M,N,K", [(128, 128, 16), (300, 257, 33), (1000, 64, 256), (77, 5, 3), (513, 700, 100),
+                                   (3000, 64, 20000), (5, 3, 9000)])
 def test_gemm_sub(env, dt, M, N, K):
     torch, hx = env
     print(f"case: gemm {dt} {M}x{N}x{K}", flush=True)
@@ -93,6 +94,25 @@ def test_getrf_f64_matches_lapack(env, n):
     b = rng.normal(size=(n, 3))
     x = lu.solve(to_dev(torch, hx, b, torch.float64)).cpu().numpy()
     assert np.abs(A @ x - b).max() / (np.abs(A).max() * np.abs(x).max() * n) < 1e-15 * 10
+
+
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (127, 3), (128, 64), (129, 65), (1000, 64), (2049, 130)])
+def test_getrs_f32_block_sweeps(env, n, nrhs):
+    """FP32 solve with the GPU's own FP32 factors: the block-inverse sweeps
+    against the same factors applied by LAPACK in float64 (lu_solve), within
+    FP32 rounding times the conditioning of the triangular factors."""
+    torch, hx = env
+    sl = pytest.importorskip("scipy.linalg")
+    print(f"case: getrs f32 n={n} nrhs={nrhs}", flush=True)
+    rng = np.random.default_rng(5 * n + nrhs)
+    A = rng.normal(size=(n, n)) + np.sqrt(n) * np.eye(n)
+    lu = hx.LU(to_dev(torch, hx, A, torch.float32))
+    LU = lu.A.cpu().numpy().astype(np.float64)
+    piv = lu.ipiv.cpu().numpy() - 1
+    b = rng.normal(size=(n, nrhs))
+    x = lu.solve(to_dev(torch, hx, b, torch.float32)).cpu().numpy().astype(np.float64)
+    xr = sl.lu_solve((LU, piv), b.astype(np.float32).astype(np.float64))
+    assert np.abs(x - xr).max() / np.abs(xr).max() < 2e-5
 
 
 @pytest.mark.parametrize("n", [300, 2049])
