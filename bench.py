@@ -39,6 +39,11 @@ def parse():
     ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-el", type=int, default=1400)
+    ap.add_argument("--workload", default="ellipse72k", choices=["ellipse72k", "wavy108k"],
+                    help="ellipse72k: the headline configuration (default); wavy108k: the non-planar "
+                         "6,000-element crack with stresses at 10^6 points")
+    ap.add_argument("--n-pts", type=int, default=1_000_000)
+    ap.add_argument("--fp64-compare", action="store_true", help="wavy108k: also run the plain FP64 path")
     return ap.parse_args()
 
 
@@ -105,6 +110,8 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(multi-GPU sharding is design-only)"}))
         sys.exit(2)
+    if args.workload == "wavy108k":
+        return run_wavy(args)
     import numpy as np
     import torch
     from hfpx3d_amd import hfp3d as hx, _lib
@@ -249,6 +256,149 @@ def main():
                 cb["solve_s_extrapolated_72k"] = round(cb["solve_s_lapack_n4000"] * (n / 4000.0) ** 3, 1)
         except Exception as ex:
             result["cpu_baseline"] = {"error": str(ex)}
+    print(json.dumps(result))
+
+
+def read_profile(ctx, steps):
+    import ctypes
+    from hfpx3d_amd import _lib
+    prof = {}
+    for name, kc in _lib.KERNEL_CLASSES.items():
+        L, ms, fl, by = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        ctx.check(ctx.lib.hfpx_profile_read(ctx.handle, kc, L, ms, fl, by))
+        prof[name] = dict(launches=L.value // steps, ms=round(ms.value / steps, 4), flops=fl.value / steps,
+                          bytes=by.value / steps)
+    return prof
+
+
+def run_wavy(args):
+    """Non-planar crack z = 0.1 sin(2 pi x) sin(2 pi y) over the unit disk,
+    6,000 elements (108,000 DoF), E = 1, nu = 0.3, p = 1 plus s_zz = -0.5
+    (tension-positive s_inf = diag(1, 1, 0.5)); FP32-MFMA LU + FP64 GMRES-IR;
+    stresses at 10^6 points uniform in [-2, 2]^3 (seed 3).  One step =
+    assembly + right-hand side + factorisation + refinement + stresses."""
+    import ctypes
+    import numpy as np
+    import torch
+    from hfpx3d_amd import hfp3d as hx, _lib, meshgen
+    G, nu = 1.0 / 2.6, 0.3
+    nods, conn = meshgen.wavy_disk(6000)
+    n_el = conn.shape[1]
+    n = 18 * n_el
+    s_inf = np.array([1.0, 1.0, 0.5, 0.0, 0.0, 0.0])
+    ctx = hx.get_context(0)
+    lib, ptr = ctx.lib, hx._ptr
+    log(f"bench: wavy crack n_el={n_el} n={n} points={args.n_pts}")
+    dm = hx.DeviceMesh(hx.Mesh_Geom_T(nods=nods, conn=conn), None, 0.125, ctx)
+    A64, lda = hx.alloc_colmajor(n, n, torch.float64, ctx)
+    A32, lda32 = hx.alloc_colmajor(n, n, torch.float32, ctx)
+    B, _ = hx.alloc_colmajor(n, 1, torch.float64, ctx)
+    X, _ = hx.alloc_colmajor(n, 1, torch.float64, ctx)
+    pts_h = np.asfortranarray(np.random.default_rng(3).uniform(-2, 2, size=(3, args.n_pts)))
+    with torch.cuda.stream(ctx.stream):
+        ipiv = torch.empty(n, dtype=torch.int32, device=ctx.device)
+        pts = torch.from_numpy(pts_h.T.copy()).to(ctx.device).t()  # 3 x P, column-major
+        S = torch.empty((6, args.n_pts), dtype=torch.float64, device=ctx.device).t()  # P x 6, column-major
+        dd = torch.zeros((3, 6 * n_el), dtype=torch.float64, device=ctx.device).t()
+    s_f = np.asfortranarray(s_inf.reshape(6, 1))
+    rep = _lib.IRReport()
+    info = ctypes.c_int64(0)
+    ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+
+    def step(rec=None):
+        e = [ev() for _ in range(5)]
+        e[0].record(ctx.stream)
+        ctx.check(lib.hfpx_assemble_mixed(ctx.handle, dm.handle, G, nu, ptr(A64), lda, ptr(A32), lda32))
+        with torch.cuda.stream(ctx.stream):
+            B.zero_()
+        ctx.check(lib.hfpx_rhs_sinf(ctx.handle, dm.handle, hx._np_ptr(s_f), 1, ptr(B), n))
+        e[1].record(ctx.stream)
+        ctx.check(lib.hfpx_getrf(ctx.handle, _lib.HFPX_F32, ptr(A32), lda32, n, ptr(ipiv), ctypes.byref(info)))
+        rc = lib.hfpx_solve_ir(ctx.handle, ptr(A64), lda, ptr(A32), lda32, ptr(ipiv), n, ptr(B), n, ptr(X), n, 1,
+                               1e-15, 10, 20, ctypes.byref(rep))
+        if rc not in (_lib.HFPX_OK, _lib.HFPX_ERR_NOT_CONVERGED):
+            ctx.check(rc)
+        e[2].record(ctx.stream)
+        # identity DoF map: the solution vector is the nodal DD array row by row
+        with torch.cuda.stream(ctx.stream):
+            dd.copy_(X.reshape(6 * n_el, 3))
+        e[3].record(ctx.stream)
+        ctx.check(lib.hfpx_stress(ctx.handle, dm.handle, G, nu, ptr(dd), 6 * n_el, ptr(pts), args.n_pts,
+                                  hx._np_ptr(s_inf), ptr(S), args.n_pts))
+        e[4].record(ctx.stream)
+        if rec is not None:
+            rec.append((e, rep.iterations, rep.inner_iterations, rep.backward_error, bool(rep.converged)))
+
+    for w in range(args.warmup):
+        step()
+        ctx.stream.synchronize()
+        log(f"bench: warmup {w + 1}/{args.warmup} done")
+    ctx.check(lib.hfpx_profile(ctx.handle, 1))
+    recs = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(recs)
+        ctx.stream.synchronize()
+        log(f"bench: step {k + 1}/{args.steps} done")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    prof = read_profile(ctx, args.steps)
+    ctx.check(lib.hfpx_profile(ctx.handle, 0))
+    t_asm = np.mean([r[0][0].elapsed_time(r[0][1]) for r in recs]) / 1e3
+    t_solve = np.mean([r[0][1].elapsed_time(r[0][2]) for r in recs]) / 1e3
+    t_str = np.mean([r[0][3].elapsed_time(r[0][4]) for r in recs]) / 1e3
+    last = recs[-1]
+    pairs = float(args.n_pts) * n_el
+    log(f"bench: assembly {t_asm * 1e3:.1f} ms, solve {t_solve:.3f} s, stress {t_str:.3f} s")
+    st = prof["stress"]
+    result = {
+        "metric": "wavy 108k: assembly Gentries/s; solve s; stress points/s",
+        "value": round(n * n / t_asm / 1e9, 3), "unit": "Gentries/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 assembly and stress; f32 LU + f64 refinement", "data": "synthetic",
+        "config": {"workload": f"wavy crack z=0.1 sin(2 pi x) sin(2 pi y) over the unit disk, {n_el} quadratic "
+                               f"triangles, {n} DoF, p=1 + s_zz=-0.5, {args.n_pts} stress points", "n_dof": n},
+        "assembly_s": round(t_asm, 5), "solve_s": round(t_solve, 4),
+        "ir": {"outer": last[1], "inner": last[2], "backward_error": last[3], "converged": last[4]},
+        "stress_s": round(t_str, 4), "stress_points_per_s": round(args.n_pts / t_str, 1),
+        "stress_pair_evals_per_s": round(pairs / t_str, 1),
+        "kernel_profile_per_step": prof,
+    }
+    if args.fp64_compare:
+        log("bench: FP64 comparison path")
+        with torch.cuda.stream(ctx.stream):
+            X64 = B.clone()
+        del A32
+        torch.cuda.empty_cache()
+        e0, e1 = ev(), ev()
+        e0.record(ctx.stream)
+        ctx.check(lib.hfpx_getrf(ctx.handle, _lib.HFPX_F64, ptr(A64), lda, n, ptr(ipiv), ctypes.byref(info)))
+        ctx.check(lib.hfpx_getrs(ctx.handle, _lib.HFPX_F64, ptr(A64), lda, ptr(ipiv), n, ptr(X64), n, 1))
+        e1.record(ctx.stream)
+        ctx.stream.synchronize()
+        with torch.cuda.stream(ctx.stream):
+            err = ((X - X64).abs().max() / X64.abs().max()).item()
+        result["fp64_path"] = {"solve_s": round(e0.elapsed_time(e1) / 1e3, 4), "max_rel_err_mixed_vs_fp64": err}
+        log(f"bench: FP64 path {result['fp64_path']}")
+    if not args.no_cpu_baseline:
+        log("bench: CPU baseline (reference stress on a point subset)")
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        ref = oracle_lib.reference_or_none()
+        sub = 2000
+        dd_h = dd.cpu().numpy()
+        threads = max(1, min(16, os.cpu_count() or 1))
+        t = time.perf_counter()
+        if ref is not None:
+            ref.stress(nods, conn, G, nu, dd_h, pts_h[:, :sub].copy(), s_inf, nthreads=threads)
+        else:
+            oracle_lib.oracle().stress(nods, conn, G, nu, dd_h, pts_h[:, :sub].copy(), s_inf, nthreads=threads)
+        dt = time.perf_counter() - t
+        result["cpu_baseline"] = {"value": round(sub / dt, 1), "unit": "stress points/s", "cores": threads,
+                                  "kind": "reference" if ref is not None else "port",
+                                  "sample": f"{sub} of the points against all {n_el} elements, {dt:.1f} s"}
     print(json.dumps(result))
 
 

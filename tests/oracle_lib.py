@@ -198,14 +198,18 @@ class Reference:
                   _p(s), _p(r))
         return r
 
-    def stress(self, nods, conn, G, nu, dd, pts, s_inf):
+    def stress(self, nods, conn, G, nu, dd, pts, s_inf, nthreads=1):
+        """The reference accumulates stresses from several OpenMP threads into
+        shared memory without synchronisation (system_assembly.cpp:415-417):
+        values are only reliable with nthreads = 1; more threads are for
+        timing."""
         nods, conn = _f(nods), _f(conn, np.int64)
         dd, pts = _f(dd), _f(pts)
         n_pts = pts.shape[1]
         out = np.zeros(6 * n_pts)
         s = np.ascontiguousarray(s_inf, dtype=np.float64)
         self._stress(_p(nods), nods.shape[0], nods.shape[1], _p(conn), conn.shape[0], conn.shape[1], G, nu,
-                     _p(dd), _p(pts), n_pts, _p(s), 1, _p(out))
+                     _p(dd), _p(pts), n_pts, _p(s), nthreads, _p(out))
         return out.reshape(n_pts, 6, order="F")
 
 
