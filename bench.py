@@ -44,6 +44,10 @@ def parse():
                          "6,000-element crack with stresses at 10^6 points")
     ap.add_argument("--n-pts", type=int, default=1_000_000)
     ap.add_argument("--fp64-compare", action="store_true", help="wavy108k: also run the plain FP64 path")
+    ap.add_argument("--no-lib-prof", action="store_true",
+                    help="do not time kernel launches with HIP events (for runs under an external profiler)")
+    ap.add_argument("--tol", type=float, default=1e-14,
+                    help="normwise backward error at which the refinement stops")
     return ap.parse_args()
 
 
@@ -151,7 +155,7 @@ def main():
         rc = lib.hfpx_getrf(ctx.handle, _lib.HFPX_F32, ptr(A32), lda32, n, ptr(ipiv), __import__("ctypes").byref(info))
         ctx.check(rc)
         rc = lib.hfpx_solve_ir(ctx.handle, ptr(A64), lda, ptr(A32), lda32, ptr(ipiv), n, ptr(B), ldb, ptr(X), ldx,
-                               args.nrhs, 1e-15, 10, 20, __import__("ctypes").byref(rep))
+                               args.nrhs, args.tol, 10, 20, __import__("ctypes").byref(rep))
         if rc not in (_lib.HFPX_OK, _lib.HFPX_ERR_NOT_CONVERGED):
             ctx.check(rc)
         e2.record(ctx.stream)
@@ -163,7 +167,7 @@ def main():
         ctx.stream.synchronize()
         log(f"bench: warmup {w + 1}/{args.warmup} done")
 
-    ctx.check(lib.hfpx_profile(ctx.handle, 1))
+    ctx.check(lib.hfpx_profile(ctx.handle, 0 if args.no_lib_prof else 1))
     recs = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -315,7 +319,7 @@ def run_wavy(args):
         e[1].record(ctx.stream)
         ctx.check(lib.hfpx_getrf(ctx.handle, _lib.HFPX_F32, ptr(A32), lda32, n, ptr(ipiv), ctypes.byref(info)))
         rc = lib.hfpx_solve_ir(ctx.handle, ptr(A64), lda, ptr(A32), lda32, ptr(ipiv), n, ptr(B), n, ptr(X), n, 1,
-                               1e-15, 10, 20, ctypes.byref(rep))
+                               args.tol, 10, 20, ctypes.byref(rep))
         if rc not in (_lib.HFPX_OK, _lib.HFPX_ERR_NOT_CONVERGED):
             ctx.check(rc)
         e[2].record(ctx.stream)
@@ -333,7 +337,7 @@ def run_wavy(args):
         step()
         ctx.stream.synchronize()
         log(f"bench: warmup {w + 1}/{args.warmup} done")
-    ctx.check(lib.hfpx_profile(ctx.handle, 1))
+    ctx.check(lib.hfpx_profile(ctx.handle, 0 if args.no_lib_prof else 1))
     recs = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()

@@ -330,6 +330,91 @@ __global__ __launch_bounds__(256) void k_sgemm32(float* __restrict__ C, int64_t 
     }
 }
 
+// ---------------------------------------------------------------------
+// FP64 P_z = -A(:, slice z) X(slice z, :) for N <= 64 and long K (the
+// residuals b - A x of the mixed-precision solve): A is streamed once, so
+// every lane loads its MFMA operands straight from HBM into registers (no
+// LDS), while the slice of X is staged through LDS for all four waves.
+// Wave w owns rows 32 w .. 32 w + 31 of the workgroup's 128; columns are
+// NB blocks of 16 (N rounded up), so even a single right-hand side keeps the
+// kernel bound by HBM rather than by v_mfma_f64_16x16x4f64.
+// Operand layout: lane l supplies A[i = l%16][k = l/16] and X[k = l/16][j =
+// l%16]; result element v of lane l is D[4v + l/16][l%16].
+// ---------------------------------------------------------------------
+template <int NB>
+__global__ __launch_bounds__(256) void k_dgemm_stream(double* __restrict__ P, const double* __restrict__ A,
+                                                      int64_t lda, const double* __restrict__ X, int64_t ldx,
+                                                      int64_t M, int64_t N, int64_t K, int64_t kslice) {
+  constexpr int KC2 = 16;
+  constexpr int XQ = KC2 * NB * 16 / 256;  // X elements staged per thread per chunk
+  __shared__ double sX[2][KC2][NB * 16 + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, l16 = lane & 15, lq = lane >> 4;
+  const int64_t row0 = (int64_t)blockIdx.x * 128 + 32 * wave;
+  const int64_t z = blockIdx.y, kb = z * kslice;
+  const int64_t ke = (K - kb) < kslice ? K : kb + kslice;
+  doublex4 acc[2][NB];
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < NB; ++cb) acc[rb][cb] = doublex4{0, 0, 0, 0};
+  const int64_t r_a = row0 + l16, r_b = row0 + 16 + l16;
+  const bool ok_a = r_a < M, ok_b = r_b < M;
+  // next chunk's operands, fetched one chunk ahead
+  double an[2][4], xn[XQ];
+  auto fetch = [&](int64_t k0) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int64_t k = k0 + 4 * s + lq;
+      const bool kin = k < ke;
+      an[0][s] = (ok_a && kin) ? A[r_a + k * lda] : 0.0;
+      an[1][s] = (ok_b && kin) ? A[r_b + k * lda] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < XQ; ++q) {
+      const int e = tid + 256 * q, kk = e % KC2, j = e / KC2;
+      const int64_t k = k0 + kk;
+      xn[q] = (k < ke && j < N) ? X[k + j * ldx] : 0.0;
+    }
+  };
+  fetch(kb);
+  int buf = 0;
+  for (int64_t k0 = kb; k0 < ke; k0 += KC2, buf ^= 1) {
+    double av[2][4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      av[0][s] = an[0][s];
+      av[1][s] = an[1][s];
+    }
+#pragma unroll
+    for (int q = 0; q < XQ; ++q) {
+      const int e = tid + 256 * q;
+      sX[buf][e % KC2][e / KC2] = xn[q];
+    }
+    __syncthreads();  // one barrier per chunk: sX is double-buffered
+    if (k0 + KC2 < ke) fetch(k0 + KC2);
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int cb = 0; cb < NB; ++cb) {
+        const double xv = sX[buf][4 * s + lq][16 * cb + l16];
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+          acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[rb][s], xv, acc[rb][cb], 0, 0, 0);
+      }
+  }
+  double* Pz = P + z * M * N;
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < NB; ++cb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t r = row0 + 16 * rb + 4 * v + lq;
+        const int64_t c = 16 * cb + l16;
+        if (r < M && c < N) Pz[r + c * M] = -acc[rb][cb][v];
+      }
+}
+
 // C (M x N, ldc) += sum over slices z = 0 .. ks-1 of P_z (M x N, ld M),
 // in slice order (deterministic).
 template <typename T>
@@ -415,6 +500,33 @@ int gemm_sub(hfpx_ctx* ctx, T* C, int64_t ldc, const T* A, int64_t lda, const T*
         case 5: launch_sgemm32<2, 2, 16, SG_INT | SG_PIPE | SG_SWZ>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
         default: launch_sgemm32<2, 2, 16, SG_INT | SG_SWZ>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
       }
+      done = true;
+    }
+  }
+  if constexpr (sizeof(T) == 8) {
+    if (!done && skinny && K >= SPLITK_MIN_K) {
+      // stream A once: 128-row tiles x K slices, partials summed in order
+      const int64_t nrt = (M + 127) / 128;
+      int64_t ks = std::min<int64_t>(SPLITK_MAX, std::max<int64_t>(1, (8 * ctx->num_cus + nrt - 1) / nrt));
+      const int64_t kslice = ((K + ks - 1) / ks + 15) / 16 * 16;
+      ks = (K + kslice - 1) / kslice;
+      const size_t slice = (size_t)M * N;
+      int rc = ensure_gemm_work(ctx, sizeof(T) * slice * ks);
+      if (rc) return rc;
+      double* P = (double*)ctx->gemm_work;
+      const dim3 g((unsigned)nrt, (unsigned)ks);
+      if (N <= 16)
+        hipLaunchKernelGGL(k_dgemm_stream<1>, g, dim3(256), 0, ctx->stream, P, (const double*)A, lda,
+                           (const double*)B, ldb, M, N, K, kslice);
+      else if (N <= 32)
+        hipLaunchKernelGGL(k_dgemm_stream<2>, g, dim3(256), 0, ctx->stream, P, (const double*)A, lda,
+                           (const double*)B, ldb, M, N, K, kslice);
+      else
+        hipLaunchKernelGGL(k_dgemm_stream<4>, g, dim3(256), 0, ctx->stream, P, (const double*)A, lda,
+                           (const double*)B, ldb, M, N, K, kslice);
+      const int64_t tot = M * N;
+      hipLaunchKernelGGL(k_splitk_sum<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream,
+                         (double*)C, ldc, P, M, N, (int)ks);
       done = true;
     }
   }

@@ -83,12 +83,24 @@ __global__ void k_scale(const double* __restrict__ a, double* __restrict__ x, in
   const int64_t j = blockIdx.y;
   if (i < n) x[i + j * ldx] *= a[j];
 }
-// row sums of |A| -> part[i]; then max
-__global__ void k_rowabs(const double* __restrict__ A, int64_t lda, int64_t n, double* __restrict__ rs) {
+// row sums of |A|: column slice blockIdx.y of every row -> part[y n + i]
+// (consecutive threads read consecutive rows of a column), then the slices
+// added in order -> rs[i]
+__global__ void k_rowabs(const double* __restrict__ A, int64_t lda, int64_t n, int64_t cw,
+                         double* __restrict__ part) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t c0 = (int64_t)blockIdx.y * cw, c1 = (c0 + cw < n) ? c0 + cw : n;
+  double s = 0.0;
+  for (int64_t c = c0; c < c1; ++c) s += fabs(A[i + c * lda]);
+  part[blockIdx.y * n + i] = s;
+}
+
+__global__ void k_rowabs_sum(const double* __restrict__ part, int64_t n, int ns, double* __restrict__ rs) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double s = 0.0;
-  for (int64_t c = 0; c < n; ++c) s += fabs(A[i + c * lda]);
+  for (int z = 0; z < ns; ++z) s += part[z * n + i];
   rs[i] = s;
 }
 
@@ -176,12 +188,18 @@ int solve_ir(hfpx_ctx* ctx, const double* A, int64_t lda, const float* LU, int64
   w.col2 = (double*)take(sizeof(double) * k);
 
   // |A|_inf (rows sums) -> part (n doubles fit: nb*k >= n/256*k ... use V as scratch)
-  hipLaunchKernelGGL(k_rowabs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, A, lda, n, w.V);
+  // V holds (m + 1) k n doubles: ns slices of partial row sums + the sums
+  const int ns = (int)std::max<int64_t>(1, std::min<int64_t>(32, (int64_t)(m + 1) * k - 1));
+  const int64_t cw = (n + ns - 1) / ns;
+  hipLaunchKernelGGL(k_rowabs, dim3((unsigned)((n + 255) / 256), (unsigned)ns), dim3(256), 0, ctx->stream, A, lda,
+                     n, cw, w.V);
+  hipLaunchKernelGGL(k_rowabs_sum, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, w.V, n, ns,
+                     w.V + (size_t)ns * n);
   HFPX_TRY_HIP(ctx, hipGetLastError());
   double anorm = 0.0;
   {
     double h;
-    if ((rc = colred(ctx, w, w.V, n, nullptr, 0, n, 1, 1, &h))) return rc;
+    if ((rc = colred(ctx, w, w.V + (size_t)ns * n, n, nullptr, 0, n, 1, 1, &h))) return rc;
     anorm = h;
   }
   std::vector<double> bnorm(k), xnorm(k), rnorm(k), omega(k), beta(k), tmp(k);

@@ -27,7 +27,7 @@ def to_dev(torch, hx, a, dtype):
 
 @pytest.mark.parametrize("dt", ["f64", "f32"])
 @pytest.mark.parametrize("M,N,K", [(128, 128, 16), (300, 257, 33), (1000, 64, 256), (77, 5, 3), (513, 700, 100),
-                                   (3000, 64, 20000), (5, 3, 9000)])
+                                   (3000, 64, 20000), (5, 3, 9000), (1000, 20, 10000), (2000, 1, 12000)])
 def test_gemm_sub(env, dt, M, N, K):
     torch, hx = env
     print(f"case: gemm {dt} {M}x{N}x{K}", flush=True)
