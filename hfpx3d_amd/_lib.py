@@ -70,6 +70,15 @@ SIGNATURES = {
     "hfpx_gemm_sub": (_c_int, [_c_void_p, _c_int, _c_void_p, _c_i64, _c_void_p, _c_i64, _c_void_p, _c_i64,
                                _c_i64, _c_i64, _c_i64]),
     "hfpx_tune_set": (_c_int, [_c_void_p, _c_int, _c_int]),
+    "hfpx_batch_create": (_c_int, [_c_void_p, _c_int, _c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_double,
+                                   _c_void_p, _c_void_p, _c_void_p, _c_i64, ctypes.POINTER(_c_void_p)]),
+    "hfpx_batch_destroy": (_c_int, [_c_void_p]),
+    "hfpx_batch_assemble": (_c_int, [_c_void_p, _c_void_p]),
+    "hfpx_batch_solve": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int]),
+    "hfpx_batch_solution": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_void_p, ctypes.POINTER(_c_i64),
+                                     ctypes.POINTER(_c_double), ctypes.POINTER(_c_i64)]),
+    "hfpx_batch_rhs": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_void_p]),
+    "hfpx_batch_matrix": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_void_p, ctypes.POINTER(_c_i64)]),
     "hfpx_tune_get": (_c_int, [_c_void_p, _c_int, ctypes.POINTER(_c_int)]),
 }
 

@@ -117,21 +117,25 @@ struct StoreBoth {
   }
 };
 
+// One wavefront's share of the matrix: collocation points 64 bx .. 64 bx + 63
+// of a mesh of n_el elements against source element 4 by + wave.  cp holds
+// the six coordinate planes of the points with plane stride cps (>= 6 n_el;
+// larger when the mesh is one crack of a batch).
 template <typename Store>
-__global__ __launch_bounds__(256) void k_assemble(const SrcElem<double>* __restrict__ elem,
-                                                  const double* __restrict__ cp, int64_t n_el,
-                                                  const int32_t* __restrict__ dof, double shear_m,
-                                                  double nu, Store store) {
+__device__ __forceinline__ void assemble_wave(const SrcElem<double>* __restrict__ elem,
+                                              const double* __restrict__ cp, int64_t cps, int64_t n_el,
+                                              const int32_t* __restrict__ dof, double shear_m, double nu,
+                                              const Store& store, int64_t bx, int64_t by) {
   const int lane = threadIdx.x & 63;
-  const int64_t es = __builtin_amdgcn_readfirstlane((int)(blockIdx.y * 4 + (threadIdx.x >> 6)));
+  const int64_t es = __builtin_amdgcn_readfirstlane((int)(by * 4 + (threadIdx.x >> 6)));
   if (es >= n_el) return;
   const int64_t ncp = 6 * n_el;
-  const int64_t icp = (int64_t)blockIdx.x * 64 + lane;
+  const int64_t icp = bx * 64 + lane;
   if (icp >= ncp) return;
   const SrcElem<double>& E = elem[es];
 
-  const double xg[3] = {cp[icp], cp[ncp + icp], cp[2 * ncp + icp]};
-  const double ng[3] = {cp[3 * ncp + icp], cp[4 * ncp + icp], cp[5 * ncp + icp]};
+  const double xg[3] = {cp[icp], cp[cps + icp], cp[2 * cps + icp]};
+  const double ng[3] = {cp[3 * cps + icp], cp[4 * cps + icp], cp[5 * cps + icp]};
   const double d[3] = {xg[0] - E.v0[0], xg[1] - E.v0[1], xg[2] - E.v0[2]};
   const double y0 = E.R[0] * d[0] + E.R[1] * d[1] + E.R[2] * d[2];
   const double y1 = E.R[3] * d[0] + E.R[4] * d[1] + E.R[5] * d[2];
@@ -171,6 +175,63 @@ __global__ __launch_bounds__(256) void k_assemble(const SrcElem<double>* __restr
       }
     }
   });
+}
+
+template <typename Store>
+__global__ __launch_bounds__(256) void k_assemble(const SrcElem<double>* __restrict__ elem,
+                                                  const double* __restrict__ cp, int64_t n_el,
+                                                  const int32_t* __restrict__ dof, double shear_m,
+                                                  double nu, Store store) {
+  assemble_wave(elem, cp, 6 * n_el, n_el, dof, shear_m, nu, store, blockIdx.x, blockIdx.y);
+}
+
+// ---------------------------------------------------------------------
+// batched assembly: workgroup w belongs to the crack c with
+// cr[c].wg_off <= w < cr[c + 1].wg_off (binary search; the table is in
+// workgroup order) and does that crack's tile (local % gx, local / gx) of
+// the same decomposition as k_assemble.  FP64 matrix always, FP32 copy for
+// the cracks that have one.
+// ---------------------------------------------------------------------
+__device__ __forceinline__ int find_crack(const BCrack* __restrict__ cr, int ncr, int64_t w, bool by_el) {
+  int lo = 0, hi = ncr - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    const int64_t start = by_el ? cr[mid].el_off : cr[mid].wg_off;
+    if (start <= w)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void k_bassemble(const SrcElem<double>* __restrict__ elem,
+                                                   const double* __restrict__ cp, int64_t cps,
+                                                   const BCrack* __restrict__ cr, int ncr, double* __restrict__ a64,
+                                                   float* __restrict__ a32) {
+  const int64_t w = blockIdx.x;
+  const BCrack c = cr[find_crack(cr, ncr, w, false)];
+  const int64_t gx = (6 * c.n_el + 63) / 64;
+  const int64_t local = w - c.wg_off;
+  StoreBoth st{a64 + c.a_off, c.n, c.a32_off >= 0 ? a32 + c.a32_off : nullptr, c.n};
+  assemble_wave(elem + c.el_off, cp + 6 * c.el_off, cps, c.n_el, nullptr, c.G, c.nu, st, local % gx, local / gx);
+}
+
+// batched right-hand sides: one thread per element of the batch,
+// b[x_off + DoF] = -(traction of the crack's far-field stress); every DoF is
+// written exactly once (all DoF free)
+__global__ void k_brhs(const SrcElem<double>* __restrict__ elem, int64_t n_el_total, const BCrack* __restrict__ cr,
+                       int ncr, double* __restrict__ b) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_el_total) return;
+  const BCrack& c = cr[find_crack(cr, ncr, e, true)];
+  const double* s = c.sinf;
+  const double n[3] = {-elem[e].R[6], -elem[e].R[7], -elem[e].R[8]};
+  const double tr[3] = {n[0] * s[0] + n[1] * s[3] + n[2] * s[4], n[0] * s[3] + n[1] * s[1] + n[2] * s[5],
+                        n[0] * s[4] + n[1] * s[5] + n[2] * s[2]};
+  double* bc = b + c.x_off + 18 * (e - c.el_off);
+  for (int ln = 0; ln < 6; ++ln)
+    for (int l = 0; l < 3; ++l) bc[3 * ln + l] = -tr[l];
 }
 
 // ---------------------------------------------------------------------
@@ -229,6 +290,27 @@ int launch_rhs(hfpx_ctx* ctx, const hfpx_mesh* m, const double* sinf_dev, int nr
   const int64_t nt = m->n_el * nrhs;
   hipLaunchKernelGGL(k_rhs, dim3((unsigned)((nt + bs - 1) / bs)), dim3(bs), 0, ctx->stream,
                      m->elem, m->n_el, m->identity_dof ? nullptr : m->dof, sinf_dev, nrhs, b, ldb);
+  HFPX_TRY_HIP(ctx, hipGetLastError());
+  return HFPX_OK;
+}
+
+}  // namespace hfpx
+
+namespace hfpx {
+
+int launch_bassemble(hfpx_ctx* ctx, const SrcElem<double>* elem, const double* cp, int64_t cps, const BCrack* cr,
+                     int ncr, int64_t total_wg, double* a64, float* a32) {
+  if (total_wg <= 0) return HFPX_OK;
+  hipLaunchKernelGGL(k_bassemble, dim3((unsigned)total_wg), dim3(256), 0, ctx->stream, elem, cp, cps, cr, ncr, a64,
+                     a32);
+  HFPX_TRY_HIP(ctx, hipGetLastError());
+  return HFPX_OK;
+}
+
+int launch_brhs(hfpx_ctx* ctx, const SrcElem<double>* elem, int64_t n_el_total, const BCrack* cr, int ncr,
+                double* b) {
+  hipLaunchKernelGGL(k_brhs, dim3((unsigned)((n_el_total + 127) / 128)), dim3(128), 0, ctx->stream, elem,
+                     n_el_total, cr, ncr, b);
   HFPX_TRY_HIP(ctx, hipGetLastError());
   return HFPX_OK;
 }

@@ -49,28 +49,18 @@ constexpr int KC = 16;   // K chunk
 
 // GTN = tile columns (N): 128, or 64 for skinny products (many right-hand
 // sides), where a 128-wide tile would leave half of every MFMA idle.
+// One 128 x GTN tile of C at (row0, col0).
 template <typename T, int GTN>
-__global__ __launch_bounds__(256) void k_gemm_sub(T* __restrict__ C, int64_t ldc,
-                                                  const T* __restrict__ A, int64_t lda,
-                                                  const T* __restrict__ B, int64_t ldb, int64_t M,
-                                                  int64_t N, int64_t K, int64_t kchunk, int64_t cz) {
+__device__ __forceinline__ void gemm_tile(T* __restrict__ C, int64_t ldc, const T* __restrict__ A, int64_t lda,
+                                          const T* __restrict__ B, int64_t ldb, int64_t M, int64_t N, int64_t K,
+                                          int64_t row0, int64_t col0) {
   typedef typename MfmaVec<T>::type vec4;
-  // split K: slice z = blockIdx.z covers k in [z kchunk, (z+1) kchunk) and
-  // writes C + z cz (cz = 0 and one slice: the plain product)
-  {
-    const int64_t z = blockIdx.z, kz = z * kchunk;
-    A += kz * lda;
-    B += kz;
-    C += z * cz;
-    K = (K - kz) < kchunk ? (K - kz) : kchunk;
-  }
   constexpr int WN = GTN / 2;      // wave tile columns
   constexpr int TC = WN / 16;      // 16x16 MFMA tiles across a wave's columns
   constexpr int BQ = KC * GTN / 256;  // B elements loaded per thread per chunk
   __shared__ T sA[2][KC][GT];      // sA[k][i] = A(row0 + i, k0 + k)
   __shared__ T sB[2][KC][GTN];     // sB[k][j] = B(k0 + k, col0 + j)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int64_t row0 = (int64_t)blockIdx.x * GT, col0 = (int64_t)blockIdx.y * GTN;
   const int wr = (wave & 1) * 64, wc = (wave >> 1) * WN;  // wave's 64 x WN sub-tile
 
   vec4 acc[TC][4];
@@ -143,6 +133,40 @@ __global__ __launch_bounds__(256) void k_gemm_sub(T* __restrict__ C, int64_t ldc
         const int64_t cc = col0 + wc + 16 * tc + MfmaVec<T>::drow(lane, v);
         if (r < M && cc < N) C[r + cc * ldc] -= acc[tc][tr][v];
       }
+}
+
+template <typename T, int GTN>
+__global__ __launch_bounds__(256) void k_gemm_sub(T* __restrict__ C, int64_t ldc,
+                                                  const T* __restrict__ A, int64_t lda,
+                                                  const T* __restrict__ B, int64_t ldb, int64_t M,
+                                                  int64_t N, int64_t K, int64_t kchunk, int64_t cz) {
+  // split K: slice z = blockIdx.z covers k in [z kchunk, (z+1) kchunk) and
+  // writes C + z cz (cz = 0 and one slice: the plain product)
+  const int64_t z = blockIdx.z, kz = z * kchunk;
+  gemm_tile<T, GTN>(C + z * cz, ldc, A + kz * lda, lda, B + kz, ldb, M, N, (K - kz) < kchunk ? (K - kz) : kchunk,
+                    (int64_t)blockIdx.x * GT, (int64_t)blockIdx.y * GTN);
+}
+
+// Batched Schur-complement update of step (k, j) of the batched LU: crack
+// list[blockIdx.z] (FP64 matrix at a_off, or its FP32 copy at a32_off);
+// tiles outside that crack's product exit at once.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bgemm(T* __restrict__ base, int which, const BCrack* __restrict__ cr,
+                                               const int32_t* __restrict__ list, int64_t k, int64_t j, int outer) {
+  const BCrack& c = cr[list[blockIdx.z]];
+  const int64_t n = c.n;
+  int64_t kb, jb;
+  if (!bstep(n, k, outer ? k : j, kb, jb)) return;
+  T* a = base + bcopy_off(c, which);
+  int64_t r0, c0, p0, M, N, K;
+  if (outer) {
+    r0 = k + kb; c0 = k + kb; p0 = k; M = n - r0; N = n - c0; K = kb;
+  } else {
+    r0 = j + jb; c0 = j + jb; p0 = j; M = n - r0; N = k + kb - c0; K = jb;
+  }
+  const int64_t row0 = (int64_t)blockIdx.x * GT, col0 = (int64_t)blockIdx.y * 128;
+  if (M <= 0 || N <= 0 || row0 >= M || col0 >= N) return;
+  gemm_tile<T, 128>(a + r0 + c0 * n, n, a + r0 + p0 * n, n, a + p0 + c0 * n, n, M, N, K, row0, col0);
 }
 
 // ---------------------------------------------------------------------
@@ -576,5 +600,29 @@ int gemm_sub_any(hfpx_ctx* ctx, int dtype, void* C, int64_t ldc, const void* A, 
     return gemm_sub<double>(ctx, (double*)C, ldc, (const double*)A, lda, (const double*)B, ldb, M, N, K);
   return gemm_sub<float>(ctx, (float*)C, ldc, (const float*)A, lda, (const float*)B, ldb, M, N, K);
 }
+
+}  // namespace hfpx
+
+namespace hfpx {
+
+template <typename T>
+int launch_bgemm(hfpx_ctx* ctx, T* base, int which, const BCrack* cr, const int32_t* list, int nact,
+                 int64_t nmax, int64_t k, int64_t j, bool outer) {
+  if (nact <= 0) return HFPX_OK;
+  // the largest crack bounds every crack's product
+  const int64_t M = outer ? nmax - k : nmax - j, N = outer ? nmax - k : BNB;
+  if (M <= 0) return HFPX_OK;
+  const dim3 grid((unsigned)((M + GT - 1) / GT), (unsigned)((N + 127) / 128), (unsigned)nact);
+  const int tok = prof_begin(ctx);
+  hipLaunchKernelGGL(k_bgemm<T>, grid, dim3(256), 0, ctx->stream, base, which, cr, list, k, j, outer ? 1 : 0);
+  prof_end(ctx, tok, sizeof(T) == 8 ? HFPX_KC_GEMM_F64 : HFPX_KC_GEMM_F32, 0.0, 0.0);
+  HFPX_TRY_HIP(ctx, hipGetLastError());
+  return HFPX_OK;
+}
+
+template int launch_bgemm<double>(hfpx_ctx*, double*, int, const BCrack*, const int32_t*, int, int64_t, int64_t,
+                                  int64_t, bool);
+template int launch_bgemm<float>(hfpx_ctx*, float*, int, const BCrack*, const int32_t*, int, int64_t, int64_t,
+                                 int64_t, bool);
 
 }  // namespace hfpx

@@ -58,6 +58,43 @@ struct hfpx_mesh {
 
 namespace hfpx {
 
+// ---- batches of independent cracks --------------------------------------
+// Blocking of the batched LU: outer blocks of BNB columns, panels of BIB.
+constexpr int BNB = 128;
+constexpr int BIB = 32;
+
+// One crack of a batch (device table).  All of a crack's DoF are free
+// (fix_tip = 0): DoF 18 e + 3 node + component of the crack's own elements.
+struct BCrack {
+  int64_t el_off;   // first element in the batch's element table
+  int64_t n_el;
+  int64_t n;        // DoF = 18 n_el
+  int64_t a_off;    // first entry of the crack's FP64 n x n matrix (ld n)
+  int64_t a32_off;  // first entry of its FP32 copy, or -1 (solved in FP64)
+  int64_t f_off;    // first entry of its FP64 working copy (the FP64 LU), or -1
+  int64_t x_off;    // first entry of its n-vectors (rhs, solution, pivots)
+  int64_t wg_off;   // first workgroup of the crack in the batched assembly grid
+  double G, nu;
+  double sinf[6];   // far-field stress (xx yy zz xy xz yz), tension positive
+};
+
+// Column ranges of one step of the batched LU for a crack of n DoF: outer
+// block [k, k + kb), panel [j, j + jb) inside it; false if the crack has no
+// work in that step.
+__host__ __device__ inline bool bstep(int64_t n, int64_t k, int64_t j, int64_t& kb, int64_t& jb) {
+  if (k >= n) return false;
+  kb = (n - k) < BNB ? (n - k) : BNB;
+  if (j >= k + kb) return false;
+  jb = (k + kb - j) < BIB ? (k + kb - j) : BIB;
+  return true;
+}
+
+// Which copy of a crack's matrix a batched kernel works on.
+enum BWhich { B_A64 = 0, B_A32 = 1, B_F64 = 2 };
+__host__ __device__ inline int64_t bcopy_off(const BCrack& c, int which) {
+  return which == B_A32 ? c.a32_off : (which == B_F64 ? c.f_off : c.a_off);
+}
+
 // Profiling bracket around one launch: prof_begin before the launch,
 // prof_end after it (no-ops unless profiling is enabled on the context).
 int prof_begin(hfpx_ctx* ctx);
@@ -92,6 +129,19 @@ int gemm_sub_any(hfpx_ctx* ctx, int dtype, void* C, int64_t ldc, const void* A, 
 size_t sweep_f32_work_bytes(int64_t n);
 int sweep_f32(hfpx_ctx* ctx, const float* LU, int64_t lda, int64_t n, float* B, int64_t ldb, int64_t nrhs,
               void* work);
+
+// batched assembly (hfpx_assemble.hip): every crack of the table; cp planes
+// have stride cps; total_wg workgroups in one 1-D grid
+int launch_bassemble(hfpx_ctx* ctx, const SrcElem<double>* elem, const double* cp, int64_t cps,
+                     const BCrack* cr, int ncr, int64_t total_wg, double* a64, float* a32);
+int launch_brhs(hfpx_ctx* ctx, const SrcElem<double>* elem, int64_t n_el_total, const BCrack* cr, int ncr,
+                double* b);
+// batched Schur-complement update (hfpx_gemm.hip) of step (k, j) for the
+// cracks list[0 .. nact): outer = false: panel update inside the outer
+// block; outer = true: trailing update with K = kb
+template <typename T>
+int launch_bgemm(hfpx_ctx* ctx, T* base, int which, const BCrack* cr, const int32_t* list, int nact,
+                 int64_t nmax, int64_t k, int64_t j, bool outer);
 
 // stress postprocessing (hfpx_stress.hip)
 int launch_stress(hfpx_ctx* ctx, hfpx_mesh* m, double shear_m, double nu, const double* dd, int64_t ldd,

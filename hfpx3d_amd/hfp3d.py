@@ -466,3 +466,65 @@ def make_3dbem_stress_f_s(shear_m, poiss_r, m_data, load, n_par, m_pts_crd, ctx=
                                   _ptr(d_pts), n_pts, _np_ptr(s_inf), _ptr(out), n_pts))
     ctx.leave()
     return out.t().cpu().numpy()
+
+
+# --------------------------------------------------------------------------
+# batches of independent cracks (no reference counterpart)
+# --------------------------------------------------------------------------
+
+class Batch:
+    """Many independent cracks assembled and solved together on the GPU
+    (hfpx_batch_*).  `cracks` is a list of dicts with nods (3+ x n_nodes),
+    conn (3+ x n_el, 0-based vertex ids in rows 0..2), G, nu and s_inf (6,
+    xx yy zz xy xz yz, tension positive); every DoF is free (fix_tip = 0).
+    Cracks with n >= mixed_min_n are solved by FP32 LU + FP64 refinement,
+    the others by FP64 LU."""
+
+    def __init__(self, cracks, mixed_min_n=2048, beta=0.125, ctx=None):
+        self.ctx = ctx or get_context()
+        self.lib = self.ctx.lib
+        self.n_cracks = len(cracks)
+        n_nodes = np.array([c["nods"].shape[1] for c in cracks], dtype=np.int64)
+        n_el = np.array([c["conn"].shape[1] for c in cracks], dtype=np.int64)
+        nods = np.ascontiguousarray(np.concatenate([np.asarray(c["nods"], dtype=np.float64)[:3].T for c in cracks]))
+        conn = np.ascontiguousarray(np.concatenate([np.asarray(c["conn"], dtype=np.int64)[:3].T for c in cracks]))
+        G = np.array([c["G"] for c in cracks], dtype=np.float64)
+        nu = np.array([c["nu"] for c in cracks], dtype=np.float64)
+        sinf = np.ascontiguousarray(np.array([np.asarray(c["s_inf"], dtype=np.float64) for c in cracks]))
+        self.n = 18 * n_el
+        h = ctypes.c_void_p()
+        self.ctx.check(self.lib.hfpx_batch_create(self.ctx.handle, self.n_cracks, _np_ptr(n_nodes), _np_ptr(n_el),
+                                                  _np_ptr(nods), _np_ptr(conn), float(beta), _np_ptr(G), _np_ptr(nu),
+                                                  _np_ptr(sinf), int(mixed_min_n), ctypes.byref(h)))
+        self.handle = h
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            self.lib.hfpx_batch_destroy(h)
+            self.handle = None
+
+    def assemble(self):
+        self.ctx.check(self.lib.hfpx_batch_assemble(self.ctx.handle, self.handle))
+
+    def solve(self, n_refine=3, use_graph=True):
+        self.ctx.check(self.lib.hfpx_batch_solve(self.ctx.handle, self.handle, int(n_refine), 1 if use_graph else 0))
+
+    def solution(self, i):
+        """(x, backward error, info) of crack i."""
+        x = np.zeros(int(self.n[i]))
+        nn, berr, info = ctypes.c_int64(0), ctypes.c_double(0), ctypes.c_int64(0)
+        self.ctx.check(self.lib.hfpx_batch_solution(self.ctx.handle, self.handle, int(i), _np_ptr(x),
+                                                    ctypes.byref(nn), ctypes.byref(berr), ctypes.byref(info)))
+        return x, berr.value, info.value
+
+    def rhs(self, i):
+        b = np.zeros(int(self.n[i]))
+        self.ctx.check(self.lib.hfpx_batch_rhs(self.ctx.handle, self.handle, int(i), _np_ptr(b)))
+        return b
+
+    def matrix(self, i):
+        n = int(self.n[i])
+        A = np.zeros((n, n), order="F")
+        self.ctx.check(self.lib.hfpx_batch_matrix(self.ctx.handle, self.handle, int(i), _np_ptr(A), None))
+        return A

@@ -154,6 +154,43 @@ int hfpx_profile(hfpx_ctx* ctx, int enable);
 int hfpx_profile_read(hfpx_ctx* ctx, int kernel_class, int64_t* launches, double* ms, double* flops,
                       double* bytes);
 
+/* ---- batches of independent cracks -------------------------------------- */
+/* No reference counterpart (the reference solves one crack per run,
+ * src/main.cpp:89-175): many cracks of uneven size assembled and solved
+ * together, each as make_3dbem_matrix_s + add_s_inf_to_3dbem_rhs + il::LU
+ * with fix_tip = 0 (all 18 n_el DoF free; at most 512 elements per crack).
+ * Crack i has n_nodes[i] nodes and n_el[i] elements; nods_host (3 x sum
+ * n_nodes, rows x y z) and conn_host (3 x sum n_el, crack-local 0-based node
+ * ids) hold the cracks one after another; shear_m[i], poiss_r[i] and
+ * sinf_host[6 i .. 6 i + 5] (xx yy zz xy xz yz, tension positive) are its
+ * material and far-field load.  Cracks with n >= mixed_min_n are solved with
+ * an FP32 LU and FP64 iterative refinement, the others with an FP64 LU.  The
+ * batch owns its device memory (FP64 matrices, FP32 copies, vectors). */
+typedef struct hfpx_batch hfpx_batch;
+int hfpx_batch_create(hfpx_ctx* ctx, int n_cracks, const int64_t* n_nodes, const int64_t* n_el,
+                      const double* nods_host, const int64_t* conn_host, double beta, const double* shear_m,
+                      const double* poiss_r, const double* sinf_host, int64_t mixed_min_n, hfpx_batch** batch);
+int hfpx_batch_destroy(hfpx_batch* batch);
+/* Assemble every crack's matrix and right-hand side (no solve). */
+int hfpx_batch_assemble(hfpx_ctx* ctx, hfpx_batch* batch);
+/* Assemble and solve every crack: batched LU (FP64 group and FP32 group),
+ * solves, n_refine refinement steps for the FP32 group, per-crack backward
+ * errors.  use_graph = 1 captures the whole launch sequence into a hipGraph
+ * on the first call (and whenever n_refine changes) and replays it. */
+int hfpx_batch_solve(hfpx_ctx* ctx, hfpx_batch* batch, int n_refine, int use_graph);
+/* Crack `crack` (input order): solution (n doubles, DoF order) to x_host (may
+ * be NULL), *n_out = its DoF count, *berr = normwise backward error
+ * max|b-Ax| / (|A|_inf max|x| + max|b|) against its FP64 matrix, *info = first
+ * zero pivot column (1-based) or 0.  Any pointer may be NULL. */
+int hfpx_batch_solution(hfpx_ctx* ctx, hfpx_batch* batch, int crack, double* x_host, int64_t* n_out,
+                        double* berr, int64_t* info);
+/* Crack `crack`'s right-hand side (n doubles) to host memory. */
+int hfpx_batch_rhs(hfpx_ctx* ctx, hfpx_batch* batch, int crack, double* b_host);
+/* Copy crack `crack`'s assembled FP64 matrix (n x n column-major) to A_host
+ * (may be NULL; *n_out = n).  Solving leaves it unchanged (the FP64 group is
+ * factorised in a working copy). */
+int hfpx_batch_matrix(hfpx_ctx* ctx, hfpx_batch* batch, int crack, double* A_host, int64_t* n_out);
+
 /* ---- tuning -------------------------------------------------------------- */
 /* Kernel-variant knobs of a context (no reference counterpart).
  * HFPX_KNOB_GEMM_F32: FP32 trailing-update GEMM (products with more than 64
