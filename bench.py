@@ -39,13 +39,16 @@ def parse():
     ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-el", type=int, default=1400)
-    ap.add_argument("--workload", default="ellipse72k", choices=["ellipse72k", "wavy108k"],
+    ap.add_argument("--workload", default="ellipse72k", choices=["ellipse72k", "wavy108k", "batch256"],
                     help="ellipse72k: the headline configuration (default); wavy108k: the non-planar "
-                         "6,000-element crack with stresses at 10^6 points")
+                         "6,000-element crack with stresses at 10^6 points; batch256: 256 independent cracks")
+    ap.add_argument("--mixed-min-n", type=int, default=2048,
+                    help="batch256: cracks with at least this many DoF use FP32 LU + FP64 refinement")
     ap.add_argument("--n-pts", type=int, default=1_000_000)
     ap.add_argument("--fp64-compare", action="store_true", help="wavy108k: also run the plain FP64 path")
     ap.add_argument("--no-lib-prof", action="store_true",
                     help="do not time kernel launches with HIP events (for runs under an external profiler)")
+    ap.add_argument("--lu-nb", type=int, default=0, help="outer block of the LU (0: library default)")
     ap.add_argument("--tol", type=float, default=1e-14,
                     help="normwise backward error at which the refinement stops")
     return ap.parse_args()
@@ -116,6 +119,8 @@ def main():
         sys.exit(2)
     if args.workload == "wavy108k":
         return run_wavy(args)
+    if args.workload == "batch256":
+        return run_batch(args)
     import numpy as np
     import torch
     from hfpx3d_amd import hfp3d as hx, _lib
@@ -127,6 +132,8 @@ def main():
     n = 18 * n_el
     ctx = hx.get_context(0)
     lib = ctx.lib
+    if args.lu_nb:
+        hx.tune_set(_lib.KNOB_LU_NB, args.lu_nb, ctx)
     log(f"bench: n_el={n_el} n={n} nrhs={args.nrhs}")
 
     mesh = hx.Mesh_Geom_T(nods=nods, conn=conn)
@@ -273,6 +280,101 @@ def read_profile(ctx, steps):
         prof[name] = dict(launches=L.value // steps, ms=round(ms.value / steps, 4), flops=fl.value / steps,
                           bytes=by.value / steps)
     return prof
+
+
+def run_batch(args):
+    """256 independent planar elliptical cracks, 40-500 elements each (seed 2),
+    semi-axes U[0.5, 2], random orientation, E = 1, nu U[0.2, 0.4], pressure
+    p U[0.5, 2] (s_inf = p I).  One step = batched assembly + right-hand
+    sides + batched LU (FP64 below --mixed-min-n DoF, FP32 + 3 FP64
+    refinement steps above) + solves + backward errors, replayed as one
+    hipGraph."""
+    import numpy as np
+    import torch
+    from hfpx3d_amd import hfp3d as hx, meshgen
+    cracks = []
+    for c in meshgen.batch_cracks(256, 2, 40, 500):
+        p = c["p"]
+        cracks.append(dict(nods=c["nods"], conn=c["conn"], G=1.0 / (2.0 * (1.0 + c["nu"])), nu=c["nu"],
+                           s_inf=np.array([p, p, p, 0.0, 0.0, 0.0])))
+    ns = np.array([18 * c["conn"].shape[1] for c in cracks], dtype=np.float64)
+    entries, flops = float((ns ** 2).sum()), float((2.0 / 3.0 * ns ** 3).sum())
+    n_mixed = int((ns >= args.mixed_min_n).sum())
+    log(f"bench: batch of {len(cracks)} cracks, n {int(ns.min())}..{int(ns.max())}, {entries:.3e} entries, "
+        f"{n_mixed} mixed")
+    ctx = hx.get_context(0)
+    bt = hx.Batch(cracks, mixed_min_n=args.mixed_min_n, ctx=ctx)
+    for w in range(max(1, args.warmup)):
+        bt.solve(n_refine=3, use_graph=True)  # the first call also captures the graph
+        log(f"bench: warmup {w + 1} done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        bt.solve(n_refine=3, use_graph=True)
+        log(f"bench: step {k + 1}/{args.steps} done")
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / args.steps
+    berrs, infos = [], []
+    for i in range(len(cracks)):
+        _, be, inf = bt.solution(i)
+        berrs.append(be)
+        infos.append(inf)
+    # spot check: three cracks against the single-crack FP64 GPU path
+    errs = {}
+    for i in (int(np.argmin(ns)), int(np.argmax(ns)), len(cracks) // 2):
+        c = cracks[i]
+        mesh = hx.Mesh_Geom_T(nods=c["nods"], conn=c["conn"])
+        md = hx.Mesh_Data_T(mesh=mesh)
+        sae = hx.SAE_T()
+        sae.matrix = hx.make_3dbem_matrix_s(c["G"], c["nu"], mesh, hx.Num_Param_T(beta=0.125, tip_type=0),
+                                            md.dof_h_dd)
+        hx.add_s_inf_to_3dbem_rhs(md, hx.Load_T(s_inf=c["s_inf"]), sae)
+        xr = hx.LU(sae.matrix).solve(sae.rhs_v).cpu().numpy()
+        x = bt.solution(i)[0]
+        errs[f"crack{i}_n{int(ns[i])}"] = float(np.abs(x - xr).max() / np.abs(xr).max())
+        del sae
+    result = {
+        "metric": "batch of 256 cracks: matrix entries/s over assembly + LU + solve",
+        "value": round(entries / t / 1e9, 3), "unit": "Gentries/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 assembly; f64 LU (small cracks), f32 LU + f64 refinement (large)", "data": "synthetic",
+        "config": {"workload": f"256 planar elliptical cracks, 40-500 quadratic triangles each (seed 2), random "
+                               f"orientation, nu U[0.2,0.4], p U[0.5,2]; {n_mixed} cracks with n >= "
+                               f"{args.mixed_min_n} in FP32 + refinement", "n_cracks": len(cracks),
+                   "sum_n2": entries, "sum_2n3_3": flops},
+        "batch_s": round(t, 4), "lu_flop_rate_TFs": round(flops / t / 1e12, 2),
+        "max_backward_error": max(berrs), "singular_cracks": int(sum(1 for v in infos if v)),
+        "max_rel_err_vs_single_fp64": errs,
+    }
+    if not args.no_cpu_baseline:
+        log("bench: CPU baseline (reference assembly + LAPACK on three cracks, scaled)")
+        import scipy.linalg as sl
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        ref = oracle_lib.reference_or_none()
+        threads = max(1, min(16, os.cpu_count() or 1))
+        ta = tl = e_s = f_s = 0.0
+        for i in np.argsort(ns)[[0, len(ns) // 2, -1]]:
+            c = cracks[int(i)]
+            tt = time.perf_counter()
+            if ref is not None:
+                A, _ = ref.assemble(c["nods"], c["conn"], c["G"], c["nu"], 0.125, 0, nthreads=threads)
+            else:
+                A, _ = oracle_lib.oracle().assemble(c["nods"], c["conn"], c["G"], c["nu"], 0.125, 0, nthreads=threads)
+            ta += time.perf_counter() - tt
+            tt = time.perf_counter()
+            sl.lu_factor(A, overwrite_a=True, check_finite=False)
+            tl += time.perf_counter() - tt
+            e_s += float(ns[i]) ** 2
+            f_s += 2.0 / 3.0 * float(ns[i]) ** 3
+        cpu_t = ta * entries / e_s + tl * flops / f_s
+        result["cpu_baseline"] = {"value": round(entries / cpu_t / 1e9, 4), "unit": "Gentries/s", "cores": threads,
+                                  "kind": "reference" if ref is not None else "port",
+                                  "sample": f"reference assembly + LAPACK LU of the smallest, median and largest "
+                                            f"crack ({ta:.1f} s + {tl:.1f} s), scaled by sum n^2 and sum n^3 to "
+                                            f"{cpu_t:.0f} s for the batch"}
+    print(json.dumps(result))
 
 
 def run_wavy(args):

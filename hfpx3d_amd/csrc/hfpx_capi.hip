@@ -384,6 +384,12 @@ int hfpx_tune_set(hfpx_ctx* ctx, int knob, int value) {
     ctx->knob[knob] = value;
     return HFPX_OK;
   }
+  if (knob == HFPX_KNOB_LU_NB) {
+    if (value != 0 && value != 256 && value != 512 && value != 1024 && value != 2048 && value != 4096)
+      return set_error(ctx, HFPX_ERR_INVALID, "tune: LU_NB must be 0, 256, 512, 1024, 2048 or 4096");
+    ctx->knob[knob] = value;
+    return HFPX_OK;
+  }
   return set_error(ctx, HFPX_ERR_INVALID, "tune: unknown knob %d", knob);
 }
 

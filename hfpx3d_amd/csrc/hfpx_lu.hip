@@ -508,8 +508,9 @@ static int panel(hfpx_ctx* ctx, T* A, int64_t lda, int64_t n, int64_t j0, int jb
 // down to IB-column panels); the trailing matrix is updated once per outer
 // block with K = NB, so it is read and written n / NB times in all.
 constexpr int LU_IB = 32;
-constexpr int LU_NB = 1024;
-constexpr int LU_MAX_PAIR_BLOCKS = LU_NB / SWAP_BLK;
+constexpr int LU_NB = 1024;      // default outer block (HFPX_KNOB_LU_NB)
+constexpr int LU_NB_MAX = 4096;
+constexpr int LU_MAX_PAIR_BLOCKS = LU_NB_MAX / SWAP_BLK;
 
 template <typename T>
 struct LuWork {
@@ -554,7 +555,8 @@ static int factor_cols(hfpx_ctx* ctx, const LuWork<T>& W, T* A, int64_t lda, int
 
 template <typename T>
 static int getrf_impl(hfpx_ctx* ctx, T* A, int64_t lda, int64_t n, int32_t* ipiv, int64_t* info_out) {
-  static_assert(LU_NB % SWAP_BLK == 0, "pair blocks per outer block");
+  static_assert(LU_NB_MAX % SWAP_BLK == 0, "pair blocks per outer block");
+  const int64_t NB = ctx->knob[HFPX_KNOB_LU_NB] > 0 ? ctx->knob[HFPX_KNOB_LU_NB] : LU_NB;
   // co-resident panel workgroups
   int occ = 0;
   HFPX_TRY_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_panel<T, LU_IB>, PWG, 0));
@@ -575,8 +577,8 @@ static int getrf_impl(hfpx_ctx* ctx, T* A, int64_t lda, int64_t n, int32_t* ipiv
   HFPX_TRY_HIP(ctx, hipMemsetAsync(ctx->sync, 0, sizeof(uint64_t), ctx->stream));
   ctx->sync_base = 0;
 
-  for (int64_t k = 0; k < n; k += LU_NB) {
-    const int64_t kb = std::min<int64_t>(LU_NB, n - k);
+  for (int64_t k = 0; k < n; k += NB) {
+    const int64_t kb = std::min<int64_t>(NB, n - k);
     if ((rc = factor_cols<T>(ctx, W, A, lda, n, k, kb, ipiv))) return rc;
     // interchanges on the columns left and right of the outer block
     if ((rc = swap_cols<T>(ctx, W, A, lda, ipiv, n, k, k + kb, 0, k, k + kb, n))) return rc;

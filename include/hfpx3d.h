@@ -200,7 +200,10 @@ int hfpx_batch_matrix(hfpx_ctx* ctx, hfpx_batch* batch, int crack, double* A_hos
  * separate launches), 4 = 3 + pipelined LDS operands, 5 = 4 + XCD-grouped
  * tile order, 6 = 3 + XCD-grouped tile order. */
 #define HFPX_KNOB_GEMM_F32 0
-#define HFPX_KNOB_COUNT 1
+/* HFPX_KNOB_LU_NB: outer block of the LU (columns per trailing update):
+ * 0 = default (1024), or one of 256, 512, 1024, 2048, 4096. */
+#define HFPX_KNOB_LU_NB 1
+#define HFPX_KNOB_COUNT 2
 /* Set / read one knob; HFPX_ERR_INVALID for an unknown knob or value. */
 int hfpx_tune_set(hfpx_ctx* ctx, int knob, int value);
 int hfpx_tune_get(const hfpx_ctx* ctx, int knob, int* value);

@@ -73,6 +73,8 @@ def test_tune_rejects_unknown_variant(env):
         hx.tune_set(_lib.KNOB_GEMM_F32, _lib.GEMM_F32_VARIANTS)
     with pytest.raises(_lib.HfpxError):
         hx.tune_set(99, 0)
+    with pytest.raises(_lib.HfpxError):
+        hx.tune_set(_lib.KNOB_LU_NB, 300)
 
 
 @pytest.mark.parametrize("n", [1, 7, 64, 300, 1000, 2500])
@@ -113,6 +115,29 @@ def test_getrs_f32_block_sweeps(env, n, nrhs):
     x = lu.solve(to_dev(torch, hx, b, torch.float32)).cpu().numpy().astype(np.float64)
     xr = sl.lu_solve((LU, piv), b.astype(np.float32).astype(np.float64))
     assert np.abs(x - xr).max() / np.abs(xr).max() < 2e-5
+
+
+@pytest.mark.parametrize("nb", [256, 512, 2048, 4096])
+def test_getrf_outer_block_sizes_match_lapack(env, nb):
+    """Every outer block size of the recursive LU gives LAPACK's pivots and
+    factors (n = 2,500 spans several outer blocks for the small sizes and a
+    partial one for the large)."""
+    torch, hx = env
+    sl = pytest.importorskip("scipy.linalg")
+    from hfpx3d_amd import _lib
+    print(f"case: getrf f64 NB={nb}", flush=True)
+    n = 2500
+    rng = np.random.default_rng(nb)
+    A = rng.normal(size=(n, n))
+    old = hx.tune_get(_lib.KNOB_LU_NB)
+    hx.tune_set(_lib.KNOB_LU_NB, nb)
+    try:
+        lu = hx.LU(to_dev(torch, hx, A, torch.float64))
+    finally:
+        hx.tune_set(_lib.KNOB_LU_NB, old)
+    lu2, piv2 = sl.lu_factor(A)
+    assert np.array_equal(lu.ipiv.cpu().numpy() - 1, piv2)
+    assert np.abs(lu.A.cpu().numpy() - lu2).max() < 1e-12 * np.abs(lu2).max() * n / 100
 
 
 @pytest.mark.parametrize("n", [300, 2049])
