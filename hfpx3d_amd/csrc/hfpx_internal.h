@@ -28,6 +28,7 @@ struct hfpx_ctx {
   uint64_t* sync = nullptr;  // grid-barrier arrival counter (device)
   uint64_t sync_base = 0;    // its value once every launched barrier completes
   int num_cus = 0;
+  bool no_coop = false;      // cooperative launches refused: launch panels normally
   int knob[HFPX_KNOB_COUNT] = {0};  // kernel-variant choices (hfpx_tune_set)
   std::string last_error;
   // opt-in per-launch timing (HIP events on the context's stream)

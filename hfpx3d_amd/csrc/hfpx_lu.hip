@@ -496,8 +496,21 @@ static int panel(hfpx_ctx* ctx, T* A, int64_t lda, int64_t n, int64_t j0, int jb
   uint64_t base = ctx->sync_base;
   void* args[] = {&A, &lda, &n, &j0, &jb, &ipiv, &info, &err, &ps, &rows, &ctr, &base};
   const int tok = prof_begin(ctx);
-  const hipError_t e = hipLaunchCooperativeKernel((const void*)k_panel<T, IB>, dim3(nwg), dim3(PWG), args, 0,
-                                                  ctx->stream);
+  // A cooperative launch guarantees that the workgroups are co-resident.
+  // The runtime refuses it while another process holds the device's
+  // cooperative capability; the grid (at most one workgroup per CU) is then
+  // launched normally, and the barrier's bounded wait turns a workgroup that
+  // never becomes resident into an error instead of a hang.
+  hipError_t e = hipErrorIllegalState;
+  if (!ctx->no_coop) e = hipLaunchCooperativeKernel((const void*)k_panel<T, IB>, dim3(nwg), dim3(PWG), args, 0,
+                                                   ctx->stream);
+  if (ctx->no_coop || e == hipErrorIllegalState) {
+    if (!ctx->no_coop) (void)hipGetLastError();
+    ctx->no_coop = true;
+    hipLaunchKernelGGL((k_panel<T, IB>), dim3(nwg), dim3(PWG), 0, ctx->stream, A, lda, n, j0, jb, ipiv, info, err,
+                       ps, rows, ctr, base);
+    e = hipGetLastError();
+  }
   prof_end(ctx, tok, HFPX_KC_PANEL, (double)m * jb * jb, 0.0);
   HFPX_TRY_HIP(ctx, e);
   ctx->sync_base += (uint64_t)jb * nwg;

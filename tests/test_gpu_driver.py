@@ -22,6 +22,16 @@ MAIN = os.path.join(ROOT, "hfpx3d_amd", "hfpx3d_main")
 @pytest.mark.parametrize("stem,tag", [("pennymesh24el_32", "psc_24"), ("pennymesh121el_32", "psc_121")])
 def test_driver_reproduces_reference_outputs(tmp_path, stem, tag):
     print(f"case: driver {tag}", flush=True)
+    # an LU in this process first: the driver's own LU then runs while this
+    # process holds the device (its cooperative panel launches included)
+    import torch
+    from hfpx3d_amd import hfp3d as hx
+    ctx = hx.get_context()
+    A, _ = hx.alloc_colmajor(300, 300, torch.float64, ctx)
+    with torch.cuda.stream(ctx.stream):
+        A.copy_(torch.randn(300, 300, dtype=torch.float64) + 20 * torch.eye(300, dtype=torch.float64))
+    hx.LU(A)
+    torch.cuda.synchronize()
     (tmp_path / "mesh").mkdir()
     (tmp_path / "obs").mkdir()
     (tmp_path / "out").mkdir()
