@@ -27,6 +27,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAKS = {"hbm_GBs": 8000.0, "fp64_mfma_TFs": 78.6, "fp32_mfma_TFs": 157.3}
+# bytes per call of the representative trailing update, from rocprofv3 --pmc (profiles/gemm72k_pmc.md)
+GEMM72K_TRAFFIC = 1.0540e+11
+GEMM72K_ALG_BYTES = 4.0882e+10
 
 
 def parse():
@@ -39,7 +42,7 @@ def parse():
     ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-el", type=int, default=1400)
-    ap.add_argument("--workload", default="ellipse72k", choices=["ellipse72k", "wavy108k", "batch256"],
+    ap.add_argument("--workload", default="ellipse72k", choices=["ellipse72k", "wavy108k", "batch256", "gemm72k"],
                     help="ellipse72k: the headline configuration (default); wavy108k: the non-planar "
                          "6,000-element crack with stresses at 10^6 points; batch256: 256 independent cracks")
     ap.add_argument("--mixed-min-n", type=int, default=2048,
@@ -49,6 +52,10 @@ def parse():
     ap.add_argument("--no-lib-prof", action="store_true",
                     help="do not time kernel launches with HIP events (for runs under an external profiler)")
     ap.add_argument("--lu-nb", type=int, default=0, help="outer block of the LU (0: library default)")
+    ap.add_argument("--gemm-variant", type=int, default=0, help="FP32 GEMM variant (0: library default)")
+    ap.add_argument("--calib-copy-gib", type=float, default=0.0,
+                    help="before the workload, copy this many GiB device-to-device (a transfer of known size "
+                         "for scaling hardware byte counters)")
     ap.add_argument("--tol", type=float, default=1e-14,
                     help="normwise backward error at which the refinement stops")
     return ap.parse_args()
@@ -121,6 +128,8 @@ def main():
         return run_wavy(args)
     if args.workload == "batch256":
         return run_batch(args)
+    if args.workload == "gemm72k":
+        return run_gemm(args)
     import numpy as np
     import torch
     from hfpx3d_amd import hfp3d as hx, _lib
@@ -134,6 +143,17 @@ def main():
     lib = ctx.lib
     if args.lu_nb:
         hx.tune_set(_lib.KNOB_LU_NB, args.lu_nb, ctx)
+    if args.gemm_variant:
+        hx.tune_set(_lib.KNOB_GEMM_F32, args.gemm_variant, ctx)
+    if args.calib_copy_gib > 0:
+        cnt = int(args.calib_copy_gib * 2 ** 30) // 4
+        with torch.cuda.stream(ctx.stream):
+            src = torch.ones(cnt, dtype=torch.float32, device=ctx.device)
+            dst = torch.empty_like(src)
+            dst.copy_(src)
+        ctx.stream.synchronize()
+        log(f"bench: calibration copy of {cnt * 4} bytes done")
+        del src, dst
     log(f"bench: n_el={n_el} n={n} nrhs={args.nrhs}")
 
     mesh = hx.Mesh_Geom_T(nods=nods, conn=conn)
@@ -204,9 +224,13 @@ def main():
     gemm_tf = g["flops"] / (g["ms"] / 1e3) / 1e12 if g["ms"] > 0 else 0.0
     a = prof["assemble"]
     asm_gbs = a["bytes"] / (a["ms"] / 1e3) / 1e9 if a["ms"] > 0 else 0.0
-    roofline = {"kernel": "k_gemm_sub<float> (FP32 LU trailing update)", "bound": "mfma",
+    roofline = {"kernel": "FP32 GEMM (k_sgemm32 / k_gemm_sub<float>, LU trailing updates)", "bound": "mfma",
                 "achieved": round(gemm_tf, 3), "peak": PEAKS["fp32_mfma_TFs"], "unit": "TFLOP/s",
-                "frac": round(gemm_tf / PEAKS["fp32_mfma_TFs"], 4), "traffic": None,
+                "frac": round(gemm_tf / PEAKS["fp32_mfma_TFs"], 4),
+                # PMC-measured bytes of one representative call (profiles/gemm72k_pmc.md)
+                "traffic": GEMM72K_TRAFFIC, "traffic_algorithmic": GEMM72K_ALG_BYTES,
+                "traffic_for": "first trailing update 70976 x 70976 x 1024 (bench.py --workload gemm72k): "
+                               "FETCH_SIZE x2.0 + WRITE_SIZE x1.0, scales from a 2 GiB device copy",
                 "launches_per_step": g["launches"], "avg_launch_ms": g["ms"] / max(1, g["launches"]),
                 "kernel_ms_per_step": g["ms"]}
     assembly_roofline = {"kernel": "k_assemble (FP64 + FP32 matrix write)", "bound": "hbm",
@@ -280,6 +304,61 @@ def read_profile(ctx, steps):
         prof[name] = dict(launches=L.value // steps, ms=round(ms.value / steps, 4), flops=fl.value / steps,
                           bytes=by.value / steps)
     return prof
+
+
+def run_gemm(args):
+    """The dominant kernel alone: the first trailing update of the 72,000-DoF
+    FP32 LU, C (70,976 x 70,976) -= A (70,976 x 1,024) B (1,024 x 70,976), on
+    random data (for hardware counters and roofline)."""
+    import numpy as np
+    import torch
+    from hfpx3d_amd import hfp3d as hx, _lib
+    M = N = 72000 - 1024
+    K = 1024
+    ctx = hx.get_context(0)
+    lib = ctx.lib
+    if args.gemm_variant:
+        hx.tune_set(_lib.KNOB_GEMM_F32, args.gemm_variant, ctx)
+    if args.calib_copy_gib > 0:
+        cnt = int(args.calib_copy_gib * 2 ** 30) // 4
+        with torch.cuda.stream(ctx.stream):
+            src = torch.ones(cnt, dtype=torch.float32, device=ctx.device)
+            dst = torch.empty_like(src)
+            dst.copy_(src)
+        ctx.stream.synchronize()
+        log(f"bench: calibration copy of {cnt * 4} bytes done")
+        del src, dst
+    with torch.cuda.stream(ctx.stream):
+        C = torch.randn(M * N, dtype=torch.float32, device=ctx.device)
+        A = torch.randn(M * K, dtype=torch.float32, device=ctx.device)
+        B = torch.randn(K * N, dtype=torch.float32, device=ctx.device)
+    ctx.stream.synchronize()
+    ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+    call = lambda: ctx.check(lib.hfpx_gemm_sub(ctx.handle, _lib.HFPX_F32, hx._ptr(C), M, hx._ptr(A), M,  # noqa: E731
+                                               hx._ptr(B), K, M, N, K))
+    for _ in range(args.warmup):
+        call()
+    ctx.stream.synchronize()
+    times = []
+    for _ in range(args.steps):
+        e0, e1 = ev(), ev()
+        e0.record(ctx.stream)
+        call()
+        e1.record(ctx.stream)
+        ctx.stream.synchronize()
+        times.append(e0.elapsed_time(e1) / 1e3)
+    t = float(np.mean(times))
+    flops = 2.0 * M * N * K
+    alg_bytes = 4.0 * (2.0 * M * N + M * K + K * N)
+    result = {"metric": "FP32 trailing-update GEMM TFLOP/s (72k LU, first update)", "value": round(flops / t / 1e12, 3),
+              "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+              "ms_per_step": round(t * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+              "dtype": "f32", "data": "synthetic",
+              "config": {"workload": f"C ({M} x {N}) -= A ({M} x {K}) B ({K} x {N}), FP32"},
+              "algorithmic_bytes": alg_bytes,
+              "roofline": {"bound": "mfma", "achieved": round(flops / t / 1e12, 3), "peak": PEAKS["fp32_mfma_TFs"],
+                           "unit": "TFLOP/s", "frac": round(flops / t / 1e12 / PEAKS["fp32_mfma_TFs"], 4)}}
+    print(json.dumps(result))
 
 
 def run_batch(args):

@@ -513,7 +513,7 @@ int gemm_sub(hfpx_ctx* ctx, T* C, int64_t ldc, const T* A, int64_t lda, const T*
   if ((N + 63) / 64 > 65535) return set_error(ctx, HFPX_ERR_INVALID, "gemm: N too large");
   const int tok = prof_begin(ctx);
   int var = ctx->knob[HFPX_KNOB_GEMM_F32];
-  if (var == 0) var = (M >= 2048 && N >= 512) ? 3 : 2;  // automatic choice
+  if (var == 0) var = (M >= 2048 && N >= 512) ? 6 : 2;  // automatic choice
   bool done = false;
   if constexpr (sizeof(T) == 4) {
     if (!skinny && var > 1) {

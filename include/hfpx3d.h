@@ -194,7 +194,7 @@ int hfpx_batch_matrix(hfpx_ctx* ctx, hfpx_batch* batch, int crack, double* A_hos
 /* ---- tuning -------------------------------------------------------------- */
 /* Kernel-variant knobs of a context (no reference counterpart).
  * HFPX_KNOB_GEMM_F32: FP32 trailing-update GEMM (products with more than 64
- * columns): 0 = automatic (default: 3 when M >= 2048 and N >= 512, else 2);
+ * columns): 0 = automatic (default: 6 when M >= 2048 and N >= 512, else 2);
  * 1 = 16x16x4 MFMA, 128 x 128 tile; 2..6 = 32x32x2 MFMA, 128 x 128 tile,
  * K chunk 16, with 2 no options, 3 unguarded interior tiles (edge bands in
  * separate launches), 4 = 3 + pipelined LDS operands, 5 = 4 + XCD-grouped
