@@ -26,6 +26,14 @@
 
 #include "hfpx_internal.h"
 
+namespace hfpx {
+// One batched Schur-complement update launch: its tile prefix table starts
+// at entry off of the batch's table, and it runs tiles workgroups.
+struct BTab {
+  int64_t off = 0, tiles = 0;
+};
+}  // namespace hfpx
+
 struct hfpx_batch {
   int ncr = 0;
   int n32 = 0, n64 = 0;            // cracks solved mixed / in FP64
@@ -53,6 +61,8 @@ struct hfpx_batch {
   double *b = nullptr, *x = nullptr, *r = nullptr, *rs = nullptr;
   float* x32 = nullptr;
   double* berr = nullptr;   // per crack (table order)
+  int32_t* tiles = nullptr;  // tile prefix tables of the batched updates
+  std::vector<hfpx::BTab> tab32, tab64;  // update launches of each group, in issue order
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graph_refine = -1;
@@ -425,12 +435,51 @@ static int count_above(const std::vector<int64_t>& sizes, int64_t v) {
   return c;
 }
 
+// The batched LU's update launches, in the order batch_lu issues them: for
+// each, the prefix sums of the cracks' tile counts (bgemm_tiles) along the
+// active prefix of the list are appended to tab.  False if one launch would
+// exceed 2^31 - 1 tiles.
+static bool plan_updates(const std::vector<int64_t>& sizes, std::vector<int32_t>& tab, std::vector<BTab>& out) {
+  if (sizes.empty()) return true;
+  const int64_t nmax = sizes[0];
+  auto add = [&](int nact, int64_t k, int64_t j, bool outer) {
+    BTab t;
+    t.off = (int64_t)tab.size();
+    int64_t sum = 0;
+    tab.push_back(0);
+    for (int e = 0; e < nact; ++e) {
+      sum += bgemm_tiles(sizes[e], k, j, outer);
+      if (sum > 0x7fffffff) return false;
+      tab.push_back((int32_t)sum);
+    }
+    t.tiles = sum;
+    out.push_back(t);
+    return true;
+  };
+  for (int64_t k = 0; k < nmax; k += BNB) {
+    for (int64_t j = k; j < k + BNB; j += BIB) {
+      const int na = count_above(sizes, j);
+      if (na == 0) break;
+      if (!add(na, k, j, false)) return false;
+    }
+    const int nrest = count_above(sizes, k + BNB);
+    if (nrest > 0 && !add(nrest, k, k, true)) return false;
+  }
+  return true;
+}
+
 template <typename T>
 static int batch_lu(hfpx_ctx* ctx, hfpx_batch* bt, T* base, int which, const int32_t* list,
-                    const std::vector<int64_t>& sizes) {
+                    const std::vector<int64_t>& sizes, const std::vector<BTab>& plan) {
   if (sizes.empty()) return HFPX_OK;
   const int64_t nmax = sizes[0];
   const int u = which;
+  size_t li = 0;  // next entry of plan (same loop structure as plan_updates)
+  auto update = [&](int nact, int64_t k, int64_t j, bool outer) -> int {
+    if (li >= plan.size()) return set_error(ctx, HFPX_ERR_INVALID, "batched LU: update plan out of step");
+    const BTab& t = plan[li++];
+    return launch_bgemm<T>(ctx, base, which, bt->cr, list, nact, bt->tiles + t.off, t.tiles, k, j, outer);
+  };
   int rc;
   for (int64_t k = 0; k < nmax; k += BNB) {
     const int nact = count_above(sizes, k);
@@ -448,7 +497,7 @@ static int batch_lu(hfpx_ctx* ctx, hfpx_batch* bt, T* base, int which, const int
       hipLaunchKernelGGL(k_btrsm<T>, dim3(1, (unsigned)na), dim3(256), 0, ctx->stream, base, u, bt->cr, list, k, j,
                          0);
       HFPX_TRY_HIP(ctx, hipGetLastError());
-      if ((rc = launch_bgemm<T>(ctx, base, which, bt->cr, list, na, nmax, k, j, false))) return rc;
+      if ((rc = update(na, k, j, false))) return rc;
     }
     hipLaunchKernelGGL(k_bswap<T>, dim3((unsigned)((nmax + 255) / 256), (unsigned)nact), dim3(256), 0, ctx->stream,
                        base, u, bt->cr, list, bt->ipiv, k, k, 1);
@@ -458,10 +507,11 @@ static int batch_lu(hfpx_ctx* ctx, hfpx_batch* bt, T* base, int which, const int
       hipLaunchKernelGGL(k_btrsm<T>, dim3((unsigned)((rest + 255) / 256), (unsigned)nrest), dim3(256), 0,
                          ctx->stream, base, u, bt->cr, list, k, k, 1);
       HFPX_TRY_HIP(ctx, hipGetLastError());
-      if ((rc = launch_bgemm<T>(ctx, base, which, bt->cr, list, nrest, nmax, k, k, true))) return rc;
+      if ((rc = update(nrest, k, k, true))) return rc;
     }
     HFPX_TRY_HIP(ctx, hipGetLastError());
   }
+  if (li != plan.size()) return set_error(ctx, HFPX_ERR_INVALID, "batched LU: update plan out of step");
   return HFPX_OK;
 }
 
@@ -488,12 +538,12 @@ static int batch_enqueue(hfpx_ctx* ctx, hfpx_batch* bt, int n_refine) {
   // FP64 group: LU in a working copy (the assembled matrix stays for the
   // backward error), solve in place in x
   if ((rc = bvec(ctx, bt->F64, nullptr, bt->A64 + bt->f_first, bt->f_total, 3))) return rc;
-  if ((rc = batch_lu<double>(ctx, bt, bt->F64, B_F64, bt->list64, bt->n64_sorted))) return rc;
+  if ((rc = batch_lu<double>(ctx, bt, bt->F64, B_F64, bt->list64, bt->n64_sorted, bt->tab64))) return rc;
   if ((rc = bvec(ctx, bt->x + bt->x32_total, nullptr, bt->b + bt->x32_total, bt->x_total - bt->x32_total, 3)))
     return rc;
   if ((rc = batch_solve_vec<double>(ctx, bt, bt->F64, B_F64, bt->list64, bt->n64, bt->x))) return rc;
   // mixed group: FP32 LU, x0 = LU^-1 b, then refinement against the FP64 matrix
-  if ((rc = batch_lu<float>(ctx, bt, bt->A32, B_A32, bt->list32, bt->n32_sorted))) return rc;
+  if ((rc = batch_lu<float>(ctx, bt, bt->A32, B_A32, bt->list32, bt->n32_sorted, bt->tab32))) return rc;
   if ((rc = bvec(ctx, nullptr, bt->x32, bt->b, bt->x32_total, 0))) return rc;
   if ((rc = batch_solve_vec<float>(ctx, bt, bt->A32, B_A32, bt->list32, bt->n32, bt->x32))) return rc;
   if ((rc = bvec(ctx, bt->x, bt->x32, nullptr, bt->x32_total, 1))) return rc;
@@ -615,6 +665,9 @@ int hfpx_batch_create(hfpx_ctx* ctx, int n_cracks, const int64_t* n_nodes, const
     hfpx_batch_destroy(bt);
     return rc;
   };
+  std::vector<int32_t> tab;
+  if (!plan_updates(bt->n32_sorted, tab, bt->tab32) || !plan_updates(bt->n64_sorted, tab, bt->tab64))
+    return fail(set_error(ctx, HFPX_ERR_INVALID, "batch: too many tiles in one update"));
   auto dalloc = [&](void** p, size_t bytes) -> int {
     HFPX_TRY_HIP(ctx, hipMalloc(p, std::max<size_t>(bytes, 256)));
     return HFPX_OK;
@@ -637,7 +690,8 @@ int hfpx_batch_create(hfpx_ctx* ctx, int n_cracks, const int64_t* n_nodes, const
       (rc = dalloc((void**)&bt->b, sizeof(double) * x_off)) || (rc = dalloc((void**)&bt->x, sizeof(double) * x_off)) ||
       (rc = dalloc((void**)&bt->r, sizeof(double) * x_off)) || (rc = dalloc((void**)&bt->rs, sizeof(double) * x_off)) ||
       (rc = dalloc((void**)&bt->x32, sizeof(float) * std::max<int64_t>(1, bt->x32_total))) ||
-      (rc = dalloc((void**)&bt->berr, sizeof(double) * n_cracks)))
+      (rc = dalloc((void**)&bt->berr, sizeof(double) * n_cracks)) ||
+      (rc = dalloc((void**)&bt->tiles, sizeof(int32_t) * tab.size())))
     return fail(rc);
   auto up = [&](void* d, const void* h, size_t bytes) -> int {
     if (bytes) HFPX_TRY_HIP(ctx, hipMemcpy(d, h, bytes, hipMemcpyHostToDevice));
@@ -650,7 +704,8 @@ int hfpx_batch_create(hfpx_ctx* ctx, int n_cracks, const int64_t* n_nodes, const
       (rc = up(bt->list64, l64.data(), sizeof(int32_t) * l64.size())) ||
       (rc = up(bt->list_all, lall.data(), sizeof(int32_t) * n_cracks)) ||
       (rc = up(bt->nods, nods3.data(), sizeof(double) * 3 * nn)) ||
-      (rc = up(bt->conn, conn_t.data(), sizeof(int32_t) * conn_t.size())))
+      (rc = up(bt->conn, conn_t.data(), sizeof(int32_t) * conn_t.size())) ||
+      (rc = up(bt->tiles, tab.data(), sizeof(int32_t) * tab.size())))
     return fail(rc);
   // element geometry of all cracks at once (the element table is in crack order)
   hfpx_mesh m;
@@ -671,7 +726,7 @@ int hfpx_batch_destroy(hfpx_batch* bt) {
   if (bt->exec) (void)hipGraphExecDestroy(bt->exec);
   if (bt->graph) (void)hipGraphDestroy(bt->graph);
   void* ps[] = {bt->cr, bt->list32, bt->list64, bt->list_all, bt->nods, bt->conn, bt->elem, bt->cp, bt->A64, bt->A32, bt->F64,
-                bt->ipiv, bt->info, bt->b, bt->x, bt->r, bt->rs, bt->x32, bt->berr};
+                bt->ipiv, bt->info, bt->b, bt->x, bt->r, bt->rs, bt->x32, bt->berr, bt->tiles};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   delete bt;

@@ -147,28 +147,6 @@ __global__ __launch_bounds__(256) void k_gemm_sub(T* __restrict__ C, int64_t ldc
                     (int64_t)blockIdx.x * GT, (int64_t)blockIdx.y * GTN);
 }
 
-// Batched Schur-complement update of step (k, j) of the batched LU: crack
-// list[blockIdx.z] (FP64 matrix at a_off, or its FP32 copy at a32_off);
-// tiles outside that crack's product exit at once.
-template <typename T>
-__global__ __launch_bounds__(256) void k_bgemm(T* __restrict__ base, int which, const BCrack* __restrict__ cr,
-                                               const int32_t* __restrict__ list, int64_t k, int64_t j, int outer) {
-  const BCrack& c = cr[list[blockIdx.z]];
-  const int64_t n = c.n;
-  int64_t kb, jb;
-  if (!bstep(n, k, outer ? k : j, kb, jb)) return;
-  T* a = base + bcopy_off(c, which);
-  int64_t r0, c0, p0, M, N, K;
-  if (outer) {
-    r0 = k + kb; c0 = k + kb; p0 = k; M = n - r0; N = n - c0; K = kb;
-  } else {
-    r0 = j + jb; c0 = j + jb; p0 = j; M = n - r0; N = k + kb - c0; K = jb;
-  }
-  const int64_t row0 = (int64_t)blockIdx.x * GT, col0 = (int64_t)blockIdx.y * 128;
-  if (M <= 0 || N <= 0 || row0 >= M || col0 >= N) return;
-  gemm_tile<T, 128>(a + r0 + c0 * n, n, a + r0 + p0 * n, n, a + p0 + c0 * n, n, M, N, K, row0, col0);
-}
-
 // ---------------------------------------------------------------------
 // FP32 C (M x N) -= A (M x K) * B (K x N) on v_mfma_f32_32x32x2f32, for the
 // large trailing updates of the LU.
@@ -195,48 +173,64 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 //  SG_SWZ   1-D grid remapped so that each of the 8 XCDs (workgroups are
 //           dealt to XCDs round-robin) owns one contiguous eighth of the
 //           tiles, walked in bands of 8 row tiles, so the A and B panels that
-//           concurrent workgroups share stay in that XCD's L2.
-constexpr int SG_INT = 1, SG_PIPE = 2, SG_SWZ = 4;
+//           concurrent workgroups share stay in that XCD's L2;
+//  SG_CPRE  C preloaded into the accumulators (no read-modify-write at the end).
+constexpr int SG_INT = 1, SG_PIPE = 2, SG_SWZ = 4, SG_CPRE = 8;
 constexpr int SG_GROUP = 8;
 
+// LDS footprint of one 32x32x2 tile (floats per buffer, two buffers).
+template <int WR, int WC, int KC>
+struct Sg32 {
+  static constexpr int GM = 64 * WR, GN = 64 * WC;
+  static constexpr int PA = GM + 32, PB = GN + 33;
+  static constexpr int LA = KC * PA, LB = KC * PB;
+};
+
+// One GM x GN tile of C at (row0, col0); sA and sB are the caller's LDS
+// buffers (2 LA and 2 LB floats).  With SG_INT the tile must lie wholly
+// inside M x N.
 template <int WR, int WC, int KC, int OPT>
-__global__ __launch_bounds__(256) void k_sgemm32(float* __restrict__ C, int64_t ldc,
-                                                 const float* __restrict__ A, int64_t lda,
-                                                 const float* __restrict__ B, int64_t ldb, int64_t M,
-                                                 int64_t N, int64_t K) {
-  constexpr int GM = 64 * WR, GN = 64 * WC;
-  constexpr int PA = GM + 32, PB = GN + 33;
+__device__ __forceinline__ void sgemm32_tile(float* __restrict__ sA, float* __restrict__ sB,
+                                             float* __restrict__ C, int64_t ldc, const float* __restrict__ A,
+                                             int64_t lda, const float* __restrict__ B, int64_t ldb, int64_t M,
+                                             int64_t N, int64_t K, int64_t row0, int64_t col0) {
+  typedef Sg32<WR, WC, KC> S;
+  constexpr int GM = S::GM, GN = S::GN, PA = S::PA, PB = S::PB, LA = S::LA, LB = S::LB;
   constexpr int QA = GM * KC / 256, QB = GN * KC / 256;
   static_assert(QA >= 1 && QB >= 1, "tile too small for the loaders");
   static_assert(256 % GM == 0 && 256 % KC == 0, "loader strides");
-  __shared__ float sA[2][KC * PA];  // sA[k * PA + i] = A(row0 + i, k0 + k)
-  __shared__ float sB[2][KC * PB];  // sB[k * PB + j] = B(k0 + k, col0 + j)
+  // sA[buf LA + k PA + i] = A(row0 + i, k0 + k); sB[buf LB + k PB + j] = B(k0 + k, col0 + j)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  int64_t tile_r, tile_c;
-  if constexpr ((OPT & SG_SWZ) != 0) {
-    const int64_t tm = (M + GM - 1) / GM, tn = (N + GN - 1) / GN, nt = tm * tn;
-    const int64_t id = blockIdx.x, xcd = id % 8, local = id / 8;
-    const int64_t fl = nt / 8, rem = nt % 8;
-    const int64_t nid = xcd * fl + (xcd < rem ? xcd : rem) + local;
-    const int64_t per = (int64_t)SG_GROUP * tn, first = (nid / per) * SG_GROUP;
-    const int64_t gs = (tm - first) < SG_GROUP ? (tm - first) : SG_GROUP;
-    const int64_t r = nid % per;
-    tile_r = first + r % gs;
-    tile_c = r / gs;
-  } else {
-    tile_r = blockIdx.x;
-    tile_c = blockIdx.y;
-  }
-  const int64_t row0 = tile_r * GM, col0 = tile_c * GN;
   const int wr = (wave & 1) * 32 * WR, wc = (wave >> 1) * 32 * WC;
 
+  const int l32 = lane & 31, hk = lane >> 5;
+  // acc[tc][tr][v] holds element (row0 + wr + 32 tr + l32, col0 + wc + 32 tc
+  // + 8 (v/4) + 4 hk + v%4) of A B, or with SG_CPRE of C - A B: the
+  // accumulators start as the tile of C (loaded before the first operands,
+  // so that its latency overlaps theirs), B is staged negated and the
+  // epilogue is a plain store.
+  // Tile addresses are 32-bit offsets from the tile's corner (the host
+  // keeps GN ldc below 2^31).
+  float* __restrict__ Ct = C + row0 + col0 * ldc;
+  const int ld = (int)ldc;
+  const int off0 = wr + l32 + (wc + 4 * hk) * ld;
+  const int mrem = (int)((M - row0) < GM ? (M - row0) : GM), nrem = (int)((N - col0) < GN ? (N - col0) : GN);
   floatx16 acc[WC][WR];
 #pragma unroll
-  for (int a = 0; a < WC; ++a)
+  for (int tc = 0; tc < WC; ++tc)
 #pragma unroll
-    for (int b = 0; b < WR; ++b)
+    for (int tr = 0; tr < WR; ++tr) {
+      const int i = wr + 32 * tr + l32;
 #pragma unroll
-      for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
+      for (int v = 0; v < 16; ++v) {
+        const int jc = wc + 32 * tc + 8 * (v >> 2) + 4 * hk + (v & 3);
+        const int off = off0 + 32 * tr + (32 * tc + 8 * (v >> 2) + (v & 3)) * ld;
+        if constexpr ((OPT & SG_CPRE) != 0)
+          acc[tc][tr][v] = ((OPT & SG_INT) != 0 || (i < mrem && jc < nrem)) ? Ct[off] : 0.f;
+        else
+          acc[tc][tr][v] = 0.f;
+      }
+    }
 
   float ra[QA], rb[QB];
   auto load_global = [&](int64_t k0) {
@@ -279,12 +273,12 @@ __global__ __launch_bounds__(256) void k_sgemm32(float* __restrict__ C, int64_t 
 #pragma unroll
     for (int q = 0; q < QA; ++q) {
       const int e = tid + 256 * q;
-      sA[buf][(e / GM) * PA + (e % GM)] = ra[q];
+      sA[buf * LA + (e / GM) * PA + (e % GM)] = ra[q];
     }
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
       const int e = tid + 256 * q;
-      sB[buf][(e % KC) * PB + (e / KC)] = rb[q];
+      sB[buf * LB + (e % KC) * PB + (e / KC)] = (OPT & SG_CPRE) != 0 ? -rb[q] : rb[q];
     }
   };
 
@@ -292,12 +286,11 @@ __global__ __launch_bounds__(256) void k_sgemm32(float* __restrict__ C, int64_t 
   load_global(0);
   store_lds(0);
   __syncthreads();
-  const int l32 = lane & 31, hk = lane >> 5;
   for (int c = 0; c < nchunks; ++c) {
     const int buf = c & 1;
     if (c + 1 < nchunks) load_global((int64_t)(c + 1) * KC);
-    const float* pa = sA[buf] + wr + l32;
-    const float* pb = sB[buf] + wc + l32;
+    const float* pa = sA + buf * LA + wr + l32;
+    const float* pb = sB + buf * LB + wc + l32;
     if constexpr ((OPT & SG_PIPE) != 0) {
       float fa[2][WC], fb[2][WR];
 #pragma unroll
@@ -339,19 +332,62 @@ __global__ __launch_bounds__(256) void k_sgemm32(float* __restrict__ C, int64_t 
     if (c + 1 < nchunks) store_lds(buf ^ 1);
     __syncthreads();
   }
-  // acc[tc][tr][v] = (A B)(row0 + wr + 32 tr + l32, col0 + wc + 32 tc + 8 (v/4) + 4 hk + v%4)
 #pragma unroll
   for (int tc = 0; tc < WC; ++tc)
 #pragma unroll
     for (int tr = 0; tr < WR; ++tr) {
-      const int64_t r = row0 + wr + 32 * tr + l32;
-      if ((OPT & SG_INT) == 0 && r >= M) continue;
+      const int i = wr + 32 * tr + l32;
+      if ((OPT & SG_INT) == 0 && i >= mrem) continue;
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
-        const int64_t cc = col0 + wc + 32 * tc + 8 * (v >> 2) + 4 * hk + (v & 3);
-        if ((OPT & SG_INT) != 0 || cc < N) C[r + cc * ldc] -= acc[tc][tr][v];
+        const int jc = wc + 32 * tc + 8 * (v >> 2) + 4 * hk + (v & 3);
+        const int off = off0 + 32 * tr + (32 * tc + 8 * (v >> 2) + (v & 3)) * ld;
+        if ((OPT & SG_INT) != 0 || jc < nrem) {
+          if constexpr ((OPT & SG_CPRE) != 0)
+            Ct[off] = acc[tc][tr][v];
+          else
+            Ct[off] -= acc[tc][tr][v];
+        }
       }
     }
+}
+
+// Tile (tile_r, tile_c) of a tm x tn tile grid for position nid of the
+// grid's walk in bands of SG_GROUP row tiles (column by column inside a
+// band), so that consecutive positions share A row blocks and B columns.
+__device__ __forceinline__ void sg_band_tile(int64_t nid, int64_t tm, int64_t tn, int64_t& tile_r, int64_t& tile_c) {
+  const int64_t per = (int64_t)SG_GROUP * tn, first = (nid / per) * SG_GROUP;
+  const int64_t gs = (tm - first) < SG_GROUP ? (tm - first) : SG_GROUP;
+  const int64_t r = nid % per;
+  tile_r = first + r % gs;
+  tile_c = r / gs;
+}
+
+// Position of workgroup id in a walk of nt tiles where each of the 8 XCDs
+// (workgroups are dealt to XCDs round-robin) owns one contiguous eighth.
+__device__ __forceinline__ int64_t sg_xcd_order(int64_t id, int64_t nt) {
+  const int64_t xcd = id % 8, local = id / 8;
+  const int64_t fl = nt / 8, rem = nt % 8;
+  return xcd * fl + (xcd < rem ? xcd : rem) + local;
+}
+
+template <int WR, int WC, int KC, int OPT>
+__global__ __launch_bounds__(256) void k_sgemm32(float* __restrict__ C, int64_t ldc,
+                                                 const float* __restrict__ A, int64_t lda,
+                                                 const float* __restrict__ B, int64_t ldb, int64_t M,
+                                                 int64_t N, int64_t K) {
+  typedef Sg32<WR, WC, KC> S;
+  __shared__ float sA[2 * S::LA];
+  __shared__ float sB[2 * S::LB];
+  int64_t tile_r, tile_c;
+  if constexpr ((OPT & SG_SWZ) != 0) {
+    const int64_t tm = (M + S::GM - 1) / S::GM, tn = (N + S::GN - 1) / S::GN;
+    sg_band_tile(sg_xcd_order(blockIdx.x, tm * tn), tm, tn, tile_r, tile_c);
+  } else {
+    tile_r = blockIdx.x;
+    tile_c = blockIdx.y;
+  }
+  sgemm32_tile<WR, WC, KC, OPT>(sA, sB, C, ldc, A, lda, B, ldb, M, N, K, tile_r * S::GM, tile_c * S::GN);
 }
 
 // ---------------------------------------------------------------------
@@ -493,7 +529,7 @@ static void launch_sgemm32(hfpx_ctx* ctx, float* C, int64_t ldc, const float* A,
   if constexpr ((OPT & SG_INT) == 0) {
     launch_sgemm32_grid<WR, WC, KC, OPT>(ctx, C, ldc, A, lda, B, ldb, M, N, K);
   } else {
-    constexpr int EDGE = OPT & ~SG_INT & ~SG_SWZ;
+    constexpr int EDGE = OPT & ~SG_INT & ~SG_SWZ & ~SG_PIPE;
     const int64_t Mi = M / (64 * WR) * (64 * WR), Ni = N / (64 * WC) * (64 * WC);
     launch_sgemm32_grid<WR, WC, KC, OPT>(ctx, C, ldc, A, lda, B, ldb, Mi, Ni, K);
     launch_sgemm32_grid<WR, WC, KC, EDGE>(ctx, C + Mi, ldc, A + Mi, lda, B, ldb, M - Mi, N, K);
@@ -501,7 +537,7 @@ static void launch_sgemm32(hfpx_ctx* ctx, float* C, int64_t ldc, const float* A,
   }
 }
 
-int gemm_f32_variants() { return 7; }
+int gemm_f32_variants() { return 9; }
 
 template <typename T>
 int gemm_sub(hfpx_ctx* ctx, T* C, int64_t ldc, const T* A, int64_t lda, const T* B,
@@ -511,9 +547,15 @@ int gemm_sub(hfpx_ctx* ctx, T* C, int64_t ldc, const T* A, int64_t lda, const T*
   const int gtn = skinny ? 64 : 128;
   dim3 grid((unsigned)((M + GT - 1) / GT), (unsigned)((N + gtn - 1) / gtn));
   if ((N + 63) / 64 > 65535) return set_error(ctx, HFPX_ERR_INVALID, "gemm: N too large");
+  if (ldc > (int64_t)1 << 24) return set_error(ctx, HFPX_ERR_INVALID, "gemm: ldc above 2^24");
   const int tok = prof_begin(ctx);
   int var = ctx->knob[HFPX_KNOB_GEMM_F32];
-  if (var == 0) var = (M >= 2048 && N >= 512) ? 6 : 2;  // automatic choice
+  if (var == 0) {
+    // automatic choice: short products are latency-bound (C preloaded),
+    // long ones MFMA-bound (the higher-occupancy kernels)
+    const bool big = M >= 2048 && N >= 512;
+    var = K <= 384 ? (big ? 8 : 7) : (big ? 6 : 2);
+  }
   bool done = false;
   if constexpr (sizeof(T) == 4) {
     if (!skinny && var > 1) {
@@ -522,6 +564,8 @@ int gemm_sub(hfpx_ctx* ctx, T* C, int64_t ldc, const T* A, int64_t lda, const T*
         case 3: launch_sgemm32<2, 2, 16, SG_INT>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
         case 4: launch_sgemm32<2, 2, 16, SG_INT | SG_PIPE>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
         case 5: launch_sgemm32<2, 2, 16, SG_INT | SG_PIPE | SG_SWZ>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
+        case 7: launch_sgemm32<2, 2, 16, SG_CPRE>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
+        case 8: launch_sgemm32<2, 2, 16, SG_INT | SG_SWZ | SG_CPRE>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
         default: launch_sgemm32<2, 2, 16, SG_INT | SG_SWZ>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
       }
       done = true;
@@ -605,24 +649,73 @@ int gemm_sub_any(hfpx_ctx* ctx, int dtype, void* C, int64_t ldc, const void* A, 
 
 namespace hfpx {
 
+// Batched Schur-complement update of step (k, j) of the batched LU over
+// the active cracks list[0 .. nact): one workgroup per BGT x BGT tile that
+// exists, no empty workgroups.  tstart[e] (e = 0 .. nact) is the prefix sum
+// of the tile counts along the list; the workgroups are dealt to the XCDs
+// in contiguous ranges of that walk, and a crack's tiles are walked in bands
+// of SG_GROUP row tiles.  FP32 tiles run on v_mfma_f32_32x32x2f32 (interior
+// tiles unguarded; BG_OPT = SG_CPRE or 0), FP64 tiles on
+// v_mfma_f64_16x16x4f64.
+template <typename T, int BG_OPT>
+__global__ __launch_bounds__(256) void k_bgemm(T* __restrict__ base, int which, const BCrack* __restrict__ cr,
+                                               const int32_t* __restrict__ list,
+                                               const int32_t* __restrict__ tstart, int nact, int64_t k, int64_t j,
+                                               int outer) {
+  __shared__ int s_e;
+  const int64_t nid = sg_xcd_order(blockIdx.x, tstart[nact]);
+  // exactly one entry e has tstart[e] <= nid < tstart[e + 1]
+  for (int e = threadIdx.x; e < nact; e += 256)
+    if (tstart[e] <= nid && nid < tstart[e + 1]) s_e = e;
+  __syncthreads();
+  const int e = s_e;
+  const BCrack& c = cr[list[e]];
+  const int64_t n = c.n;
+  int64_t r0, c0, p0, M, N, K;
+  bgemm_shape(n, k, j, outer != 0, r0, c0, p0, M, N, K);
+  const int64_t tm = (M + BGT - 1) / BGT, tn = (N + BGT - 1) / BGT;
+  int64_t tile_r, tile_c;
+  sg_band_tile(nid - tstart[e], tm, tn, tile_r, tile_c);
+  const int64_t row0 = tile_r * BGT, col0 = tile_c * BGT;
+  T* a = base + bcopy_off(c, which);
+  T* C = a + r0 + c0 * n;
+  const T* A = a + r0 + p0 * n;
+  const T* B = a + p0 + c0 * n;
+  if constexpr (sizeof(T) == 4) {
+    typedef Sg32<2, 2, 16> S;
+    static_assert(S::GM == BGT && S::GN == BGT, "batched tile");
+    __shared__ float sA[2 * S::LA];
+    __shared__ float sB[2 * S::LB];
+    if (row0 + BGT <= M && col0 + BGT <= N)
+      sgemm32_tile<2, 2, 16, SG_INT | BG_OPT>(sA, sB, C, n, A, n, B, n, M, N, K, row0, col0);
+    else
+      sgemm32_tile<2, 2, 16, BG_OPT>(sA, sB, C, n, A, n, B, n, M, N, K, row0, col0);
+  } else {
+    static_assert(GT == BGT, "batched tile");
+    gemm_tile<T, BGT>(C, n, A, n, B, n, M, N, K, row0, col0);
+  }
+}
+
 template <typename T>
 int launch_bgemm(hfpx_ctx* ctx, T* base, int which, const BCrack* cr, const int32_t* list, int nact,
-                 int64_t nmax, int64_t k, int64_t j, bool outer) {
-  if (nact <= 0) return HFPX_OK;
-  // the largest crack bounds every crack's product
-  const int64_t M = outer ? nmax - k : nmax - j, N = outer ? nmax - k : BNB;
-  if (M <= 0) return HFPX_OK;
-  const dim3 grid((unsigned)((M + GT - 1) / GT), (unsigned)((N + 127) / 128), (unsigned)nact);
+                 const int32_t* tstart, int64_t tiles, int64_t k, int64_t j, bool outer) {
+  if (nact <= 0 || tiles <= 0) return HFPX_OK;
+  if (tiles > 0x7fffffff) return set_error(ctx, HFPX_ERR_INVALID, "batched update: too many tiles");
   const int tok = prof_begin(ctx);
-  hipLaunchKernelGGL(k_bgemm<T>, grid, dim3(256), 0, ctx->stream, base, which, cr, list, k, j, outer ? 1 : 0);
+  if (ctx->knob[HFPX_KNOB_BGEMM_F32] == 1)
+    hipLaunchKernelGGL((k_bgemm<T, 0>), dim3((unsigned)tiles), dim3(256), 0, ctx->stream, base, which, cr, list,
+                       tstart, nact, k, j, outer ? 1 : 0);
+  else
+    hipLaunchKernelGGL((k_bgemm<T, SG_CPRE>), dim3((unsigned)tiles), dim3(256), 0, ctx->stream, base, which, cr,
+                       list, tstart, nact, k, j, outer ? 1 : 0);
   prof_end(ctx, tok, sizeof(T) == 8 ? HFPX_KC_GEMM_F64 : HFPX_KC_GEMM_F32, 0.0, 0.0);
   HFPX_TRY_HIP(ctx, hipGetLastError());
   return HFPX_OK;
 }
 
-template int launch_bgemm<double>(hfpx_ctx*, double*, int, const BCrack*, const int32_t*, int, int64_t, int64_t,
-                                  int64_t, bool);
-template int launch_bgemm<float>(hfpx_ctx*, float*, int, const BCrack*, const int32_t*, int, int64_t, int64_t,
-                                 int64_t, bool);
+template int launch_bgemm<double>(hfpx_ctx*, double*, int, const BCrack*, const int32_t*, int, const int32_t*,
+                                  int64_t, int64_t, int64_t, bool);
+template int launch_bgemm<float>(hfpx_ctx*, float*, int, const BCrack*, const int32_t*, int, const int32_t*,
+                                 int64_t, int64_t, int64_t, bool);
 
 }  // namespace hfpx

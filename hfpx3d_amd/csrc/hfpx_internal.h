@@ -90,6 +90,34 @@ __host__ __device__ inline bool bstep(int64_t n, int64_t k, int64_t j, int64_t& 
   return true;
 }
 
+// Schur-complement product C -= A B of step (k, j) of the batched LU for a
+// crack of n DoF (local column-major indices, ld n): outer = false is the
+// update inside the outer block after panel [j, j + jb), outer = true the
+// trailing update after the whole block.  C starts at (r0, c0), A at
+// (r0, p0), B at (p0, c0); false if the product is empty.
+__host__ __device__ inline bool bgemm_shape(int64_t n, int64_t k, int64_t j, bool outer, int64_t& r0, int64_t& c0,
+                                            int64_t& p0, int64_t& M, int64_t& N, int64_t& K) {
+  int64_t kb, jb;
+  M = N = K = 0;
+  r0 = c0 = p0 = 0;
+  if (!bstep(n, k, outer ? k : j, kb, jb)) return false;
+  if (outer) {
+    r0 = k + kb; c0 = k + kb; p0 = k; M = n - r0; N = n - c0; K = kb;
+  } else {
+    r0 = j + jb; c0 = j + jb; p0 = j; M = n - r0; N = k + kb - c0; K = jb;
+  }
+  return M > 0 && N > 0 && K > 0;
+}
+
+// Tile (rows and columns) of the batched Schur-complement updates, and the
+// number of tiles of one crack's product.
+constexpr int BGT = 128;
+__host__ __device__ inline int64_t bgemm_tiles(int64_t n, int64_t k, int64_t j, bool outer) {
+  int64_t r0, c0, p0, M, N, K;
+  if (!bgemm_shape(n, k, j, outer, r0, c0, p0, M, N, K)) return 0;
+  return ((M + BGT - 1) / BGT) * ((N + BGT - 1) / BGT);
+}
+
 // Which copy of a crack's matrix a batched kernel works on.
 enum BWhich { B_A64 = 0, B_A32 = 1, B_F64 = 2 };
 __host__ __device__ inline int64_t bcopy_off(const BCrack& c, int which) {
@@ -138,11 +166,11 @@ int launch_bassemble(hfpx_ctx* ctx, const SrcElem<double>* elem, const double* c
 int launch_brhs(hfpx_ctx* ctx, const SrcElem<double>* elem, int64_t n_el_total, const BCrack* cr, int ncr,
                 double* b);
 // batched Schur-complement update (hfpx_gemm.hip) of step (k, j) for the
-// cracks list[0 .. nact): outer = false: panel update inside the outer
-// block; outer = true: trailing update with K = kb
+// cracks list[0 .. nact) (bgemm_shape); tstart (device, nact + 1 entries)
+// is the prefix sum of bgemm_tiles along the list, tiles = tstart[nact]
 template <typename T>
 int launch_bgemm(hfpx_ctx* ctx, T* base, int which, const BCrack* cr, const int32_t* list, int nact,
-                 int64_t nmax, int64_t k, int64_t j, bool outer);
+                 const int32_t* tstart, int64_t tiles, int64_t k, int64_t j, bool outer);
 
 // stress postprocessing (hfpx_stress.hip)
 int launch_stress(hfpx_ctx* ctx, hfpx_mesh* m, double shear_m, double nu, const double* dd, int64_t ldd,

@@ -194,16 +194,23 @@ int hfpx_batch_matrix(hfpx_ctx* ctx, hfpx_batch* batch, int crack, double* A_hos
 /* ---- tuning -------------------------------------------------------------- */
 /* Kernel-variant knobs of a context (no reference counterpart).
  * HFPX_KNOB_GEMM_F32: FP32 trailing-update GEMM (products with more than 64
- * columns): 0 = automatic (default: 6 when M >= 2048 and N >= 512, else 2);
+ * columns): 0 = automatic (default: for K <= 384, 8 when M >= 2048 and
+ * N >= 512, else 7; for longer K, 6 when M >= 2048 and N >= 512, else 2);
  * 1 = 16x16x4 MFMA, 128 x 128 tile; 2..6 = 32x32x2 MFMA, 128 x 128 tile,
  * K chunk 16, with 2 no options, 3 unguarded interior tiles (edge bands in
  * separate launches), 4 = 3 + pipelined LDS operands, 5 = 4 + XCD-grouped
- * tile order, 6 = 3 + XCD-grouped tile order. */
+ * tile order, 6 = 3 + XCD-grouped tile order; 7 = 2 and 8 = 6 with C loaded
+ * into the accumulators before the product (no read-modify-write at the
+ * end). */
 #define HFPX_KNOB_GEMM_F32 0
 /* HFPX_KNOB_LU_NB: outer block of the LU (columns per trailing update):
  * 0 = default (1024), or one of 256, 512, 1024, 2048, 4096. */
 #define HFPX_KNOB_LU_NB 1
-#define HFPX_KNOB_COUNT 2
+/* HFPX_KNOB_BGEMM_F32: FP32 Schur-complement updates of the batched LU:
+ * 0 = default (2), 1 = read-modify-write epilogue, 2 = C loaded into the
+ * accumulators first. */
+#define HFPX_KNOB_BGEMM_F32 2
+#define HFPX_KNOB_COUNT 3
 /* Set / read one knob; HFPX_ERR_INVALID for an unknown knob or value. */
 int hfpx_tune_set(hfpx_ctx* ctx, int knob, int value);
 int hfpx_tune_get(const hfpx_ctx* ctx, int knob, int* value);

@@ -53,16 +53,22 @@ def test_batch_assembly_matches_single_crack(env):
         assert np.abs(bt.rhs(i) - b1).max() <= 1e-15 * np.abs(b1).max(), i
 
 
-@pytest.mark.parametrize("use_graph", [False, True])
-def test_batch_solve_matches_lapack(env, use_graph):
+@pytest.mark.parametrize("use_graph,bgemm", [(False, 0), (True, 0), (False, 1)])
+def test_batch_solve_matches_lapack(env, use_graph, bgemm):
     """n from 360 to 1,620 DoF; cracks with n >= 900 take FP32 LU + 3
-    refinement steps, the others FP64 LU."""
+    refinement steps, the others FP64 LU.  bgemm selects the FP32 update
+    epilogue (0 default: C preloaded, 1: read-modify-write)."""
     torch, hx = env
+    from hfpx3d_amd import _lib
     sl = pytest.importorskip("scipy.linalg")
-    print(f"case: batch solve graph={use_graph}", flush=True)
+    print(f"case: batch solve graph={use_graph} bgemm={bgemm}", flush=True)
     cracks = make_cracks(10, 7, 20, 90)
     bt = hx.Batch(cracks, mixed_min_n=900)
-    bt.solve(n_refine=3, use_graph=use_graph)
+    hx.tune_set(_lib.KNOB_BGEMM_F32, bgemm)
+    try:
+        bt.solve(n_refine=3, use_graph=use_graph)
+    finally:
+        hx.tune_set(_lib.KNOB_BGEMM_F32, 0)
     groups = set()
     for i, c in enumerate(cracks):
         A1, b1 = single_crack(hx, c)

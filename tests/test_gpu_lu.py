@@ -42,7 +42,7 @@ def test_gemm_sub(env, dt, M, N, K):
     assert np.abs(got - ref).max() / (np.abs(A).max() * np.abs(B).max() * K) < tol
 
 
-@pytest.mark.parametrize("variant", range(1, 7))
+@pytest.mark.parametrize("variant", range(1, 9))
 @pytest.mark.parametrize("M,N,K", [(300, 257, 33), (513, 700, 100), (1000, 300, 256), (130, 129, 17),
                                    (256, 256, 32), (77, 200, 20), (512, 384, 256)])
 def test_sgemm_variants(env, variant, M, N, K):
