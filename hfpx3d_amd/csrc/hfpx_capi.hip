@@ -390,6 +390,11 @@ int hfpx_tune_set(hfpx_ctx* ctx, int knob, int value) {
     ctx->knob[knob] = value;
     return HFPX_OK;
   }
+  if (knob == HFPX_KNOB_PANEL) {
+    if (value < 0 || value > 2) return set_error(ctx, HFPX_ERR_INVALID, "tune: PANEL must be 0, 1 or 2");
+    ctx->knob[knob] = value;
+    return HFPX_OK;
+  }
   if (knob == HFPX_KNOB_BGEMM_F32) {
     if (value < 0 || value > 2) return set_error(ctx, HFPX_ERR_INVALID, "tune: BGEMM_F32 must be 0, 1 or 2");
     ctx->knob[knob] = value;

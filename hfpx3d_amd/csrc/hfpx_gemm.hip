@@ -390,6 +390,28 @@ __global__ __launch_bounds__(256) void k_sgemm32(float* __restrict__ C, int64_t 
   sgemm32_tile<WR, WC, KC, OPT>(sA, sB, C, ldc, A, lda, B, ldb, M, N, K, tile_r * S::GM, tile_c * S::GN);
 }
 
+// The same for the larger tiles (WR WC > 4), whose accumulators would
+// otherwise be placed in AGPRs beside a full VGPR set (one wave per SIMD):
+// bounded to two waves per SIMD, the compiler keeps everything in VGPRs.
+template <int WR, int WC, int KC, int OPT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_sgemm32_w2(float* __restrict__ C, int64_t ldc,
+                                                 const float* __restrict__ A, int64_t lda,
+                                                 const float* __restrict__ B, int64_t ldb, int64_t M,
+                                                 int64_t N, int64_t K) {
+  typedef Sg32<WR, WC, KC> S;
+  __shared__ float sA[2 * S::LA];
+  __shared__ float sB[2 * S::LB];
+  int64_t tile_r, tile_c;
+  if constexpr ((OPT & SG_SWZ) != 0) {
+    const int64_t tm = (M + S::GM - 1) / S::GM, tn = (N + S::GN - 1) / S::GN;
+    sg_band_tile(sg_xcd_order(blockIdx.x, tm * tn), tm, tn, tile_r, tile_c);
+  } else {
+    tile_r = blockIdx.x;
+    tile_c = blockIdx.y;
+  }
+  sgemm32_tile<WR, WC, KC, OPT>(sA, sB, C, ldc, A, lda, B, ldb, M, N, K, tile_r * S::GM, tile_c * S::GN);
+}
+
 // ---------------------------------------------------------------------
 // FP64 P_z = -A(:, slice z) X(slice z, :) for N <= 64 and long K (the
 // residuals b - A x of the mixed-precision solve): A is streamed once, so
@@ -517,8 +539,12 @@ static void launch_sgemm32_grid(hfpx_ctx* ctx, float* C, int64_t ldc, const floa
   if (M <= 0 || N <= 0) return;
   const int64_t tm = (M + 64 * WR - 1) / (64 * WR), tn = (N + 64 * WC - 1) / (64 * WC);
   const dim3 grid = (OPT & SG_SWZ) ? dim3((unsigned)(tm * tn)) : dim3((unsigned)tm, (unsigned)tn);
-  hipLaunchKernelGGL((k_sgemm32<WR, WC, KC, OPT>), grid, dim3(256), 0, ctx->stream, C, ldc, A, lda, B, ldb, M,
-                     N, K);
+  if constexpr (WR * WC > 4)
+    hipLaunchKernelGGL((k_sgemm32_w2<WR, WC, KC, OPT>), grid, dim3(256), 0, ctx->stream, C, ldc, A, lda, B, ldb,
+                       M, N, K);
+  else
+    hipLaunchKernelGGL((k_sgemm32<WR, WC, KC, OPT>), grid, dim3(256), 0, ctx->stream, C, ldc, A, lda, B, ldb, M,
+                       N, K);
 }
 
 // With SG_INT: the whole tiles by the interior kernel, then the bottom band
@@ -537,7 +563,7 @@ static void launch_sgemm32(hfpx_ctx* ctx, float* C, int64_t ldc, const float* A,
   }
 }
 
-int gemm_f32_variants() { return 9; }
+int gemm_f32_variants() { return 13; }
 
 template <typename T>
 int gemm_sub(hfpx_ctx* ctx, T* C, int64_t ldc, const T* A, int64_t lda, const T* B,
@@ -566,6 +592,10 @@ int gemm_sub(hfpx_ctx* ctx, T* C, int64_t ldc, const T* A, int64_t lda, const T*
         case 5: launch_sgemm32<2, 2, 16, SG_INT | SG_PIPE | SG_SWZ>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
         case 7: launch_sgemm32<2, 2, 16, SG_CPRE>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
         case 8: launch_sgemm32<2, 2, 16, SG_INT | SG_SWZ | SG_CPRE>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
+        case 9: launch_sgemm32<4, 2, 16, SG_INT | SG_SWZ>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
+        case 10: launch_sgemm32<2, 4, 16, SG_INT | SG_SWZ>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
+        case 11: launch_sgemm32<4, 2, 16, SG_INT | SG_SWZ | SG_CPRE>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
+        case 12: launch_sgemm32<4, 2, 16, SG_INT | SG_SWZ | SG_PIPE>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
         default: launch_sgemm32<2, 2, 16, SG_INT | SG_SWZ>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
       }
       done = true;

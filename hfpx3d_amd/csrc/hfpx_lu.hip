@@ -85,14 +85,14 @@ struct PanelCtx {
   int32_t row;  // row (relative to j0) held by this thread, -1 if none
   PanelScratch<T>* ps;
   T* rows;
-  int32_t* ipiv;
-  int32_t* info;
   int32_t* err;
   uint64_t* ctr;
   uint64_t base;
   T* red_v;
   int32_t* red_i;
   T* s_piv;
+  int32_t* s_ipiv;  // pivots of the panel's columns (LDS, written out at the end)
+  int32_t* s_info;  // first column with a zero pivot, -1: none
 };
 
 // (bv, bi) := better of itself and (ov, oi): larger value, then smaller row;
@@ -107,15 +107,101 @@ __device__ __forceinline__ bool pick(T& bv, int32_t& bi, T ov, int32_t oi) {
   return false;
 }
 
+// Wave-wide maxima on DPP lane moves (no LDS round trips): xor 1 and 2
+// inside quads (quad_perm), then row_half_mirror and row_mirror inside rows
+// of 16, then row_bcast15 / row_bcast31 fold the rows into lane 63, which
+// is broadcast.  Lanes a move does not write keep 0, the identity.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v) {
+  return ((uint64_t)dpp_u32<CTRL, ROWS>((uint32_t)(v >> 32)) << 32) | dpp_u32<CTRL, ROWS>((uint32_t)v);
+}
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
+  uint64_t o;
+  o = dpp_u64<0xB1, 0xf>(v);  v = o > v ? o : v;   // quad_perm [1,0,3,2]
+  o = dpp_u64<0x4E, 0xf>(v);  v = o > v ? o : v;   // quad_perm [2,3,0,1]
+  o = dpp_u64<0x141, 0xf>(v); v = o > v ? o : v;   // row_half_mirror
+  o = dpp_u64<0x140, 0xf>(v); v = o > v ? o : v;   // row_mirror
+  o = dpp_u64<0x142, 0xa>(v); v = o > v ? o : v;   // row_bcast15 -> rows 1, 3
+  o = dpp_u64<0x143, 0xc>(v); v = o > v ? o : v;   // row_bcast31 -> rows 2, 3
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 63);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+  uint32_t o;
+  o = dpp_u32<0xB1, 0xf>(v);  v = o > v ? o : v;
+  o = dpp_u32<0x4E, 0xf>(v);  v = o > v ? o : v;
+  o = dpp_u32<0x141, 0xf>(v); v = o > v ? o : v;
+  o = dpp_u32<0x140, 0xf>(v); v = o > v ? o : v;
+  o = dpp_u32<0x142, 0xa>(v); v = o > v ? o : v;
+  o = dpp_u32<0x143, 0xc>(v); v = o > v ? o : v;
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// Wave-wide arg-max of the lanes' (bv >= 0, row bi; bi < 0 = none), the
+// same order as pick: largest value, then smallest row.  Non-negative IEEE
+// values order as their bit patterns, so FP32 is one maximum of the 64-bit
+// key bits(bv):~bi; FP64 takes the largest bits(bv), then the largest ~bi
+// among the lanes holding it.  The result is uniform over the wave.
 template <typename T>
-__device__ __forceinline__ void wave_argmax(T& bv, int32_t& bi, int32_t& bw) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const T ov = __shfl_xor(bv, off);
-    const int32_t oi = __shfl_xor(bi, off);
-    const int32_t ow = __shfl_xor(bw, off);
-    if (pick(bv, bi, ov, oi)) bw = ow;
+__device__ __forceinline__ void wave_argmax(T& bv, int32_t& bi) {
+  const bool valid = bi >= 0;
+  if constexpr (sizeof(T) == 4) {
+    uint64_t k = valid ? (((uint64_t)__float_as_uint(bv) << 32) | (uint32_t)~bi) : 0;
+    k = wave_max_u64(k);
+    bv = k ? __uint_as_float((uint32_t)(k >> 32)) : T(-1);
+    bi = k ? (int32_t)~(uint32_t)k : -1;
+  } else {
+    const uint64_t bits = valid ? (uint64_t)__double_as_longlong(bv) : 0;
+    const uint64_t mb = wave_max_u64(bits);
+    uint32_t k2 = (valid && bits == mb) ? ~(uint32_t)bi : 0u;
+    k2 = wave_max_u32(k2);
+    bv = k2 ? __longlong_as_double((long long)mb) : T(-1);
+    bi = k2 ? (int32_t)~k2 : -1;
   }
+}
+
+// Workgroup arg-max of column J over the rows the threads hold (uniform
+// result; one workgroup barrier).
+template <typename T, int IB>
+__device__ __forceinline__ void wg_argmax(const T (&a)[IB], int J, int32_t row, int t, T* red_v, int32_t* red_i,
+                                          T& bv, int32_t& bi) {
+  const int lane = t & 63;
+  bv = T(-1);
+  bi = -1;
+  if (row >= J) {
+    bv = fabs(a[J]);
+    bi = row;
+  }
+  wave_argmax(bv, bi);
+  if (lane == 0) {
+    red_v[t >> 6] = bv;
+    red_i[t >> 6] = bi;
+  }
+  __syncthreads();
+  bv = T(-1);
+  bi = -1;
+  if (lane < PWG / 64) {
+    bv = red_v[lane];
+    bi = red_i[lane];
+  }
+  wave_argmax(bv, bi);
+}
+
+// The panel's pivots are kept in LDS and written once at the end:
+// s_ipiv[J] = 1-based global pivot row of column J, s_info = first column
+// with an exactly zero pivot (-1: none).
+template <typename T>
+__device__ __forceinline__ void panel_finish(int w, int t, int jb, int64_t j0, const int32_t* s_ipiv,
+                                             const int32_t* s_info, int32_t* ipiv, int32_t* info) {
+  if (w != 0) return;
+  __syncthreads();
+  if (t < jb) ipiv[j0 + t] = s_ipiv[t];
+  if (t == 0 && *s_info >= 0 && *info == 0) *info = (int32_t)(j0 + *s_info + 1);
 }
 
 template <typename T, int IB, int J>
@@ -125,25 +211,9 @@ __device__ __forceinline__ void panel_step(T (&a)[IB], const PanelCtx<T>& P) {
       const int par = J & 1;
       const int t = P.t, lane = t & 63;
       // 1. candidate of this workgroup
-      T bv = T(-1);
-      int32_t bi = -1, bw = 0;
-      if (P.row >= J) {
-        bv = fabs(a[J]);
-        bi = P.row;
-      }
-      wave_argmax(bv, bi, bw);
-      if (lane == 0) {
-        P.red_v[t >> 6] = bv;
-        P.red_i[t >> 6] = bi;
-      }
-      __syncthreads();
-      bv = T(-1);
-      bi = -1;
-      if (lane < PWG / 64) {
-        bv = P.red_v[lane];
-        bi = P.red_i[lane];
-      }
-      wave_argmax(bv, bi, bw);  // now uniform over the workgroup
+      T bv;
+      int32_t bi;
+      wg_argmax<T, IB>(a, J, P.row, t, P.red_v, P.red_i, bv, bi);
       T* cand = P.rows + ((size_t)par * (P.nwg + 1) + P.w) * IB;
       T* rowj = P.rows + ((size_t)par * (P.nwg + 1) + P.nwg) * IB;
       if (t == 0) {
@@ -159,20 +229,35 @@ __device__ __forceinline__ void panel_step(T (&a)[IB], const PanelCtx<T>& P) {
         for (int c = 0; c < IB; ++c) rowj[c] = a[c];
       }
       grid_barrier(P.ctr, P.base + (uint64_t)(J + 1) * P.nwg, P.err);
-      // 2. global pivot, reduced redundantly by every wave
+      // 2. global pivot, reduced redundantly by every wave: four slots per
+      // lane, all loads issued before any is used
       T gv = T(-1);
-      int32_t gi = -1, gw = 0;
-      for (int q = lane; q < P.nwg; q += 64)
-        if (pick(gv, gi, P.ps->val[par][q], P.ps->idx[par][q])) gw = q;
-      wave_argmax(gv, gi, gw);
+      int32_t gi = -1;
+      for (int q0 = 0; q0 < P.nwg; q0 += 256) {
+        T cv[4];
+        int32_t ci[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int q = q0 + lane + 64 * k;
+          ci[k] = -1;
+          if (q < P.nwg) {
+            cv[k] = P.ps->val[par][q];
+            ci[k] = P.ps->idx[par][q];
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) pick(gv, gi, cv[k], ci[k]);
+      }
+      wave_argmax(gv, gi);
+      const int gw = gi >= 0 ? gi / PWG : 0;  // the workgroup holding the pivot row
       const T* prow = P.rows + ((size_t)par * (P.nwg + 1) + gw) * IB;
       if (t < IB) P.s_piv[t] = prow[t];
       __syncthreads();
       const T pv = P.s_piv[J];
       const bool zero = (pv == T(0));
-      if (P.w == 0 && t == 0) {
-        P.ipiv[P.j0 + J] = (int32_t)(P.j0 + (zero ? J : gi) + 1);
-        if (zero && *P.info == 0) *P.info = (int32_t)(P.j0 + J + 1);
+      if (t == 0) {
+        P.s_ipiv[J] = (int32_t)(P.j0 + (zero ? J : gi) + 1);
+        if (zero && *P.s_info < 0) *P.s_info = J;
       }
       const T inv = zero ? T(0) : T(1) / pv;
       // 3. interchange + rank-1 update of this thread's row
@@ -208,17 +293,236 @@ __global__ __launch_bounds__(PWG) void k_panel(T* __restrict__ A, int64_t lda, i
   __shared__ T red_v[PWG / 64];
   __shared__ int32_t red_i[PWG / 64];
   __shared__ T s_piv[IB];
+  __shared__ int32_t s_ipiv[IB], s_info;
+  if (t == 0) s_info = -1;  // read by thread 0 only until the closing barrier
   T a[IB];
   const bool ok = r < m;
 #pragma unroll
   for (int c = 0; c < IB; ++c) a[c] = (ok && c < jb) ? A[(j0 + r) + (j0 + c) * lda] : T(0);
-  PanelCtx<T> P{nwg, w, t, jb, j0, ok ? (int32_t)r : -1, ps, rows, ipiv, info, err, ctr, base,
-                red_v, red_i, s_piv};
+  PanelCtx<T> P{nwg, w, t, jb, j0, ok ? (int32_t)r : -1, ps, rows, err, ctr, base,
+                red_v, red_i, s_piv, s_ipiv, &s_info};
   panel_step<T, IB, 0>(a, P);
   if (ok)
 #pragma unroll
     for (int c = 0; c < IB; ++c)
       if (c < jb) A[(j0 + r) + (j0 + c) * lda] = a[c];
+  panel_finish<T>(w, t, jb, j0, s_ipiv, &s_info, ipiv, info);
+}
+
+// ---------------------------------------------------------------------
+// The same panel factorisation without a grid barrier.  Per column every
+// workgroup publishes its candidate row (the full row of jb values) and
+// its index as 64-bit words whose high half is the column's sequence number
+// and low half 32 bits of payload; workgroup 0 also publishes the current
+// row J (the row the pivot row is exchanged with).  Every workgroup then
+// polls, with agent-scope atomic loads, the column-J value and the index of
+// every candidate until each carries the current sequence number (staged in
+// LDS), picks the pivot, and reads the pivot row's words (and, in the
+// pivot's workgroup, row J's) the same way.  A word is single-copy atomic,
+// so it validates itself: no fences and no arrival counter, two short
+// round trips per column.  The last four waves publish and the first ones
+// poll, so that no polling wave waits on its own stores.  Two buffers
+// alternate by column parity: a workgroup writes column J + 2 only after
+// reading every word of column J + 1 it needs, which each workgroup
+// publishes only after it has read what it needs of column J.  Sequence
+// numbers are the global column + 1, unique within one factorisation (the
+// buffer is cleared before it), so stale words are never taken for current
+// ones.  Polls are bounded like the grid barrier: past ~2^22 polls *err is
+// raised and the panel completes with garbage (reported by the host), never
+// a hang.  Measured on MI355X the sc1 polls cost more than the barrier's
+// fences, so this variant is selectable (HFPX_KNOB_PANEL = 2), not the
+// default.
+// ---------------------------------------------------------------------
+constexpr int PX_MAX_WG = 256;  // LDS staging bound (one 1024-thread workgroup per CU)
+
+template <typename T, int IB>
+struct PxFmt {
+  static constexpr int W = (int)(sizeof(T) / 4);  // payload words per value
+  static constexpr int WORDS = IB * W + 1;        // row words + index word
+};
+
+template <typename T>
+__device__ __forceinline__ T px_val(const uint32_t* w, int c) {
+  if constexpr (sizeof(T) == 4)
+    return __uint_as_float(w[c]);
+  else
+    return __hiloint2double((int)w[2 * c + 1], (int)w[2 * c]);
+}
+
+template <typename T>
+__device__ __forceinline__ void px_put(uint32_t* w, int c, T v) {
+  if constexpr (sizeof(T) == 4) {
+    w[c] = __float_as_uint(v);
+  } else {
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    w[2 * c] = (uint32_t)u;
+    w[2 * c + 1] = (uint32_t)(u >> 32);
+  }
+}
+
+// Payload of the tagged word at p once it carries sequence number seq
+// (bounded wait; *err raised on time-out).
+__device__ __forceinline__ uint32_t px_get(uint64_t* p, uint32_t seq, int32_t* err) {
+  uint64_t v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint32_t polls = 0;
+  while ((uint32_t)(v >> 32) != seq) {
+    __builtin_amdgcn_s_sleep(1);
+    if (++polls > (1u << 22)) {
+      __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      break;
+    }
+    v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return (uint32_t)v;
+}
+
+template <typename T, int IB>
+struct PxCtx {
+  int nwg, w, t, jb;
+  int64_t j0;
+  int32_t row;
+  uint64_t* xw;  // [2][nwg + 1][WORDS]
+  int32_t* err;
+  T* red_v;
+  int32_t* red_i;
+  uint32_t* s_cand;  // IB W words: this workgroup's candidate row
+  uint32_t* s_rowj;  // IB W words: row J (workgroup 0)
+  uint32_t* s_all;   // (nwg + 1) WORDS staged payloads
+  int32_t* s_ipiv;
+  int32_t* s_info;
+};
+
+constexpr int PX_PUB = PWG - 128;   // first publishing thread (waves 14, 15)
+constexpr int PX_ROWJ = PWG - 256;  // first thread publishing row J (waves 12, 13)
+
+template <typename T, int IB, int J>
+__device__ __forceinline__ void panel_step_x(T (&a)[IB], const PxCtx<T, IB>& P) {
+  if constexpr (J < IB) {
+    if (J < P.jb) {
+      typedef PxFmt<T, IB> F;
+      constexpr int RW = IB * F::W;  // row words
+      static_assert(RW + 1 <= 128 && RW <= 128, "publisher waves");
+      const int t = P.t, lane = t & 63;
+      const uint32_t seq = (uint32_t)(P.j0 + J + 1);
+      uint64_t* buf = P.xw + (size_t)(J & 1) * (P.nwg + 1) * F::WORDS;
+      // 1. candidate of this workgroup
+      T bv;
+      int32_t bi;
+      wg_argmax<T, IB>(a, J, P.row, t, P.red_v, P.red_i, bv, bi);
+      if (P.row >= 0 && P.row == bi) {
+#pragma unroll
+        for (int c = 0; c < IB; ++c) px_put<T>(P.s_cand, c, a[c]);
+      }
+      if (P.row == J) {
+#pragma unroll
+        for (int c = 0; c < IB; ++c) px_put<T>(P.s_rowj, c, a[c]);
+      }
+      __syncthreads();
+      // 2. publish
+      const uint64_t tag = (uint64_t)seq << 32;
+      if (t >= PX_PUB) {
+        const int k = t - PX_PUB;
+        if (k < RW)
+          __hip_atomic_store(buf + (size_t)P.w * F::WORDS + k, tag | P.s_cand[k], __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+        else if (k == RW)
+          __hip_atomic_store(buf + (size_t)P.w * F::WORDS + RW, tag | (uint32_t)bi, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+      } else if (P.w == 0 && t >= PX_ROWJ && t < PX_ROWJ + RW) {
+        const int k = t - PX_ROWJ;
+        __hip_atomic_store(buf + (size_t)P.nwg * F::WORDS + k, tag | P.s_rowj[k], __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      }
+      // 3a. column J and the index of every candidate, one word per thread
+      constexpr int PA = F::W + 1;  // words per slot in this phase
+      if (t < P.nwg * PA) {
+        const int q = t / PA, k = t % PA;
+        const int wd = k < F::W ? J * F::W + k : RW;
+        P.s_all[q * F::WORDS + wd] = px_get(buf + (size_t)q * F::WORDS + wd, seq, P.err);
+      }
+      __syncthreads();
+      // 4. global pivot, reduced redundantly by every wave from LDS
+      T gv = T(-1);
+      int32_t gi = -1;
+#pragma unroll
+      for (int k = 0; k < PX_MAX_WG / 64; ++k) {
+        const int q = lane + 64 * k;
+        if (q < P.nwg) {
+          const uint32_t* sq = P.s_all + q * F::WORDS;
+          pick(gv, gi, (T)fabs(px_val<T>(sq, J)), (int32_t)sq[RW]);
+        }
+      }
+      wave_argmax(gv, gi);
+      const int gw = gi >= 0 ? gi / PWG : 0;  // the workgroup holding the pivot row
+      // 3b. the pivot row (held locally by its own workgroup) and, in that
+      // workgroup only, row J (held locally by workgroup 0)
+      uint32_t* prow = P.s_all + gw * F::WORDS;
+      const uint32_t* rowj = P.w == 0 ? P.s_rowj : P.s_all + P.nwg * F::WORDS;
+      if (t < RW) {
+        if (gw == P.w)
+          prow[t] = P.s_cand[t];
+        else if (t / F::W != J)
+          prow[t] = px_get(buf + (size_t)gw * F::WORDS + t, seq, P.err);
+      } else if (t >= PWG / 2 && t < PWG / 2 + RW && gw == P.w && P.w != 0) {
+        const int k = t - PWG / 2;
+        P.s_all[P.nwg * F::WORDS + k] = px_get(buf + (size_t)P.nwg * F::WORDS + k, seq, P.err);
+      }
+      __syncthreads();
+      const T pv = px_val<T>(prow, J);
+      const bool zero = (pv == T(0));
+      if (t == 0) {
+        P.s_ipiv[J] = (int32_t)(P.j0 + (zero ? J : gi) + 1);
+        if (zero && *P.s_info < 0) *P.s_info = J;
+      }
+      const T inv = zero ? T(0) : T(1) / pv;
+      // 5. interchange + rank-1 update of this thread's row
+      if (P.row >= J) {
+        if (!zero && P.row == J) {
+#pragma unroll
+          for (int c = 0; c < IB; ++c) a[c] = px_val<T>(prow, c);
+        } else {
+          if (!zero && P.row == gi) {
+#pragma unroll
+            for (int c = 0; c < IB; ++c) a[c] = px_val<T>(rowj, c);
+          }
+          const T l = a[J] * inv;
+          a[J] = l;
+#pragma unroll
+          for (int c = J + 1; c < IB; ++c) a[c] -= l * px_val<T>(prow, c);
+        }
+      }
+    }
+    panel_step_x<T, IB, J + 1>(a, P);
+  }
+}
+
+template <typename T, int IB>
+__global__ __launch_bounds__(PWG) void k_panel_x(T* __restrict__ A, int64_t lda, int64_t n, int64_t j0, int jb,
+                                                 int32_t* __restrict__ ipiv, int32_t* __restrict__ info,
+                                                 int32_t* __restrict__ err, uint64_t* __restrict__ xw) {
+  typedef PxFmt<T, IB> F;
+  static_assert(PX_MAX_WG * (F::W + 1) <= PX_ROWJ, "polling threads overlap the publishing waves");
+  const int nwg = gridDim.x, w = blockIdx.x, t = threadIdx.x;
+  const int64_t m = n - j0;
+  const int64_t r = (int64_t)w * PWG + t;
+  __shared__ T red_v[PWG / 64];
+  __shared__ int32_t red_i[PWG / 64];
+  __shared__ uint32_t s_cand[IB * F::W], s_rowj[IB * F::W];
+  __shared__ uint32_t s_all[(PX_MAX_WG + 1) * F::WORDS];
+  __shared__ int32_t s_ipiv[IB], s_info;
+  if (t == 0) s_info = -1;
+  T a[IB];
+  const bool ok = r < m;
+#pragma unroll
+  for (int c = 0; c < IB; ++c) a[c] = (ok && c < jb) ? A[(j0 + r) + (j0 + c) * lda] : T(0);
+  PxCtx<T, IB> P{nwg, w, t, jb, j0, ok ? (int32_t)r : -1, xw, err, red_v, red_i, s_cand, s_rowj, s_all,
+                 s_ipiv, &s_info};
+  panel_step_x<T, IB, 0>(a, P);
+  if (ok)
+#pragma unroll
+    for (int c = 0; c < IB; ++c)
+      if (c < jb) A[(j0 + r) + (j0 + c) * lda] = a[c];
+  panel_finish<T>(w, t, jb, j0, s_ipiv, &s_info, ipiv, info);
 }
 
 // ---------------------------------------------------------------------
@@ -483,7 +787,7 @@ static int ensure_lu_work(hfpx_ctx* ctx, size_t bytes) {
 
 template <typename T, int IB>
 static int panel(hfpx_ctx* ctx, T* A, int64_t lda, int64_t n, int64_t j0, int jb, int32_t* ipiv, int32_t* dflags,
-                 PanelScratch<T>* ps, T* rows, int max_wg) {
+                 PanelScratch<T>* ps, T* rows, uint64_t* xw, int max_wg) {
   const int64_t m = n - j0;
   const int64_t nwg64 = (m + PWG - 1) / PWG;
   if (nwg64 > max_wg)
@@ -494,7 +798,13 @@ static int panel(hfpx_ctx* ctx, T* A, int64_t lda, int64_t n, int64_t j0, int jb
   int32_t* err = dflags + 1;
   uint64_t* ctr = ctx->sync;
   uint64_t base = ctx->sync_base;
+  // the grid-barrier panel unless the exchange panel is asked for (and the
+  // grid fits its LDS staging); on MI355X the barrier panel is the faster
+  // (72,000 rows FP32: 396 against 559 ms per factorisation)
+  const bool xchg = ctx->knob[HFPX_KNOB_PANEL] == 2 && nwg <= PX_MAX_WG;
   void* args[] = {&A, &lda, &n, &j0, &jb, &ipiv, &info, &err, &ps, &rows, &ctr, &base};
+  void* xargs[] = {&A, &lda, &n, &j0, &jb, &ipiv, &info, &err, &xw};
+  const void* kern = xchg ? (const void*)k_panel_x<T, IB> : (const void*)k_panel<T, IB>;
   const int tok = prof_begin(ctx);
   // A cooperative launch guarantees that the workgroups are co-resident.
   // The runtime refuses it while another process holds the device's
@@ -502,18 +812,22 @@ static int panel(hfpx_ctx* ctx, T* A, int64_t lda, int64_t n, int64_t j0, int jb
   // launched normally, and the barrier's bounded wait turns a workgroup that
   // never becomes resident into an error instead of a hang.
   hipError_t e = hipErrorIllegalState;
-  if (!ctx->no_coop) e = hipLaunchCooperativeKernel((const void*)k_panel<T, IB>, dim3(nwg), dim3(PWG), args, 0,
-                                                   ctx->stream);
+  if (!ctx->no_coop)
+    e = hipLaunchCooperativeKernel(kern, dim3(nwg), dim3(PWG), xchg ? xargs : args, 0, ctx->stream);
   if (ctx->no_coop || e == hipErrorIllegalState) {
     if (!ctx->no_coop) (void)hipGetLastError();
     ctx->no_coop = true;
-    hipLaunchKernelGGL((k_panel<T, IB>), dim3(nwg), dim3(PWG), 0, ctx->stream, A, lda, n, j0, jb, ipiv, info, err,
-                       ps, rows, ctr, base);
+    if (xchg)
+      hipLaunchKernelGGL((k_panel_x<T, IB>), dim3(nwg), dim3(PWG), 0, ctx->stream, A, lda, n, j0, jb, ipiv, info,
+                         err, xw);
+    else
+      hipLaunchKernelGGL((k_panel<T, IB>), dim3(nwg), dim3(PWG), 0, ctx->stream, A, lda, n, j0, jb, ipiv, info,
+                         err, ps, rows, ctr, base);
     e = hipGetLastError();
   }
   prof_end(ctx, tok, HFPX_KC_PANEL, (double)m * jb * jb, 0.0);
   HFPX_TRY_HIP(ctx, e);
-  ctx->sync_base += (uint64_t)jb * nwg;
+  if (!xchg) ctx->sync_base += (uint64_t)jb * nwg;
   return HFPX_OK;
 }
 
@@ -530,6 +844,7 @@ struct LuWork {
   int32_t* dflags;  // [0] info, [1] barrier timeout
   PanelScratch<T>* ps;
   T* rows;
+  uint64_t* xw;  // exchange words of k_panel_x, cleared per factorisation
   PermPairs* pp;  // LU_MAX_PAIR_BLOCKS blocks
   int max_wg;
 };
@@ -552,7 +867,8 @@ static int swap_cols(hfpx_ctx* ctx, const LuWork<T>& W, T* A, int64_t lda, const
 template <typename T>
 static int factor_cols(hfpx_ctx* ctx, const LuWork<T>& W, T* A, int64_t lda, int64_t n, int64_t j, int64_t w,
                        int32_t* ipiv) {
-  if (w <= LU_IB) return panel<T, LU_IB>(ctx, A, lda, n, j, (int)w, ipiv, W.dflags, W.ps, W.rows, W.max_wg);
+  if (w <= LU_IB)
+    return panel<T, LU_IB>(ctx, A, lda, n, j, (int)w, ipiv, W.dflags, W.ps, W.rows, W.xw, W.max_wg);
   const int64_t w1 = ((w / 2 + LU_IB - 1) / LU_IB) * LU_IB;
   int rc;
   if ((rc = factor_cols<T>(ctx, W, A, lda, n, j, w1, ipiv))) return rc;
@@ -571,22 +887,28 @@ static int getrf_impl(hfpx_ctx* ctx, T* A, int64_t lda, int64_t n, int32_t* ipiv
   static_assert(LU_NB_MAX % SWAP_BLK == 0, "pair blocks per outer block");
   const int64_t NB = ctx->knob[HFPX_KNOB_LU_NB] > 0 ? ctx->knob[HFPX_KNOB_LU_NB] : LU_NB;
   // co-resident panel workgroups
-  int occ = 0;
+  int occ = 0, occx = 0;
   HFPX_TRY_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_panel<T, LU_IB>, PWG, 0));
+  HFPX_TRY_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occx, k_panel_x<T, LU_IB>, PWG, 0));
+  occ = std::min(occ, occx);
   if (occ < 1) return set_error(ctx, HFPX_ERR_HIP, "getrf: panel kernel cannot be resident");
   LuWork<T> W;
   W.max_wg = std::min(PANEL_MAX_WG, occ * std::max(1, ctx->num_cus));
   const size_t rows_bytes = sizeof(T) * 2 * (size_t)(W.max_wg + 1) * LU_IB;
   const size_t off_ps = 256, off_rows = off_ps + ((sizeof(PanelScratch<T>) + 255) & ~(size_t)255);
   const size_t off_pp = off_rows + ((rows_bytes + 255) & ~(size_t)255);
-  int rc = ensure_lu_work(ctx, off_pp + LU_MAX_PAIR_BLOCKS * sizeof(PermPairs));
+  const size_t xw_bytes = sizeof(uint64_t) * 2 * (PX_MAX_WG + 1) * PxFmt<T, LU_IB>::WORDS;
+  const size_t off_xw = off_pp + ((LU_MAX_PAIR_BLOCKS * sizeof(PermPairs) + 255) & ~(size_t)255);
+  int rc = ensure_lu_work(ctx, off_xw + xw_bytes);
   if (rc) return rc;
   char* w = (char*)ctx->lu_work;
   W.dflags = (int32_t*)w;
   W.ps = (PanelScratch<T>*)(w + off_ps);
   W.rows = (T*)(w + off_rows);
   W.pp = (PermPairs*)(w + off_pp);
+  W.xw = (uint64_t*)(w + off_xw);
   HFPX_TRY_HIP(ctx, hipMemsetAsync(W.dflags, 0, 2 * sizeof(int32_t), ctx->stream));
+  HFPX_TRY_HIP(ctx, hipMemsetAsync(W.xw, 0, xw_bytes, ctx->stream));
   HFPX_TRY_HIP(ctx, hipMemsetAsync(ctx->sync, 0, sizeof(uint64_t), ctx->stream));
   ctx->sync_base = 0;
 

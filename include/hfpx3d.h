@@ -210,7 +210,11 @@ int hfpx_batch_matrix(hfpx_ctx* ctx, hfpx_batch* batch, int crack, double* A_hos
  * 0 = default (2), 1 = read-modify-write epilogue, 2 = C loaded into the
  * accumulators first. */
 #define HFPX_KNOB_BGEMM_F32 2
-#define HFPX_KNOB_COUNT 3
+/* HFPX_KNOB_PANEL: LU panel kernel: 0 = default (1), 1 = grid-barrier
+ * panel, 2 = exchange panel (per-column tagged words, no barrier; used for
+ * grids up to 256 workgroups, i.e. n up to 262,144). */
+#define HFPX_KNOB_PANEL 3
+#define HFPX_KNOB_COUNT 4
 /* Set / read one knob; HFPX_ERR_INVALID for an unknown knob or value. */
 int hfpx_tune_set(hfpx_ctx* ctx, int knob, int value);
 int hfpx_tune_get(const hfpx_ctx* ctx, int knob, int* value);

@@ -140,6 +140,54 @@ def test_getrf_outer_block_sizes_match_lapack(env, nb):
     assert np.abs(lu.A.cpu().numpy() - lu2).max() < 1e-12 * np.abs(lu2).max() * n / 100
 
 
+@pytest.mark.parametrize("dt,n", [("f64", 2500), ("f32", 2500), ("f32", 5000), ("f64", 1100)])
+def test_getrf_panel_kernels_agree(env, dt, n):
+    """The exchange panel (tagged words, no grid barrier) and the grid-
+    barrier panel do the same arithmetic in the same order: bitwise equal
+    factors and pivots, and LAPACK's pivots in FP64.  n = 5,000 takes five
+    panel workgroups, n = 1,100 two (the second mostly idle)."""
+    torch, hx = env
+    sl = pytest.importorskip("scipy.linalg")
+    from hfpx3d_amd import _lib
+    print(f"case: getrf panels {dt} n={n}", flush=True)
+    dtype = torch.float64 if dt == "f64" else torch.float32
+    A = np.random.default_rng(n + 17).normal(size=(n, n))
+    out = {}
+    try:
+        for knob in (1, 2):
+            hx.tune_set(_lib.KNOB_PANEL, knob)
+            lu = hx.LU(to_dev(torch, hx, A, dtype))
+            out[knob] = (lu.A.cpu().numpy(), lu.ipiv.cpu().numpy())
+    finally:
+        hx.tune_set(_lib.KNOB_PANEL, 0)
+    assert np.array_equal(out[1][1], out[2][1])
+    assert np.array_equal(out[1][0], out[2][0])
+    if dt == "f64":
+        lu2, piv2 = sl.lu_factor(A)
+        assert np.array_equal(out[2][1] - 1, piv2)
+
+
+@pytest.mark.parametrize("dt", ["f64", "f32"])
+@pytest.mark.parametrize("panel", [1, 2])
+def test_getrf_pivot_ties_take_first_row(env, dt, panel):
+    """Equal |a| in the first column, spread over several panel workgroups
+    (n = 3,000: rows 1,500, 1,700 (negative), 2,100 and 2,999): the first
+    such row is the pivot, as in LAPACK (idamax takes the first maximum)."""
+    torch, hx = env
+    from hfpx3d_amd import _lib
+    print(f"case: getrf ties {dt} panel={panel}", flush=True)
+    n = 3000
+    A = np.random.default_rng(3).uniform(-1, 1, size=(n, n))
+    for r, v in ((2100, 5.0), (1700, -5.0), (2999, 5.0), (1500, 5.0)):
+        A[r, 0] = v
+    try:
+        hx.tune_set(_lib.KNOB_PANEL, panel)
+        lu = hx.LU(to_dev(torch, hx, A, torch.float64 if dt == "f64" else torch.float32))
+    finally:
+        hx.tune_set(_lib.KNOB_PANEL, 0)
+    assert int(lu.ipiv.cpu().numpy()[0]) == 1501
+
+
 @pytest.mark.parametrize("n", [300, 2049])
 def test_getrf_f32_backward_error(env, n):
     torch, hx = env
