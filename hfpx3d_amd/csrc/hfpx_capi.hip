@@ -151,6 +151,7 @@ int hfpx_destroy(hfpx_ctx* ctx) {
   if (ctx->work) (void)hipFree(ctx->work);
   if (ctx->lu_work) (void)hipFree(ctx->lu_work);
   if (ctx->gemm_work) (void)hipFree(ctx->gemm_work);
+  if (ctx->x3_work) (void)hipFree(ctx->x3_work);
   if (ctx->sync) (void)hipFree(ctx->sync);
   if (ctx->hwork) (void)hipHostFree(ctx->hwork);
   prof_clear(ctx);

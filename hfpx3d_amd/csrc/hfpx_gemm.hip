@@ -176,7 +176,6 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 //           concurrent workgroups share stay in that XCD's L2;
 //  SG_CPRE  C preloaded into the accumulators (no read-modify-write at the end).
 constexpr int SG_INT = 1, SG_PIPE = 2, SG_SWZ = 4, SG_CPRE = 8;
-constexpr int SG_GROUP = 8;
 
 // LDS footprint of one 32x32x2 tile (floats per buffer, two buffers).
 template <int WR, int WC, int KC>
@@ -350,25 +349,6 @@ __device__ __forceinline__ void sgemm32_tile(float* __restrict__ sA, float* __re
         }
       }
     }
-}
-
-// Tile (tile_r, tile_c) of a tm x tn tile grid for position nid of the
-// grid's walk in bands of SG_GROUP row tiles (column by column inside a
-// band), so that consecutive positions share A row blocks and B columns.
-__device__ __forceinline__ void sg_band_tile(int64_t nid, int64_t tm, int64_t tn, int64_t& tile_r, int64_t& tile_c) {
-  const int64_t per = (int64_t)SG_GROUP * tn, first = (nid / per) * SG_GROUP;
-  const int64_t gs = (tm - first) < SG_GROUP ? (tm - first) : SG_GROUP;
-  const int64_t r = nid % per;
-  tile_r = first + r % gs;
-  tile_c = r / gs;
-}
-
-// Position of workgroup id in a walk of nt tiles where each of the 8 XCDs
-// (workgroups are dealt to XCDs round-robin) owns one contiguous eighth.
-__device__ __forceinline__ int64_t sg_xcd_order(int64_t id, int64_t nt) {
-  const int64_t xcd = id % 8, local = id / 8;
-  const int64_t fl = nt / 8, rem = nt % 8;
-  return xcd * fl + (xcd < rem ? xcd : rem) + local;
 }
 
 template <int WR, int WC, int KC, int OPT>
@@ -563,7 +543,7 @@ static void launch_sgemm32(hfpx_ctx* ctx, float* C, int64_t ldc, const float* A,
   }
 }
 
-int gemm_f32_variants() { return 13; }
+int gemm_f32_variants() { return 14; }
 
 template <typename T>
 int gemm_sub(hfpx_ctx* ctx, T* C, int64_t ldc, const T* A, int64_t lda, const T* B,
@@ -577,14 +557,18 @@ int gemm_sub(hfpx_ctx* ctx, T* C, int64_t ldc, const T* A, int64_t lda, const T*
   const int tok = prof_begin(ctx);
   int var = ctx->knob[HFPX_KNOB_GEMM_F32];
   if (var == 0) {
-    // automatic choice: short products are latency-bound (C preloaded),
-    // long ones MFMA-bound (the higher-occupancy kernels)
-    const bool big = M >= 2048 && N >= 512;
-    var = K <= 384 ? (big ? 8 : 7) : (big ? 6 : 2);
+    // automatic choice: the bf16-split product wherever it has tiles to
+    // fill (faster on every LU shape measured, 1.5x on the large trailing
+    // updates); small products on FP32 MFMA, C preloaded for short K
+    var = (M >= 256 && N >= 128 && K >= 64) ? 13 : (K <= 384 ? 7 : 2);
   }
   bool done = false;
   if constexpr (sizeof(T) == 4) {
-    if (!skinny && var > 1) {
+    if (!skinny && var == 13) {
+      int rc = gemm_x3(ctx, C, ldc, A, lda, B, ldb, M, N, K);
+      if (rc) return rc;
+      done = true;
+    } else if (!skinny && var > 1) {
       switch (var) {
         case 2: launch_sgemm32<2, 2, 16, 0>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;
         case 3: launch_sgemm32<2, 2, 16, SG_INT>(ctx, C, ldc, A, lda, B, ldb, M, N, K); break;

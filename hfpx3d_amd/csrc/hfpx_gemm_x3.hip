@@ -1,0 +1,246 @@
+// hfpx_gemm_x3.hip -- FP32 C -= A B on the bf16 MFMA units of gfx950 by
+// three-way operand splitting, for the large trailing updates of the FP32
+// LU.
+//
+// Every FP32 operand x is written exactly as x = h + m + l + r with h, m, l
+// bf16 (h = bf16(x), m = bf16(x - h), l = bf16(x - h - m); each difference
+// is exact in FP32) and |r| <= 2^-24 |x| or so.  With |m| ~ 2^-9 |x| and
+// |l| ~ 2^-18 |x|, the product a b is h_a h_b + h_a m_b + m_a h_b + h_a l_b +
+// l_a h_b + m_a m_b up to terms of relative size 2^-27, below FP32 rounding:
+// six bf16 products (each exact, accumulated in FP32 by the MFMA) give an
+// FP32-accurate product at up to 16/6 the rate of v_mfma_f32_32x32x2f32.
+//
+// Two passes: the split kernels write the parts of A (M x K) and B (K x N)
+// in blocks of 128 rows (columns of B) by 16 k, each block [part][row][k]
+// contiguous (12 KB) and zero-padded, B negated; the product kernel preloads
+// the tile of C into its accumulators and adds the six products of the
+// negated B parts and the A parts.  No direct reference counterpart: inside
+// LAPACKE_dgetrf (il/linear_algebra/dense/factorization/LU.h:79-138) this is
+// the dgemm of the trailing update, done here in FP32 for the mixed solve.
+#include <algorithm>
+
+#include "hfpx_internal.h"
+
+namespace hfpx {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16_x3 __attribute__((ext_vector_type(16)));
+
+constexpr int X3_T = 128;                  // rows (A) / columns (B) of a block
+constexpr int X3_KC = 16;                  // k of a block (one MFMA k-step)
+constexpr int X3_PIECES = 3 * X3_T * 2;    // 16-byte pieces of a block (768)
+constexpr int X3_BLK = 3 * X3_T * X3_KC;   // bf16 per block
+constexpr int X3_ROW = 24;                 // LDS pitch of a 16-k row (bf16): 48 B keeps
+                                           // the 16-byte reads of 8 lanes on distinct banks
+
+__device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l) {
+  h = (__bf16)x;
+  const float r1 = x - (float)h;
+  m = (__bf16)r1;
+  const float r2 = r1 - (float)m;
+  l = (__bf16)r2;
+}
+
+// A (M x K, lda) -> blocks [row tile it][chunk kc][part][row][k], sign * A.
+// Thread t writes piece t of each part: row t / 2, k 8 (t % 2) .. + 7.
+__global__ __launch_bounds__(256) void k_split_a(const float* __restrict__ A, int64_t lda, int64_t M, int64_t K,
+                                                 int nkc, __bf16* __restrict__ out, float sign) {
+  __shared__ float s[X3_KC][X3_T + 1];
+  const int it = blockIdx.x, kc = blockIdx.y, t = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < X3_T * X3_KC / 256; ++q) {
+    const int e = t + 256 * q, i = e % X3_T, k = e / X3_T;
+    const int64_t gi = (int64_t)it * X3_T + i, gk = (int64_t)kc * X3_KC + k;
+    s[k][i] = (gi < M && gk < K) ? A[gi + gk * lda] : 0.f;
+  }
+  __syncthreads();
+  const int il = t >> 1, half = t & 1;
+  bf16x8 ph, pm, pl;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    __bf16 h, m, l;
+    split3(sign * s[8 * half + e][il], h, m, l);
+    ph[e] = h;
+    pm[e] = m;
+    pl[e] = l;
+  }
+  bf16x8* blk = (bf16x8*)(out + ((size_t)it * nkc + kc) * X3_BLK);
+  blk[t] = ph;
+  blk[256 + t] = pm;
+  blk[512 + t] = pl;
+}
+
+// B (K x N, ldb) -> blocks [column tile jt][chunk kc][part][column][k].
+__global__ __launch_bounds__(256) void k_split_b(const float* __restrict__ B, int64_t ldb, int64_t K, int64_t N,
+                                                 int nkc, __bf16* __restrict__ out, float sign) {
+  const int jt = blockIdx.x, kc = blockIdx.y, t = threadIdx.x;
+  const int jl = t >> 1, half = t & 1;
+  const int64_t gj = (int64_t)jt * X3_T + jl, k0 = (int64_t)kc * X3_KC + 8 * half;
+  bf16x8 ph, pm, pl;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const float x = (gj < N && k0 + e < K) ? B[(k0 + e) + gj * ldb] : 0.f;
+    __bf16 h, m, l;
+    split3(sign * x, h, m, l);
+    ph[e] = h;
+    pm[e] = m;
+    pl[e] = l;
+  }
+  bf16x8* blk = (bf16x8*)(out + ((size_t)jt * nkc + kc) * X3_BLK);
+  blk[t] = ph;
+  blk[256 + t] = pm;
+  blk[512 + t] = pl;
+}
+
+// C (M x N) += As Bs over nkc chunks: 128 x 128 tiles, 4 waves of 64 x 64
+// (2 x 2 MFMA blocks of 32 x 32), one 16-k chunk per LDS buffer (double
+// buffered, one barrier per chunk).  As the FP32 kernels, the MFMA forms the
+// transposed product: operand a is a block of Bs^T (C columns), operand b a
+// block of As^T (C rows), so lane l % 32 runs down 32 consecutive rows of C.
+// v_mfma_f32_32x32x16_bf16: lane l supplies a[l%32][8(l/32) + e] and
+// b[8(l/32) + e][l%32], e = 0..7; result element v of lane l is
+// D[8(v/4) + 4(l/32) + v%4][l%32].  Tiles are walked as in the FP32 kernel
+// (each XCD a contiguous range, bands of SG_GROUP row tiles).
+// Bounded to two waves per SIMD (the LDS of two workgroups), the compiler
+// keeps accumulators and operands in VGPRs.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_x3gemm(
+    float* __restrict__ C, int64_t ldc, const __bf16* __restrict__ As, const __bf16* __restrict__ Bs, int64_t M,
+    int64_t N, int nkc, int64_t tm, int64_t tn) {
+  __shared__ __attribute__((aligned(16))) __bf16 sA[2][3 * X3_T * X3_ROW];
+  __shared__ __attribute__((aligned(16))) __bf16 sB[2][3 * X3_T * X3_ROW];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int l32 = lane & 31, hk = lane >> 5;
+  int64_t tile_r, tile_c;
+  sg_band_tile(sg_xcd_order(blockIdx.x, tm * tn), tm, tn, tile_r, tile_c);
+  const int64_t row0 = tile_r * X3_T, col0 = tile_c * X3_T;
+  const int wr = (wave & 1) * 64, wc = (wave >> 1) * 64;
+  const __bf16* pa = As + (size_t)tile_r * nkc * X3_BLK;
+  const __bf16* pb = Bs + (size_t)tile_c * nkc * X3_BLK;
+
+  // accumulators start as the tile of C (32-bit offsets from its corner)
+  float* __restrict__ Ct = C + row0 + col0 * ldc;
+  const int ld = (int)ldc;
+  const int off0 = wr + l32 + (wc + 4 * hk) * ld;
+  const int mrem = (int)((M - row0) < X3_T ? (M - row0) : X3_T);
+  const int nrem = (int)((N - col0) < X3_T ? (N - col0) : X3_T);
+  floatx16_x3 acc[2][2];
+#pragma unroll
+  for (int tc = 0; tc < 2; ++tc)
+#pragma unroll
+    for (int tr = 0; tr < 2; ++tr) {
+      const int i = wr + 32 * tr + l32;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int jc = wc + 32 * tc + 8 * (v >> 2) + 4 * hk + (v & 3);
+        const int off = off0 + 32 * tr + (32 * tc + 8 * (v >> 2) + (v & 3)) * ld;
+        acc[tc][tr][v] = (i < mrem && jc < nrem) ? Ct[off] : 0.f;
+      }
+    }
+
+  // staging: piece t + 256 q of a block is part q, row t / 2, k 8 (t % 2) .. + 7
+  typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+  const u32x4* pa4 = (const u32x4*)pa;
+  const u32x4* pb4 = (const u32x4*)pb;
+  u32x4 ra[3], rb[3];
+  auto gload = [&](int kc) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      ra[q] = pa4[(size_t)kc * X3_PIECES + t + 256 * q];
+      rb[q] = pb4[(size_t)kc * X3_PIECES + t + 256 * q];
+    }
+  };
+  const int soff = (t >> 1) * X3_ROW + 8 * (t & 1);
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      *(u32x4*)&sA[buf][q * X3_T * X3_ROW + soff] = ra[q];
+      *(u32x4*)&sB[buf][q * X3_T * X3_ROW + soff] = rb[q];
+    }
+  };
+
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  for (int kc = 0; kc < nkc; ++kc) {
+    const int buf = kc & 1;
+    if (kc + 1 < nkc) gload(kc + 1);
+    bf16x8 fa[2][3], fb[2][3];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        fa[u][p] = *(const bf16x8*)&sB[buf][(p * X3_T + wc + 32 * u + l32) * X3_ROW + 8 * hk];
+        fb[u][p] = *(const bf16x8*)&sA[buf][(p * X3_T + wr + 32 * u + l32) * X3_ROW + 8 * hk];
+      }
+#pragma unroll
+    for (int tc = 0; tc < 2; ++tc)
+#pragma unroll
+      for (int tr = 0; tr < 2; ++tr) {
+        floatx16_x3 c = acc[tc][tr];
+        // small terms first
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[tc][1], fb[tr][1], c, 0, 0, 0);  // m m
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[tc][2], fb[tr][0], c, 0, 0, 0);  // l h
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[tc][0], fb[tr][2], c, 0, 0, 0);  // h l
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[tc][1], fb[tr][0], c, 0, 0, 0);  // m h
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[tc][0], fb[tr][1], c, 0, 0, 0);  // h m
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[tc][0], fb[tr][0], c, 0, 0, 0);  // h h
+        acc[tc][tr] = c;
+      }
+    if (kc + 1 < nkc) lstore(buf ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int tc = 0; tc < 2; ++tc)
+#pragma unroll
+    for (int tr = 0; tr < 2; ++tr) {
+      const int i = wr + 32 * tr + l32;
+      if (i >= mrem) continue;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int jc = wc + 32 * tc + 8 * (v >> 2) + 4 * hk + (v & 3);
+        const int off = off0 + 32 * tr + (32 * tc + 8 * (v >> 2) + (v & 3)) * ld;
+        if (jc < nrem) Ct[off] = acc[tc][tr][v];
+      }
+    }
+}
+
+// Grow-only workspace of the split operands.
+static int ensure_x3_work(hfpx_ctx* ctx, size_t bytes) {
+  if (ctx->x3_work_bytes >= bytes) return HFPX_OK;
+  if (ctx->x3_work) {
+    HFPX_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    HFPX_TRY_HIP(ctx, hipFree(ctx->x3_work));
+    ctx->x3_work = nullptr;
+    ctx->x3_work_bytes = 0;
+  }
+  HFPX_TRY_HIP(ctx, hipMalloc(&ctx->x3_work, bytes));
+  ctx->x3_work_bytes = bytes;
+  return HFPX_OK;
+}
+
+size_t gemm_x3_work_bytes(int64_t M, int64_t N, int64_t K) {
+  const int64_t tm = (M + X3_T - 1) / X3_T, tn = (N + X3_T - 1) / X3_T, nkc = (K + X3_KC - 1) / X3_KC;
+  return sizeof(__bf16) * (size_t)(tm + tn) * nkc * X3_BLK;
+}
+
+int gemm_x3(hfpx_ctx* ctx, float* C, int64_t ldc, const float* A, int64_t lda, const float* B, int64_t ldb,
+            int64_t M, int64_t N, int64_t K) {
+  if (M <= 0 || N <= 0 || K <= 0) return HFPX_OK;
+  if (ldc > (int64_t)1 << 24) return set_error(ctx, HFPX_ERR_INVALID, "gemm: ldc above 2^24");
+  const int64_t tm = (M + X3_T - 1) / X3_T, tn = (N + X3_T - 1) / X3_T, nkc = (K + X3_KC - 1) / X3_KC;
+  if (tm > 65535 || tn > 65535 || nkc > 65535) return set_error(ctx, HFPX_ERR_INVALID, "gemm: too large");
+  int rc = ensure_x3_work(ctx, gemm_x3_work_bytes(M, N, K));
+  if (rc) return rc;
+  __bf16* As = (__bf16*)ctx->x3_work;
+  __bf16* Bs = As + (size_t)tm * nkc * X3_BLK;
+  hipLaunchKernelGGL(k_split_a, dim3((unsigned)tm, (unsigned)nkc), dim3(256), 0, ctx->stream, A, lda, M, K,
+                     (int)nkc, As, 1.0f);
+  hipLaunchKernelGGL(k_split_b, dim3((unsigned)tn, (unsigned)nkc), dim3(256), 0, ctx->stream, B, ldb, K, N,
+                     (int)nkc, Bs, -1.0f);
+  hipLaunchKernelGGL(k_x3gemm, dim3((unsigned)(tm * tn)), dim3(256), 0, ctx->stream, C, ldc, As, Bs, M, N,
+                     (int)nkc, tm, tn);
+  HFPX_TRY_HIP(ctx, hipGetLastError());
+  return HFPX_OK;
+}
+
+}  // namespace hfpx

@@ -25,6 +25,8 @@ struct hfpx_ctx {
   size_t lu_work_bytes = 0;
   void* gemm_work = nullptr;  // grow-only device workspace of split-K products
   size_t gemm_work_bytes = 0;
+  void* x3_work = nullptr;    // grow-only workspace of the bf16-split GEMM operands
+  size_t x3_work_bytes = 0;
   uint64_t* sync = nullptr;  // grid-barrier arrival counter (device)
   uint64_t sync_base = 0;    // its value once every launched barrier completes
   int num_cus = 0;
@@ -109,6 +111,28 @@ __host__ __device__ inline bool bgemm_shape(int64_t n, int64_t k, int64_t j, boo
   return M > 0 && N > 0 && K > 0;
 }
 
+// Walk of a GEMM's tile grid (shared by the FP32, bf16-split and batched
+// kernels).
+constexpr int SG_GROUP = 8;
+// Tile (tile_r, tile_c) of a tm x tn tile grid for position nid of the
+// grid's walk in bands of SG_GROUP row tiles (column by column inside a
+// band), so that consecutive positions share A row blocks and B columns.
+__device__ inline void sg_band_tile(int64_t nid, int64_t tm, int64_t tn, int64_t& tile_r, int64_t& tile_c) {
+  const int64_t per = (int64_t)SG_GROUP * tn, first = (nid / per) * SG_GROUP;
+  const int64_t gs = (tm - first) < SG_GROUP ? (tm - first) : SG_GROUP;
+  const int64_t r = nid % per;
+  tile_r = first + r % gs;
+  tile_c = r / gs;
+}
+
+// Position of workgroup id in a walk of nt tiles where each of the 8 XCDs
+// (workgroups are dealt to XCDs round-robin) owns one contiguous eighth.
+__device__ inline int64_t sg_xcd_order(int64_t id, int64_t nt) {
+  const int64_t xcd = id % 8, local = id / 8;
+  const int64_t fl = nt / 8, rem = nt % 8;
+  return xcd * fl + (xcd < rem ? xcd : rem) + local;
+}
+
 // Tile (rows and columns) of the batched Schur-complement updates, and the
 // number of tiles of one crack's product.
 constexpr int BGT = 128;
@@ -146,6 +170,11 @@ int lu_getrf(hfpx_ctx* ctx, int dtype, void* A, int64_t lda, int64_t n, int32_t*
 int lu_getrs(hfpx_ctx* ctx, int dtype, const void* LU, int64_t lda, const int32_t* ipiv, int64_t n,
              void* B, int64_t ldb, int64_t nrhs);
 int gemm_f32_variants();
+// FP32 C -= A B by three-way bf16 operand splitting on bf16 MFMA
+// (hfpx_gemm_x3.hip); workspace bytes for an M x N x K product
+int gemm_x3(hfpx_ctx* ctx, float* C, int64_t ldc, const float* A, int64_t lda, const float* B, int64_t ldb,
+            int64_t M, int64_t N, int64_t K);
+size_t gemm_x3_work_bytes(int64_t M, int64_t N, int64_t K);
 // C (M x N) -= A (M x K) B (K x N), column-major, on the context stream
 // (hfpx_gemm.hip; instantiated for float and double).
 template <typename T>

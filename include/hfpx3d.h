@@ -194,14 +194,17 @@ int hfpx_batch_matrix(hfpx_ctx* ctx, hfpx_batch* batch, int crack, double* A_hos
 /* ---- tuning -------------------------------------------------------------- */
 /* Kernel-variant knobs of a context (no reference counterpart).
  * HFPX_KNOB_GEMM_F32: FP32 trailing-update GEMM (products with more than 64
- * columns): 0 = automatic (default: for K <= 384, 8 when M >= 2048 and
- * N >= 512, else 7; for longer K, 6 when M >= 2048 and N >= 512, else 2);
- * 1 = 16x16x4 MFMA, 128 x 128 tile; 2..6 = 32x32x2 MFMA, 128 x 128 tile,
- * K chunk 16, with 2 no options, 3 unguarded interior tiles (edge bands in
- * separate launches), 4 = 3 + pipelined LDS operands, 5 = 4 + XCD-grouped
- * tile order, 6 = 3 + XCD-grouped tile order; 7 = 2 and 8 = 6 with C loaded
- * into the accumulators before the product (no read-modify-write at the
- * end). */
+ * columns): 0 = automatic (13 when M >= 256, N >= 128 and K >= 64; else 7
+ * for K <= 384, 2 for longer K); 1 = 16x16x4 FP32 MFMA, 128 x 128 tile;
+ * 2..12 = 32x32x2 FP32 MFMA, K chunk 16, with 2 = 128 x 128 tile, no
+ * options, 3 = unguarded interior tiles (edge bands in separate launches),
+ * 4 = 3 + pipelined LDS operands, 5 = 4 + XCD-grouped tile order, 6 = 3 +
+ * XCD-grouped tile order, 7 = 2 and 8 = 6 with C loaded into the
+ * accumulators first, 9 = 6 and 11 = 8 with 256 x 128 tiles, 10 = 6 with
+ * 128 x 256 tiles, 12 = 9 + pipelined LDS operands; 13 = bf16 MFMA
+ * (32x32x16) on three-way bf16 splits of both operands, six products per
+ * pair, FP32 accumulation: FP32-accurate, not bitwise equal to the FP32
+ * MFMA kernels. */
 #define HFPX_KNOB_GEMM_F32 0
 /* HFPX_KNOB_LU_NB: outer block of the LU (columns per trailing update):
  * 0 = default (1024), or one of 256, 512, 1024, 2048, 4096. */

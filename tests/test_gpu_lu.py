@@ -42,7 +42,7 @@ def test_gemm_sub(env, dt, M, N, K):
     assert np.abs(got - ref).max() / (np.abs(A).max() * np.abs(B).max() * K) < tol
 
 
-@pytest.mark.parametrize("variant", range(1, 9))
+@pytest.mark.parametrize("variant", range(1, 14))
 @pytest.mark.parametrize("M,N,K", [(300, 257, 33), (513, 700, 100), (1000, 300, 256), (130, 129, 17),
                                    (256, 256, 32), (77, 200, 20), (512, 384, 256)])
 def test_sgemm_variants(env, variant, M, N, K):
@@ -64,6 +64,40 @@ def test_sgemm_variants(env, variant, M, N, K):
     ref = C.astype(np.float32).astype(np.float64) - A.astype(np.float32).astype(np.float64) @ \
         B.astype(np.float32).astype(np.float64)
     assert np.abs(got - ref).max() / (np.abs(A).max() * np.abs(B).max() * K) < 2e-6
+
+
+@pytest.mark.parametrize("scaled", [False, True])
+def test_sgemm_bf16_split_is_fp32_accurate(env, scaled):
+    """The three-way bf16 split product (variant 13) against float64 on the
+    same FP32 inputs: its error stays within a small factor of the FP32
+    MFMA kernel's (variant 2), i.e. FP32-level, also when the rows of A and
+    the columns of B span 10^-8 .. 10^8 (products stay inside FP32's range;
+    a dropped low part would cost a factor 2^6 or more)."""
+    torch, hx = env
+    from hfpx3d_amd import _lib
+    print(f"case: sgemm bf16 split accuracy scaled={scaled}", flush=True)
+    M, N, K = 700, 650, 1024
+    rng = np.random.default_rng(11)
+    A, B, C = rng.normal(size=(M, K)), rng.normal(size=(K, N)), rng.normal(size=(M, N))
+    if scaled:
+        A *= 10.0 ** rng.uniform(-8, 8, size=(M, 1))
+        B *= 10.0 ** rng.uniform(-8, 8, size=(1, N))
+        C *= np.abs(A).max(axis=1, keepdims=True) * np.abs(B).max(axis=0, keepdims=True)
+    A32, B32, C32 = (x.astype(np.float32).astype(np.float64) for x in (A, B, C))
+    ref = C32 - A32 @ B32
+    scale = (np.abs(A32) @ np.abs(B32)) + np.abs(C32)  # elementwise error scale
+    err = {}
+    old = hx.tune_get(_lib.KNOB_GEMM_F32)
+    try:
+        for v in (2, 13):
+            hx.tune_set(_lib.KNOB_GEMM_F32, v)
+            Cd = to_dev(torch, hx, C32, torch.float32)
+            hx.gemm_sub(Cd, to_dev(torch, hx, A32, torch.float32), to_dev(torch, hx, B32, torch.float32))
+            err[v] = (np.abs(Cd.cpu().numpy() - ref) / scale).max()
+    finally:
+        hx.tune_set(_lib.KNOB_GEMM_F32, old)
+    assert err[2] < 1e-6
+    assert err[13] < 3 * err[2] + 1e-7, err
 
 
 def test_tune_rejects_unknown_variant(env):
