@@ -391,6 +391,11 @@ int hfpx_tune_set(hfpx_ctx* ctx, int knob, int value) {
     ctx->knob[knob] = value;
     return HFPX_OK;
   }
+  if (knob == HFPX_KNOB_X3_BAND) {
+    if (value < 0 || value > 64) return set_error(ctx, HFPX_ERR_INVALID, "tune: X3_BAND must be in [0, 64]");
+    ctx->knob[knob] = value;
+    return HFPX_OK;
+  }
   if (knob == HFPX_KNOB_PANEL) {
     if (value < 0 || value > 2) return set_error(ctx, HFPX_ERR_INVALID, "tune: PANEL must be 0, 1 or 2");
     ctx->knob[knob] = value;

@@ -100,18 +100,19 @@ __global__ __launch_bounds__(256) void k_split_b(const float* __restrict__ B, in
 // v_mfma_f32_32x32x16_bf16: lane l supplies a[l%32][8(l/32) + e] and
 // b[8(l/32) + e][l%32], e = 0..7; result element v of lane l is
 // D[8(v/4) + 4(l/32) + v%4][l%32].  Tiles are walked as in the FP32 kernel
-// (each XCD a contiguous range, bands of SG_GROUP row tiles).
+// (each XCD a contiguous range, in bands of row tiles), the band height
+// chosen by the host so that the band's A parts fit an XCD's L2.
 // Bounded to two waves per SIMD (the LDS of two workgroups), the compiler
 // keeps accumulators and operands in VGPRs.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_x3gemm(
     float* __restrict__ C, int64_t ldc, const __bf16* __restrict__ As, const __bf16* __restrict__ Bs, int64_t M,
-    int64_t N, int nkc, int64_t tm, int64_t tn) {
+    int64_t N, int nkc, int64_t tm, int64_t tn, int64_t band) {
   __shared__ __attribute__((aligned(16))) __bf16 sA[2][3 * X3_T * X3_ROW];
   __shared__ __attribute__((aligned(16))) __bf16 sB[2][3 * X3_T * X3_ROW];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int l32 = lane & 31, hk = lane >> 5;
   int64_t tile_r, tile_c;
-  sg_band_tile(sg_xcd_order(blockIdx.x, tm * tn), tm, tn, tile_r, tile_c);
+  sg_band_tile(sg_xcd_order(blockIdx.x, tm * tn), tm, tn, tile_r, tile_c, band);
   const int64_t row0 = tile_r * X3_T, col0 = tile_c * X3_T;
   const int wr = (wave & 1) * 64, wc = (wave >> 1) * 64;
   const __bf16* pa = As + (size_t)tile_r * nkc * X3_BLK;
@@ -138,6 +139,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     }
 
   // staging: piece t + 256 q of a block is part q, row t / 2, k 8 (t % 2) .. + 7
+  // (one chunk ahead: loading two ahead measured slower)
   typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
   const u32x4* pa4 = (const u32x4*)pa;
   const u32x4* pb4 = (const u32x4*)pb;
@@ -237,8 +239,12 @@ int gemm_x3(hfpx_ctx* ctx, float* C, int64_t ldc, const float* A, int64_t lda, c
                      (int)nkc, As, 1.0f);
   hipLaunchKernelGGL(k_split_b, dim3((unsigned)tn, (unsigned)nkc), dim3(256), 0, ctx->stream, B, ldb, K, N,
                      (int)nkc, Bs, -1.0f);
+  // band height: the band's A parts (128 rows x K, 6 bytes per entry) in
+  // the 4 MB L2 of an XCD, between 1 and 16 row tiles
+  int64_t band = ctx->knob[HFPX_KNOB_X3_BAND];
+  if (band <= 0) band = std::max<int64_t>(1, std::min<int64_t>(16, (4 << 20) / ((int64_t)X3_T * nkc * X3_KC * 6)));
   hipLaunchKernelGGL(k_x3gemm, dim3((unsigned)(tm * tn)), dim3(256), 0, ctx->stream, C, ldc, As, Bs, M, N,
-                     (int)nkc, tm, tn);
+                     (int)nkc, tm, tn, band);
   HFPX_TRY_HIP(ctx, hipGetLastError());
   return HFPX_OK;
 }

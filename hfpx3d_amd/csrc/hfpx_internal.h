@@ -115,11 +115,12 @@ __host__ __device__ inline bool bgemm_shape(int64_t n, int64_t k, int64_t j, boo
 // kernels).
 constexpr int SG_GROUP = 8;
 // Tile (tile_r, tile_c) of a tm x tn tile grid for position nid of the
-// grid's walk in bands of SG_GROUP row tiles (column by column inside a
+// grid's walk in bands of group row tiles (column by column inside a
 // band), so that consecutive positions share A row blocks and B columns.
-__device__ inline void sg_band_tile(int64_t nid, int64_t tm, int64_t tn, int64_t& tile_r, int64_t& tile_c) {
-  const int64_t per = (int64_t)SG_GROUP * tn, first = (nid / per) * SG_GROUP;
-  const int64_t gs = (tm - first) < SG_GROUP ? (tm - first) : SG_GROUP;
+__device__ inline void sg_band_tile(int64_t nid, int64_t tm, int64_t tn, int64_t& tile_r, int64_t& tile_c,
+                                    int64_t group = SG_GROUP) {
+  const int64_t per = group * tn, first = (nid / per) * group;
+  const int64_t gs = (tm - first) < group ? (tm - first) : group;
   const int64_t r = nid % per;
   tile_r = first + r % gs;
   tile_c = r / gs;

@@ -217,7 +217,11 @@ int hfpx_batch_matrix(hfpx_ctx* ctx, hfpx_batch* batch, int crack, double* A_hos
  * panel, 2 = exchange panel (per-column tagged words, no barrier; used for
  * grids up to 256 workgroups, i.e. n up to 262,144). */
 #define HFPX_KNOB_PANEL 3
-#define HFPX_KNOB_COUNT 4
+/* HFPX_KNOB_X3_BAND: row tiles per band of the bf16-split product's tile
+ * walk: 0 = automatic (the band's split A panel in an XCD's L2),
+ * 1 .. 64 = fixed. */
+#define HFPX_KNOB_X3_BAND 4
+#define HFPX_KNOB_COUNT 5
 /* Set / read one knob; HFPX_ERR_INVALID for an unknown knob or value. */
 int hfpx_tune_set(hfpx_ctx* ctx, int knob, int value);
 int hfpx_tune_get(const hfpx_ctx* ctx, int knob, int* value);
