@@ -26,9 +26,15 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAKS = {"hbm_GBs": 8000.0, "fp64_mfma_TFs": 78.6, "fp32_mfma_TFs": 157.3}
+PEAKS = {"hbm_GBs": 8000.0, "fp64_mfma_TFs": 78.6, "fp32_mfma_TFs": 157.3, "bf16_mfma_TFs": 2500.0}
+# The FP32 trailing updates run as six bf16 MFMA products of three-way operand
+# splits: their FP32-equivalent ceiling is the dense bf16 peak / 6.
+X3_PRODUCTS = 6
+X3_PEAK = PEAKS["bf16_mfma_TFs"] / X3_PRODUCTS
 # bytes per call of the representative trailing update, from rocprofv3 --pmc (profiles/gemm72k_pmc.md)
-GEMM72K_TRAFFIC = 1.0540e+11
+# bytes per call of the first trailing update (bench.py --workload gemm72k),
+# from the FETCH_SIZE / WRITE_SIZE passes summarised in profiles/
+GEMM72K_TRAFFIC = 2.106e+11
 GEMM72K_ALG_BYTES = 4.0882e+10
 
 
@@ -194,6 +200,9 @@ def main():
         ctx.stream.synchronize()
         log(f"bench: warmup {w + 1}/{args.warmup} done")
 
+    # timed steps: HIP events only around the dominant kernel class (the
+    # trailing-update GEMM), for its live roofline
+    ctx.check(lib.hfpx_profile_classes(ctx.handle, 1 << _lib.KERNEL_CLASSES["gemm_f32"]))
     ctx.check(lib.hfpx_profile(ctx.handle, 0 if args.no_lib_prof else 1))
     recs = []
     torch.cuda.synchronize()
@@ -205,14 +214,17 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_step = wall / args.steps * 1e3
-    prof = {}
-    for name, kc in _lib.KERNEL_CLASSES.items():
-        L, ms, fl, by = (__import__("ctypes").c_int64(), __import__("ctypes").c_double(),
-                         __import__("ctypes").c_double(), __import__("ctypes").c_double())
-        ctx.check(lib.hfpx_profile_read(ctx.handle, kc, L, ms, fl, by))
-        prof[name] = dict(launches=L.value // args.steps, ms=ms.value / args.steps,
-                          flops=fl.value / args.steps, bytes=by.value / args.steps)
+    g = read_profile(ctx, args.steps)["gemm_f32"]
     ctx.check(lib.hfpx_profile(ctx.handle, 0))
+    # per-class breakdown from one more, untimed step with every class timed
+    prof = {}
+    if not args.no_lib_prof:
+        ctx.check(lib.hfpx_profile_classes(ctx.handle, 0xFFFFFFFF))
+        ctx.check(lib.hfpx_profile(ctx.handle, 1))
+        step(None)
+        ctx.stream.synchronize()
+        prof = read_profile(ctx, 1)
+        ctx.check(lib.hfpx_profile(ctx.handle, 0))
     t_asm = np.mean([a.elapsed_time(b) for a, b, *_ in recs]) / 1e3
     t_solve = np.mean([b.elapsed_time(c) for _, b, c, *_ in recs]) / 1e3
     last = recs[-1]
@@ -220,13 +232,16 @@ def main():
         f"backward error {last[5]:.2e}")
 
     # dominant kernel by time: the FP32 trailing-update GEMM of the LU
-    g = prof["gemm_f32"]
     gemm_tf = g["flops"] / (g["ms"] / 1e3) / 1e12 if g["ms"] > 0 else 0.0
-    a = prof["assemble"]
+    a = prof.get("assemble", dict(ms=0.0, bytes=0.0))
     asm_gbs = a["bytes"] / (a["ms"] / 1e3) / 1e9 if a["ms"] > 0 else 0.0
-    roofline = {"kernel": "FP32 GEMM (k_sgemm32 / k_gemm_sub<float>, LU trailing updates)", "bound": "mfma",
-                "achieved": round(gemm_tf, 3), "peak": PEAKS["fp32_mfma_TFs"], "unit": "TFLOP/s",
-                "frac": round(gemm_tf / PEAKS["fp32_mfma_TFs"], 4),
+    roofline = {"kernel": "FP32 trailing-update GEMM of the LU: k_split_a + k_split_b + k_x3gemm (bf16 MFMA "
+                          "32x32x16, six products of three-way operand splits, FP32 accumulation); products "
+                          "below 256 x 128 x 64 on FP32 MFMA (k_sgemm32 / k_gemm_sub<float>)",
+                "bound": "mfma", "achieved": round(gemm_tf, 3), "peak": round(X3_PEAK, 1), "unit": "TFLOP/s",
+                "frac": round(gemm_tf / X3_PEAK, 4),
+                "peak_basis": "FP32-equivalent: dense bf16 MFMA 2500 TFLOP/s / 6 products (FP32 MFMA peak 157.3)",
+                "achieved_bf16_mfma_TFs": round(gemm_tf * X3_PRODUCTS, 1),
                 # PMC-measured bytes of one representative call (profiles/gemm72k_pmc.md)
                 "traffic": GEMM72K_TRAFFIC, "traffic_algorithmic": GEMM72K_ALG_BYTES,
                 "traffic_for": "first trailing update 70976 x 70976 x 1024 (bench.py --workload gemm72k): "
@@ -244,7 +259,8 @@ def main():
         "unit": "Gentries/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64 assembly; f32 LU + f64 refinement",
+        "dtype": "f64 assembly; f32 LU (trailing updates as FP32-accurate products on bf16 MFMA, three-way "
+                 "splits) + f64 refinement",
         "data": "synthetic",
         "config": {"workload": f"rotated elliptical crack 1x0.5 (30 deg about x), {n_el} quadratic triangles, "
                                f"{n} DoF, {args.nrhs} load cases, E=1 nu=0.3",
@@ -252,6 +268,8 @@ def main():
         "assembly_s": round(t_asm, 5),
         "solve_s": round(t_solve, 4),
         "ir": {"outer": last[3], "inner": last[4], "backward_error": last[5], "converged": bool(last[6])},
+        "kernel_profile_note": "per kernel class, from one untimed step after the timed ones (HIP events around "
+                               "every launch); the timed steps carry events around the GEMM class only",
         "kernel_profile_per_step": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                                     for k, v in prof.items()},
         "roofline": roofline,
@@ -350,14 +368,19 @@ def run_gemm(args):
     t = float(np.mean(times))
     flops = 2.0 * M * N * K
     alg_bytes = 4.0 * (2.0 * M * N + M * K + K * N)
+    if not args.gemm_variant or args.gemm_variant == 13:
+        peak, basis = X3_PEAK, "FP32-equivalent: dense bf16 MFMA 2500 TFLOP/s / 6 products"
+    else:
+        peak, basis = PEAKS["fp32_mfma_TFs"], "FP32 MFMA"
     result = {"metric": "FP32 trailing-update GEMM TFLOP/s (72k LU, first update)", "value": round(flops / t / 1e12, 3),
               "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
               "ms_per_step": round(t * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-              "dtype": "f32", "data": "synthetic",
+              "dtype": "f32 (bf16 MFMA on three-way splits)" if not args.gemm_variant or args.gemm_variant == 13
+              else "f32", "data": "synthetic",
               "config": {"workload": f"C ({M} x {N}) -= A ({M} x {K}) B ({K} x {N}), FP32"},
               "algorithmic_bytes": alg_bytes,
-              "roofline": {"bound": "mfma", "achieved": round(flops / t / 1e12, 3), "peak": PEAKS["fp32_mfma_TFs"],
-                           "unit": "TFLOP/s", "frac": round(flops / t / 1e12 / PEAKS["fp32_mfma_TFs"], 4)}}
+              "roofline": {"bound": "mfma", "achieved": round(flops / t / 1e12, 3), "peak": round(peak, 1),
+                           "unit": "TFLOP/s", "frac": round(flops / t / 1e12 / peak, 4), "peak_basis": basis}}
     print(json.dumps(result))
 
 

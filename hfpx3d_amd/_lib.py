@@ -69,6 +69,7 @@ SIGNATURES = {
                                _c_i64, _c_void_p, _c_i64, _c_i64, _c_double, _c_int, _c_int,
                                ctypes.POINTER(IRReport)]),
     "hfpx_profile": (_c_int, [_c_void_p, _c_int]),
+    "hfpx_profile_classes": (_c_int, [_c_void_p, ctypes.c_uint32]),
     "hfpx_profile_read": (_c_int, [_c_void_p, _c_int, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_double),
                                    ctypes.POINTER(_c_double), ctypes.POINTER(_c_double)]),
     "hfpx_gemm_sub": (_c_int, [_c_void_p, _c_int, _c_void_p, _c_i64, _c_void_p, _c_i64, _c_void_p, _c_i64,

@@ -274,7 +274,7 @@ int launch_assemble(hfpx_ctx* ctx, const hfpx_mesh* m, double shear_m, double po
   const unsigned gy = (unsigned)((m->n_el + 3) / 4);
   if (gy > 65535u) return set_error(ctx, HFPX_ERR_INVALID, "too many elements for one launch");
   StoreBoth st{a64, lda64, a32, lda32};
-  const int tok = prof_begin(ctx);
+  const int tok = prof_begin(ctx, HFPX_KC_ASSEMBLE);
   hipLaunchKernelGGL(k_assemble<StoreBoth>, dim3(gx, gy), dim3(256), 0, ctx->stream, m->elem,
                      m->cp, m->n_el, m->identity_dof ? nullptr : m->dof, shear_m, poiss_r, st);
   // algorithmic HBM bytes: every matrix entry written once (FP64 and/or FP32)

@@ -487,7 +487,7 @@ static int batch_lu(hfpx_ctx* ctx, hfpx_batch* bt, T* base, int which, const int
       const int na = count_above(sizes, j);
       if (na == 0) break;
       {
-        const int tok = prof_begin(ctx);
+        const int tok = prof_begin(ctx, HFPX_KC_PANEL);
         hipLaunchKernelGGL(k_bpanel<T>, dim3((unsigned)na), dim3(PT), 0, ctx->stream, base, u, bt->cr, list,
                            bt->ipiv, bt->info, k, j);
         prof_end(ctx, tok, HFPX_KC_PANEL, 0.0, 0.0);

@@ -54,8 +54,8 @@ int ensure_hwork(hfpx_ctx* ctx, size_t bytes) {
   return HFPX_OK;
 }
 
-int prof_begin(hfpx_ctx* ctx) {
-  if (!ctx->prof) return -1;
+int prof_begin(hfpx_ctx* ctx, int cls) {
+  if (!ctx->prof || !(ctx->prof_mask & (1u << cls))) return -1;
   hipEvent_t a, b;
   if (ctx->prof_pool.size() >= 2) {
     a = ctx->prof_pool.back();
@@ -352,6 +352,12 @@ int hfpx_profile(hfpx_ctx* ctx, int enable) {
   HFPX_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (enable) prof_clear(ctx);
   ctx->prof = enable != 0;
+  return HFPX_OK;
+}
+
+int hfpx_profile_classes(hfpx_ctx* ctx, uint32_t mask) {
+  if (!ctx) return HFPX_ERR_INVALID;
+  ctx->prof_mask = mask;
   return HFPX_OK;
 }
 

@@ -554,7 +554,7 @@ int gemm_sub(hfpx_ctx* ctx, T* C, int64_t ldc, const T* A, int64_t lda, const T*
   dim3 grid((unsigned)((M + GT - 1) / GT), (unsigned)((N + gtn - 1) / gtn));
   if ((N + 63) / 64 > 65535) return set_error(ctx, HFPX_ERR_INVALID, "gemm: N too large");
   if (ldc > (int64_t)1 << 24) return set_error(ctx, HFPX_ERR_INVALID, "gemm: ldc above 2^24");
-  const int tok = prof_begin(ctx);
+  const int tok = prof_begin(ctx, sizeof(T) == 8 ? HFPX_KC_GEMM_F64 : HFPX_KC_GEMM_F32);
   int var = ctx->knob[HFPX_KNOB_GEMM_F32];
   if (var == 0) {
     // automatic choice: the bf16-split product wherever it has tiles to
@@ -715,7 +715,7 @@ int launch_bgemm(hfpx_ctx* ctx, T* base, int which, const BCrack* cr, const int3
                  const int32_t* tstart, int64_t tiles, int64_t k, int64_t j, bool outer) {
   if (nact <= 0 || tiles <= 0) return HFPX_OK;
   if (tiles > 0x7fffffff) return set_error(ctx, HFPX_ERR_INVALID, "batched update: too many tiles");
-  const int tok = prof_begin(ctx);
+  const int tok = prof_begin(ctx, sizeof(T) == 8 ? HFPX_KC_GEMM_F64 : HFPX_KC_GEMM_F32);
   if (ctx->knob[HFPX_KNOB_BGEMM_F32] == 1)
     hipLaunchKernelGGL((k_bgemm<T, 0>), dim3((unsigned)tiles), dim3(256), 0, ctx->stream, base, which, cr, list,
                        tstart, nact, k, j, outer ? 1 : 0);

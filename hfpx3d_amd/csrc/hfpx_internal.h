@@ -35,6 +35,7 @@ struct hfpx_ctx {
   std::string last_error;
   // opt-in per-launch timing (HIP events on the context's stream)
   bool prof = false;
+  uint32_t prof_mask = ~0u;  // kernel classes recorded (bit HFPX_KC_*)
   struct ProfRec {
     int cls;
     hipEvent_t a, b;
@@ -151,7 +152,7 @@ __host__ __device__ inline int64_t bcopy_off(const BCrack& c, int which) {
 
 // Profiling bracket around one launch: prof_begin before the launch,
 // prof_end after it (no-ops unless profiling is enabled on the context).
-int prof_begin(hfpx_ctx* ctx);
+int prof_begin(hfpx_ctx* ctx, int cls);
 void prof_end(hfpx_ctx* ctx, int token, int cls, double flops, double bytes);
 
 int set_error(hfpx_ctx* ctx, int code, const char* fmt, ...);

@@ -710,7 +710,7 @@ __global__ __launch_bounds__(256) void k_trsm_leaf(const T* __restrict__ Lm, int
 static int perm_pairs(hfpx_ctx* ctx, const int32_t* ipiv, int64_t k0, int64_t k1, int64_t n, PermPairs* pp) {
   if (k1 <= k0) return HFPX_OK;
   const unsigned nblk = (unsigned)((k1 - k0 + SWAP_BLK - 1) / SWAP_BLK);
-  const int tok = prof_begin(ctx);
+  const int tok = prof_begin(ctx, HFPX_KC_LASWP);
   hipLaunchKernelGGL(k_perm_pairs, dim3(nblk), dim3(256), 0, ctx->stream, ipiv, k0, k1, n, pp);
   prof_end(ctx, tok, HFPX_KC_LASWP, 0.0, 0.0);
   HFPX_TRY_HIP(ctx, hipGetLastError());
@@ -725,7 +725,7 @@ static int apply_pairs(hfpx_ctx* ctx, T* A, int64_t lda, const PermPairs* pp, in
   if (ncol <= 0) return HFPX_OK;
   if (c1 < c0) c1 = c0;
   if (c3 < c2) c3 = c2;
-  const int tok = prof_begin(ctx);
+  const int tok = prof_begin(ctx, HFPX_KC_LASWP);
   hipLaunchKernelGGL(k_apply_pairs<T>, dim3((unsigned)((ncol + 3) / 4)), dim3(256), 0, ctx->stream, A, lda, pp, c0,
                      c1, c2, c3);
   prof_end(ctx, tok, HFPX_KC_LASWP, 0.0, 2.0 * sizeof(T) * ncol * 2 * SWAP_BLK);
@@ -735,7 +735,7 @@ static int apply_pairs(hfpx_ctx* ctx, T* A, int64_t lda, const PermPairs* pp, in
 
 template <typename T, bool LOWER>
 static int trsm_leaf(hfpx_ctx* ctx, const T* L, int64_t ldl, T* B, int64_t ldb, int kb, int64_t ncols) {
-  const int tok = prof_begin(ctx);
+  const int tok = prof_begin(ctx, HFPX_KC_TRSM);
   hipLaunchKernelGGL((k_trsm_leaf<T, LOWER>), dim3((unsigned)((ncols + 255) / 256)), dim3(256), 0, ctx->stream,
                      L, ldl, B, ldb, kb, ncols);
   prof_end(ctx, tok, HFPX_KC_TRSM, (double)kb * kb * ncols, 0.0);
@@ -805,7 +805,7 @@ static int panel(hfpx_ctx* ctx, T* A, int64_t lda, int64_t n, int64_t j0, int jb
   void* args[] = {&A, &lda, &n, &j0, &jb, &ipiv, &info, &err, &ps, &rows, &ctr, &base};
   void* xargs[] = {&A, &lda, &n, &j0, &jb, &ipiv, &info, &err, &xw};
   const void* kern = xchg ? (const void*)k_panel_x<T, IB> : (const void*)k_panel<T, IB>;
-  const int tok = prof_begin(ctx);
+  const int tok = prof_begin(ctx, HFPX_KC_PANEL);
   // A cooperative launch guarantees that the workgroups are co-resident.
   // The runtime refuses it while another process holds the device's
   // cooperative capability; the grid (at most one workgroup per CU) is then
@@ -949,7 +949,7 @@ static int getrs_impl(hfpx_ctx* ctx, const T* LU, int64_t lda, const int32_t* ip
   PermPairs* pp = (PermPairs*)ctx->lu_work;
   if ((rc = perm_pairs(ctx, ipiv, 0, n, n, pp))) return rc;
   {
-    const int tok = prof_begin(ctx);
+    const int tok = prof_begin(ctx, HFPX_KC_LASWP);
     hipLaunchKernelGGL(k_apply_pairs_seq<T>, dim3((unsigned)nrhs), dim3(256), 0, ctx->stream, B, ldb, pp,
                        (int)npb);
     prof_end(ctx, tok, HFPX_KC_LASWP, 0.0, 0.0);

@@ -106,7 +106,7 @@ int launch_stress(hfpx_ctx* ctx, hfpx_mesh* m, double shear_m, double nu, const 
   HFPX_TRY_HIP(ctx, hipMemcpyAsync(sinf, ctx->hwork, 6 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   hipLaunchKernelGGL(k_dd_local, dim3((unsigned)((6 * m->n_el + 255) / 256)), dim3(256), 0, ctx->stream,
                      m->elem, dd, ldd, m->n_el, dloc);
-  const int tok = prof_begin(ctx);
+  const int tok = prof_begin(ctx, HFPX_KC_STRESS);
   hipLaunchKernelGGL(k_stress, dim3((unsigned)pblocks, (unsigned)nchunk), dim3(256), 0, ctx->stream, m->elem,
                      dloc, m->n_el, pts, n_pts, chunk, shear_m, nu, part);
   prof_end(ctx, tok, HFPX_KC_STRESS, 0.0, 0.0);

@@ -163,12 +163,12 @@ int sweep_f32(hfpx_ctx* ctx, const float* LU, int64_t lda, int64_t n, float* B, 
   const unsigned gy = (unsigned)((nrhs + SWN - 1) / SWN);
   float* invL = (float*)work;
   float* invU = invL + (size_t)nblk * SB * SB;
-  int tok = prof_begin(ctx);
+  int tok = prof_begin(ctx, HFPX_KC_TRSM);
   hipLaunchKernelGGL(k_block_inv<true>, dim3((unsigned)nblk), dim3(SB), 0, ctx->stream, LU, lda, n, invL);
   hipLaunchKernelGGL(k_block_inv<false>, dim3((unsigned)nblk), dim3(SB), 0, ctx->stream, LU, lda, n, invU);
   prof_end(ctx, tok, HFPX_KC_TRSM, 2.0 * nblk * SB * SB * SB / 3.0, 0.0);
   HFPX_TRY_HIP(ctx, hipGetLastError());
-  tok = prof_begin(ctx);
+  tok = prof_begin(ctx, HFPX_KC_TRSM);
   for (int64_t c = -1; c <= nblk - 2; ++c) {
     const unsigned gx = (unsigned)(c < 0 ? 1 : nblk - 1 - c);
     hipLaunchKernelGGL(k_sweep<true>, dim3(gx, gy), dim3(SWT), 0, ctx->stream, LU, lda, n, nblk, invL, B, ldb,

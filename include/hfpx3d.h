@@ -149,6 +149,10 @@ int hfpx_gemm_sub(hfpx_ctx* ctx, int dtype, void* C, int64_t ldc, const void* A,
  * previous records.  No reference counterpart (the reference times whole
  * stages with il::Timer, main.cpp:89-175). */
 int hfpx_profile(hfpx_ctx* ctx, int enable);
+/* Restrict recording to the classes whose bit (1 << HFPX_KC_*) is set in
+ * mask (default: every class), so that a timed run carries events only
+ * around the launches it reports.  No reference counterpart. */
+int hfpx_profile_classes(hfpx_ctx* ctx, uint32_t mask);
 /* Totals for one kernel class since the last start: launches, summed kernel
  * time (ms), algorithmic flops and algorithmic HBM bytes.  Synchronises. */
 int hfpx_profile_read(hfpx_ctx* ctx, int kernel_class, int64_t* launches, double* ms, double* flops,
