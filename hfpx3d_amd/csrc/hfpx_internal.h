@@ -186,7 +186,7 @@ int gemm_sub_any(hfpx_ctx* ctx, int dtype, void* C, int64_t ldc, const void* A, 
                  const void* B, int64_t ldb, int64_t M, int64_t N, int64_t K);
 
 // FP32 triangular sweeps with block inverses (hfpx_sweep.hip)
-size_t sweep_f32_work_bytes(int64_t n);
+size_t sweep_f32_work_bytes(int64_t n, int64_t nrhs);
 int sweep_f32(hfpx_ctx* ctx, const float* LU, int64_t lda, int64_t n, float* B, int64_t ldb, int64_t nrhs,
               void* work);
 

@@ -945,7 +945,7 @@ static int getrs_impl(hfpx_ctx* ctx, const T* LU, int64_t lda, const int32_t* ip
   const int64_t npb = (n + SWAP_BLK - 1) / SWAP_BLK;
   const size_t pairs_bytes = (sizeof(PermPairs) * (size_t)npb + 255) & ~(size_t)255;
   const bool sweep = std::is_same<T, float>::value;  // FP32: block-inverse sweeps
-  if ((rc = ensure_lu_work(ctx, pairs_bytes + (sweep ? sweep_f32_work_bytes(n) : 0)))) return rc;
+  if ((rc = ensure_lu_work(ctx, pairs_bytes + (sweep ? sweep_f32_work_bytes(n, nrhs) : 0)))) return rc;
   PermPairs* pp = (PermPairs*)ctx->lu_work;
   if ((rc = perm_pairs(ctx, ipiv, 0, n, n, pp))) return rc;
   {

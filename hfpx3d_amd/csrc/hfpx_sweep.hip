@@ -5,12 +5,14 @@
 // LAPACKE_dgetrs (il/linear_algebra/dense/factorization/LU.h:106-138).
 //
 // Blocks of SB = 128 rows.  One launch inverts every diagonal block of L
-// (unit lower) and of U; each sweep step is then one launch: block c's
-// solution rows are final in B, every workgroup subtracts T(tile, c) X_c from
-// its 128-row tile (MFMA), and the workgroup whose tile is the next block
-// multiplies its freshly updated rows by that block's inverse, which makes
-// them final in place.  A sweep is nblk launches whose critical path is one
-// tile update plus one 128 x 128 product; the factor is read once per sweep.
+// (unit lower) and of U.  Each sweep is then one launch in dependency-flow
+// form: workgroups take row blocks in sweep order from an atomic ticket
+// counter; the workgroup of block i accumulates T(i, c) X_c over the blocks
+// c before it in sweep order as each one's flag appears, then multiplies
+// B_i - sum by block i's inverse,
+// which makes X_i final in place, and raises block i's flag.  The critical
+// path is one tile product plus one 128 x 128 product per block, instead of
+// one launch per block; the factor is read once per sweep.
 #include "hfpx_internal.h"
 
 namespace hfpx {
@@ -63,125 +65,197 @@ __global__ __launch_bounds__(SB) void k_block_inv(const float* __restrict__ LU, 
   for (int e = t; e < SB * SB; e += SB) out[e] = sX[(e % SB) + (e / SB) * P];
 }
 
-// One sweep step (see the file comment).  LOWER: forward with unit L, tiles
-// c+1+t; UPPER: backward with U, tiles c-1-t.  c = -1 (LOWER) or nblk
-// (UPPER) is the first step: no update, the first block is only solved.
-// Eight waves, one 32 x 32 block of the 128 x 64 tile each; every lane
-// fetches all of its factor (and inverse) operands before the first MFMA, so
-// a step costs one memory round trip plus 64 MFMAs per phase.  Products use
+// One sweep (see the file comment).  LOWER: forward with unit L, blocks in
+// increasing order; UPPER: backward with U, decreasing order.  A ticket t is
+// the pair (q = t / gy, right-hand-side block t % gy) with q the position of
+// the row block in sweep order; flags[rb nblk + block] = want once X_block
+// of right-hand-side block rb is final.  Tickets are taken in order and a
+// workgroup only waits on lower tickets, which were handed to workgroups
+// already running: any grid size makes progress.
+// Memory ordering: the producer's workgroup barrier, then an agent-scope
+// release fence and a relaxed flag store by thread 0; the consumer's thread 0
+// polls relaxed (bounded: past ~2^20 polls *err is raised, after which every
+// other wait of the sweep gives up too and the sweep completes with garbage,
+// reported by the host) and ends with an agent-scope
+// acquire fence, which the workgroup barrier extends to the other threads.
+// Eight waves, one 32 x 32 block of the 128 x 64 tile each.  Products use
 // the transposed MFMA form (v_mfma_f32_32x32x2f32): lane l holds row
 // 32 (wave % 4) + l % 32 of the tile and column 8 (v/4) + 4 (l/32) + v%4 of
 // the wave's 32-column block, so tile loads and stores are column segments.
 constexpr int SWT = 512;  // threads per sweep workgroup
 
 template <bool LOWER>
-__global__ __launch_bounds__(SWT) void k_sweep(const float* __restrict__ LU, int64_t lda, int64_t n, int64_t nblk,
-                                               const float* __restrict__ inv, float* __restrict__ B, int64_t ldb,
-                                               int64_t nrhs, int64_t c) {
+__global__ __launch_bounds__(SWT) void k_sweep_flow(const float* __restrict__ LU, int64_t lda, int64_t n,
+                                                    int64_t nblk, const float* __restrict__ inv,
+                                                    float* __restrict__ B, int64_t ldb, int64_t nrhs, int64_t gy,
+                                                    uint64_t* __restrict__ flags, uint64_t want,
+                                                    uint64_t* __restrict__ ticket, int32_t* __restrict__ err) {
   __shared__ float sX[SB][SWN + 1];  // solution rows of block c: sX[k][j]
-  __shared__ float sC[SB][SWN + 1];  // updated tile of the solving workgroup: sC[i][j]
+  __shared__ float sC[SB][SWN + 1];  // B_i - sum, then X_i: sC[i][j]
+  __shared__ uint64_t s_tk;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, l32 = lane & 31, hk = lane >> 5;
-  const int64_t t = blockIdx.x;
-  const int64_t tile = LOWER ? c + 1 + t : c - 1 - t;
-  const int64_t r0 = tile * SB;
-  const int rows = (int)std::min<int64_t>(SB, n - r0);
-  const int64_t j0 = (int64_t)blockIdx.y * SWN;
-  const int ncol = (int)std::min<int64_t>(SWN, nrhs - j0);
-  const bool has_update = LOWER ? (c >= 0) : (c < nblk);
   const int i = 32 * (wave & 3) + l32;  // tile row of this lane
   const int jb = wave >> 2;             // 32-column block of this wave
-  const bool rok = i < rows;
-
-  floatx16 acc;
-#pragma unroll
-  for (int v = 0; v < 16; ++v) acc[v] = 0.f;
-
-  if (has_update) {
-    const int64_t k0 = c * SB;
-    const int kc = (int)std::min<int64_t>(SB, n - k0);
-    const float* lrow = LU + (r0 + i) + k0 * lda;
-    float lv[SB / 2];
-#pragma unroll
-    for (int s = 0; s < SB / 2; ++s) {
-      const int k = 2 * s + hk;
-      lv[s] = (rok && k < kc) ? lrow[(int64_t)k * lda] : 0.f;
-    }
-    for (int e = tid; e < SB * SWN; e += SWT) {
-      const int k = e % SB, j = e / SB;
-      sX[k][j] = (k < kc && j < ncol) ? B[(k0 + k) + (j0 + j) * ldb] : 0.f;
-    }
+  const int64_t total = nblk * gy;
+  // Thread 0 takes the tickets; every thread reads them as wave-uniform
+  // (scalar) values.  The ticket is taken at the end of the loop body, not at
+  // its head: with the divergent "thread 0 takes a ticket" block at the loop
+  // head, the compiler's structurisation split the loop so that lanes 1..63
+  // of wave 0 kept iterating on the stale ticket while lane 0 waited to take
+  // the next one (a hang).
+  auto next_ticket = [&]() -> int64_t {
+    if (tid == 0) s_tk = __hip_atomic_fetch_add(ticket, (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
-#pragma unroll
-    for (int s = 0; s < SB / 2; ++s)
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(sX[2 * s + hk][32 * jb + l32], lv[s], acc, 0, 0, 0);
-  }
+    const uint64_t u = s_tk;
+    const uint64_t t = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32)) << 32) |
+                       (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u);
+    __syncthreads();  // every thread has s_tk before thread 0 may take the next ticket
+    return (int64_t)t;
+  };
+  for (int64_t tk = next_ticket(); tk < total; tk = next_ticket()) {
+    const int64_t q = tk / gy, rb = tk % gy;
+    const int64_t blk = LOWER ? q : nblk - 1 - q;
+    const int64_t r0 = blk * SB;
+    const int rows = (int)std::min<int64_t>(SB, n - r0);
+    const int64_t j0 = rb * SWN;
+    const int ncol = (int)std::min<int64_t>(SWN, nrhs - j0);
+    const bool rok = i < rows;
+    uint64_t* fl = flags + rb * nblk;
 
-  if (t != 0) {
-    // plain update of this tile
+    floatx16 acc;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+    // the block's inverse, fetched before the waits (off the critical path)
+    const float* iv = inv + blk * SB * SB + i;  // inverse(i, k) at iv[k SB]
+    float ivv[SB / 2];
+#pragma unroll
+    for (int s = 0; s < SB / 2; ++s) ivv[s] = iv[(2 * s + hk) * SB];
+    int64_t ready = 0;  // predecessors 0 .. ready-1 (in sweep order) are known final
+    for (int64_t p = 0; p < q; ++p) {
+      const int64_t c = LOWER ? p : nblk - 1 - p;
+      const int64_t k0 = c * SB;
+      const int kc = (int)std::min<int64_t>(SB, n - k0);
+      // the factor tile does not depend on the flag: its loads are issued
+      // before the wait.  The column pointer steps by two columns (a table
+      // of the 64 offsets would be hoisted out of the loop and held in
+      // registers).
+      const float* lp = LU + (r0 + i) + (k0 + hk) * lda;
+      float lv[SB / 2];
+#pragma unroll
+      for (int s = 0; s < SB / 2; ++s) {
+        lv[s] = (rok && 2 * s + hk < kc) ? *lp : 0.f;
+        lp += 2 * lda;
+      }
+      if (p >= ready) {
+        // wait for this predecessor, then take every later one already
+        // final, so that one acquire fence covers the whole run
+        if (tid == 0) {
+          uint32_t polls = 0;
+          while (__hip_atomic_load(fl + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != want) {
+            __builtin_amdgcn_s_sleep(1);
+            // give up past ~2^20 polls, or at once when another wait already has
+            if (++polls > (1u << 20) ||
+                ((polls & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+              __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              break;
+            }
+          }
+          int64_t r = p + 1;
+          while (r < q && __hip_atomic_load(fl + (LOWER ? r : nblk - 1 - r), __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT) == want)
+            ++r;
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+          s_tk = (uint64_t)r;
+        }
+        __syncthreads();
+        const uint64_t u = s_tk;
+        ready = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32)) << 32) |
+                          (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u));
+      }
+      for (int e = tid; e < SB * SWN; e += SWT) {
+        const int k = e % SB, j = e / SB;
+        sX[k][j] = (k < kc && j < ncol) ? B[(k0 + k) + (j0 + j) * ldb] : 0.f;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < SB / 2; ++s)
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(sX[2 * s + hk][32 * jb + l32], lv[s], acc, 0, 0, 0);
+      __syncthreads();  // sX is rewritten by the next predecessor
+    }
+    // X_i = inverse(i) (B_i - sum)
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
       const int j = 32 * jb + 8 * (v >> 2) + 4 * hk + (v & 3);
-      if (rok && j < ncol) B[(r0 + i) + (j0 + j) * ldb] -= acc[v];
+      sC[i][j] = (rok && j < ncol) ? B[(r0 + i) + (j0 + j) * ldb] - acc[v] : 0.f;
     }
-    return;
-  }
-  // the next block: updated rows -> LDS, then times the block inverse
-  const float* iv = inv + tile * SB * SB + i;  // inverse(i, k) at iv[k SB]
-  float ivv[SB / 2];
+    __syncthreads();
+    floatx16 y;
 #pragma unroll
-  for (int s = 0; s < SB / 2; ++s) ivv[s] = iv[(2 * s + hk) * SB];
+    for (int v = 0; v < 16; ++v) y[v] = 0.f;
 #pragma unroll
-  for (int v = 0; v < 16; ++v) {
-    const int j = 32 * jb + 8 * (v >> 2) + 4 * hk + (v & 3);
-    sC[i][j] = (rok && j < ncol) ? B[(r0 + i) + (j0 + j) * ldb] - acc[v] : 0.f;
-  }
-  __syncthreads();
-  floatx16 y;
+    for (int s = 0; s < SB / 2; ++s)
+      y = __builtin_amdgcn_mfma_f32_32x32x2f32(sC[2 * s + hk][32 * jb + l32], ivv[s], y, 0, 0, 0);
 #pragma unroll
-  for (int v = 0; v < 16; ++v) y[v] = 0.f;
-#pragma unroll
-  for (int s = 0; s < SB / 2; ++s)
-    y = __builtin_amdgcn_mfma_f32_32x32x2f32(sC[2 * s + hk][32 * jb + l32], ivv[s], y, 0, 0, 0);
-#pragma unroll
-  for (int v = 0; v < 16; ++v) {
-    const int j = 32 * jb + 8 * (v >> 2) + 4 * hk + (v & 3);
-    if (rok && j < ncol) B[(r0 + i) + (j0 + j) * ldb] = y[v];
+    for (int v = 0; v < 16; ++v) {
+      const int j = 32 * jb + 8 * (v >> 2) + 4 * hk + (v & 3);
+      if (rok && j < ncol) B[(r0 + i) + (j0 + j) * ldb] = y[v];
+    }
+    __syncthreads();  // every store of X_i, and every read of sC, before the flag
+    if (tid == 0) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __hip_atomic_store(fl + blk, want, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
 }
 
-size_t sweep_f32_work_bytes(int64_t n) {
-  const int64_t nblk = (n + SB - 1) / SB;
-  return sizeof(float) * 2 * (size_t)nblk * SB * SB;
+__global__ void k_fill_u64(uint64_t* __restrict__ p, int64_t cnt) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e < cnt) p[e] = 0;
+}
+
+// work: the block inverses of L and U, then the flags (one per row block
+// and right-hand-side block, per sweep), two ticket counters and two error
+// flags
+size_t sweep_f32_work_bytes(int64_t n, int64_t nrhs) {
+  const int64_t nblk = (n + SB - 1) / SB, gy = (nrhs + SWN - 1) / SWN;
+  return sizeof(float) * 2 * (size_t)nblk * SB * SB + sizeof(uint64_t) * (2 * (size_t)nblk * gy + 4);
 }
 
 // B := U^{-1} L^{-1} B for the FP32 factors (row interchanges already
-// applied to B); work holds sweep_f32_work_bytes(n).
+// applied to B); work holds sweep_f32_work_bytes(n, nrhs).
 int sweep_f32(hfpx_ctx* ctx, const float* LU, int64_t lda, int64_t n, float* B, int64_t ldb, int64_t nrhs,
               void* work) {
   if (n <= 0 || nrhs <= 0) return HFPX_OK;
-  const int64_t nblk = (n + SB - 1) / SB;
-  const unsigned gy = (unsigned)((nrhs + SWN - 1) / SWN);
+  const int64_t nblk = (n + SB - 1) / SB, gy = (nrhs + SWN - 1) / SWN;
   float* invL = (float*)work;
   float* invU = invL + (size_t)nblk * SB * SB;
+  uint64_t* flags = (uint64_t*)(invU + (size_t)nblk * SB * SB);  // [2][gy][nblk]
+  uint64_t* tickets = flags + 2 * nblk * gy;                      // [2]
+  int32_t* errs = (int32_t*)(tickets + 2);                       // [2]
+  const int64_t nclear = 2 * nblk * gy + 4;
+  hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nclear + 255) / 256)), dim3(256), 0, ctx->stream, flags, nclear);
   int tok = prof_begin(ctx, HFPX_KC_TRSM);
   hipLaunchKernelGGL(k_block_inv<true>, dim3((unsigned)nblk), dim3(SB), 0, ctx->stream, LU, lda, n, invL);
   hipLaunchKernelGGL(k_block_inv<false>, dim3((unsigned)nblk), dim3(SB), 0, ctx->stream, LU, lda, n, invU);
   prof_end(ctx, tok, HFPX_KC_TRSM, 2.0 * nblk * SB * SB * SB / 3.0, 0.0);
   HFPX_TRY_HIP(ctx, hipGetLastError());
   tok = prof_begin(ctx, HFPX_KC_TRSM);
-  for (int64_t c = -1; c <= nblk - 2; ++c) {
-    const unsigned gx = (unsigned)(c < 0 ? 1 : nblk - 1 - c);
-    hipLaunchKernelGGL(k_sweep<true>, dim3(gx, gy), dim3(SWT), 0, ctx->stream, LU, lda, n, nblk, invL, B, ldb,
-                       nrhs, c);
-  }
-  for (int64_t c = nblk; c >= 1; --c) {
-    const unsigned gx = (unsigned)(c == nblk ? 1 : c);
-    hipLaunchKernelGGL(k_sweep<false>, dim3(gx, gy), dim3(SWT), 0, ctx->stream, LU, lda, n, nblk, invU, B, ldb,
-                       nrhs, c);
-  }
+  // as many workgroups as are resident at once (any grid size would make
+  // progress: a workgroup takes tickets only once it runs)
+  int occ = 0;
+  HFPX_TRY_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_sweep_flow<true>, SWT, 0));
+  const unsigned nwg = (unsigned)std::min<int64_t>(nblk * gy, std::max(1, occ) * std::max(1, ctx->num_cus));
+  hipLaunchKernelGGL(k_sweep_flow<true>, dim3(nwg), dim3(SWT), 0, ctx->stream, LU, lda, n, nblk, invL, B, ldb,
+                     nrhs, gy, flags, (uint64_t)1, tickets, errs);
+  hipLaunchKernelGGL(k_sweep_flow<false>, dim3(nwg), dim3(SWT), 0, ctx->stream, LU, lda, n, nblk, invU, B, ldb,
+                     nrhs, gy, flags + nblk * gy, (uint64_t)1, tickets + 1, errs + 1);
   // algorithmic: 2 n^2 nrhs flops, the factor read once (4 n^2 bytes)
   prof_end(ctx, tok, HFPX_KC_TRSM, 2.0 * n * n * nrhs, 4.0 * n * n);
   HFPX_TRY_HIP(ctx, hipGetLastError());
+  int32_t herr[2] = {0, 0};
+  HFPX_TRY_HIP(ctx, hipMemcpyAsync(herr, errs, sizeof herr, hipMemcpyDeviceToHost, ctx->stream));
+  HFPX_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (herr[0] || herr[1]) return set_error(ctx, HFPX_ERR_HIP, "getrs: sweep dependency wait timed out");
   return HFPX_OK;
 }
 
