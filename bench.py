@@ -59,6 +59,8 @@ def parse():
                     help="do not time kernel launches with HIP events (for runs under an external profiler)")
     ap.add_argument("--lu-nb", type=int, default=0, help="outer block of the LU (0: library default)")
     ap.add_argument("--gemm-variant", type=int, default=0, help="FP32 GEMM variant (0: library default)")
+    ap.add_argument("--autotune", default="", help="apply the knob settings of an autotune table "
+                                                   "(python -m hfpx3d_amd.autotune) for the LU's n")
     ap.add_argument("--calib-copy-gib", type=float, default=0.0,
                     help="before the workload, copy this many GiB device-to-device (a transfer of known size "
                          "for scaling hardware byte counters)")
@@ -147,6 +149,9 @@ def main():
     n = 18 * n_el
     ctx = hx.get_context(0)
     lib = ctx.lib
+    if args.autotune:
+        from hfpx3d_amd import autotune
+        log(f"bench: autotune settings {autotune.apply(autotune.load(args.autotune), n, 'f32', ctx)}")
     if args.lu_nb:
         hx.tune_set(_lib.KNOB_LU_NB, args.lu_nb, ctx)
     if args.gemm_variant:

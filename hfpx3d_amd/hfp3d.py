@@ -104,6 +104,7 @@ class Context:
         self.device = torch.device("cuda", device)
         self.lib = _lib.load()
         self.stream = torch.cuda.Stream(device=self.device)
+        self.autotune = None  # table of knob settings by (n-range, precision), see autotune.py
         h = ctypes.c_void_p()
         _lib.check(self.lib.hfpx_create(ctypes.byref(h), device, ctypes.c_void_p(self.stream.cuda_stream)))
         self.handle = h
@@ -344,6 +345,9 @@ class LU:
         self.code = _dtype_code(torch, A)
         with torch.cuda.stream(self.ctx.stream):
             self.ipiv = torch.empty(self.n, dtype=torch.int32, device=self.ctx.device)
+        if self.ctx.autotune is not None:
+            from . import autotune
+            autotune.apply(self.ctx.autotune, self.n, "f32" if A.dtype == torch.float32 else "f64", self.ctx)
         info = ctypes.c_int64(0)
         self.ctx.enter()
         rc = self.ctx.lib.hfpx_getrf(self.ctx.handle, self.code, _ptr(A), self.lda, self.n, _ptr(self.ipiv),
