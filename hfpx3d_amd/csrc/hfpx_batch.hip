@@ -57,6 +57,7 @@ struct hfpx_batch {
   double* F64 = nullptr;    // working copies of the FP64 group (their LU factors)
   int64_t f_first = 0, f_total = 0;  // A64 range of the FP64 group, copied to F64
   int32_t* ipiv = nullptr;
+  int32_t* perm = nullptr;  // per crack: the interchanges composed, b_permuted[i] = b[perm[i]]
   int32_t* info = nullptr;  // per crack (table order)
   double *b = nullptr, *x = nullptr, *r = nullptr, *rs = nullptr;
   float* x32 = nullptr;
@@ -259,32 +260,47 @@ __global__ __launch_bounds__(256) void k_btrsm(T* __restrict__ base, int which, 
 // shuffles (its rows preloaded into registers), then every thread updates its
 // rows with the block's 32 values.
 // ---------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(PT) void k_bsolve(const T* __restrict__ base, int which, const BCrack* __restrict__ cr,
-                                               const int32_t* __restrict__ list, const int32_t* __restrict__ ipiv,
-                                               T* __restrict__ vec) {
-  __shared__ T v[BATCH_MAX_N];
-  __shared__ int32_t sp[BATCH_MAX_N];
+// The sequential interchanges ipiv of each listed crack composed into one
+// gather permutation: after them, row i holds the original row perm[i]
+// (thread 0 walks them in order on an index array in LDS).
+__global__ __launch_bounds__(PT) void k_bperm(const BCrack* __restrict__ cr, const int32_t* __restrict__ list,
+                                              const int32_t* __restrict__ ipiv, int32_t* __restrict__ perm) {
+  __shared__ int32_t idx[BATCH_MAX_N];
   const BCrack& c = cr[list[blockIdx.x]];
   const int64_t n = c.n;
-  const T* a = base + bcopy_off(c, which);
   const int32_t* piv = ipiv + c.x_off;
-  T* g = vec + c.x_off;
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  for (int64_t i = t; i < n; i += PT) {
-    v[i] = g[i];
-    sp[i] = piv[i] - 1;
-  }
+  const int t = threadIdx.x;
+  for (int64_t i = t; i < n; i += PT) idx[i] = (int32_t)i;
   __syncthreads();
   if (t == 0)
     for (int64_t s = 0; s < n; ++s) {
-      const int64_t p = sp[s];
-      if (p != s) {
-        const T tmp = v[s];
-        v[s] = v[p];
-        v[p] = tmp;
+      const int32_t p = piv[s] - 1;
+      if (p != s && p >= 0 && p < n) {
+        const int32_t tmp = idx[s];
+        idx[s] = idx[p];
+        idx[p] = tmp;
       }
     }
+  __syncthreads();
+  for (int64_t i = t; i < n; i += PT) perm[c.x_off + i] = idx[i];
+}
+
+// In-place x := U^-1 L^-1 P x for every listed crack, one workgroup each,
+// the vector in LDS: the interchanges as one parallel gather (perm), then
+// 32-column blocks: wave 0 solves the diagonal block with shuffles, then
+// every thread updates its rows below (above) it.
+template <typename T>
+__global__ __launch_bounds__(PT) void k_bsolve(const T* __restrict__ base, int which, const BCrack* __restrict__ cr,
+                                               const int32_t* __restrict__ list, const int32_t* __restrict__ perm,
+                                               T* __restrict__ vec) {
+  __shared__ T v[BATCH_MAX_N];
+  const BCrack& c = cr[list[blockIdx.x]];
+  const int64_t n = c.n;
+  const T* a = base + bcopy_off(c, which);
+  const int32_t* pm = perm + c.x_off;
+  T* g = vec + c.x_off;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  for (int64_t i = t; i < n; i += PT) v[i] = g[pm[i]];
   __syncthreads();
   // forward, unit lower
   for (int64_t c0 = 0; c0 < n; c0 += 32) {
@@ -520,7 +536,7 @@ static int batch_solve_vec(hfpx_ctx* ctx, hfpx_batch* bt, const T* base, int whi
                            T* vec) {
   if (cnt <= 0) return HFPX_OK;
   hipLaunchKernelGGL(k_bsolve<T>, dim3((unsigned)cnt), dim3(PT), 0, ctx->stream, base, which, bt->cr,
-                     list, bt->ipiv, vec);
+                     list, bt->perm, vec);
   HFPX_TRY_HIP(ctx, hipGetLastError());
   return HFPX_OK;
 }
@@ -539,11 +555,17 @@ static int batch_enqueue(hfpx_ctx* ctx, hfpx_batch* bt, int n_refine) {
   // backward error), solve in place in x
   if ((rc = bvec(ctx, bt->F64, nullptr, bt->A64 + bt->f_first, bt->f_total, 3))) return rc;
   if ((rc = batch_lu<double>(ctx, bt, bt->F64, B_F64, bt->list64, bt->n64_sorted, bt->tab64))) return rc;
+  if (bt->n64 > 0)
+    hipLaunchKernelGGL(k_bperm, dim3((unsigned)bt->n64), dim3(PT), 0, ctx->stream, bt->cr, bt->list64, bt->ipiv,
+                       bt->perm);
   if ((rc = bvec(ctx, bt->x + bt->x32_total, nullptr, bt->b + bt->x32_total, bt->x_total - bt->x32_total, 3)))
     return rc;
   if ((rc = batch_solve_vec<double>(ctx, bt, bt->F64, B_F64, bt->list64, bt->n64, bt->x))) return rc;
   // mixed group: FP32 LU, x0 = LU^-1 b, then refinement against the FP64 matrix
   if ((rc = batch_lu<float>(ctx, bt, bt->A32, B_A32, bt->list32, bt->n32_sorted, bt->tab32))) return rc;
+  if (bt->n32 > 0)
+    hipLaunchKernelGGL(k_bperm, dim3((unsigned)bt->n32), dim3(PT), 0, ctx->stream, bt->cr, bt->list32, bt->ipiv,
+                       bt->perm);
   if ((rc = bvec(ctx, nullptr, bt->x32, bt->b, bt->x32_total, 0))) return rc;
   if ((rc = batch_solve_vec<float>(ctx, bt, bt->A32, B_A32, bt->list32, bt->n32, bt->x32))) return rc;
   if ((rc = bvec(ctx, bt->x, bt->x32, nullptr, bt->x32_total, 1))) return rc;
@@ -686,6 +708,7 @@ int hfpx_batch_create(hfpx_ctx* ctx, int n_cracks, const int64_t* n_nodes, const
       (rc = dalloc((void**)&bt->A32, sizeof(float) * a32_off)) ||
       (rc = dalloc((void**)&bt->F64, sizeof(double) * bt->f_total)) ||
       (rc = dalloc((void**)&bt->ipiv, sizeof(int32_t) * x_off)) ||
+      (rc = dalloc((void**)&bt->perm, sizeof(int32_t) * x_off)) ||
       (rc = dalloc((void**)&bt->info, sizeof(int32_t) * n_cracks)) ||
       (rc = dalloc((void**)&bt->b, sizeof(double) * x_off)) || (rc = dalloc((void**)&bt->x, sizeof(double) * x_off)) ||
       (rc = dalloc((void**)&bt->r, sizeof(double) * x_off)) || (rc = dalloc((void**)&bt->rs, sizeof(double) * x_off)) ||
@@ -726,7 +749,7 @@ int hfpx_batch_destroy(hfpx_batch* bt) {
   if (bt->exec) (void)hipGraphExecDestroy(bt->exec);
   if (bt->graph) (void)hipGraphDestroy(bt->graph);
   void* ps[] = {bt->cr, bt->list32, bt->list64, bt->list_all, bt->nods, bt->conn, bt->elem, bt->cp, bt->A64, bt->A32, bt->F64,
-                bt->ipiv, bt->info, bt->b, bt->x, bt->r, bt->rs, bt->x32, bt->berr, bt->tiles};
+                bt->ipiv, bt->perm, bt->info, bt->b, bt->x, bt->r, bt->rs, bt->x32, bt->berr, bt->tiles};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   delete bt;
