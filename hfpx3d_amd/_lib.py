@@ -23,7 +23,7 @@ KNOB_LU_NB = 1
 KNOB_BGEMM_F32 = 2
 KNOB_PANEL = 3
 KNOB_X3_BAND = 4
-GEMM_F32_VARIANTS = 14
+GEMM_F32_VARIANTS = 15
 
 KERNEL_CLASSES = {"assemble": 0, "gemm_f64": 1, "gemm_f32": 2, "panel": 3, "laswp": 4, "trsm": 5, "stress": 6}
 

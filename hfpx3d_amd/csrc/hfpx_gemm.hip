@@ -543,7 +543,7 @@ static void launch_sgemm32(hfpx_ctx* ctx, float* C, int64_t ldc, const float* A,
   }
 }
 
-int gemm_f32_variants() { return 14; }
+int gemm_f32_variants() { return 15; }
 
 template <typename T>
 int gemm_sub(hfpx_ctx* ctx, T* C, int64_t ldc, const T* A, int64_t lda, const T* B,
@@ -559,13 +559,16 @@ int gemm_sub(hfpx_ctx* ctx, T* C, int64_t ldc, const T* A, int64_t lda, const T*
   if (var == 0) {
     // automatic choice: the bf16-split product wherever it has tiles to
     // fill (faster on every LU shape measured, 1.5x on the large trailing
-    // updates); small products on FP32 MFMA, C preloaded for short K
-    var = (M >= 256 && N >= 128 && K >= 64) ? 13 : (K <= 384 ? 7 : 2);
+    // updates), with 128 x 256 tiles on the large square updates (+5 to 7 %
+    // there, slower on narrow products); small products on FP32 MFMA, C
+    // preloaded for short K
+    var = (M >= 256 && N >= 128 && K >= 64) ? ((M >= 4096 && N >= 4096 && K >= 512) ? 14 : 13)
+                                             : (K <= 384 ? 7 : 2);
   }
   bool done = false;
   if constexpr (sizeof(T) == 4) {
-    if (!skinny && var == 13) {
-      int rc = gemm_x3(ctx, C, ldc, A, lda, B, ldb, M, N, K);
+    if (!skinny && (var == 13 || var == 14)) {
+      int rc = gemm_x3(ctx, C, ldc, A, lda, B, ldb, M, N, K, var == 14 ? 1 : 0);
       if (rc) return rc;
       done = true;
     } else if (!skinny && var > 1) {

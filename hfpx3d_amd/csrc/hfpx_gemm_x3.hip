@@ -92,41 +92,48 @@ __global__ __launch_bounds__(256) void k_split_b(const float* __restrict__ B, in
   blk[512 + t] = pl;
 }
 
-// C (M x N) += As Bs over nkc chunks: 128 x 128 tiles, 4 waves of 64 x 64
-// (2 x 2 MFMA blocks of 32 x 32), one 16-k chunk per LDS buffer (double
-// buffered, one barrier per chunk).  As the FP32 kernels, the MFMA forms the
-// transposed product: operand a is a block of Bs^T (C columns), operand b a
-// block of As^T (C rows), so lane l % 32 runs down 32 consecutive rows of C.
-// v_mfma_f32_32x32x16_bf16: lane l supplies a[l%32][8(l/32) + e] and
+// C (M x N) += As Bs over nkc chunks: tiles of 128 rows by 64 WCB columns,
+// 4 waves of 64 x 32 WCB (2 x WCB MFMA blocks of 32 x 32), one 16-k chunk
+// per LDS buffer (double buffered, one barrier per chunk; loading two chunks
+// ahead measured slower).  WCB = 2 reads 0.5 LDS operands per MFMA, WCB = 4
+// 0.375.  ROW is the LDS pitch of a 16-k row (bf16): 24 (48 B) or 16
+// (unpadded, less LDS per workgroup).  As the FP32 kernels, the MFMA forms
+// the transposed product: operand a is a block of Bs^T (C columns), operand
+// b a block of As^T (C rows), so lane l % 32 runs down 32 consecutive rows
+// of C.  v_mfma_f32_32x32x16_bf16: lane l supplies a[l%32][8(l/32) + e] and
 // b[8(l/32) + e][l%32], e = 0..7; result element v of lane l is
 // D[8(v/4) + 4(l/32) + v%4][l%32].  Tiles are walked as in the FP32 kernel
 // (each XCD a contiguous range, in bands of row tiles), the band height
-// chosen by the host so that the band's A parts fit an XCD's L2.
-// Bounded to two waves per SIMD (the LDS of two workgroups), the compiler
-// keeps accumulators and operands in VGPRs.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_x3gemm(
+// chosen by the host so that the band's A parts fit an XCD's L2.  Bounded
+// to two waves per SIMD, the compiler keeps accumulators and operands in
+// VGPRs.  Bs has an even number of column blocks when WCB = 4 (the split
+// kernel zero-fills the last one).
+template <int ROW, int WCB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_x3gemm(
     float* __restrict__ C, int64_t ldc, const __bf16* __restrict__ As, const __bf16* __restrict__ Bs, int64_t M,
     int64_t N, int nkc, int64_t tm, int64_t tn, int64_t band) {
-  __shared__ __attribute__((aligned(16))) __bf16 sA[2][3 * X3_T * X3_ROW];
-  __shared__ __attribute__((aligned(16))) __bf16 sB[2][3 * X3_T * X3_ROW];
+  constexpr int GN = 64 * WCB;       // tile columns
+  constexpr int NBB = GN / X3_T;     // B blocks per tile (1 or 2)
+  __shared__ __attribute__((aligned(16))) __bf16 sA[2][3 * X3_T * ROW];
+  __shared__ __attribute__((aligned(16))) __bf16 sB[2][NBB][3 * X3_T * ROW];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int l32 = lane & 31, hk = lane >> 5;
   int64_t tile_r, tile_c;
   sg_band_tile(sg_xcd_order(blockIdx.x, tm * tn), tm, tn, tile_r, tile_c, band);
-  const int64_t row0 = tile_r * X3_T, col0 = tile_c * X3_T;
-  const int wr = (wave & 1) * 64, wc = (wave >> 1) * 64;
+  const int64_t row0 = tile_r * X3_T, col0 = tile_c * GN;
+  const int wr = (wave & 1) * 64, wc = (wave >> 1) * 32 * WCB;
   const __bf16* pa = As + (size_t)tile_r * nkc * X3_BLK;
-  const __bf16* pb = Bs + (size_t)tile_c * nkc * X3_BLK;
+  const __bf16* pb = Bs + (size_t)tile_c * NBB * nkc * X3_BLK;
 
   // accumulators start as the tile of C (32-bit offsets from its corner)
   float* __restrict__ Ct = C + row0 + col0 * ldc;
   const int ld = (int)ldc;
   const int off0 = wr + l32 + (wc + 4 * hk) * ld;
   const int mrem = (int)((M - row0) < X3_T ? (M - row0) : X3_T);
-  const int nrem = (int)((N - col0) < X3_T ? (N - col0) : X3_T);
-  floatx16_x3 acc[2][2];
+  const int nrem = (int)((N - col0) < GN ? (N - col0) : GN);
+  floatx16_x3 acc[WCB][2];
 #pragma unroll
-  for (int tc = 0; tc < 2; ++tc)
+  for (int tc = 0; tc < WCB; ++tc)
 #pragma unroll
     for (int tr = 0; tr < 2; ++tr) {
       const int i = wr + 32 * tr + l32;
@@ -139,25 +146,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     }
 
   // staging: piece t + 256 q of a block is part q, row t / 2, k 8 (t % 2) .. + 7
-  // (one chunk ahead: loading two ahead measured slower)
   typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
   const u32x4* pa4 = (const u32x4*)pa;
   const u32x4* pb4 = (const u32x4*)pb;
-  u32x4 ra[3], rb[3];
+  u32x4 ra[3], rb[NBB][3];
   auto gload = [&](int kc) {
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      ra[q] = pa4[(size_t)kc * X3_PIECES + t + 256 * q];
-      rb[q] = pb4[(size_t)kc * X3_PIECES + t + 256 * q];
-    }
+    for (int q = 0; q < 3; ++q) ra[q] = pa4[(size_t)kc * X3_PIECES + t + 256 * q];
+#pragma unroll
+    for (int h = 0; h < NBB; ++h)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) rb[h][q] = pb4[((size_t)h * nkc + kc) * X3_PIECES + t + 256 * q];
   };
-  const int soff = (t >> 1) * X3_ROW + 8 * (t & 1);
+  const int soff = (t >> 1) * ROW + 8 * (t & 1);
   auto lstore = [&](int buf) {
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      *(u32x4*)&sA[buf][q * X3_T * X3_ROW + soff] = ra[q];
-      *(u32x4*)&sB[buf][q * X3_T * X3_ROW + soff] = rb[q];
-    }
+    for (int q = 0; q < 3; ++q) *(u32x4*)&sA[buf][q * X3_T * ROW + soff] = ra[q];
+#pragma unroll
+    for (int h = 0; h < NBB; ++h)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) *(u32x4*)&sB[buf][h][q * X3_T * ROW + soff] = rb[h][q];
   };
 
   gload(0);
@@ -166,33 +174,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   for (int kc = 0; kc < nkc; ++kc) {
     const int buf = kc & 1;
     if (kc + 1 < nkc) gload(kc + 1);
-    bf16x8 fa[2][3], fb[2][3];
+    bf16x8 fb[2][3];
 #pragma unroll
     for (int p = 0; p < 3; ++p)
 #pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        fa[u][p] = *(const bf16x8*)&sB[buf][(p * X3_T + wc + 32 * u + l32) * X3_ROW + 8 * hk];
-        fb[u][p] = *(const bf16x8*)&sA[buf][(p * X3_T + wr + 32 * u + l32) * X3_ROW + 8 * hk];
-      }
+      for (int u = 0; u < 2; ++u)
+        fb[u][p] = *(const bf16x8*)&sA[buf][(p * X3_T + wr + 32 * u + l32) * ROW + 8 * hk];
 #pragma unroll
-    for (int tc = 0; tc < 2; ++tc)
+    for (int tc = 0; tc < WCB; ++tc) {
+      // the B operands of one 32-column block at a time (register pressure)
+      const int jb = wc + 32 * tc, h = jb / X3_T, jl = jb % X3_T;
+      bf16x8 fa[3];
+#pragma unroll
+      for (int p = 0; p < 3; ++p) fa[p] = *(const bf16x8*)&sB[buf][h][(p * X3_T + jl + l32) * ROW + 8 * hk];
 #pragma unroll
       for (int tr = 0; tr < 2; ++tr) {
         floatx16_x3 c = acc[tc][tr];
         // small terms first
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[tc][1], fb[tr][1], c, 0, 0, 0);  // m m
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[tc][2], fb[tr][0], c, 0, 0, 0);  // l h
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[tc][0], fb[tr][2], c, 0, 0, 0);  // h l
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[tc][1], fb[tr][0], c, 0, 0, 0);  // m h
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[tc][0], fb[tr][1], c, 0, 0, 0);  // h m
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[tc][0], fb[tr][0], c, 0, 0, 0);  // h h
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[1], fb[tr][1], c, 0, 0, 0);  // m m
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[2], fb[tr][0], c, 0, 0, 0);  // l h
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0], fb[tr][2], c, 0, 0, 0);  // h l
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[1], fb[tr][0], c, 0, 0, 0);  // m h
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0], fb[tr][1], c, 0, 0, 0);  // h m
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0], fb[tr][0], c, 0, 0, 0);  // h h
         acc[tc][tr] = c;
       }
+    }
     if (kc + 1 < nkc) lstore(buf ^ 1);
     __syncthreads();
   }
 #pragma unroll
-  for (int tc = 0; tc < 2; ++tc)
+  for (int tc = 0; tc < WCB; ++tc)
 #pragma unroll
     for (int tr = 0; tr < 2; ++tr) {
       const int i = wr + 32 * tr + l32;
@@ -221,30 +233,38 @@ static int ensure_x3_work(hfpx_ctx* ctx, size_t bytes) {
 }
 
 size_t gemm_x3_work_bytes(int64_t M, int64_t N, int64_t K) {
-  const int64_t tm = (M + X3_T - 1) / X3_T, tn = (N + X3_T - 1) / X3_T, nkc = (K + X3_KC - 1) / X3_KC;
-  return sizeof(__bf16) * (size_t)(tm + tn) * nkc * X3_BLK;
+  // B blocks rounded up to an even count (the 256-column tiles)
+  const int64_t tm = (M + X3_T - 1) / X3_T, tnb = ((N + 2 * X3_T - 1) / (2 * X3_T)) * 2,
+                nkc = (K + X3_KC - 1) / X3_KC;
+  return sizeof(__bf16) * (size_t)(tm + tnb) * nkc * X3_BLK;
 }
 
 int gemm_x3(hfpx_ctx* ctx, float* C, int64_t ldc, const float* A, int64_t lda, const float* B, int64_t ldb,
-            int64_t M, int64_t N, int64_t K) {
+            int64_t M, int64_t N, int64_t K, int wide) {
   if (M <= 0 || N <= 0 || K <= 0) return HFPX_OK;
   if (ldc > (int64_t)1 << 24) return set_error(ctx, HFPX_ERR_INVALID, "gemm: ldc above 2^24");
-  const int64_t tm = (M + X3_T - 1) / X3_T, tn = (N + X3_T - 1) / X3_T, nkc = (K + X3_KC - 1) / X3_KC;
-  if (tm > 65535 || tn > 65535 || nkc > 65535) return set_error(ctx, HFPX_ERR_INVALID, "gemm: too large");
+  const int64_t tm = (M + X3_T - 1) / X3_T, nkc = (K + X3_KC - 1) / X3_KC;
+  const int64_t tnb = ((N + 2 * X3_T - 1) / (2 * X3_T)) * 2;  // B blocks, even
+  const int64_t tn = wide ? tnb / 2 : (N + X3_T - 1) / X3_T;  // product tiles along N
+  if (tm > 65535 || tnb > 65535 || nkc > 65535) return set_error(ctx, HFPX_ERR_INVALID, "gemm: too large");
   int rc = ensure_x3_work(ctx, gemm_x3_work_bytes(M, N, K));
   if (rc) return rc;
   __bf16* As = (__bf16*)ctx->x3_work;
   __bf16* Bs = As + (size_t)tm * nkc * X3_BLK;
   hipLaunchKernelGGL(k_split_a, dim3((unsigned)tm, (unsigned)nkc), dim3(256), 0, ctx->stream, A, lda, M, K,
                      (int)nkc, As, 1.0f);
-  hipLaunchKernelGGL(k_split_b, dim3((unsigned)tn, (unsigned)nkc), dim3(256), 0, ctx->stream, B, ldb, K, N,
+  hipLaunchKernelGGL(k_split_b, dim3((unsigned)tnb, (unsigned)nkc), dim3(256), 0, ctx->stream, B, ldb, K, N,
                      (int)nkc, Bs, -1.0f);
   // band height: the band's A parts (128 rows x K, 6 bytes per entry) in
   // the 4 MB L2 of an XCD, between 1 and 16 row tiles
   int64_t band = ctx->knob[HFPX_KNOB_X3_BAND];
   if (band <= 0) band = std::max<int64_t>(1, std::min<int64_t>(16, (4 << 20) / ((int64_t)X3_T * nkc * X3_KC * 6)));
-  hipLaunchKernelGGL(k_x3gemm, dim3((unsigned)(tm * tn)), dim3(256), 0, ctx->stream, C, ldc, As, Bs, M, N,
-                     (int)nkc, tm, tn, band);
+  if (wide)
+    hipLaunchKernelGGL((k_x3gemm<16, 4>), dim3((unsigned)(tm * tn)), dim3(256), 0, ctx->stream, C, ldc, As, Bs, M,
+                       N, (int)nkc, tm, tn, band);
+  else
+    hipLaunchKernelGGL((k_x3gemm<24, 2>), dim3((unsigned)(tm * tn)), dim3(256), 0, ctx->stream, C, ldc, As, Bs, M,
+                       N, (int)nkc, tm, tn, band);
   HFPX_TRY_HIP(ctx, hipGetLastError());
   return HFPX_OK;
 }

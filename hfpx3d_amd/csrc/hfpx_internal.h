@@ -175,7 +175,7 @@ int gemm_f32_variants();
 // FP32 C -= A B by three-way bf16 operand splitting on bf16 MFMA
 // (hfpx_gemm_x3.hip); workspace bytes for an M x N x K product
 int gemm_x3(hfpx_ctx* ctx, float* C, int64_t ldc, const float* A, int64_t lda, const float* B, int64_t ldb,
-            int64_t M, int64_t N, int64_t K);
+            int64_t M, int64_t N, int64_t K, int wide);
 size_t gemm_x3_work_bytes(int64_t M, int64_t N, int64_t K);
 // C (M x N) -= A (M x K) B (K x N), column-major, on the context stream
 // (hfpx_gemm.hip; instantiated for float and double).

@@ -198,8 +198,8 @@ int hfpx_batch_matrix(hfpx_ctx* ctx, hfpx_batch* batch, int crack, double* A_hos
 /* ---- tuning -------------------------------------------------------------- */
 /* Kernel-variant knobs of a context (no reference counterpart).
  * HFPX_KNOB_GEMM_F32: FP32 trailing-update GEMM (products with more than 64
- * columns): 0 = automatic (13 when M >= 256, N >= 128 and K >= 64; else 7
- * for K <= 384, 2 for longer K); 1 = 16x16x4 FP32 MFMA, 128 x 128 tile;
+ * columns): 0 = automatic (14 when M, N >= 4096 and K >= 512; 13 when
+ * M >= 256, N >= 128 and K >= 64; else 7 for K <= 384, 2 for longer K); 1 = 16x16x4 FP32 MFMA, 128 x 128 tile;
  * 2..12 = 32x32x2 FP32 MFMA, K chunk 16, with 2 = 128 x 128 tile, no
  * options, 3 = unguarded interior tiles (edge bands in separate launches),
  * 4 = 3 + pipelined LDS operands, 5 = 4 + XCD-grouped tile order, 6 = 3 +
@@ -208,7 +208,8 @@ int hfpx_batch_matrix(hfpx_ctx* ctx, hfpx_batch* batch, int crack, double* A_hos
  * 128 x 256 tiles, 12 = 9 + pipelined LDS operands; 13 = bf16 MFMA
  * (32x32x16) on three-way bf16 splits of both operands, six products per
  * pair, FP32 accumulation: FP32-accurate, not bitwise equal to the FP32
- * MFMA kernels. */
+ * MFMA kernels; 14 = 13 with 128 x 256 tiles (waves of 64 x 128, unpadded
+ * LDS rows). */
 #define HFPX_KNOB_GEMM_F32 0
 /* HFPX_KNOB_LU_NB: outer block of the LU (columns per trailing update):
  * 0 = default (1024), or one of 256, 512, 1024, 2048, 4096. */

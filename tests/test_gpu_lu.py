@@ -42,7 +42,7 @@ def test_gemm_sub(env, dt, M, N, K):
     assert np.abs(got - ref).max() / (np.abs(A).max() * np.abs(B).max() * K) < tol
 
 
-@pytest.mark.parametrize("variant", range(1, 14))
+@pytest.mark.parametrize("variant", range(1, 15))
 @pytest.mark.parametrize("M,N,K", [(300, 257, 33), (513, 700, 100), (1000, 300, 256), (130, 129, 17),
                                    (256, 256, 32), (77, 200, 20), (512, 384, 256)])
 def test_sgemm_variants(env, variant, M, N, K):
@@ -68,7 +68,7 @@ def test_sgemm_variants(env, variant, M, N, K):
 
 @pytest.mark.parametrize("scaled", [False, True])
 def test_sgemm_bf16_split_is_fp32_accurate(env, scaled):
-    """The three-way bf16 split product (variant 13) against float64 on the
+    """The three-way bf16 split product (variants 13, 14) against float64 on the
     same FP32 inputs: its error stays within a small factor of the FP32
     MFMA kernel's (variant 2), i.e. FP32-level, also when the rows of A and
     the columns of B span 10^-8 .. 10^8 (products stay inside FP32's range;
@@ -89,7 +89,7 @@ def test_sgemm_bf16_split_is_fp32_accurate(env, scaled):
     err = {}
     old = hx.tune_get(_lib.KNOB_GEMM_F32)
     try:
-        for v in (2, 13):
+        for v in (2, 13, 14):
             hx.tune_set(_lib.KNOB_GEMM_F32, v)
             Cd = to_dev(torch, hx, C32, torch.float32)
             hx.gemm_sub(Cd, to_dev(torch, hx, A32, torch.float32), to_dev(torch, hx, B32, torch.float32))
@@ -98,6 +98,7 @@ def test_sgemm_bf16_split_is_fp32_accurate(env, scaled):
         hx.tune_set(_lib.KNOB_GEMM_F32, old)
     assert err[2] < 1e-6
     assert err[13] < 3 * err[2] + 1e-7, err
+    assert err[14] < 3 * err[2] + 1e-7, err
 
 
 def test_tune_rejects_unknown_variant(env):
