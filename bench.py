@@ -34,7 +34,7 @@ X3_PEAK = PEAKS["bf16_mfma_TFs"] / X3_PRODUCTS
 # bytes per call of the representative trailing update, from rocprofv3 --pmc (profiles/gemm72k_pmc.md)
 # bytes per call of the first trailing update (bench.py --workload gemm72k),
 # from the FETCH_SIZE / WRITE_SIZE passes summarised in profiles/
-GEMM72K_TRAFFIC = 2.106e+11
+GEMM72K_TRAFFIC = 1.378e+11
 GEMM72K_ALG_BYTES = 4.0882e+10
 
 
