@@ -31,6 +31,8 @@ namespace hfpx {
 // at entry off of the batch's table, and it runs tiles workgroups.
 struct BTab {
   int64_t off = 0, tiles = 0;
+  int64_t roff = 0, rtot = 0;  // prefix of the cracks' 128-row tiles of A (split product)
+  int64_t coff = 0, ctot = 0;  // prefix of their 128-column tiles of B
 };
 }  // namespace hfpx
 
@@ -63,6 +65,7 @@ struct hfpx_batch {
   float* x32 = nullptr;
   double* berr = nullptr;   // per crack (table order)
   int32_t* tiles = nullptr;  // tile prefix tables of the batched updates
+  __bf16* xsplit = nullptr;  // bf16 parts of the FP32 outer updates (largest launch)
   std::vector<hfpx::BTab> tab32, tab64;  // update launches of each group, in issue order
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -460,15 +463,27 @@ static bool plan_updates(const std::vector<int64_t>& sizes, std::vector<int32_t>
   const int64_t nmax = sizes[0];
   auto add = [&](int nact, int64_t k, int64_t j, bool outer) {
     BTab t;
-    t.off = (int64_t)tab.size();
-    int64_t sum = 0;
-    tab.push_back(0);
-    for (int e = 0; e < nact; ++e) {
-      sum += bgemm_tiles(sizes[e], k, j, outer);
-      if (sum > 0x7fffffff) return false;
-      tab.push_back((int32_t)sum);
+    // three prefix tables: tiles of the product, row tiles of A, column
+    // tiles of B
+    int64_t tot[3];
+    int64_t* offs[3] = {&t.off, &t.roff, &t.coff};
+    for (int w = 0; w < 3; ++w) {
+      *offs[w] = (int64_t)tab.size();
+      int64_t sum = 0;
+      tab.push_back(0);
+      for (int e = 0; e < nact; ++e) {
+        int64_t r0, c0, p0, M, N, K;
+        if (bgemm_shape(sizes[e], k, j, outer, r0, c0, p0, M, N, K))
+          sum += w == 0 ? ((M + BGT - 1) / BGT) * ((N + BGT - 1) / BGT) : (w == 1 ? (M + BGT - 1) / BGT
+                                                                                  : (N + BGT - 1) / BGT);
+        if (sum > 0x7fffffff) return false;
+        tab.push_back((int32_t)sum);
+      }
+      tot[w] = sum;
     }
-    t.tiles = sum;
+    t.tiles = tot[0];
+    t.rtot = tot[1];
+    t.ctot = tot[2];
     out.push_back(t);
     return true;
   };
@@ -494,6 +509,13 @@ static int batch_lu(hfpx_ctx* ctx, hfpx_batch* bt, T* base, int which, const int
   auto update = [&](int nact, int64_t k, int64_t j, bool outer) -> int {
     if (li >= plan.size()) return set_error(ctx, HFPX_ERR_INVALID, "batched LU: update plan out of step");
     const BTab& t = plan[li++];
+    if constexpr (sizeof(T) == 4) {
+      // FP32 outer updates: the bf16-split product unless an FP32 epilogue is asked for
+      const int kn = ctx->knob[HFPX_KNOB_BGEMM_F32];
+      if (outer && (kn == 0 || kn == 3) && bt->xsplit)
+        return launch_bgemm_x3(ctx, base, which, bt->cr, list, nact, bt->tiles + t.off, t.tiles,
+                               bt->tiles + t.roff, t.rtot, bt->tiles + t.coff, t.ctot, k, bt->xsplit);
+    }
     return launch_bgemm<T>(ctx, base, which, bt->cr, list, nact, bt->tiles + t.off, t.tiles, k, j, outer);
   };
   int rc;
@@ -690,6 +712,8 @@ int hfpx_batch_create(hfpx_ctx* ctx, int n_cracks, const int64_t* n_nodes, const
   std::vector<int32_t> tab;
   if (!plan_updates(bt->n32_sorted, tab, bt->tab32) || !plan_updates(bt->n64_sorted, tab, bt->tab64))
     return fail(set_error(ctx, HFPX_ERR_INVALID, "batch: too many tiles in one update"));
+  size_t split_bytes = 0;  // the largest FP32 outer update's bf16 parts
+  for (const BTab& t : bt->tab32) split_bytes = std::max(split_bytes, bgemm_x3_split_bytes(t.rtot, t.ctot));
   auto dalloc = [&](void** p, size_t bytes) -> int {
     HFPX_TRY_HIP(ctx, hipMalloc(p, std::max<size_t>(bytes, 256)));
     return HFPX_OK;
@@ -714,7 +738,8 @@ int hfpx_batch_create(hfpx_ctx* ctx, int n_cracks, const int64_t* n_nodes, const
       (rc = dalloc((void**)&bt->r, sizeof(double) * x_off)) || (rc = dalloc((void**)&bt->rs, sizeof(double) * x_off)) ||
       (rc = dalloc((void**)&bt->x32, sizeof(float) * std::max<int64_t>(1, bt->x32_total))) ||
       (rc = dalloc((void**)&bt->berr, sizeof(double) * n_cracks)) ||
-      (rc = dalloc((void**)&bt->tiles, sizeof(int32_t) * tab.size())))
+      (rc = dalloc((void**)&bt->tiles, sizeof(int32_t) * tab.size())) ||
+      (split_bytes && (rc = dalloc((void**)&bt->xsplit, split_bytes))))
     return fail(rc);
   auto up = [&](void* d, const void* h, size_t bytes) -> int {
     if (bytes) HFPX_TRY_HIP(ctx, hipMemcpy(d, h, bytes, hipMemcpyHostToDevice));
@@ -749,7 +774,7 @@ int hfpx_batch_destroy(hfpx_batch* bt) {
   if (bt->exec) (void)hipGraphExecDestroy(bt->exec);
   if (bt->graph) (void)hipGraphDestroy(bt->graph);
   void* ps[] = {bt->cr, bt->list32, bt->list64, bt->list_all, bt->nods, bt->conn, bt->elem, bt->cp, bt->A64, bt->A32, bt->F64,
-                bt->ipiv, bt->perm, bt->info, bt->b, bt->x, bt->r, bt->rs, bt->x32, bt->berr, bt->tiles};
+                bt->ipiv, bt->perm, bt->info, bt->b, bt->x, bt->r, bt->rs, bt->x32, bt->berr, bt->tiles, bt->xsplit};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   delete bt;

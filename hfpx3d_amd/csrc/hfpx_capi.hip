@@ -408,7 +408,7 @@ int hfpx_tune_set(hfpx_ctx* ctx, int knob, int value) {
     return HFPX_OK;
   }
   if (knob == HFPX_KNOB_BGEMM_F32) {
-    if (value < 0 || value > 2) return set_error(ctx, HFPX_ERR_INVALID, "tune: BGEMM_F32 must be 0, 1 or 2");
+    if (value < 0 || value > 3) return set_error(ctx, HFPX_ERR_INVALID, "tune: BGEMM_F32 must be 0 .. 3");
     ctx->knob[knob] = value;
     return HFPX_OK;
   }

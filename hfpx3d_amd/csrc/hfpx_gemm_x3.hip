@@ -41,16 +41,17 @@ __device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l)
   l = (__bf16)r2;
 }
 
-// A (M x K, lda) -> blocks [row tile it][chunk kc][part][row][k], sign * A.
-// Thread t writes piece t of each part: row t / 2, k 8 (t % 2) .. + 7.
-__global__ __launch_bounds__(256) void k_split_a(const float* __restrict__ A, int64_t lda, int64_t M, int64_t K,
-                                                 int nkc, __bf16* __restrict__ out, float sign) {
-  __shared__ float s[X3_KC][X3_T + 1];
-  const int it = blockIdx.x, kc = blockIdx.y, t = threadIdx.x;
+// Row tile it, chunk kc of A (M x K, lda) -> one block [part][row][k] at
+// blk, sign * A.  Thread t writes piece t of each part: row t / 2, k 8 (t % 2)
+// .. + 7.  s is the caller's LDS staging.
+__device__ __forceinline__ void split_a_block(const float* __restrict__ A, int64_t lda, int64_t M, int64_t K,
+                                              int64_t it, int64_t kc, bf16x8* __restrict__ blk, float sign,
+                                              float (*s)[X3_T + 1]) {
+  const int t = threadIdx.x;
 #pragma unroll
   for (int q = 0; q < X3_T * X3_KC / 256; ++q) {
     const int e = t + 256 * q, i = e % X3_T, k = e / X3_T;
-    const int64_t gi = (int64_t)it * X3_T + i, gk = (int64_t)kc * X3_KC + k;
+    const int64_t gi = it * X3_T + i, gk = kc * X3_KC + k;
     s[k][i] = (gi < M && gk < K) ? A[gi + gk * lda] : 0.f;
   }
   __syncthreads();
@@ -64,18 +65,17 @@ __global__ __launch_bounds__(256) void k_split_a(const float* __restrict__ A, in
     pm[e] = m;
     pl[e] = l;
   }
-  bf16x8* blk = (bf16x8*)(out + ((size_t)it * nkc + kc) * X3_BLK);
   blk[t] = ph;
   blk[256 + t] = pm;
   blk[512 + t] = pl;
 }
 
-// B (K x N, ldb) -> blocks [column tile jt][chunk kc][part][column][k].
-__global__ __launch_bounds__(256) void k_split_b(const float* __restrict__ B, int64_t ldb, int64_t K, int64_t N,
-                                                 int nkc, __bf16* __restrict__ out, float sign) {
-  const int jt = blockIdx.x, kc = blockIdx.y, t = threadIdx.x;
+// Column tile jt, chunk kc of B (K x N, ldb) -> one block [part][column][k].
+__device__ __forceinline__ void split_b_block(const float* __restrict__ B, int64_t ldb, int64_t K, int64_t N,
+                                              int64_t jt, int64_t kc, bf16x8* __restrict__ blk, float sign) {
+  const int t = threadIdx.x;
   const int jl = t >> 1, half = t & 1;
-  const int64_t gj = (int64_t)jt * X3_T + jl, k0 = (int64_t)kc * X3_KC + 8 * half;
+  const int64_t gj = jt * X3_T + jl, k0 = kc * X3_KC + 8 * half;
   bf16x8 ph, pm, pl;
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
@@ -86,10 +86,24 @@ __global__ __launch_bounds__(256) void k_split_b(const float* __restrict__ B, in
     pm[e] = m;
     pl[e] = l;
   }
-  bf16x8* blk = (bf16x8*)(out + ((size_t)jt * nkc + kc) * X3_BLK);
   blk[t] = ph;
   blk[256 + t] = pm;
   blk[512 + t] = pl;
+}
+
+// A (M x K, lda) -> blocks [row tile it][chunk kc], sign * A.
+__global__ __launch_bounds__(256) void k_split_a(const float* __restrict__ A, int64_t lda, int64_t M, int64_t K,
+                                                 int nkc, __bf16* __restrict__ out, float sign) {
+  __shared__ float s[X3_KC][X3_T + 1];
+  const int64_t it = blockIdx.x, kc = blockIdx.y;
+  split_a_block(A, lda, M, K, it, kc, (bf16x8*)(out + ((size_t)it * nkc + kc) * X3_BLK), sign, s);
+}
+
+// B (K x N, ldb) -> blocks [column tile jt][chunk kc], sign * B.
+__global__ __launch_bounds__(256) void k_split_b(const float* __restrict__ B, int64_t ldb, int64_t K, int64_t N,
+                                                 int nkc, __bf16* __restrict__ out, float sign) {
+  const int64_t jt = blockIdx.x, kc = blockIdx.y;
+  split_b_block(B, ldb, K, N, jt, kc, (bf16x8*)(out + ((size_t)jt * nkc + kc) * X3_BLK), sign);
 }
 
 // C (M x N) += As Bs over nkc chunks: tiles of 128 rows by 64 WCB columns,
@@ -109,21 +123,26 @@ __global__ __launch_bounds__(256) void k_split_b(const float* __restrict__ B, in
 // VGPRs.  Bs has an even number of column blocks when WCB = 4 (the split
 // kernel zero-fills the last one).
 template <int ROW, int WCB>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_x3gemm(
-    float* __restrict__ C, int64_t ldc, const __bf16* __restrict__ As, const __bf16* __restrict__ Bs, int64_t M,
-    int64_t N, int nkc, int64_t tm, int64_t tn, int64_t band) {
-  constexpr int GN = 64 * WCB;       // tile columns
-  constexpr int NBB = GN / X3_T;     // B blocks per tile (1 or 2)
-  __shared__ __attribute__((aligned(16))) __bf16 sA[2][3 * X3_T * ROW];
-  __shared__ __attribute__((aligned(16))) __bf16 sB[2][NBB][3 * X3_T * ROW];
+struct X3Lds {
+  static constexpr int NBB = 64 * WCB / X3_T;  // B blocks per tile (1 or 2)
+  __bf16 a[2][3 * X3_T * ROW];
+  __bf16 b[2][NBB][3 * X3_T * ROW];
+};
+
+// One tile of C at (row0, col0): pa = the row tile's A blocks (nkc of them),
+// pb = the column tile's B blocks (NBB x nkc, consecutive), lds = the
+// caller's LDS.
+template <int ROW, int WCB>
+__device__ __forceinline__ void x3_tile(X3Lds<ROW, WCB>& lds, float* __restrict__ C, int64_t ldc,
+                                        const __bf16* __restrict__ pa, const __bf16* __restrict__ pb, int64_t M,
+                                        int64_t N, int nkc, int64_t row0, int64_t col0) {
+  constexpr int GN = 64 * WCB;                 // tile columns
+  constexpr int NBB = X3Lds<ROW, WCB>::NBB;
+  auto& sA = lds.a;
+  auto& sB = lds.b;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int l32 = lane & 31, hk = lane >> 5;
-  int64_t tile_r, tile_c;
-  sg_band_tile(sg_xcd_order(blockIdx.x, tm * tn), tm, tn, tile_r, tile_c, band);
-  const int64_t row0 = tile_r * X3_T, col0 = tile_c * GN;
   const int wr = (wave & 1) * 64, wc = (wave >> 1) * 32 * WCB;
-  const __bf16* pa = As + (size_t)tile_r * nkc * X3_BLK;
-  const __bf16* pb = Bs + (size_t)tile_c * NBB * nkc * X3_BLK;
 
   // accumulators start as the tile of C (32-bit offsets from its corner)
   float* __restrict__ Ct = C + row0 + col0 * ldc;
@@ -216,6 +235,105 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         if (jc < nrem) Ct[off] = acc[tc][tr][v];
       }
     }
+}
+
+template <int ROW, int WCB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_x3gemm(
+    float* __restrict__ C, int64_t ldc, const __bf16* __restrict__ As, const __bf16* __restrict__ Bs, int64_t M,
+    int64_t N, int nkc, int64_t tm, int64_t tn, int64_t band) {
+  __shared__ __attribute__((aligned(16))) X3Lds<ROW, WCB> lds;
+  int64_t tile_r, tile_c;
+  sg_band_tile(sg_xcd_order(blockIdx.x, tm * tn), tm, tn, tile_r, tile_c, band);
+  constexpr int NBB = X3Lds<ROW, WCB>::NBB;
+  x3_tile<ROW, WCB>(lds, C, ldc, As + (size_t)tile_r * nkc * X3_BLK, Bs + (size_t)tile_c * NBB * nkc * X3_BLK, M,
+                    N, nkc, tile_r * X3_T, tile_c * 64 * WCB);
+}
+
+// ---------------------------------------------------------------------
+// Batched form: the outer Schur-complement updates of the FP32 group of the
+// batched LU (bgemm_shape with outer = true, K = BNB).  For one launch,
+// rstart / cstart (nact + 1 entries) are the prefix sums of the active
+// cracks' 128-row tiles of A and 128-column tiles of B along the list; the
+// split buffer holds the A blocks of all of them, then the B blocks.
+// ---------------------------------------------------------------------
+// entry e of the list with start[e] <= id < start[e + 1] (one exists)
+__device__ __forceinline__ int find_entry(const int32_t* __restrict__ start, int nact, int64_t id, int* s_e) {
+  for (int e = threadIdx.x; e < nact; e += 256)
+    if (start[e] <= id && id < start[e + 1]) *s_e = e;
+  __syncthreads();
+  return *s_e;
+}
+
+__global__ __launch_bounds__(256) void k_bsplit_a(float* __restrict__ base, int which, const BCrack* __restrict__ cr,
+                                                  const int32_t* __restrict__ list, const int32_t* __restrict__ rstart,
+                                                  int nact, int64_t k, int nkc, __bf16* __restrict__ out) {
+  __shared__ float s[X3_KC][X3_T + 1];
+  __shared__ int s_e;
+  const int64_t rb = blockIdx.x, kc = blockIdx.y;
+  const int e = find_entry(rstart, nact, rb, &s_e);
+  const BCrack& c = cr[list[e]];
+  int64_t r0, c0, p0, M, N, K;
+  bgemm_shape(c.n, k, k, true, r0, c0, p0, M, N, K);
+  const float* a = base + bcopy_off(c, which);
+  split_a_block(a + r0 + p0 * c.n, c.n, M, K, rb - rstart[e], kc, (bf16x8*)(out + ((size_t)rb * nkc + kc) * X3_BLK),
+                1.0f, s);
+}
+
+__global__ __launch_bounds__(256) void k_bsplit_b(float* __restrict__ base, int which, const BCrack* __restrict__ cr,
+                                                  const int32_t* __restrict__ list, const int32_t* __restrict__ cstart,
+                                                  int nact, int64_t k, int nkc, __bf16* __restrict__ out) {
+  __shared__ int s_e;
+  const int64_t cb = blockIdx.x, kc = blockIdx.y;
+  const int e = find_entry(cstart, nact, cb, &s_e);
+  const BCrack& c = cr[list[e]];
+  int64_t r0, c0, p0, M, N, K;
+  bgemm_shape(c.n, k, k, true, r0, c0, p0, M, N, K);
+  const float* a = base + bcopy_off(c, which);
+  split_b_block(a + p0 + c0 * c.n, c.n, K, N, cb - cstart[e], kc, (bf16x8*)(out + ((size_t)cb * nkc + kc) * X3_BLK),
+                -1.0f);
+}
+
+// one workgroup per 128 x 128 tile of the launch (tstart: tile prefix sums)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_bx3gemm(
+    float* __restrict__ base, int which, const BCrack* __restrict__ cr, const int32_t* __restrict__ list,
+    const int32_t* __restrict__ tstart, const int32_t* __restrict__ rstart, const int32_t* __restrict__ cstart,
+    int nact, int64_t k, int nkc, const __bf16* __restrict__ As, const __bf16* __restrict__ Bs) {
+  __shared__ __attribute__((aligned(16))) X3Lds<24, 2> lds;
+  __shared__ int s_e;
+  const int64_t nid = sg_xcd_order(blockIdx.x, tstart[nact]);
+  const int e = find_entry(tstart, nact, nid, &s_e);
+  const BCrack& c = cr[list[e]];
+  int64_t r0, c0, p0, M, N, K;
+  bgemm_shape(c.n, k, k, true, r0, c0, p0, M, N, K);
+  const int64_t tm = (M + X3_T - 1) / X3_T, tn = (N + X3_T - 1) / X3_T;
+  int64_t tile_r, tile_c;
+  sg_band_tile(nid - tstart[e], tm, tn, tile_r, tile_c);
+  float* a = base + bcopy_off(c, which);
+  x3_tile<24, 2>(lds, a + r0 + c0 * c.n, c.n, As + (size_t)(rstart[e] + tile_r) * nkc * X3_BLK,
+                 Bs + (size_t)(cstart[e] + tile_c) * nkc * X3_BLK, M, N, nkc, tile_r * X3_T, tile_c * X3_T);
+}
+
+size_t bgemm_x3_split_bytes(int64_t rtot, int64_t ctot) {
+  return sizeof(__bf16) * (size_t)(rtot + ctot) * (BNB / X3_KC) * X3_BLK;
+}
+
+int launch_bgemm_x3(hfpx_ctx* ctx, float* base, int which, const BCrack* cr, const int32_t* list, int nact,
+                    const int32_t* tstart, int64_t tiles, const int32_t* rstart, int64_t rtot, const int32_t* cstart,
+                    int64_t ctot, int64_t k, __bf16* split) {
+  if (nact <= 0 || tiles <= 0) return HFPX_OK;
+  constexpr int nkc = BNB / X3_KC;
+  __bf16* As = split;
+  __bf16* Bs = split + (size_t)rtot * nkc * X3_BLK;
+  const int tok = prof_begin(ctx, HFPX_KC_GEMM_F32);
+  hipLaunchKernelGGL(k_bsplit_a, dim3((unsigned)rtot, (unsigned)nkc), dim3(256), 0, ctx->stream, base, which, cr,
+                     list, rstart, nact, k, nkc, As);
+  hipLaunchKernelGGL(k_bsplit_b, dim3((unsigned)ctot, (unsigned)nkc), dim3(256), 0, ctx->stream, base, which, cr,
+                     list, cstart, nact, k, nkc, Bs);
+  hipLaunchKernelGGL(k_bx3gemm, dim3((unsigned)tiles), dim3(256), 0, ctx->stream, base, which, cr, list, tstart,
+                     rstart, cstart, nact, k, nkc, (const __bf16*)As, (const __bf16*)Bs);
+  prof_end(ctx, tok, HFPX_KC_GEMM_F32, 0.0, 0.0);
+  HFPX_TRY_HIP(ctx, hipGetLastError());
+  return HFPX_OK;
 }
 
 // Grow-only workspace of the split operands.

@@ -203,6 +203,16 @@ template <typename T>
 int launch_bgemm(hfpx_ctx* ctx, T* base, int which, const BCrack* cr, const int32_t* list, int nact,
                  const int32_t* tstart, int64_t tiles, int64_t k, int64_t j, bool outer);
 
+// batched FP32 outer update on bf16 MFMA by three-way splitting
+// (hfpx_gemm_x3.hip): step k of the FP32 group; rstart / cstart (nact + 1
+// entries) are the prefix sums of the cracks' 128-row tiles of A and
+// 128-column tiles of B, rtot / ctot their totals; split holds
+// bgemm_x3_split_bytes(rtot, ctot)
+int launch_bgemm_x3(hfpx_ctx* ctx, float* base, int which, const BCrack* cr, const int32_t* list, int nact,
+                    const int32_t* tstart, int64_t tiles, const int32_t* rstart, int64_t rtot, const int32_t* cstart,
+                    int64_t ctot, int64_t k, __bf16* split);
+size_t bgemm_x3_split_bytes(int64_t rtot, int64_t ctot);
+
 // stress postprocessing (hfpx_stress.hip)
 int launch_stress(hfpx_ctx* ctx, hfpx_mesh* m, double shear_m, double nu, const double* dd, int64_t ldd,
                   const double* pts, int64_t n_pts, const double* sinf_host, double* out, int64_t ldo);

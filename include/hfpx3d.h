@@ -215,8 +215,10 @@ int hfpx_batch_matrix(hfpx_ctx* ctx, hfpx_batch* batch, int crack, double* A_hos
  * 0 = default (1024), or one of 256, 512, 1024, 2048, 4096. */
 #define HFPX_KNOB_LU_NB 1
 /* HFPX_KNOB_BGEMM_F32: FP32 Schur-complement updates of the batched LU:
- * 0 = default (2), 1 = read-modify-write epilogue, 2 = C loaded into the
- * accumulators first. */
+ * 0 = default (3), 1 = FP32 MFMA with a read-modify-write epilogue, 2 =
+ * FP32 MFMA with C loaded into the accumulators first, 3 = outer updates as
+ * bf16 MFMA products of three-way splits (as HFPX_KNOB_GEMM_F32 13), the
+ * updates inside an outer block as 2. */
 #define HFPX_KNOB_BGEMM_F32 2
 /* HFPX_KNOB_PANEL: LU panel kernel: 0 = default (1), 1 = grid-barrier
  * panel, 2 = exchange panel (per-column tagged words, no barrier; used for

@@ -53,11 +53,12 @@ def test_batch_assembly_matches_single_crack(env):
         assert np.abs(bt.rhs(i) - b1).max() <= 1e-15 * np.abs(b1).max(), i
 
 
-@pytest.mark.parametrize("use_graph,bgemm", [(False, 0), (True, 0), (False, 1)])
+@pytest.mark.parametrize("use_graph,bgemm", [(False, 0), (True, 0), (False, 1), (False, 2)])
 def test_batch_solve_matches_lapack(env, use_graph, bgemm):
     """n from 360 to 1,620 DoF; cracks with n >= 900 take FP32 LU + 3
-    refinement steps, the others FP64 LU.  bgemm selects the FP32 update
-    epilogue (0 default: C preloaded, 1: read-modify-write)."""
+    refinement steps, the others FP64 LU.  bgemm selects the FP32 updates
+    (0 default: outer updates as bf16-split products; 1: FP32 MFMA with a
+    read-modify-write epilogue; 2: FP32 MFMA with C preloaded)."""
     torch, hx = env
     from hfpx3d_amd import _lib
     sl = pytest.importorskip("scipy.linalg")
