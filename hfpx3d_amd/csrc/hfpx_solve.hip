@@ -227,6 +227,11 @@ int solve_ir(hfpx_ctx* ctx, const double* A, int64_t lda, const float* LU, int64
       break;
     }
     if (outer == max_outer) break;
+    // the correction has to take the backward error from `worst` below tol:
+    // the inner iteration stops at a preconditioned-residual reduction of
+    // 0.01 tol / worst (clamped to [1e-12, 1e-6]); the outer loop still runs
+    // until the backward error itself is below tol
+    const double inner_tol = std::max(1e-12, std::min(1e-6, 0.01 * tol / worst));
     // GMRES on M^-1 A d = M^-1 r, zero initial guess
     double* V0 = w.V;
     if ((rc = precond(ctx, w, LU, ldlu, ipiv, n, w.R, n, V0, n, k))) return rc;
@@ -289,9 +294,7 @@ int solve_ir(hfpx_ctx* ctx, const double* A, int64_t lda, const float* LU, int64
       if ((rc = put_col(ctx, w.col2, tmp.data(), k))) return rc;
       hipLaunchKernelGGL(k_scale, grid2(n, k, 256), dim3(256), 0, ctx->stream, w.col2, Vn, n, n);
       steps = s + 1;
-      // the preconditioned residual has dropped far below what one FP64
-      // outer step can use
-      if (worst_res < 1e-10) break;
+      if (worst_res < inner_tol) break;
     }
     inner_total += steps;
     // y = H^-1 g (upper triangular, per column); x += V y
