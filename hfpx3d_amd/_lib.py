@@ -25,7 +25,8 @@ KNOB_PANEL = 3
 KNOB_X3_BAND = 4
 GEMM_F32_VARIANTS = 15
 
-KERNEL_CLASSES = {"assemble": 0, "gemm_f64": 1, "gemm_f32": 2, "panel": 3, "laswp": 4, "trsm": 5, "stress": 6}
+KERNEL_CLASSES = {"assemble": 0, "gemm_f64": 1, "gemm_f32": 2, "panel": 3, "laswp": 4, "trsm": 5, "stress": 6,
+                  "vc": 7}
 
 _c_void_p = ctypes.c_void_p
 _c_i64 = ctypes.c_int64
@@ -61,6 +62,11 @@ SIGNATURES = {
     "hfpx_assemble_mixed": (_c_int, [_c_void_p, _c_void_p, _c_double, _c_double, _c_void_p, _c_i64,
                                      _c_void_p, _c_i64]),
     "hfpx_rhs_sinf": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_int, _c_void_p, _c_i64]),
+    "hfpx_assemble_vc": (_c_int, [_c_void_p, _c_void_p, _c_double, _c_double, _c_void_p, _c_i64, _c_void_p,
+                                  _c_i64]),
+    "hfpx_vc_truncate": (_c_int, [_c_void_p, _c_void_p, _c_i64, _c_i64, _c_void_p, _c_void_p, _c_i64, _c_i64,
+                                  _c_i64, _c_void_p, _c_i64, _c_double, _c_void_p, _c_i64, _c_void_p, _c_i64,
+                                  _c_void_p]),
     "hfpx_stress": (_c_int, [_c_void_p, _c_void_p, _c_double, _c_double, _c_void_p, _c_i64, _c_void_p, _c_i64,
                              _c_void_p, _c_void_p, _c_i64]),
     "hfpx_getrf": (_c_int, [_c_void_p, _c_int, _c_void_p, _c_i64, _c_i64, _c_void_p, ctypes.POINTER(_c_i64)]),

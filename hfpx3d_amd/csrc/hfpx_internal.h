@@ -153,6 +153,7 @@ __host__ __device__ inline int64_t bcopy_off(const BCrack& c, int which) {
 // Profiling bracket around one launch: prof_begin before the launch,
 // prof_end after it (no-ops unless profiling is enabled on the context).
 int prof_begin(hfpx_ctx* ctx, int cls);
+int launch_vc_border(hfpx_ctx* ctx, const hfpx_mesh* m, double* a64, int64_t lda64, float* a32, int64_t lda32);
 void prof_end(hfpx_ctx* ctx, int token, int cls, double flops, double bytes);
 
 int set_error(hfpx_ctx* ctx, int code, const char* fmt, ...);

@@ -76,6 +76,7 @@ class SAE_T:
     n_dof: int = 0
     matrix: object = None     # torch tensor, n_dof x n_dof, column-major, device
     rhs_v: object = None      # torch tensor, n_dof (or n_dof x nrhs), device
+    matrix32: object = None   # optional FP32 rounding of matrix (same shape)
 
 
 class Status:
@@ -293,6 +294,84 @@ def add_s_inf_to_3dbem_rhs(mesh_data, load, sae, ctx=None, device_mesh=None):
     sae.rhs_v = rhs
     sae.n_dof = n
     return rhs
+
+
+# --------------------------------------------------------------------------
+# volume control
+# --------------------------------------------------------------------------
+
+def make_3dbem_matrix_vc(shear_m, poiss_r, mesh, n_par, dof_hndl=None, ctx=None, dtype=None, device_mesh=None,
+                         with_f32=False):
+    """Volume-control matrix ((n_dof+1) x (n_dof+1), device, column-major):
+    the influence matrix of make_3dbem_matrix_s bordered by the opening-volume
+    row (n_dof) and the traction-per-pressure column (n_dof), corner 0.
+    Mirrors hfp3d::make_3dbem_matrix_vc (system_assembly.cpp:458-557),
+    including the io fill-in of an empty dof_hndl.  dtype=torch.float32 gives
+    the FP32 rounding; with_f32=True returns (FP64 matrix, FP32 matrix) from
+    one assembly pass."""
+    ctx = ctx or get_context()
+    torch = ctx.torch
+    dtype = dtype or torch.float64
+    if dof_hndl is not None and (dof_hndl.n_dof == 0 or dof_hndl.dof_h is None):
+        d = make_dof_h_crack(mesh, 2, n_par.tip_type)
+        dof_hndl.n_dof, dof_hndl.dof_h = d.n_dof, d.dof_h
+    if n_par.is_dd_local:
+        raise ValueError("is_dd_local = true is not supported (global DD only)")
+    dm = device_mesh or DeviceMesh(mesh, dof_hndl, n_par.beta, ctx)
+    n1 = dm.n_dof + 1
+    A64 = A32 = None
+    if with_f32 or dtype == torch.float64:
+        A64, _ = alloc_colmajor(n1, n1, torch.float64, ctx)
+    if with_f32 or dtype == torch.float32:
+        A32, _ = alloc_colmajor(n1, n1, torch.float32, ctx)
+    if A64 is None and A32 is None:
+        raise ValueError("dtype must be float64 or float32")
+    ctx.enter()
+    ctx.check(ctx.lib.hfpx_assemble_vc(ctx.handle, dm.handle, float(shear_m), float(poiss_r),
+                                       None if A64 is None else _ptr(A64), n1,
+                                       None if A32 is None else _ptr(A32), n1))
+    ctx.leave()
+    if with_f32:
+        return A64, A32
+    return A64 if A64 is not None else A32
+
+
+def mod_3dbem_system_vc(orig_matrix, orig_dof_hndl, dof_hndl, delta_t, delta_v, ctx=None, with_f32=False):
+    """Truncated volume-control system gathered on the device from the
+    resident (orig_ndof+1)^2 matrix orig_matrix: SAE_T with n_dof = used DoF,
+    matrix (used+1)^2 FP64 (and matrix32, its FP32 rounding, when with_f32),
+    rhs_v (used+1) = [delta_t on the used DoF, delta_v].  delta_t is indexed
+    per slot, per original DoF or per used DoF by its length (n_el * 18,
+    orig n_dof or used n_dof).  Mirrors hfp3d::mod_3dbem_system_vc
+    (system_assembly.cpp:560-629); the border is read at row/column
+    orig n_dof (the reference's n_el * 18 when no original DoF is fixed)."""
+    ctx = ctx or get_context()
+    torch = ctx.torch
+    n_o, lda = _check_colmajor(orig_matrix)
+    if orig_matrix.dtype != torch.float64:
+        raise ValueError("orig_matrix must be the FP64 volume-control matrix")
+    if n_o != orig_dof_hndl.n_dof + 1:
+        raise ValueError(f"orig_matrix is {n_o} x {n_o}, expected orig n_dof + 1 = {orig_dof_hndl.n_dof + 1}")
+    od = np.asfortranarray(orig_dof_hndl.dof_h, dtype=np.int64)
+    ud = np.asfortranarray(dof_hndl.dof_h, dtype=np.int64)
+    if od.shape != ud.shape:
+        raise ValueError("DoF handles of different shapes")
+    n_el, ndpe = ud.shape
+    used = int(dof_hndl.n_dof)
+    nu = used + 1
+    dt = np.ascontiguousarray(np.asarray(delta_t, dtype=np.float64).reshape(-1))
+    M, ldm = alloc_colmajor(nu, nu, torch.float64, ctx)
+    M32 = None
+    if with_f32:
+        M32, _ = alloc_colmajor(nu, nu, torch.float32, ctx)
+    with torch.cuda.stream(ctx.stream):
+        rhs = torch.empty(nu, dtype=torch.float64, device=ctx.device)
+    ctx.enter()
+    ctx.check(ctx.lib.hfpx_vc_truncate(ctx.handle, _ptr(orig_matrix), lda, int(orig_dof_hndl.n_dof), _np_ptr(od),
+                                       _np_ptr(ud), n_el, ndpe, used, _np_ptr(dt), dt.size, float(delta_v), _ptr(M),
+                                       ldm, None if M32 is None else _ptr(M32), nu, _ptr(rhs)))
+    ctx.leave()
+    return SAE_T(n_dof=used, matrix=M, rhs_v=rhs, matrix32=M32)
 
 
 def write_dd_vector_to_md(sol_v, dof_h_dd, n_el=None):

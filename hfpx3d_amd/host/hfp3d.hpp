@@ -10,6 +10,7 @@
 //   make_dof_h_crack, write_dd_vector_to_md     src/Core/surface_mesh_utilities.cpp:28-112, 256-328
 //   make_3dbem_matrix_s, add_s_inf_to_3dbem_rhs,
 //   make_3dbem_stress_f_s                       src/Solvers/system_assembly.cpp:30-455
+//   make_3dbem_matrix_vc, mod_3dbem_system_vc   src/Solvers/system_assembly.cpp:458-629
 //   load_mesh_from_numpy_32/_64, load_mesh_from_csv,
 //   load_crd_from_csv/_numpy                    src/IO/mesh_file_io.cpp:28-471
 //   save_data_to_csv                            src/IO/data_file_io.h:171-204
@@ -140,6 +141,12 @@ void write_dd_vector_to_md(const std::vector<double>& sol_v, const DoF_Handle_T&
 // ---- the GPU stages --------------------------------------------------------
 DeviceMatrix make_3dbem_matrix_s(double shear_m, double poiss_r, const Mesh_Geom_T& mesh,
                                  const Num_Param_T& n_par, DoF_Handle_T& dof_hndl);
+// Volume control: (n_dof+1)^2 bordered matrix (corner 0) and the truncated
+// system gathered from it on the device (rhs_v copied to the host).
+DeviceMatrix make_3dbem_matrix_vc(double shear_m, double poiss_r, const Mesh_Geom_T& mesh,
+                                  const Num_Param_T& n_par, DoF_Handle_T& dof_hndl);
+SAE_T mod_3dbem_system_vc(const DeviceMatrix& orig_matrix, const DoF_Handle_T& orig_dof_hndl,
+                          const DoF_Handle_T& dof_hndl, const std::vector<double>& delta_t, double delta_v);
 void add_s_inf_to_3dbem_rhs(const Mesh_Data_T& mesh_data, const Load_T& load, SAE_T& sae);
 Array2D<double> make_3dbem_stress_f_s(double shear_m, double poiss_r, const Mesh_Data_T& m_data,
                                       const Load_T& load, const Num_Param_T& n_par,

@@ -153,6 +153,39 @@ DeviceMatrix make_3dbem_matrix_s(double shear_m, double poiss_r, const Mesh_Geom
   return A;
 }
 
+DeviceMatrix make_3dbem_matrix_vc(double shear_m, double poiss_r, const Mesh_Geom_T& mesh,
+                                  const Num_Param_T& n_par, DoF_Handle_T& dof_hndl) {
+  if (dof_hndl.n_dof == 0 || dof_hndl.dof_h.size(0) == 0) dof_hndl = make_dof_h_crack(mesh, 2, n_par.tip_type);
+  MeshHandle mh(mesh, &dof_hndl, n_par.beta);
+  const int64_t n1 = dof_hndl.n_dof + 1;
+  DeviceMatrix A(n1);
+  check(hfpx_assemble_vc(context(), mh.m, shear_m, poiss_r, A.data(), n1, nullptr, 0), "volume-control assembly");
+  check(hfpx_synchronize(context()), "volume-control assembly");
+  return A;
+}
+
+SAE_T mod_3dbem_system_vc(const DeviceMatrix& orig_matrix, const DoF_Handle_T& orig_dof_hndl,
+                          const DoF_Handle_T& dof_hndl, const std::vector<double>& delta_t, double delta_v) {
+  const int64_t n_el = dof_hndl.dof_h.size(0), ndpe = dof_hndl.dof_h.size(1);
+  if (orig_dof_hndl.dof_h.size(0) != n_el || orig_dof_hndl.dof_h.size(1) != ndpe)
+    check(HFPX_ERR_INVALID, "mod_3dbem_system_vc: DoF handles of different shapes");
+  if (orig_matrix.size(0) != orig_dof_hndl.n_dof + 1)
+    check(HFPX_ERR_INVALID, "mod_3dbem_system_vc: matrix is not orig n_dof + 1 square");
+  const int64_t nu = dof_hndl.n_dof + 1;
+  SAE_T sae;
+  sae.n_dof = dof_hndl.n_dof;
+  sae.matrix = DeviceMatrix(nu);
+  sae.rhs_v.assign((size_t)nu, 0.0);
+  double* rhs = dalloc<double>((size_t)nu);
+  check(hfpx_vc_truncate(context(), orig_matrix.data(), orig_matrix.size(0), orig_dof_hndl.n_dof,
+                         orig_dof_hndl.dof_h.data(), dof_hndl.dof_h.data(), n_el, ndpe, dof_hndl.n_dof,
+                         delta_t.data(), (int64_t)delta_t.size(), delta_v, sae.matrix.data(), nu, nullptr, 0, rhs),
+        "volume-control truncation");
+  d2h(sae.rhs_v.data(), rhs, sizeof(double) * nu);
+  (void)hipFree(rhs);
+  return sae;
+}
+
 void add_s_inf_to_3dbem_rhs(const Mesh_Data_T& mesh_data, const Load_T& load, SAE_T& sae) {
   const int64_t n = mesh_data.dof_h_dd.n_dof;
   if ((int64_t)sae.rhs_v.size() != n) sae.rhs_v.assign((size_t)n, 0.0);

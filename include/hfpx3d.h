@@ -90,6 +90,36 @@ int hfpx_assemble_mixed(hfpx_ctx* ctx, hfpx_mesh* mesh, double shear_m, double p
 int hfpx_rhs_sinf(hfpx_ctx* ctx, hfpx_mesh* mesh, const double* sinf_host, int nrhs,
                   double* B, int64_t ldb);
 
+/* ---- volume control -------------------------------------------------------- */
+/* Volume-control matrix (device, (n_dof+1) x (n_dof+1), lda >= n_dof + 1):
+ * the influence matrix of hfpx_assemble in rows/columns 0..n_dof-1, row
+ * n_dof = opening volume per unit DD (area integral of the DoF's shape
+ * function times the element normal's component), column n_dof = -(element
+ * normal component) (traction per unit pressure), corner 0.  FP64 (A64)
+ * and/or its FP32 rounding (A32) in one pass; either may be NULL.  Global DD
+ * only (Num_Param_T::is_dd_local = false).  Replaces
+ * hfp3d::make_3dbem_matrix_vc (src/Solvers/system_assembly.cpp:458-557,
+ * declared system_assembly.h:59-64); the reference leaves the corner unset. */
+int hfpx_assemble_vc(hfpx_ctx* ctx, hfpx_mesh* mesh, double shear_m, double poiss_r, double* A64,
+                     int64_t lda64, float* A32, int64_t lda32);
+/* Truncated volume-control system from a resident one.  A (device, lda):
+ * the (orig_ndof+1)^2 volume-control matrix of the original DoF handle
+ * orig_dof_h_host (n_el x ndpe, col-major, -1 = fixed); dof_h_host (same
+ * shape) numbers the used DoF 0..used_ndof-1 (-1 = not used).  Writes
+ *   M(t, s) = sum of A(o_a, o_b) over used slots a -> t, b -> s,
+ *   M(used, s) / M(s, used) = sums of the border row / column, M(used, used) = 0
+ * to M64 (ldm64) and/or its FP32 rounding M32 (ldm32), and rhs (device,
+ * used_ndof + 1): rhs[s] = delta_t_host[...] (indexed by slot e * ndpe + j,
+ * original DoF or used DoF when tsize is n_el * ndpe, orig_ndof or
+ * used_ndof; the last slot of a DoF wins), rhs[used_ndof] = delta_v.
+ * Synchronises the stream.  Replaces hfp3d::mod_3dbem_system_vc
+ * (src/Solvers/system_assembly.cpp:560-629); the border is read at index
+ * orig_ndof, which is the reference's n_el * ndpe whenever the original
+ * handle has no fixed DoF (the only case the reference keeps in bounds). */
+int hfpx_vc_truncate(hfpx_ctx* ctx, const double* A, int64_t lda, int64_t orig_ndof,
+                     const int64_t* orig_dof_h_host, const int64_t* dof_h_host, int64_t n_el, int64_t ndpe,
+                     int64_t used_ndof, const double* delta_t_host, int64_t tsize, double delta_v, double* M64,
+                     int64_t ldm64, float* M32, int64_t ldm32, double* rhs);
 /* ---- postprocessing ------------------------------------------------------ */
 /* Stress at n_pts observation points: S (device, n_pts x 6, col-major, ldS,
  * order Sxx Syy Szz Sxy Sxz Syz, tension positive) = sinf_host (6 values) +
@@ -143,7 +173,8 @@ int hfpx_gemm_sub(hfpx_ctx* ctx, int dtype, void* C, int64_t ldc, const void* A,
 #define HFPX_KC_LASWP 4
 #define HFPX_KC_TRSM 5
 #define HFPX_KC_STRESS 6
-#define HFPX_KC_COUNT 7
+#define HFPX_KC_VC 7
+#define HFPX_KC_COUNT 8
 /* Start (enable = 1) or stop (0) recording a HIP event pair around every
  * launch of the classes above on the context's stream; starting clears the
  * previous records.  No reference counterpart (the reference times whole

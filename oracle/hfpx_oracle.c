@@ -17,6 +17,8 @@
  *   LU (dgetrf / dgetrs)    il/linear_algebra/dense/factorization/LU.h:79-138
  *                           (LAPACK right-looking LU with partial pivoting)
  *   Sneddon opening         src/Solvers/test_radial_crack_static.cpp:316-337
+ *   volume control          src/Solvers/system_assembly.cpp:458-629,
+ *                           src/Core/element_utilities.cpp:484-595
  *
  * The per-pair kernel.  The stress at X due to a displacement discontinuity
  * D_i P(xi) on a flat element (plane x3 = 0 of the element frame) follows
@@ -658,4 +660,113 @@ void hfpo_sneddon_dd(double shear_m, double poiss_r, double a, double p,
   dd[0] = dd[1] = dd[2] = 0.0;
   if (r2 < a * a)
     dd[2] = 4.0 * (1.0 - poiss_r) * a * p / (kPi * shear_m) * sqrt(1.0 - r2 / (a * a));
+}
+
+/* ---------------------------------------------------------------------- */
+/* volume control (system_assembly.cpp:458-629, element_utilities.cpp:484-595) */
+/* ---------------------------------------------------------------------- */
+
+/* Area integrals of the six shape functions over the element: the
+ * reference's el_p2_sf_integral (element_utilities.cpp:573-595) sums, over
+ * the three edges a -> b, the Cauchy-Pompeiu contour form of the monomial
+ * integrals (el_p2_cbp_integral, :484-510) and maps them through sfm.  Here
+ * the same contour form is evaluated by Gauss-Legendre quadrature instead of
+ * the reference's closed-form edge polynomials: with dG/dtb = t^p tb^q,
+ * G = t^p tb^(q+1) / (q+1), the area integral is (1 / 2i) sum over edges of
+ * int G dt, a polynomial of degree <= 3 in the edge parameter (3-point rule
+ * exact).  Monomial order [1, t, tb, t^2, tb^2, t*tb]. */
+void hfpo_sf_integral(const double* tau, const double* sfm, double* out) {
+  static const int P[6] = {0, 1, 0, 2, 0, 1}, Q[6] = {0, 0, 1, 0, 2, 1};
+  const double gx[3] = {0.5 - 0.5 * sqrt(0.6), 0.5, 0.5 + 0.5 * sqrt(0.6)};
+  const double gw[3] = {5.0 / 18.0, 8.0 / 18.0, 5.0 / 18.0};
+  cplx t[3], m[6] = {0, 0, 0, 0, 0, 0};
+  for (int j = 0; j < 3; ++j) t[j] = tau[2 * j] + I * tau[2 * j + 1];
+  for (int e = 0; e < 3; ++e) {
+    const cplx a = t[e], b = t[(e + 1) % 3];
+    for (int g = 0; g < 3; ++g) {
+      const cplx z = a + gx[g] * (b - a), zb = conj(z);
+      for (int k = 0; k < 6; ++k)
+        m[k] += gw[g] * cpow(z, P[k]) * cpow(zb, Q[k] + 1) / (Q[k] + 1.0) * (b - a) / (2.0 * I);
+    }
+  }
+  for (int r = 0; r < 6; ++r) {
+    cplx s = 0;
+    for (int c = 0; c < 6; ++c) s += (sfm[2 * (r + 6 * c)] + I * sfm[2 * (r + 6 * c) + 1]) * m[c];
+    out[r] = creal(s);
+  }
+}
+
+/* The border that make_3dbem_matrix_vc (system_assembly.cpp:497-553) adds to
+ * the n_dof x n_dof matrix: for each free slot (element e, node n_s,
+ * component j) with DoF d,
+ *   row[d] = (int N_{n_s} dA) * n_j   (volume vs DD; n = row 2 of the
+ *                                      element's rotation tensor)
+ *   col[d] = -n_j                     (traction vs pressure),
+ * written in element order (the last writer wins for a DoF shared between
+ * slots).  Global DD only (is_dd_local = false). */
+void hfpo_vc_border(const double* nods, int64_t nods_rows, int64_t n_nodes,
+                    const int64_t* conn, int64_t conn_rows, int64_t n_el,
+                    const int64_t* dof_h, double* row, double* col) {
+  (void)n_nodes;
+  for (int64_t e = 0; e < n_el; ++e) {
+    double v[9], rt[9], tau[6], sfm[72], cp[18], sf[6];
+    el_vert(nods, nods_rows, conn, conn_rows, e, v);
+    hfpo_el_geom(v, 0.0, rt, tau, sfm, cp);
+    hfpo_sf_integral(tau, sfm, sf);
+    for (int ns = 0; ns < 6; ++ns)
+      for (int j = 0; j < 3; ++j) {
+        const int64_t d = dof_h[e + n_el * (3 * ns + j)];
+        if (d < 0) continue;
+        row[d] = sf[ns] * rt[2 + 3 * j];
+        col[d] = -rt[2 + 3 * j];
+      }
+  }
+}
+
+/* Truncated volume-control system, mod_3dbem_system_vc
+ * (system_assembly.cpp:560-629).  A: the (orig_ndof + 1)^2 volume-control
+ * matrix (lda), its border at row/column orig_ndof (the reference indexes it
+ * with n_el * ndpe, the same index whenever the original handle has no fixed
+ * DoF, the only case in which the reference stays in bounds).  For used
+ * slots (e, j) (dof_h >= 0) in element order:
+ *   M(t, s)    += A(o_t, o_s)      over every pair of used slots
+ *   M(used, s) += A(orig_ndof, o_s),  M(s, used) += A(o_s, orig_ndof)
+ *   rhs[s] = delta_t[f] with f = e * ndpe + j if tsize == n_el * ndpe,
+ *            o_s if tsize == orig_ndof, s otherwise;  rhs[used] = delta_v.
+ * M ((used+1)^2, ld used+1) and rhs (used+1) are zero-filled here.  Returns
+ * 0, or -1 when a used slot is fixed in the original handle or tsize fits
+ * none of the three lengths. */
+int hfpo_mod_system_vc(const double* A, int64_t lda, int64_t orig_ndof,
+                       const int64_t* orig_dof_h, const int64_t* dof_h,
+                       int64_t n_el, int64_t ndpe, int64_t used_ndof,
+                       const double* delta_t, int64_t tsize, double delta_v,
+                       double* M, double* rhs) {
+  const int64_t full = n_el * ndpe, ldm = used_ndof + 1;
+  if (tsize != full && tsize != orig_ndof && tsize != used_ndof) return -1;
+  for (int64_t k = 0; k < n_el * ndpe; ++k)
+    if (dof_h[k] >= 0 && orig_dof_h[k] < 0) return -1;
+  memset(M, 0, sizeof(double) * (size_t)(ldm * ldm));
+  memset(rhs, 0, sizeof(double) * (size_t)ldm);
+  rhs[used_ndof] = delta_v;
+  for (int64_t se = 0; se < n_el; ++se)
+    for (int64_t j = 0; j < ndpe; ++j) {
+      const int64_t s = dof_h[se + n_el * j];
+      if (s < 0) continue;
+      const int64_t os = orig_dof_h[se + n_el * j];
+      for (int64_t te = 0; te < n_el; ++te)
+        for (int64_t k = 0; k < ndpe; ++k) {
+          const int64_t t = dof_h[te + n_el * k];
+          if (t < 0) continue;
+          M[t + ldm * s] += A[orig_dof_h[te + n_el * k] + lda * os];
+        }
+      M[used_ndof + ldm * s] += A[orig_ndof + lda * os];
+      M[s + ldm * used_ndof] += A[os + lda * orig_ndof];
+      if (tsize == full)
+        rhs[s] = delta_t[se * ndpe + j];
+      else if (tsize == orig_ndof)
+        rhs[s] = delta_t[os];
+      else
+        rhs[s] = delta_t[s];
+    }
+  return 0;
 }

@@ -76,6 +76,23 @@ void hfpo_getrs(int64_t n, int64_t nrhs, const double* LU, int64_t lda,
 void hfpo_sneddon_dd(double shear_m, double poiss_r, double a, double p,
                      const double* xyz, double* dd);
 
+/* Volume control (system_assembly.cpp:458-629).  Area integrals of the six
+ * shape functions (el_p2_sf_integral, element_utilities.cpp:573-595; tau and
+ * sfm as returned by hfpo_el_geom). */
+void hfpo_sf_integral(const double* tau, const double* sfm, double* out);
+/* Border of make_3dbem_matrix_vc: row[d] = volume per unit DD of DoF d,
+ * col[d] = -normal component (both n_dof long, written for free DoF). */
+void hfpo_vc_border(const double* nods, int64_t nods_rows, int64_t n_nodes,
+                    const int64_t* conn, int64_t conn_rows, int64_t n_el,
+                    const int64_t* dof_h, double* row, double* col);
+/* mod_3dbem_system_vc: M ((used+1)^2 col-major) and rhs (used+1) of the
+ * truncated system; dof handles n_el x ndpe col-major.  0 or -1 (invalid). */
+int hfpo_mod_system_vc(const double* A, int64_t lda, int64_t orig_ndof,
+                       const int64_t* orig_dof_h, const int64_t* dof_h,
+                       int64_t n_el, int64_t ndpe, int64_t used_ndof,
+                       const double* delta_t, int64_t tsize, double delta_v,
+                       double* M, double* rhs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,6 +15,9 @@
 //   hfp3d::make_3dbem_matrix_s          src/Solvers/system_assembly.cpp:30-219
 //   hfp3d::add_s_inf_to_3dbem_rhs       src/Solvers/system_assembly.cpp:222-274
 //   hfp3d::make_3dbem_stress_f_s        src/Solvers/system_assembly.cpp:279-455
+//   hfp3d::el_p2_sf_integral            src/Core/element_utilities.cpp:573-595
+//   hfp3d::make_3dbem_matrix_vc         src/Solvers/system_assembly.cpp:458-557
+//   hfp3d::mod_3dbem_system_vc          src/Solvers/system_assembly.cpp:560-629
 //
 // All arrays are column-major, like il::Array2D.
 #include <complex>
@@ -185,6 +188,72 @@ int ref_stress(const double* nods, int64_t nods_rows, int64_t n_nodes,
     for (int64_t i = 0; i < 3; ++i) p(i, j) = pts[i + 3 * j];
   il::Array2D<double> s = hfp3d::make_3dbem_stress_f_s(shear_m, poiss_r, md, load, np, p);
   std::memcpy(out, s.data(), sizeof(double) * n_pts * 6);
+  return 0;
+}
+
+// Area integrals of the six shape functions; tau, sfm as in ref_el_geom.
+int ref_sf_integral(const double* tau, const double* sfm, double* out) {
+  il::StaticArray<std::complex<double>, 3> t;
+  for (int k = 0; k < 3; ++k) t[k] = std::complex<double>(tau[2 * k], tau[2 * k + 1]);
+  il::StaticArray2D<std::complex<double>, 6, 6> s;
+  for (int c = 0; c < 6; ++c)
+    for (int r = 0; r < 6; ++r)
+      s(r, c) = std::complex<double>(sfm[2 * (r + 6 * c)], sfm[2 * (r + 6 * c) + 1]);
+  il::StaticArray<double, 6> v = hfp3d::el_p2_sf_integral(s, t);
+  for (int k = 0; k < 6; ++k) out[k] = v[k];
+  return 0;
+}
+
+// Volume-control matrix, (n_dof+1)^2 col-major into A (n_dof as returned by
+// ref_dof_h for the same tip_type).  The reference leaves the corner entry
+// (n_dof, n_dof) unset; it is written as 0 here.
+int64_t ref_matrix_vc(const double* nods, int64_t nods_rows, int64_t n_nodes,
+                      const int64_t* conn, int64_t conn_rows, int64_t n_el,
+                      double shear_m, double poiss_r, double beta, int tip_type,
+                      int nthreads, double* A) {
+  set_threads(nthreads);
+  hfp3d::Mesh_Geom_T mesh = make_mesh(nods, nods_rows, n_nodes, conn, conn_rows, n_el);
+  hfp3d::Num_Param_T np;
+  np.beta = beta;
+  np.tip_type = tip_type;
+  np.is_dd_local = false;
+  hfp3d::DoF_Handle_T d;
+  il::Array2D<double> m = hfp3d::make_3dbem_matrix_vc(shear_m, poiss_r, mesh, np, il::io, d);
+  const il::int_t n = m.size(0);
+  m(n - 1, n - 1) = 0.0;
+  for (il::int_t j = 0; j < n; ++j)
+    for (il::int_t i = 0; i < n; ++i) A[i + n * j] = m(i, j);
+  return d.n_dof;
+}
+
+// Truncated volume-control system: A is (orig_ndof+1)^2 col-major; dof
+// handles n_el x 18 col-major; M (used+1)^2 col-major, rhs (used+1).
+int ref_mod_system_vc(const double* A, int64_t orig_ndof, const int64_t* orig_dof_h,
+                      const int64_t* dof_h, int64_t n_el, int64_t used_ndof,
+                      const double* delta_t, int64_t tsize, double delta_v, double* M,
+                      double* rhs) {
+  set_threads(1);
+  const il::int_t no = orig_ndof + 1;
+  il::Array2D<double> a{no, no, 0.0};
+  for (il::int_t j = 0; j < no; ++j)
+    for (il::int_t i = 0; i < no; ++i) a(i, j) = A[i + no * j];
+  hfp3d::DoF_Handle_T oh, uh;
+  oh.n_dof = orig_ndof;
+  uh.n_dof = used_ndof;
+  oh.dof_h = il::Array2D<il::int_t>{n_el, 18, 0};
+  uh.dof_h = il::Array2D<il::int_t>{n_el, 18, 0};
+  for (int64_t c = 0; c < 18; ++c)
+    for (int64_t e = 0; e < n_el; ++e) {
+      oh.dof_h(e, c) = orig_dof_h[e + n_el * c];
+      uh.dof_h(e, c) = dof_h[e + n_el * c];
+    }
+  il::Array<double> dt{tsize, 0.0};
+  for (il::int_t i = 0; i < tsize; ++i) dt[i] = delta_t[i];
+  hfp3d::SAE_T sae = hfp3d::mod_3dbem_system_vc(a, oh, uh, dt, delta_v);
+  const il::int_t nu = used_ndof + 1;
+  for (il::int_t j = 0; j < nu; ++j)
+    for (il::int_t i = 0; i < nu; ++i) M[i + nu * j] = sae.matrix(i, j);
+  for (il::int_t i = 0; i < nu; ++i) rhs[i] = sae.rhs_v[i];
   return 0;
 }
 

@@ -68,6 +68,10 @@ class Oracle:
         self._getrf = L.fn("getrf", _i64, [_i64, _dp, _i64, _dp, _int])
         self._getrs = L.fn("getrs", None, [_i64, _i64, _dp, _i64, _dp, _dp, _i64])
         self._sned = L.fn("sneddon_dd", None, [_d, _d, _d, _d, _dp, _dp])
+        self._sfi = L.fn("sf_integral", None, [_dp, _dp, _dp])
+        self._vcb = L.fn("vc_border", None, [_dp, _i64, _i64, _dp, _i64, _i64, _dp, _dp, _dp])
+        self._mvc = L.fn("mod_system_vc", _int, [_dp, _i64, _i64, _dp, _dp, _i64, _i64, _i64, _dp, _i64, _d,
+                                                  _dp, _dp])
 
     def el_geom(self, vert, beta=0.125):
         v = _f(vert)
@@ -141,6 +145,37 @@ class Oracle:
         self._getrs(n, nrhs, _p(_f(LU)), n, _p(np.ascontiguousarray(ipiv, dtype=np.int32)), _p(X), n)
         return X
 
+    def sf_integral(self, tau, sfm):
+        t, s = _sub_pack(tau, sfm)
+        out = np.zeros(6)
+        self._sfi(_p(t), _p(s), _p(out))
+        return out
+
+    def vc_border(self, nods, conn, dof):
+        """(row, col) of the volume-control border (n_dof each)."""
+        nods, conn = _f(nods), _f(conn, np.int64)
+        dof_f = _f(dof, np.int64)
+        n_dof = int(dof.max()) + 1
+        row, col = np.zeros(n_dof), np.zeros(n_dof)
+        self._vcb(_p(nods), nods.shape[0], nods.shape[1], _p(conn), conn.shape[0], conn.shape[1], _p(dof_f),
+                  _p(row), _p(col))
+        return row, col
+
+    def mod_system_vc(self, A, orig_dof, dof, used_ndof, delta_t, delta_v):
+        """(M, rhs) of the truncated volume-control system; A is the
+        (orig_ndof+1)^2 volume-control matrix."""
+        A = _f(A)
+        od, ud = _f(orig_dof, np.int64), _f(dof, np.int64)
+        n_el, ndpe = ud.shape
+        nu = used_ndof + 1
+        M, rhs = np.zeros(nu * nu), np.zeros(nu)
+        dt = np.ascontiguousarray(delta_t, dtype=np.float64)
+        rc = self._mvc(_p(A), A.shape[0], A.shape[0] - 1, _p(od), _p(ud), n_el, ndpe, used_ndof, _p(dt), dt.size,
+                       float(delta_v), _p(M), _p(rhs))
+        if rc != 0:
+            raise ValueError("invalid truncation")
+        return M.reshape(nu, nu, order="F"), rhs
+
     def sneddon_dd(self, G, nu, a, p, xyz):
         dd = np.zeros(3)
         self._sned(G, nu, a, p, _p(np.ascontiguousarray(xyz, dtype=np.float64)), _p(dd))
@@ -159,6 +194,38 @@ class Reference:
         self._rhs = L.fn("rhs", _i64, [_dp, _i64, _i64, _dp, _i64, _i64, _int, _dp, _dp])
         self._stress = L.fn("stress", _int, [_dp, _i64, _i64, _dp, _i64, _i64, _d, _d, _dp, _dp, _i64, _dp,
                                              _int, _dp])
+        self._sfi = L.fn("sf_integral", _int, [_dp, _dp, _dp])
+        self._mvcm = L.fn("matrix_vc", _i64, [_dp, _i64, _i64, _dp, _i64, _i64, _d, _d, _d, _int, _int, _dp])
+        self._mvc = L.fn("mod_system_vc", _int, [_dp, _i64, _dp, _dp, _i64, _i64, _dp, _i64, _d, _dp, _dp])
+
+    def sf_integral(self, tau, sfm):
+        t, s = _sub_pack(tau, sfm)
+        out = np.zeros(6)
+        self._sfi(_p(t), _p(s), _p(out))
+        return out
+
+    def matrix_vc(self, nods, conn, G, nu, beta=0.125, tip_type=0, nthreads=8):
+        """The reference's volume-control matrix ((n_dof+1)^2, corner 0)."""
+        if tip_type == 2:
+            raise ValueError("the reference's tip_type = 2 numbering writes past n_dof")
+        nods, conn = _f(nods), _f(conn, np.int64)
+        n_dof, dof = self.dof_h(nods, conn, tip_type)
+        A = np.zeros((n_dof + 1) ** 2)
+        self._mvcm(_p(nods), nods.shape[0], nods.shape[1], _p(conn), conn.shape[0], conn.shape[1], G, nu, beta,
+                   tip_type, nthreads, _p(A))
+        return A.reshape(n_dof + 1, n_dof + 1, order="F"), dof
+
+    def mod_system_vc(self, A, orig_dof, dof, used_ndof, delta_t, delta_v):
+        """The reference's truncation; defined only when the original handle
+        has no fixed DoF (its border index is n_el * 18)."""
+        A = _f(A)
+        od, ud = _f(orig_dof, np.int64), _f(dof, np.int64)
+        nu = used_ndof + 1
+        M, rhs = np.zeros(nu * nu), np.zeros(nu)
+        dt = np.ascontiguousarray(delta_t, dtype=np.float64)
+        self._mvc(_p(A), A.shape[0] - 1, _p(od), _p(ud), ud.shape[0], used_ndof, _p(dt), dt.size, float(delta_v),
+                  _p(M), _p(rhs))
+        return M.reshape(nu, nu, order="F"), rhs
 
     def el_geom(self, vert, beta=0.125):
         v = _f(vert)
