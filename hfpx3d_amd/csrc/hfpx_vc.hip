@@ -95,6 +95,40 @@ __global__ __launch_bounds__(256) void k_vc_gather(const double* __restrict__ A,
   }
 }
 
+// The common case, every used DoF from exactly one slot: M(t, s) =
+// A(idx[t], idx[s]) with idx[nu - 1] the border index and M(nu - 1, nu - 1)
+// = 0.  Four columns per trip keep four independent loads in flight per lane;
+// the outputs are written with non-temporal stores (streamed, not re-read by
+// this kernel).
+template <bool F64, bool F32>
+__global__ __launch_bounds__(256) void k_vc_gather1(const double* __restrict__ A, int64_t lda,
+                                                    const int32_t* __restrict__ idx, int64_t nu,
+                                                    double* __restrict__ M, int64_t ldm,
+                                                    float* __restrict__ M32, int64_t ldm32) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= nu) return;
+  const double* At = A + idx[t];
+  const int64_t G = gridDim.y;
+  int64_t s = blockIdx.y;
+  for (; s + 3 * G < nu; s += 4 * G) {
+    double v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = At[(int64_t)idx[s + k * G] * lda];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int64_t c = s + k * G;
+      const double x = (t == nu - 1 && c == nu - 1) ? 0.0 : v[k];
+      if (F64) __builtin_nontemporal_store(x, M + t + c * ldm);
+      if (F32) __builtin_nontemporal_store((float)x, M32 + t + c * ldm32);
+    }
+  }
+  for (; s < nu; s += G) {
+    const double x = (t == nu - 1 && s == nu - 1) ? 0.0 : At[(int64_t)idx[s] * lda];
+    if (F64) __builtin_nontemporal_store(x, M + t + s * ldm);
+    if (F32) __builtin_nontemporal_store((float)x, M32 + t + s * ldm32);
+  }
+}
+
 int launch_vc_border(hfpx_ctx* ctx, const hfpx_mesh* m, double* a64, int64_t lda64, float* a32,
                      int64_t lda32) {
   const int bs = 128;
@@ -159,6 +193,8 @@ int hfpx_vc_truncate(hfpx_ctx* ctx, const double* A, int64_t lda, int64_t orig_n
   off[(size_t)nu] = 1;  // the border
   for (int64_t u = 0; u < nu; ++u) off[(size_t)u + 1] += off[(size_t)u];
   const int64_t total = off[(size_t)nu];
+  bool one_slot_each = true;
+  for (int64_t u = 0; u < nu; ++u) one_slot_each = one_slot_each && off[(size_t)u + 1] - off[(size_t)u] == 1;
   std::vector<int32_t> idx((size_t)total), fill(off.begin(), off.end() - 1);
   std::vector<double> r((size_t)nu, 0.0);
   for (int64_t e = 0; e < n_el; ++e)
@@ -188,9 +224,19 @@ int hfpx_vc_truncate(hfpx_ctx* ctx, const double* A, int64_t lda, int64_t orig_n
   HFPX_TRY_HIP(ctx, hipMemcpyAsync(rhs, ctx->work, rb, hipMemcpyDeviceToDevice, ctx->stream));
   const int32_t* d_off = (const int32_t*)((char*)ctx->work + rb);
   const int32_t* d_idx = (const int32_t*)((char*)ctx->work + rb + ob);
-  const dim3 grid((unsigned)((nu + 255) / 256), (unsigned)std::min<int64_t>(nu, 8192));
+  const dim3 grid((unsigned)((nu + 255) / 256), (unsigned)std::min<int64_t>(nu, 32768));
   const int tok = prof_begin(ctx, HFPX_KC_VC);
-  if (M64 && M32)
+  if (one_slot_each) {
+    if (M64 && M32)
+      hipLaunchKernelGGL((k_vc_gather1<true, true>), grid, dim3(256), 0, ctx->stream, A, lda, d_idx, nu, M64, ldm64,
+                         M32, ldm32);
+    else if (M64)
+      hipLaunchKernelGGL((k_vc_gather1<true, false>), grid, dim3(256), 0, ctx->stream, A, lda, d_idx, nu, M64, ldm64,
+                         M32, ldm32);
+    else
+      hipLaunchKernelGGL((k_vc_gather1<false, true>), grid, dim3(256), 0, ctx->stream, A, lda, d_idx, nu, M64, ldm64,
+                         M32, ldm32);
+  } else if (M64 && M32)
     hipLaunchKernelGGL((k_vc_gather<true, true>), grid, dim3(256), 0, ctx->stream, A, lda, d_off, d_idx, nu, M64,
                        ldm64, M32, ldm32);
   else if (M64)

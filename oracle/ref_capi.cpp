@@ -20,6 +20,7 @@
 //   hfp3d::mod_3dbem_system_vc          src/Solvers/system_assembly.cpp:560-629
 //
 // All arrays are column-major, like il::Array2D.
+#include <chrono>
 #include <complex>
 #include <cstdint>
 #include <cstring>
@@ -227,12 +228,15 @@ int64_t ref_matrix_vc(const double* nods, int64_t nods_rows, int64_t n_nodes,
 }
 
 // Truncated volume-control system: A is (orig_ndof+1)^2 col-major; dof
-// handles n_el x 18 col-major; M (used+1)^2 col-major, rhs (used+1).
+// handles n_el x 18 col-major; M (used+1)^2 col-major, rhs (used+1).  The
+// reference's OpenMP loop over source elements is race-free when no used DoF
+// is shared between elements.  *seconds (may be NULL) = duration of the
+// reference call alone, without the copies in and out.
 int ref_mod_system_vc(const double* A, int64_t orig_ndof, const int64_t* orig_dof_h,
                       const int64_t* dof_h, int64_t n_el, int64_t used_ndof,
-                      const double* delta_t, int64_t tsize, double delta_v, double* M,
-                      double* rhs) {
-  set_threads(1);
+                      const double* delta_t, int64_t tsize, double delta_v, int nthreads,
+                      double* M, double* rhs, double* seconds) {
+  set_threads(nthreads);
   const il::int_t no = orig_ndof + 1;
   il::Array2D<double> a{no, no, 0.0};
   for (il::int_t j = 0; j < no; ++j)
@@ -249,7 +253,9 @@ int ref_mod_system_vc(const double* A, int64_t orig_ndof, const int64_t* orig_do
     }
   il::Array<double> dt{tsize, 0.0};
   for (il::int_t i = 0; i < tsize; ++i) dt[i] = delta_t[i];
+  const auto t0 = std::chrono::steady_clock::now();
   hfp3d::SAE_T sae = hfp3d::mod_3dbem_system_vc(a, oh, uh, dt, delta_v);
+  if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   const il::int_t nu = used_ndof + 1;
   for (il::int_t j = 0; j < nu; ++j)
     for (il::int_t i = 0; i < nu; ++i) M[i + nu * j] = sae.matrix(i, j);

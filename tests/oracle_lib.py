@@ -196,7 +196,8 @@ class Reference:
                                              _int, _dp])
         self._sfi = L.fn("sf_integral", _int, [_dp, _dp, _dp])
         self._mvcm = L.fn("matrix_vc", _i64, [_dp, _i64, _i64, _dp, _i64, _i64, _d, _d, _d, _int, _int, _dp])
-        self._mvc = L.fn("mod_system_vc", _int, [_dp, _i64, _dp, _dp, _i64, _i64, _dp, _i64, _d, _dp, _dp])
+        self._mvc = L.fn("mod_system_vc", _int, [_dp, _i64, _dp, _dp, _i64, _i64, _dp, _i64, _d, _int, _dp, _dp,
+                                                  _dp])
 
     def sf_integral(self, tau, sfm):
         t, s = _sub_pack(tau, sfm)
@@ -215,7 +216,7 @@ class Reference:
                    tip_type, nthreads, _p(A))
         return A.reshape(n_dof + 1, n_dof + 1, order="F"), dof
 
-    def mod_system_vc(self, A, orig_dof, dof, used_ndof, delta_t, delta_v):
+    def mod_system_vc(self, A, orig_dof, dof, used_ndof, delta_t, delta_v, nthreads=1):
         """The reference's truncation; defined only when the original handle
         has no fixed DoF (its border index is n_el * 18)."""
         A = _f(A)
@@ -223,8 +224,10 @@ class Reference:
         nu = used_ndof + 1
         M, rhs = np.zeros(nu * nu), np.zeros(nu)
         dt = np.ascontiguousarray(delta_t, dtype=np.float64)
+        sec = np.zeros(1)
         self._mvc(_p(A), A.shape[0] - 1, _p(od), _p(ud), ud.shape[0], used_ndof, _p(dt), dt.size, float(delta_v),
-                  _p(M), _p(rhs))
+                  nthreads, _p(M), _p(rhs), _p(sec))
+        self.last_seconds = float(sec[0])
         return M.reshape(nu, nu, order="F"), rhs
 
     def el_geom(self, vert, beta=0.125):

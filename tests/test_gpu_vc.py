@@ -127,6 +127,24 @@ def test_vc_truncate_from_fixed_original_and_merged_dof(hx):
     assert np.array_equal(sae.rhs_v.cpu().numpy(), ro)
 
 
+def test_vc_truncate_unreferenced_dof(hx):
+    """A used DoF that no slot maps to gets a zero row, column and rhs entry
+    (the general gather path; the oracle zero-fills the same way)."""
+    print("case: vc truncate unreferenced DoF", flush=True)
+    nods, conn = load_mesh("pennymesh24el_32")
+    A64, _, dof0 = gpu_vc(hx, nods, conn, 1.0, 0.35, 0)
+    dof1 = hx.make_dof_h_crack(hx.Mesh_Geom_T(nods=nods, conn=conn), 2, 1)
+    used = hx.DoF_Handle_T(n_dof=dof1.n_dof + 1, dof_h=dof1.dof_h)
+    dt = np.ones(dof1.dof_h.size)
+    sae = hx.mod_3dbem_system_vc(A64, dof0, used, dt, 3.0, with_f32=True)
+    Mo, ro = O.mod_system_vc(A64.cpu().numpy(), dof0.dof_h, dof1.dof_h, used.n_dof, dt, 3.0)
+    M = sae.matrix.cpu().numpy()
+    assert np.array_equal(M, Mo)
+    assert np.array_equal(sae.matrix32.cpu().numpy(), Mo.astype(np.float32))
+    assert np.array_equal(sae.rhs_v.cpu().numpy(), ro)
+    assert (M[dof1.n_dof] == 0).all() and (M[:, dof1.n_dof] == 0).all()
+
+
 def test_vc_truncate_rejects_invalid(hx):
     print("case: vc truncate invalid", flush=True)
     nods, conn = load_mesh("pennymesh24el_32")
