@@ -177,6 +177,10 @@ void load_mesh_from_csv(const std::string& src_dir, const std::string& conn_f_na
 Array2D<double> load_crd_from_csv(const std::string& src_dir, const std::string& f_name, Status& status);
 Array2D<double> load_crd_from_numpy(const std::string& src_dir, const std::string& f_name, Status& status);
 bool save_data_to_csv(const Array2D<double>& a, const std::string& trg_dir, const std::string& of_name);
+// Binary output for output_format "npy64" / "npy32" (the reference's branches
+// for these formats are empty, main.cpp:132-152, 215-230, 270-285): NumPy
+// .npy v1.0, '<f8' or '<f4', fortran_order True (the array's own layout).
+bool save_data_to_npy(const Array2D<double>& a, const std::string& trg_dir, const std::string& of_name, bool as_f32);
 
 IO_param_T set_default_paths(const std::string& src_path);
 void read_config(const std::string& f_path, Mesh_Data_T& mesh_data, Properties_T& props, Load_T& load,

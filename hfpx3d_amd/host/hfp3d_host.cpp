@@ -436,6 +436,31 @@ bool save_data_to_csv(const Array2D<double>& a, const std::string& trg_dir, cons
   return (bool)o;
 }
 
+bool save_data_to_npy(const Array2D<double>& a, const std::string& trg_dir, const std::string& of_name, bool as_f32) {
+  std::ofstream o(join(trg_dir, of_name), std::ios::binary);
+  if (!o) return false;
+  std::string dict = std::string("{'descr': '") + (as_f32 ? "<f4" : "<f8") + "', 'fortran_order': True, 'shape': (" +
+                     std::to_string(a.size(0)) + ", " + std::to_string(a.size(1)) + "), }";
+  // magic (6) + version (2) + header length (2) + dict + padding + '\n' is a multiple of 64
+  const size_t base = 10 + dict.size() + 1;
+  dict.append((64 - base % 64) % 64, ' ');
+  dict.push_back('\n');
+  const uint16_t hl = (uint16_t)dict.size();
+  o.write("\x93NUMPY\x01\x00", 8);
+  const char hb[2] = {(char)(hl & 0xff), (char)(hl >> 8)};
+  o.write(hb, 2);
+  o.write(dict.data(), (std::streamsize)dict.size());
+  const size_t n = (size_t)(a.size(0) * a.size(1));
+  if (as_f32) {
+    std::vector<float> f(n);
+    for (size_t i = 0; i < n; ++i) f[i] = (float)a.data()[i];
+    o.write((const char*)f.data(), (std::streamsize)(n * sizeof(float)));
+  } else {
+    o.write((const char*)a.data(), (std::streamsize)(n * sizeof(double)));
+  }
+  return (bool)o;
+}
+
 // ------------------------------------------------------------------ config
 // A TOML subset: key = value pairs (strings, integers, floats, booleans),
 // '#' comments and [table] headers.  Values keep their type: like the

@@ -1,12 +1,23 @@
 // hfpx3d_host_check.cpp -- prints what the host layer parses, for the CPU
 // tests (no device call): read_config results, mesh sizes and checksums,
-// and the DoF handle.  Usage: hfpx3d_host_check <config.toml>
+// and the DoF handle.  Usage: hfpx3d_host_check <config.toml>; with
+// "--npy <dir>" it writes a small array through save_data_to_npy instead.
 #include <cstdio>
+#include <string>
 
 #include "hfp3d.hpp"
 
 int main(int argc, char** argv) {
   if (argc < 2) return 2;
+  if (argc == 3 && std::string(argv[1]) == "--npy") {
+    // writes a 3 x 5 array (entry (i, j) = i + 10 j + 0.1) as check64.npy and check32.npy
+    hfp3d::Array2D<double> a(3, 5);
+    for (int64_t j = 0; j < 5; ++j)
+      for (int64_t i = 0; i < 3; ++i) a(i, j) = (double)i + 10.0 * (double)j + 0.1;
+    const bool ok = hfp3d::save_data_to_npy(a, argv[2], "check64.npy", false) &&
+                    hfp3d::save_data_to_npy(a, argv[2], "check32.npy", true);
+    return ok ? 0 : 1;
+  }
   hfp3d::Mesh_Data_T md;
   hfp3d::Properties_T props;
   hfp3d::Load_T load;

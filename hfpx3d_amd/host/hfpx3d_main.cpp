@@ -3,7 +3,8 @@
 //   Assembly       = DoF handle + influence matrix + right-hand side
 //   Solution       = LU factorisation + solve
 //   Postprocessing = stresses at the observation points
-// and the CSV outputs the configuration asks for.  The configuration path is
+// and the CSV (or, for output_format npy64 / npy32, .npy) outputs the
+// configuration asks for.  The configuration path is
 // the first argument (default ./config.toml) instead of being derived from
 // the source file's location.
 #include <chrono>
@@ -13,6 +14,18 @@
 
 using clk = std::chrono::steady_clock;
 static double secs(clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); }
+
+// output_format "npy64" / "npy32" writes NumPy files (name with ".npy" in
+// place of the configured ".csv"); anything else writes CSV, like the
+// reference's default branch.
+static bool save_out(const hfp3d::Array2D<double>& a, const hfp3d::IO_param_T& io, std::string& name) {
+  if (io.out_f_format == "npy64" || io.out_f_format == "npy32") {
+    if (name.size() >= 4 && name.compare(name.size() - 4, 4, ".csv") == 0) name.resize(name.size() - 4);
+    name += ".npy";
+    return hfp3d::save_data_to_npy(a, io.output_dir, name, io.out_f_format == "npy32");
+  }
+  return hfp3d::save_data_to_csv(a, io.output_dir, name);
+}
 
 int main(int argc, char** argv) {
   const std::string cfg = argc > 1 ? argv[1] : "config.toml";
@@ -39,8 +52,9 @@ int main(int argc, char** argv) {
   std::cout << 18 * n_el - sae.n_dof << " Fixed DoF" << std::endl;
 
   if (sp.do_save_matrix) {
-    const bool ok = hfp3d::save_data_to_csv(sae.matrix.to_host(), io.output_dir, io.matr_f_name);
-    std::cout << (ok ? "Matrix saved to " + io.output_dir + io.matr_f_name : std::string("Cannot save the matrix"))
+    std::string name = io.matr_f_name;
+    const bool ok = save_out(sae.matrix.to_host(), io, name);
+    std::cout << (ok ? "Matrix saved to " + io.output_dir + name : std::string("Cannot save the matrix"))
               << std::endl;
   }
 
@@ -67,8 +81,9 @@ int main(int argc, char** argv) {
         }
       }
     }
-    const bool ok = hfp3d::save_data_to_csv(out, io.output_dir, io.out_f_name);
-    std::cout << (ok ? "Solution (nodes + DD) saved to " + io.output_dir + io.out_f_name
+    std::string name = io.out_f_name;
+    const bool ok = save_out(out, io, name);
+    std::cout << (ok ? "Solution (nodes + DD) saved to " + io.output_dir + name
                      : std::string("Cannot save the solution"))
               << std::endl;
   }
@@ -84,8 +99,9 @@ int main(int argc, char** argv) {
     hfp3d::Array2D<double> s =
         hfp3d::make_3dbem_stress_f_s(props.shear_m[0], props.poiss_r[0], md, load, np, pts);
     std::cout << ": " << secs(t) << "s" << std::endl;
-    const bool ok = hfp3d::save_data_to_csv(s, io.output_dir, io.strs_f_name);
-    std::cout << (ok ? "Stresses saved to " + io.output_dir + io.strs_f_name : std::string("Cannot save stresses"))
+    std::string name = io.strs_f_name;
+    const bool ok = save_out(s, io, name);
+    std::cout << (ok ? "Stresses saved to " + io.output_dir + name : std::string("Cannot save stresses"))
               << std::endl;
   }
   return 0;

@@ -135,3 +135,18 @@ def test_missing_required_key_is_reported(tmp_path):
     cfg, _, _ = write_case(tmp_path, body=body)
     r = subprocess.run([CHECK, cfg], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "mesh_conn_fname" in r.stdout
+
+
+def test_save_data_to_npy(tmp_path):
+    """save_data_to_npy writes .npy v1.0 files numpy reads back: FP64 exactly,
+    FP32 as the rounding, column-major with the array's shape."""
+    r = subprocess.run([CHECK, "--npy", str(tmp_path) + "/"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0
+    want = np.array([[i + 10.0 * j + 0.1 for j in range(5)] for i in range(3)])
+    a = np.load(tmp_path / "check64.npy", allow_pickle=False)
+    b = np.load(tmp_path / "check32.npy", allow_pickle=False)
+    assert a.dtype == np.float64 and a.shape == (3, 5) and a.flags["F_CONTIGUOUS"]
+    assert b.dtype == np.float32 and b.shape == (3, 5)
+    assert np.array_equal(a, want) and np.array_equal(b, want.astype(np.float32))
+    raw = (tmp_path / "check64.npy").read_bytes()
+    assert raw[:8] == b"\x93NUMPY\x01\x00" and (10 + int.from_bytes(raw[8:10], "little")) % 64 == 0
