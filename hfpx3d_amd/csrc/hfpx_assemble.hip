@@ -125,7 +125,7 @@ template <typename Store>
 __device__ __forceinline__ void assemble_wave(const SrcElem<double>* __restrict__ elem,
                                               const double* __restrict__ cp, int64_t cps, int64_t n_el,
                                               const int32_t* __restrict__ dof, double shear_m, double nu,
-                                              const Store& store, int64_t bx, int64_t by) {
+                                              bool dd_local, const Store& store, int64_t bx, int64_t by) {
   const int lane = threadIdx.x & 63;
   const int64_t es = __builtin_amdgcn_readfirstlane((int)(by * 4 + (threadIdx.x >> 6)));
   if (es >= n_el) return;
@@ -170,7 +170,9 @@ __device__ __forceinline__ void assemble_wave(const SrcElem<double>* __restrict_
       if (col < 0) continue;
       for (int i = 0; i < 3; ++i) {
         if (row[i] < 0) continue;
-        const double v = tg[i][0] * E.R[jg] + tg[i][1] * E.R[3 + jg] + tg[i][2] * E.R[6 + jg];
+        // global DD: T R per source node; local DD keep T (wave-uniform branch)
+        const double v =
+            dd_local ? tg[i][jg] : tg[i][0] * E.R[jg] + tg[i][1] * E.R[3 + jg] + tg[i][2] * E.R[6 + jg];
         store(row[i], col, v);
       }
     }
@@ -181,8 +183,8 @@ template <typename Store>
 __global__ __launch_bounds__(256) void k_assemble(const SrcElem<double>* __restrict__ elem,
                                                   const double* __restrict__ cp, int64_t n_el,
                                                   const int32_t* __restrict__ dof, double shear_m,
-                                                  double nu, Store store) {
-  assemble_wave(elem, cp, 6 * n_el, n_el, dof, shear_m, nu, store, blockIdx.x, blockIdx.y);
+                                                  double nu, bool dd_local, Store store) {
+  assemble_wave(elem, cp, 6 * n_el, n_el, dof, shear_m, nu, dd_local, store, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------
@@ -214,7 +216,8 @@ __global__ __launch_bounds__(256) void k_bassemble(const SrcElem<double>* __rest
   const int64_t gx = (6 * c.n_el + 63) / 64;
   const int64_t local = w - c.wg_off;
   StoreBoth st{a64 + c.a_off, c.n, c.a32_off >= 0 ? a32 + c.a32_off : nullptr, c.n};
-  assemble_wave(elem + c.el_off, cp + 6 * c.el_off, cps, c.n_el, nullptr, c.G, c.nu, st, local % gx, local / gx);
+  assemble_wave(elem + c.el_off, cp + 6 * c.el_off, cps, c.n_el, nullptr, c.G, c.nu, false, st, local % gx,
+                local / gx);
 }
 
 // batched right-hand sides: one thread per element of the batch,
@@ -276,7 +279,7 @@ int launch_assemble(hfpx_ctx* ctx, const hfpx_mesh* m, double shear_m, double po
   StoreBoth st{a64, lda64, a32, lda32};
   const int tok = prof_begin(ctx, HFPX_KC_ASSEMBLE);
   hipLaunchKernelGGL(k_assemble<StoreBoth>, dim3(gx, gy), dim3(256), 0, ctx->stream, m->elem,
-                     m->cp, m->n_el, m->identity_dof ? nullptr : m->dof, shear_m, poiss_r, st);
+                     m->cp, m->n_el, m->identity_dof ? nullptr : m->dof, shear_m, poiss_r, m->dd_local, st);
   // algorithmic HBM bytes: every matrix entry written once (FP64 and/or FP32)
   const double entries = (double)m->n_dof * (double)m->n_dof;
   prof_end(ctx, tok, HFPX_KC_ASSEMBLE, 0.0, entries * ((a64 ? 8.0 : 0.0) + (a32 ? 4.0 : 0.0)));

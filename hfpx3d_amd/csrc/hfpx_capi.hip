@@ -259,6 +259,12 @@ int hfpx_mesh_destroy(hfpx_mesh* m) {
 
 int64_t hfpx_mesh_n_dof(const hfpx_mesh* m) { return m ? m->n_dof : -1; }
 
+int hfpx_mesh_set_dd_local(hfpx_mesh* m, int is_dd_local) {
+  if (!m || (is_dd_local != 0 && is_dd_local != 1)) return HFPX_ERR_INVALID;
+  m->dd_local = is_dd_local != 0;
+  return HFPX_OK;
+}
+
 int hfpx_assemble(hfpx_ctx* ctx, hfpx_mesh* m, double shear_m, double poiss_r, double* A,
                   int64_t lda) {
   return hfpx_assemble_mixed(ctx, m, shear_m, poiss_r, A, lda, nullptr, 0);

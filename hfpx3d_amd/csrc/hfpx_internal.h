@@ -52,6 +52,7 @@ struct hfpx_mesh {
   int64_t n_dof = 0;
   double beta = 0.125;
   bool identity_dof = true;
+  bool dd_local = false;       // DD in each element's own frame (Num_Param_T::is_dd_local)
   double* nods = nullptr;      // 3 x n_nodes (x, y, z rows only)
   int64_t n_nodes = 0;
   int32_t* conn = nullptr;     // 3 x n_el (vertex ids, 0-based)

@@ -15,15 +15,18 @@
 
 namespace hfpx {
 
-// DD of every node in its element's frame: dloc[3*(6e+j)+k] = (R_e d_glob)_k
+// DD of every node in its element's frame: dloc[3*(6e+j)+k] = (R_e d_glob)_k,
+// or the given DD unchanged when they are already local (is_dd_local,
+// system_assembly.cpp:358-387)
 __global__ void k_dd_local(const SrcElem<double>* __restrict__ elem, const double* __restrict__ dd,
-                           int64_t ldd, int64_t n_el, double* __restrict__ dloc) {
+                           int64_t ldd, int64_t n_el, bool already_local, double* __restrict__ dloc) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 6 * n_el) return;
   const int64_t e = t / 6;
   const double g[3] = {dd[t], dd[t + ldd], dd[t + 2 * ldd]};
   const double* R = elem[e].R;
-  for (int k = 0; k < 3; ++k) dloc[3 * t + k] = R[3 * k] * g[0] + R[3 * k + 1] * g[1] + R[3 * k + 2] * g[2];
+  for (int k = 0; k < 3; ++k)
+    dloc[3 * t + k] = already_local ? g[k] : R[3 * k] * g[0] + R[3 * k + 1] * g[1] + R[3 * k + 2] * g[2];
 }
 
 __global__ __launch_bounds__(256) void k_stress(const SrcElem<double>* __restrict__ elem,
@@ -105,7 +108,7 @@ int launch_stress(hfpx_ctx* ctx, hfpx_mesh* m, double shear_m, double nu, const 
   memcpy(ctx->hwork, sinf_host, 6 * sizeof(double));
   HFPX_TRY_HIP(ctx, hipMemcpyAsync(sinf, ctx->hwork, 6 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   hipLaunchKernelGGL(k_dd_local, dim3((unsigned)((6 * m->n_el + 255) / 256)), dim3(256), 0, ctx->stream,
-                     m->elem, dd, ldd, m->n_el, dloc);
+                     m->elem, dd, ldd, m->n_el, m->dd_local, dloc);
   const int tok = prof_begin(ctx, HFPX_KC_STRESS);
   hipLaunchKernelGGL(k_stress, dim3((unsigned)pblocks, (unsigned)nchunk), dim3(256), 0, ctx->stream, m->elem,
                      dloc, m->n_el, pts, n_pts, chunk, shear_m, nu, part);

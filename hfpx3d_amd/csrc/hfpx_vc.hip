@@ -30,9 +30,11 @@ namespace hfpx {
 // as contour integrals of the monomials.  Thread 0 also writes the corner
 // A(n, n) = 0 (the reference leaves it unset).  A DoF shared between
 // elements keeps one of their values, as in the reference's OpenMP loop.
+// With DD in element frames (dd_local) the volume row holds the area
+// integral on the opening component only (system_assembly.cpp:529-536).
 __global__ void k_vc_border(const SrcElem<double>* __restrict__ elem, int64_t n_el,
-                            const int32_t* __restrict__ dof, int64_t n, double* __restrict__ a64,
-                            int64_t lda64, float* __restrict__ a32, int64_t lda32) {
+                            const int32_t* __restrict__ dof, int64_t n, bool dd_local,
+                            double* __restrict__ a64, int64_t lda64, float* __restrict__ a32, int64_t lda32) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e == 0) {
     if (a64) a64[n + n * lda64] = 0.0;
@@ -48,12 +50,14 @@ __global__ void k_vc_border(const SrcElem<double>* __restrict__ elem, int64_t n_
       const int64_t d = dof ? (int64_t)dof[18 * e + l] : 18 * e + l;
       if (d < 0) continue;
       const double nj = E.R[6 + j];
+      // local DD: only the opening component (2) carries volume
+      const double vol = dd_local ? (j == 2 ? sf : 0.0) : sf * nj;
       if (a64) {
-        a64[n + d * lda64] = sf * nj;
+        a64[n + d * lda64] = vol;
         a64[d + n * lda64] = -nj;
       }
       if (a32) {
-        a32[n + d * lda32] = (float)(sf * nj);
+        a32[n + d * lda32] = (float)vol;
         a32[d + n * lda32] = (float)(-nj);
       }
     }
@@ -133,7 +137,7 @@ int launch_vc_border(hfpx_ctx* ctx, const hfpx_mesh* m, double* a64, int64_t lda
                      int64_t lda32) {
   const int bs = 128;
   hipLaunchKernelGGL(k_vc_border, dim3((unsigned)((m->n_el + bs - 1) / bs)), dim3(bs), 0, ctx->stream, m->elem,
-                     m->n_el, m->identity_dof ? nullptr : m->dof, m->n_dof, a64, lda64, a32, lda32);
+                     m->n_el, m->identity_dof ? nullptr : m->dof, m->n_dof, m->dd_local, a64, lda64, a32, lda32);
   HFPX_TRY_HIP(ctx, hipGetLastError());
   return HFPX_OK;
 }

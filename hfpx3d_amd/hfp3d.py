@@ -155,7 +155,7 @@ def _np_ptr(a):
 class DeviceMesh:
     """A mesh (plus DoF map and collocation offset) resident on the GPU."""
 
-    def __init__(self, mesh, dof_hndl=None, beta=0.125, ctx=None):
+    def __init__(self, mesh, dof_hndl=None, beta=0.125, ctx=None, dd_local=False):
         self.ctx = ctx or get_context()
         nods = np.asfortranarray(mesh.nods, dtype=np.float64)
         conn = np.asfortranarray(mesh.conn, dtype=np.int64)
@@ -175,6 +175,12 @@ class DeviceMesh:
             self.n_el, dof_ptr, n_dof, float(beta), ctypes.byref(h)))
         self.handle = h
         self.n_dof = int(self.ctx.lib.hfpx_mesh_n_dof(h))
+        self.set_dd_local(dd_local)
+
+    def set_dd_local(self, dd_local):
+        """DD in each element's own frame (Num_Param_T::is_dd_local)."""
+        self.ctx.check(self.ctx.lib.hfpx_mesh_set_dd_local(self.handle, int(bool(dd_local))))
+        self.dd_local = bool(dd_local)
 
     def close(self):
         if getattr(self, "handle", None):
@@ -253,9 +259,8 @@ def make_3dbem_matrix_s(shear_m, poiss_r, mesh, n_par, dof_hndl=None, ctx=None, 
     if dof_hndl is not None and (dof_hndl.n_dof == 0 or dof_hndl.dof_h is None):
         d = make_dof_h_crack(mesh, 2, n_par.tip_type)
         dof_hndl.n_dof, dof_hndl.dof_h = d.n_dof, d.dof_h
-    if n_par.is_dd_local:
-        raise ValueError("is_dd_local = true is not supported (global DD only)")
     dm = device_mesh or DeviceMesh(mesh, dof_hndl, n_par.beta, ctx)
+    dm.set_dd_local(n_par.is_dd_local)
     A, lda = alloc_colmajor(dm.n_dof, dm.n_dof, dtype, ctx)
     ctx.enter()
     if dtype == torch.float64:
@@ -315,9 +320,8 @@ def make_3dbem_matrix_vc(shear_m, poiss_r, mesh, n_par, dof_hndl=None, ctx=None,
     if dof_hndl is not None and (dof_hndl.n_dof == 0 or dof_hndl.dof_h is None):
         d = make_dof_h_crack(mesh, 2, n_par.tip_type)
         dof_hndl.n_dof, dof_hndl.dof_h = d.n_dof, d.dof_h
-    if n_par.is_dd_local:
-        raise ValueError("is_dd_local = true is not supported (global DD only)")
     dm = device_mesh or DeviceMesh(mesh, dof_hndl, n_par.beta, ctx)
+    dm.set_dd_local(n_par.is_dd_local)
     n1 = dm.n_dof + 1
     A64 = A32 = None
     if with_f32 or dtype == torch.float64:
@@ -524,12 +528,11 @@ def solve_mixed(A64, B, tol=1e-14, max_outer=10, gmres_m=20, ctx=None, LU32=None
 def make_3dbem_stress_f_s(shear_m, poiss_r, m_data, load, n_par, m_pts_crd, ctx=None, device_mesh=None):
     """Stresses (n_pts x 6: Sxx Syy Szz Sxy Sxz Syz, tension positive) at the
     points m_pts_crd (3 x n_pts) due to the nodal DD m_data.dd ((6 n_el) x 3,
-    global components) plus load.s_inf, like the reference (the far-field
-    stress is added to every point).  Returns a numpy array."""
+    global components, or each element's own frame when n_par.is_dd_local)
+    plus load.s_inf, like the reference (the far-field stress is added to
+    every point).  Returns a numpy array."""
     ctx = ctx or get_context()
     torch = ctx.torch
-    if n_par is not None and n_par.is_dd_local:
-        raise ValueError("is_dd_local = true is not supported (global DD only)")
     pts = np.asarray(m_pts_crd, dtype=np.float64)
     if pts.ndim != 2 or pts.shape[0] != 3:
         raise ValueError("m_pts_crd must be 3 x n_pts")
@@ -539,6 +542,7 @@ def make_3dbem_stress_f_s(shear_m, poiss_r, m_data, load, n_par, m_pts_crd, ctx=
     if dd.shape != (6 * n_el, 3):
         raise ValueError("m_data.dd must be (6 n_el) x 3")
     dm = device_mesh or DeviceMesh(m_data.mesh, None, 0.0, ctx)
+    dm.set_dd_local(n_par is not None and n_par.is_dd_local)
     with torch.cuda.stream(ctx.stream):
         d_dd = torch.from_numpy(np.ascontiguousarray(dd.T)).to(ctx.device)      # (3, 6 n_el): col-major dd
         d_pts = torch.from_numpy(np.ascontiguousarray(pts.T)).to(ctx.device)    # (n_pts, 3): col-major pts

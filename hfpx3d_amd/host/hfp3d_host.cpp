@@ -87,12 +87,13 @@ Array2D<double> DeviceMatrix::to_host() const {
 // RAII device mesh
 struct MeshHandle {
   hfpx_mesh* m = nullptr;
-  MeshHandle(const Mesh_Geom_T& mesh, const DoF_Handle_T* dof, double beta) {
+  MeshHandle(const Mesh_Geom_T& mesh, const DoF_Handle_T* dof, double beta, bool dd_local = false) {
     const int64_t* dh = (dof && dof->n_dof > 0 && dof->dof_h.size(0) == mesh.conn.size(1)) ? dof->dof_h.data()
                                                                                             : nullptr;
     check(hfpx_mesh_create(context(), mesh.nods.data(), mesh.nods.size(0), mesh.nods.size(1), mesh.conn.data(),
                            mesh.conn.size(0), mesh.conn.size(1), dh, dh ? dof->n_dof : 0, beta, &m),
           "mesh upload");
+    check(hfpx_mesh_set_dd_local(m, dd_local ? 1 : 0), "mesh upload");
   }
   ~MeshHandle() { hfpx_mesh_destroy(m); }
 };
@@ -146,7 +147,7 @@ void write_dd_vector_to_md(const std::vector<double>& sol_v, const DoF_Handle_T&
 DeviceMatrix make_3dbem_matrix_s(double shear_m, double poiss_r, const Mesh_Geom_T& mesh,
                                  const Num_Param_T& n_par, DoF_Handle_T& dof_hndl) {
   if (dof_hndl.n_dof == 0 || dof_hndl.dof_h.size(0) == 0) dof_hndl = make_dof_h_crack(mesh, 2, n_par.tip_type);
-  MeshHandle mh(mesh, &dof_hndl, n_par.beta);
+  MeshHandle mh(mesh, &dof_hndl, n_par.beta, n_par.is_dd_local);
   DeviceMatrix A(dof_hndl.n_dof);
   check(hfpx_assemble(context(), mh.m, shear_m, poiss_r, A.data(), dof_hndl.n_dof), "assembly");
   check(hfpx_synchronize(context()), "assembly");
@@ -156,7 +157,7 @@ DeviceMatrix make_3dbem_matrix_s(double shear_m, double poiss_r, const Mesh_Geom
 DeviceMatrix make_3dbem_matrix_vc(double shear_m, double poiss_r, const Mesh_Geom_T& mesh,
                                   const Num_Param_T& n_par, DoF_Handle_T& dof_hndl) {
   if (dof_hndl.n_dof == 0 || dof_hndl.dof_h.size(0) == 0) dof_hndl = make_dof_h_crack(mesh, 2, n_par.tip_type);
-  MeshHandle mh(mesh, &dof_hndl, n_par.beta);
+  MeshHandle mh(mesh, &dof_hndl, n_par.beta, n_par.is_dd_local);
   const int64_t n1 = dof_hndl.n_dof + 1;
   DeviceMatrix A(n1);
   check(hfpx_assemble_vc(context(), mh.m, shear_m, poiss_r, A.data(), n1, nullptr, 0), "volume-control assembly");
@@ -201,11 +202,10 @@ void add_s_inf_to_3dbem_rhs(const Mesh_Data_T& mesh_data, const Load_T& load, SA
 Array2D<double> make_3dbem_stress_f_s(double shear_m, double poiss_r, const Mesh_Data_T& m_data,
                                       const Load_T& load, const Num_Param_T& n_par,
                                       const Array2D<double>& m_pts_crd) {
-  (void)n_par;
   const int64_t n_el = m_data.mesh.conn.size(1), np = m_pts_crd.size(1);
   Array2D<double> out(np, 6, 0.0);
   if (np == 0) return out;
-  MeshHandle mh(m_data.mesh, nullptr, 0.0);
+  MeshHandle mh(m_data.mesh, nullptr, 0.0, n_par.is_dd_local);
   double* dd = dalloc<double>(18 * (size_t)n_el);
   double* pts = dalloc<double>(3 * (size_t)np);
   double* s = dalloc<double>(6 * (size_t)np);

@@ -68,6 +68,12 @@ int hfpx_mesh_create(hfpx_ctx* ctx, const double* nods_host, int64_t nods_rows,
                      int64_t n_el, const int64_t* dof_h_host, int64_t n_dof,
                      double beta, hfpx_mesh** mesh);
 int hfpx_mesh_destroy(hfpx_mesh* mesh);
+/* DD in each element's own frame (1) or global (0, the default): the
+ * assembly's columns, the volume-control row and the DD given to
+ * hfpx_stress follow it.  Replaces Num_Param_T::is_dd_local
+ * (src/Core/model_parameters.h:89; branches at system_assembly.cpp:186-195,
+ * 358-387, 529-536). */
+int hfpx_mesh_set_dd_local(hfpx_mesh* mesh, int is_dd_local);
 int64_t hfpx_mesh_n_dof(const hfpx_mesh* mesh);
 
 /* ---- assembly ---------------------------------------------------------- */
@@ -96,8 +102,9 @@ int hfpx_rhs_sinf(hfpx_ctx* ctx, hfpx_mesh* mesh, const double* sinf_host, int n
  * n_dof = opening volume per unit DD (area integral of the DoF's shape
  * function times the element normal's component), column n_dof = -(element
  * normal component) (traction per unit pressure), corner 0.  FP64 (A64)
- * and/or its FP32 rounding (A32) in one pass; either may be NULL.  Global DD
- * only (Num_Param_T::is_dd_local = false).  Replaces
+ * and/or its FP32 rounding (A32) in one pass; either may be NULL.  With
+ * hfpx_mesh_set_dd_local(mesh, 1) the row holds the area integral on the
+ * opening (local component 2) only.  Replaces
  * hfp3d::make_3dbem_matrix_vc (src/Solvers/system_assembly.cpp:458-557,
  * declared system_assembly.h:59-64); the reference leaves the corner unset. */
 int hfpx_assemble_vc(hfpx_ctx* ctx, hfpx_mesh* mesh, double shear_m, double poiss_r, double* A64,

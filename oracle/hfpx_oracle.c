@@ -399,7 +399,8 @@ static void el_vert(const double* nods, int64_t nods_rows, const int64_t* conn,
 /* 18x18 traction influence of source element es on the 6 collocation
  * points of target element et, global frame (system_assembly.cpp:88-199). */
 static void el2el_block(const double* vs, const double* vt, double shear_m,
-                        double poiss_r, double beta, double* blk /* 18x18 col-major */) {
+                        double poiss_r, double beta, int dd_local,
+                        double* blk /* 18x18 col-major */) {
   double rs[9], taus[6], sfms[72], cps[18];
   hfpo_el_geom(vs, beta, rs, taus, sfms, cps);
   double rt[9], taut[6], sfmt[72], cpt[18];
@@ -422,8 +423,9 @@ static void el2el_block(const double* vs, const double* vt, double shear_m,
       for (int i = 0; i < 3; ++i) tg[i] = rs[0 + 3 * i] * tl[0] + rs[1 + 3 * i] * tl[1] + rs[2 + 3 * i] * tl[2];
       for (int i = 0; i < 3; ++i) blk[3 * nt + i + 18 * c] = tg[i];
     }
-    /* DD to global: per source node, T . R_s (system_assembly.cpp:164-185) */
-    for (int ns = 0; ns < 6; ++ns) {
+    /* DD to global: per source node, T . R_s (system_assembly.cpp:164-185);
+     * local DD keep the block as it is (:186-195) */
+    for (int ns = 0; ns < 6 && !dd_local; ++ns) {
       double tmp[3][3];
       for (int i = 0; i < 3; ++i)
         for (int jj = 0; jj < 3; ++jj) tmp[i][jj] = blk[3 * nt + i + 18 * (3 * ns + jj)];
@@ -439,7 +441,7 @@ int64_t hfpo_assemble(const double* nods, int64_t nods_rows, int64_t n_nodes,
                       const int64_t* conn, int64_t conn_rows, int64_t n_el,
                       const int64_t* dof_h, int64_t n_dof, double shear_m,
                       double poiss_r, double beta, int64_t t0, int64_t t1,
-                      int nthreads, double* A) {
+                      int is_dd_local, int nthreads, double* A) {
   (void)n_nodes;
   if (t1 <= t0) { t0 = 0; t1 = n_el; }
   memset(A, 0, sizeof(double) * (size_t)n_dof * (size_t)n_dof);
@@ -452,7 +454,7 @@ int64_t hfpo_assemble(const double* nods, int64_t nods_rows, int64_t n_nodes,
     el_vert(nods, nods_rows, conn, conn_rows, es, vs);
     for (int64_t et = t0; et < t1; ++et) {
       el_vert(nods, nods_rows, conn, conn_rows, et, vt);
-      el2el_block(vs, vt, shear_m, poiss_r, beta, blk);
+      el2el_block(vs, vt, shear_m, poiss_r, beta, is_dd_local, blk);
       for (int c = 0; c < 18; ++c) {
         int64_t gc = dof_h[es + n_el * c];
         if (gc < 0) continue;
@@ -510,7 +512,7 @@ void hfpo_stress(const double* nods, int64_t nods_rows, int64_t n_nodes,
                  const int64_t* conn, int64_t conn_rows, int64_t n_el,
                  double shear_m, double poiss_r, const double* dd,
                  const double* pts, int64_t n_pts, const double* s_inf,
-                 int nthreads, double* out) {
+                 int is_dd_local, int nthreads, double* out) {
   (void)n_nodes;
   const int64_t nr = 6 * n_el;
   double* geo = (double*)malloc(sizeof(double) * 96 * (size_t)n_el);
@@ -534,11 +536,13 @@ void hfpo_stress(const double* nods, int64_t nods_rows, int64_t n_nodes,
       to_local(rs, g + 87, pts + 3 * p, y);
       double S[108];
       hfpo_local_submatrix(shear_m, poiss_r, -y[2], y[0], y[1], g + 9, g + 15, S);
-      /* DD of the element in its local frame: d_loc = R d_glob */
+      /* DD of the element in its local frame: d_loc = R d_glob, or the
+       * given DD as they are when they are local (system_assembly.cpp:358-387) */
       for (int ns = 0; ns < 6; ++ns) {
         double dg[3] = {dd[6 * e + ns], dd[6 * e + ns + nr], dd[6 * e + ns + 2 * nr]};
         double dl[3];
-        for (int i = 0; i < 3; ++i) dl[i] = rs[i] * dg[0] + rs[i + 3] * dg[1] + rs[i + 6] * dg[2];
+        for (int i = 0; i < 3; ++i)
+          dl[i] = is_dd_local ? dg[i] : rs[i] * dg[0] + rs[i + 3] * dg[1] + rs[i + 6] * dg[2];
         /* local stress, then R^T S R (tensor_utilities.cpp:57-91) */
         double sl[6] = {0, 0, 0, 0, 0, 0};
         for (int k = 0; k < 3; ++k)
@@ -703,10 +707,10 @@ void hfpo_sf_integral(const double* tau, const double* sfm, double* out) {
  *                                      element's rotation tensor)
  *   col[d] = -n_j                     (traction vs pressure),
  * written in element order (the last writer wins for a DoF shared between
- * slots).  Global DD only (is_dd_local = false). */
+ * slots).  is_dd_local: the row holds the opening component only. */
 void hfpo_vc_border(const double* nods, int64_t nods_rows, int64_t n_nodes,
                     const int64_t* conn, int64_t conn_rows, int64_t n_el,
-                    const int64_t* dof_h, double* row, double* col) {
+                    const int64_t* dof_h, int is_dd_local, double* row, double* col) {
   (void)n_nodes;
   for (int64_t e = 0; e < n_el; ++e) {
     double v[9], rt[9], tau[6], sfm[72], cp[18], sf[6];
@@ -717,7 +721,8 @@ void hfpo_vc_border(const double* nods, int64_t nods_rows, int64_t n_nodes,
       for (int j = 0; j < 3; ++j) {
         const int64_t d = dof_h[e + n_el * (3 * ns + j)];
         if (d < 0) continue;
-        row[d] = sf[ns] * rt[2 + 3 * j];
+        /* local DD: the opening is component 2 (system_assembly.cpp:529-536) */
+        row[d] = is_dd_local ? (j == 2 ? sf[ns] : 0.0) : sf[ns] * rt[2 + 3 * j];
         col[d] = -rt[2 + 3 * j];
       }
   }

@@ -39,12 +39,14 @@ int64_t hfpo_dof_h(const double* nods, int64_t nods_rows, int64_t n_nodes,
 
 /* Dense influence matrix, n_dof x n_dof col-major (lda = n_dof), zero-filled
  * by the callee.  Target rows [t0, t1) of ELEMENTS only are computed when
- * t1 > t0 (rows of other targets are left zero); pass t0 = t1 = 0 for all. */
+ * t1 > t0 (rows of other targets are left zero); pass t0 = t1 = 0 for all.
+ * is_dd_local: columns are DD in each source element's own frame
+ * (Num_Param_T::is_dd_local). */
 int64_t hfpo_assemble(const double* nods, int64_t nods_rows, int64_t n_nodes,
                       const int64_t* conn, int64_t conn_rows, int64_t n_el,
                       const int64_t* dof_h, int64_t n_dof, double shear_m,
                       double poiss_r, double beta, int64_t t0, int64_t t1,
-                      int nthreads, double* A);
+                      int is_dd_local, int nthreads, double* A);
 
 /* rhs[dof] -= (n . s_inf)_l for every free DoF (caller zero-fills rhs). */
 void hfpo_rhs(const double* nods, int64_t nods_rows, int64_t n_nodes,
@@ -56,12 +58,13 @@ void hfpo_write_dd(const double* x, const int64_t* dof_h, int64_t n_el,
                    double* dd);
 
 /* Stress at n_pts points (pts 3 x n_pts), out n_pts x 6 col-major
- * [Sxx Syy Szz Sxy Sxz Syz], s_inf added, deterministic. */
+ * [Sxx Syy Szz Sxy Sxz Syz], s_inf added, deterministic; is_dd_local: dd are
+ * in each element's own frame. */
 void hfpo_stress(const double* nods, int64_t nods_rows, int64_t n_nodes,
                  const int64_t* conn, int64_t conn_rows, int64_t n_el,
                  double shear_m, double poiss_r, const double* dd,
                  const double* pts, int64_t n_pts, const double* s_inf,
-                 int nthreads, double* out);
+                 int is_dd_local, int nthreads, double* out);
 
 /* LU with partial pivoting (LAPACK dgetrf semantics): A n x n col-major
  * (lda), ipiv 1-based.  Returns info (0 ok, k > 0: U(k,k) exactly zero). */
@@ -84,7 +87,7 @@ void hfpo_sf_integral(const double* tau, const double* sfm, double* out);
  * col[d] = -normal component (both n_dof long, written for free DoF). */
 void hfpo_vc_border(const double* nods, int64_t nods_rows, int64_t n_nodes,
                     const int64_t* conn, int64_t conn_rows, int64_t n_el,
-                    const int64_t* dof_h, double* row, double* col);
+                    const int64_t* dof_h, int is_dd_local, double* row, double* col);
 /* mod_3dbem_system_vc: M ((used+1)^2 col-major) and rhs (used+1) of the
  * truncated system; dof handles n_el x ndpe col-major.  0 or -1 (invalid). */
 int hfpo_mod_system_vc(const double* A, int64_t lda, int64_t orig_ndof,

@@ -138,13 +138,13 @@ int64_t ref_dof_h(const double* nods, int64_t nods_rows, int64_t n_nodes,
 int64_t ref_assemble(const double* nods, int64_t nods_rows, int64_t n_nodes,
                      const int64_t* conn, int64_t conn_rows, int64_t n_el,
                      double shear_m, double poiss_r, double beta, int tip_type,
-                     int nthreads, double* A) {
+                     int is_dd_local, int nthreads, double* A) {
   set_threads(nthreads);
   hfp3d::Mesh_Geom_T mesh = make_mesh(nods, nods_rows, n_nodes, conn, conn_rows, n_el);
   hfp3d::Num_Param_T np;
   np.beta = beta;
   np.tip_type = tip_type;
-  np.is_dd_local = false;
+  np.is_dd_local = is_dd_local != 0;
   hfp3d::DoF_Handle_T d;
   il::Array2D<double> m = hfp3d::make_3dbem_matrix_s(shear_m, poiss_r, mesh, np, il::io, d);
   std::memcpy(A, m.data(), sizeof(double) * m.size(0) * m.size(1));
@@ -174,7 +174,7 @@ int ref_stress(const double* nods, int64_t nods_rows, int64_t n_nodes,
                const int64_t* conn, int64_t conn_rows, int64_t n_el,
                double shear_m, double poiss_r, const double* dd,
                const double* pts, int64_t n_pts, const double* s_inf,
-               int nthreads, double* out) {
+               int is_dd_local, int nthreads, double* out) {
   set_threads(nthreads);
   hfp3d::Mesh_Data_T md;
   md.mesh = make_mesh(nods, nods_rows, n_nodes, conn, conn_rows, n_el);
@@ -184,6 +184,7 @@ int ref_stress(const double* nods, int64_t nods_rows, int64_t n_nodes,
   hfp3d::Load_T load;
   for (int k = 0; k < 6; ++k) load.s_inf[k] = s_inf[k];
   hfp3d::Num_Param_T np;
+  np.is_dd_local = is_dd_local != 0;
   il::Array2D<double> p{3, n_pts, 0.0};
   for (int64_t j = 0; j < n_pts; ++j)
     for (int64_t i = 0; i < 3; ++i) p(i, j) = pts[i + 3 * j];
@@ -211,13 +212,13 @@ int ref_sf_integral(const double* tau, const double* sfm, double* out) {
 int64_t ref_matrix_vc(const double* nods, int64_t nods_rows, int64_t n_nodes,
                       const int64_t* conn, int64_t conn_rows, int64_t n_el,
                       double shear_m, double poiss_r, double beta, int tip_type,
-                      int nthreads, double* A) {
+                      int is_dd_local, int nthreads, double* A) {
   set_threads(nthreads);
   hfp3d::Mesh_Geom_T mesh = make_mesh(nods, nods_rows, n_nodes, conn, conn_rows, n_el);
   hfp3d::Num_Param_T np;
   np.beta = beta;
   np.tip_type = tip_type;
-  np.is_dd_local = false;
+  np.is_dd_local = is_dd_local != 0;
   hfp3d::DoF_Handle_T d;
   il::Array2D<double> m = hfp3d::make_3dbem_matrix_vc(shear_m, poiss_r, mesh, np, il::io, d);
   const il::int_t n = m.size(0);

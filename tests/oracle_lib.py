@@ -60,16 +60,16 @@ class Oracle:
         self._sub = L.fn("local_submatrix", None, [_d, _d, _d, _d, _d, _dp, _dp, _dp])
         self._dof = L.fn("dof_h", _i64, [_dp, _i64, _i64, _dp, _i64, _i64, _int, _dp])
         self._asm = L.fn("assemble", _i64, [_dp, _i64, _i64, _dp, _i64, _i64, _dp, _i64, _d, _d, _d,
-                                            _i64, _i64, _int, _dp])
+                                            _i64, _i64, _int, _int, _dp])
         self._rhs = L.fn("rhs", None, [_dp, _i64, _i64, _dp, _i64, _i64, _dp, _dp, _dp])
         self._wdd = L.fn("write_dd", None, [_dp, _dp, _i64, _dp])
         self._stress = L.fn("stress", None, [_dp, _i64, _i64, _dp, _i64, _i64, _d, _d, _dp, _dp, _i64,
-                                             _dp, _int, _dp])
+                                             _dp, _int, _int, _dp])
         self._getrf = L.fn("getrf", _i64, [_i64, _dp, _i64, _dp, _int])
         self._getrs = L.fn("getrs", None, [_i64, _i64, _dp, _i64, _dp, _dp, _i64])
         self._sned = L.fn("sneddon_dd", None, [_d, _d, _d, _d, _dp, _dp])
         self._sfi = L.fn("sf_integral", None, [_dp, _dp, _dp])
-        self._vcb = L.fn("vc_border", None, [_dp, _i64, _i64, _dp, _i64, _i64, _dp, _dp, _dp])
+        self._vcb = L.fn("vc_border", None, [_dp, _i64, _i64, _dp, _i64, _i64, _dp, _int, _dp, _dp])
         self._mvc = L.fn("mod_system_vc", _int, [_dp, _i64, _i64, _dp, _dp, _i64, _i64, _i64, _dp, _i64, _d,
                                                   _dp, _dp])
 
@@ -92,7 +92,7 @@ class Oracle:
         n = self._dof(_p(nods), nods.shape[0], nods.shape[1], _p(conn), conn.shape[0], n_el, tip_type, _p(dof))
         return int(n), dof.reshape(n_el, 18, order="F")
 
-    def assemble(self, nods, conn, G, nu, beta=0.125, tip_type=0, targets=None, nthreads=8):
+    def assemble(self, nods, conn, G, nu, beta=0.125, tip_type=0, targets=None, nthreads=8, dd_local=False):
         nods, conn = _f(nods), _f(conn, np.int64)
         n_el = conn.shape[1]
         n_dof, dof = self.dof_h(nods, conn, tip_type)
@@ -100,7 +100,7 @@ class Oracle:
         A = np.zeros(n_dof * n_dof)
         t0, t1 = (0, 0) if targets is None else targets
         self._asm(_p(nods), nods.shape[0], nods.shape[1], _p(conn), conn.shape[0], n_el, _p(dof_f), n_dof,
-                  G, nu, beta, t0, t1, nthreads, _p(A))
+                  G, nu, beta, t0, t1, int(dd_local), nthreads, _p(A))
         return A.reshape(n_dof, n_dof, order="F"), dof
 
     def rhs(self, nods, conn, dof, s_inf):
@@ -121,14 +121,14 @@ class Oracle:
         self._wdd(_p(xx), _p(dof_f), n_el, _p(dd))
         return dd.reshape(6 * n_el, 3, order="F")
 
-    def stress(self, nods, conn, G, nu, dd, pts, s_inf, nthreads=8):
+    def stress(self, nods, conn, G, nu, dd, pts, s_inf, nthreads=8, dd_local=False):
         nods, conn = _f(nods), _f(conn, np.int64)
         dd, pts = _f(dd), _f(pts)
         n_pts = pts.shape[1]
         out = np.zeros(6 * n_pts)
         s = np.ascontiguousarray(s_inf, dtype=np.float64)
         self._stress(_p(nods), nods.shape[0], nods.shape[1], _p(conn), conn.shape[0], conn.shape[1], G, nu,
-                     _p(dd), _p(pts), n_pts, _p(s), nthreads, _p(out))
+                     _p(dd), _p(pts), n_pts, _p(s), int(dd_local), nthreads, _p(out))
         return out.reshape(n_pts, 6, order="F")
 
     def getrf(self, A, nthreads=8):
@@ -151,14 +151,14 @@ class Oracle:
         self._sfi(_p(t), _p(s), _p(out))
         return out
 
-    def vc_border(self, nods, conn, dof):
+    def vc_border(self, nods, conn, dof, dd_local=False):
         """(row, col) of the volume-control border (n_dof each)."""
         nods, conn = _f(nods), _f(conn, np.int64)
         dof_f = _f(dof, np.int64)
         n_dof = int(dof.max()) + 1
         row, col = np.zeros(n_dof), np.zeros(n_dof)
         self._vcb(_p(nods), nods.shape[0], nods.shape[1], _p(conn), conn.shape[0], conn.shape[1], _p(dof_f),
-                  _p(row), _p(col))
+                  int(dd_local), _p(row), _p(col))
         return row, col
 
     def mod_system_vc(self, A, orig_dof, dof, used_ndof, delta_t, delta_v):
@@ -190,12 +190,12 @@ class Reference:
         self._sub = L.fn("local_submatrix", _int, [_d, _d, _d, _d, _d, _dp, _dp, _dp])
         self._geom = L.fn("el_geom", _int, [_dp, _d, _dp, _dp, _dp, _dp])
         self._dof = L.fn("dof_h", _i64, [_dp, _i64, _i64, _dp, _i64, _i64, _int, _dp])
-        self._asm = L.fn("assemble", _i64, [_dp, _i64, _i64, _dp, _i64, _i64, _d, _d, _d, _int, _int, _dp])
+        self._asm = L.fn("assemble", _i64, [_dp, _i64, _i64, _dp, _i64, _i64, _d, _d, _d, _int, _int, _int, _dp])
         self._rhs = L.fn("rhs", _i64, [_dp, _i64, _i64, _dp, _i64, _i64, _int, _dp, _dp])
         self._stress = L.fn("stress", _int, [_dp, _i64, _i64, _dp, _i64, _i64, _d, _d, _dp, _dp, _i64, _dp,
-                                             _int, _dp])
+                                             _int, _int, _dp])
         self._sfi = L.fn("sf_integral", _int, [_dp, _dp, _dp])
-        self._mvcm = L.fn("matrix_vc", _i64, [_dp, _i64, _i64, _dp, _i64, _i64, _d, _d, _d, _int, _int, _dp])
+        self._mvcm = L.fn("matrix_vc", _i64, [_dp, _i64, _i64, _dp, _i64, _i64, _d, _d, _d, _int, _int, _int, _dp])
         self._mvc = L.fn("mod_system_vc", _int, [_dp, _i64, _dp, _dp, _i64, _i64, _dp, _i64, _d, _int, _dp, _dp,
                                                   _dp])
 
@@ -205,7 +205,7 @@ class Reference:
         self._sfi(_p(t), _p(s), _p(out))
         return out
 
-    def matrix_vc(self, nods, conn, G, nu, beta=0.125, tip_type=0, nthreads=8):
+    def matrix_vc(self, nods, conn, G, nu, beta=0.125, tip_type=0, nthreads=8, dd_local=False):
         """The reference's volume-control matrix ((n_dof+1)^2, corner 0)."""
         if tip_type == 2:
             raise ValueError("the reference's tip_type = 2 numbering writes past n_dof")
@@ -213,7 +213,7 @@ class Reference:
         n_dof, dof = self.dof_h(nods, conn, tip_type)
         A = np.zeros((n_dof + 1) ** 2)
         self._mvcm(_p(nods), nods.shape[0], nods.shape[1], _p(conn), conn.shape[0], conn.shape[1], G, nu, beta,
-                   tip_type, nthreads, _p(A))
+                   tip_type, int(dd_local), nthreads, _p(A))
         return A.reshape(n_dof + 1, n_dof + 1, order="F"), dof
 
     def mod_system_vc(self, A, orig_dof, dof, used_ndof, delta_t, delta_v, nthreads=1):
@@ -249,14 +249,14 @@ class Reference:
         n = self._dof(_p(nods), nods.shape[0], nods.shape[1], _p(conn), conn.shape[0], n_el, tip_type, _p(dof))
         return int(n), dof.reshape(n_el, 18, order="F")
 
-    def assemble(self, nods, conn, G, nu, beta=0.125, tip_type=0, nthreads=8):
+    def assemble(self, nods, conn, G, nu, beta=0.125, tip_type=0, nthreads=8, dd_local=False):
         if tip_type == 2:
             raise ValueError("the reference's tip_type = 2 numbering writes past n_dof")
         nods, conn = _f(nods), _f(conn, np.int64)
         n_dof, dof = self.dof_h(nods, conn, tip_type)
         A = np.zeros(n_dof * n_dof)
         self._asm(_p(nods), nods.shape[0], nods.shape[1], _p(conn), conn.shape[0], conn.shape[1], G, nu, beta,
-                  tip_type, nthreads, _p(A))
+                  tip_type, int(dd_local), nthreads, _p(A))
         return A.reshape(n_dof, n_dof, order="F"), dof
 
     def rhs(self, nods, conn, tip_type, s_inf):
@@ -268,7 +268,7 @@ class Reference:
                   _p(s), _p(r))
         return r
 
-    def stress(self, nods, conn, G, nu, dd, pts, s_inf, nthreads=1):
+    def stress(self, nods, conn, G, nu, dd, pts, s_inf, nthreads=1, dd_local=False):
         """The reference accumulates stresses from several OpenMP threads into
         shared memory without synchronisation (system_assembly.cpp:415-417):
         values are only reliable with nthreads = 1; more threads are for
@@ -279,7 +279,7 @@ class Reference:
         out = np.zeros(6 * n_pts)
         s = np.ascontiguousarray(s_inf, dtype=np.float64)
         self._stress(_p(nods), nods.shape[0], nods.shape[1], _p(conn), conn.shape[0], conn.shape[1], G, nu,
-                     _p(dd), _p(pts), n_pts, _p(s), nthreads, _p(out))
+                     _p(dd), _p(pts), n_pts, _p(s), int(dd_local), nthreads, _p(out))
         return out.reshape(n_pts, 6, order="F")
 
 

@@ -1,0 +1,102 @@
+"""DD in each element's own frame (Num_Param_T::is_dd_local = true) through
+the C ABI against the oracle (GPU): assembly, volume-control border,
+stresses, and the physical invariance of the solved stress field."""
+import numpy as np
+import pytest
+
+import oracle_lib
+from oracle_lib import golden
+
+pytestmark = pytest.mark.gpu
+
+O = oracle_lib.oracle()
+
+
+def load_mesh(stem):
+    nods = np.load(golden(f"Nodes_{stem}.npy"))
+    conn = np.load(golden(f"Elems_{stem}.npy")).astype(np.int64)
+    conn[:3] -= 1
+    return nods, conn
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+@pytest.fixture(scope="module")
+def hx():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a device"
+    from hfpx3d_amd import hfp3d
+    return hfp3d
+
+
+def meshes():
+    d = np.load(golden("ref_assembly.npz"))
+    nods, conn = load_mesh("pennymesh24el_32")
+    return {"penny24": (nods, conn, 1.0, 0.35), "wavy24": (d["wavy24_nods"], d["wavy24_conn"], 1 / 2.6, 0.3)}
+
+
+@pytest.mark.parametrize("case,tip", [("penny24", 0), ("penny24", 1), ("wavy24", 0)])
+def test_assembly_dd_local_matches_oracle(hx, case, tip):
+    import torch
+    print(f"case: dd_local assembly {case} tip {tip}", flush=True)
+    nods, conn, G, nu = meshes()[case]
+    mesh = hx.Mesh_Geom_T(nods=nods, conn=conn)
+    npar = hx.Num_Param_T(tip_type=tip, is_dd_local=True)
+    A = hx.make_3dbem_matrix_s(G, nu, mesh, npar, hx.DoF_Handle_T()).cpu().numpy()
+    A32 = hx.make_3dbem_matrix_s(G, nu, mesh, npar, hx.DoF_Handle_T(), dtype=torch.float32).cpu().numpy()
+    Ao, _ = O.assemble(nods, conn, G, nu, tip_type=tip, dd_local=True)
+    assert rel(A, Ao) < 1e-13
+    assert np.array_equal(A32, A.astype(np.float32))
+    Ag = hx.make_3dbem_matrix_s(G, nu, mesh, hx.Num_Param_T(tip_type=tip), hx.DoF_Handle_T()).cpu().numpy()
+    assert rel(A, Ag) > 1e-3
+
+
+def test_vc_border_dd_local_matches_oracle(hx):
+    print("case: dd_local vc border", flush=True)
+    nods, conn = load_mesh("pennymesh121el_32")
+    mesh = hx.Mesh_Geom_T(nods=nods, conn=conn)
+    dof = hx.DoF_Handle_T()
+    A = hx.make_3dbem_matrix_vc(1.0, 0.35, mesh, hx.Num_Param_T(tip_type=1, is_dd_local=True), dof).cpu().numpy()
+    n = dof.n_dof
+    row, col = O.vc_border(nods, conn, dof.dof_h, dd_local=True)
+    assert np.abs(A[n, :n] - row).max() <= 1e-13 * np.abs(row).max()
+    assert np.array_equal(A[:n, n], col)
+    Ao, _ = O.assemble(nods, conn, 1.0, 0.35, tip_type=1, dd_local=True)
+    assert rel(A[:n, :n], Ao) < 1e-13
+
+
+def test_stress_dd_local_matches_oracle(hx):
+    print("case: dd_local stress", flush=True)
+    d = np.load(golden("ref_rhs_stress.npz"))
+    nods, conn = load_mesh("pennymesh24el_32")
+    md = hx.Mesh_Data_T(mesh=hx.Mesh_Geom_T(nods=nods, conn=conn), dd=d["dd"])
+    S = hx.make_3dbem_stress_f_s(1.0, 0.35, md, hx.Load_T(s_inf=d["s_inf"]), hx.Num_Param_T(is_dd_local=True),
+                                 d["pts"])
+    So = O.stress(nods, conn, 1.0, 0.35, d["dd"], d["pts"], d["s_inf"], dd_local=True)
+    assert np.abs(S - So).max() <= 1e-12 * np.abs(So).max()
+
+
+def test_solved_stress_field_is_frame_independent(hx):
+    """Solving for local DD and evaluating stresses with local DD gives the
+    stress field of the global-DD solve (ellipse 440 elements, tip DoF
+    fixed, far-field load with shear)."""
+    print("case: dd_local end to end", flush=True)
+    nods, conn = load_mesh("elliptical_440_el_32")
+    mesh = hx.Mesh_Geom_T(nods=nods, conn=conn)
+    load = hx.Load_T(s_inf=np.array([0.1, -0.2, 1.0, 0.05, 0.3, -0.15]))
+    pts = np.random.default_rng(11).uniform(-1.5, 1.5, size=(3, 300))
+    out = {}
+    for local in (False, True):
+        npar = hx.Num_Param_T(tip_type=1, is_dd_local=local)
+        dof = hx.DoF_Handle_T()
+        A = hx.make_3dbem_matrix_s(1.0, 0.3, mesh, npar, dof)
+        md = hx.Mesh_Data_T(mesh=mesh, dof_h_dd=dof)
+        sae = hx.SAE_T()
+        hx.add_s_inf_to_3dbem_rhs(md, load, sae)
+        x = hx.LU(A).solve(sae.rhs_v)
+        md.dd = hx.write_dd_vector_to_md(x, dof, conn.shape[1])
+        out[local] = (md.dd, hx.make_3dbem_stress_f_s(1.0, 0.3, md, load, npar, pts))
+    assert np.abs(out[True][1] - out[False][1]).max() <= 1e-9 * np.abs(out[False][1]).max()
+    assert rel(out[True][0], out[False][0]) > 1e-3   # the DD themselves differ (frames)
