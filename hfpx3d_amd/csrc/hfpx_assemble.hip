@@ -4,6 +4,8 @@
 // Replaces, on the device:
 //   hfp3d::make_el_r_tensor / make_el_tau_crd / make_el_sfm_uniform /
 //   el_cp_uniform          src/Core/element_utilities.cpp:62-238, 364-395
+//   hfp3d::make_el_sfm_nonuniform / el_cp_nonuniform (per-node vertex
+//   weights)               src/Core/element_utilities.cpp:240-322, 397-431
 //   hfp3d::make_3dbem_matrix_s   src/Solvers/system_assembly.cpp:30-219
 //   hfp3d::add_s_inf_to_3dbem_rhs src/Solvers/system_assembly.cpp:222-274
 //
@@ -24,15 +26,18 @@ namespace hfpx {
 // ---------------------------------------------------------------------
 // element geometry: one thread per element
 // ---------------------------------------------------------------------
+// wts: per-node vertex weights of a non-uniform edge partition, or null for
+// the uniform element (edge nodes at mid-edge).
 __global__ void k_geometry(const double* __restrict__ nods, const int32_t* __restrict__ conn,
-                           int64_t n_el, double beta, SrcElem<double>* __restrict__ elem,
-                           double* __restrict__ cp) {
+                           const double* __restrict__ wts, int64_t n_el, double beta,
+                           SrcElem<double>* __restrict__ elem, double* __restrict__ cp) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n_el) return;
-  double v[3][3];
+  double v[3][3], w[3] = {1.0, 1.0, 1.0};
   for (int j = 0; j < 3; ++j) {
     const int64_t n = conn[3 * e + j];
     for (int k = 0; k < 3; ++k) v[j][k] = nods[3 * n + k];
+    if (wts) w[j] = wts[n];
   }
   // frame: e1 = (v1-v0)/|.|, e3 = normalize(e1 x (v2-v0)), e2 = e3 x e1
   double a1[3], a2[3], e1[3], e2[3], e3[3];
@@ -86,6 +91,14 @@ __global__ void k_geometry(const double* __restrict__ nods, const int32_t* __res
     E.l1[2 * k] = xi / (2.0 * area2);
     E.l1[2 * k + 1] = -xr / (2.0 * area2);
   }
+  // Lagrange basis of the six nodes: the edge node opposite v sits at
+  // L_a = w_a / (w_a + w_b) on the edge (a, b) = (v+1, v+2)
+  for (int q = 0; q < 3; ++q) {
+    const int a = (q + 1) % 3, b = (q + 2) % 3;
+    E.sw[2 * q] = w[q] / w[a];
+    E.sw[2 * q + 1] = w[q] / w[b];
+    E.sw[6 + q] = (w[a] + w[b]) * (w[a] + w[b]) / (w[a] * w[b]);
+  }
   elem[e] = E;
 
   // collocation points (vertex-ward and mid-edge, pulled towards the centroid)
@@ -96,7 +109,9 @@ __global__ void k_geometry(const double* __restrict__ nods, const int32_t* __res
     const int m = (q + 1) % 3, l = (m + 1) % 3;
     for (int k = 0; k < 3; ++k) {
       cp[k * ncp + 6 * e + q] = (1.0 - beta) * v[q][k] + beta * c[k];
-      cp[k * ncp + 6 * e + 3 + q] = 0.5 * (1.0 - beta) * (v[m][k] + v[l][k]) + beta * c[k];
+      cp[k * ncp + 6 * e + 3 + q] =
+          wts ? (1.0 - beta) * (w[m] * v[m][k] + w[l] * v[l][k]) / (w[m] + w[l]) + beta * c[k]
+              : 0.5 * (1.0 - beta) * (v[m][k] + v[l][k]) + beta * c[k];
     }
   }
   for (int q = 0; q < 6; ++q)
@@ -121,7 +136,7 @@ struct StoreBoth {
 // of a mesh of n_el elements against source element 4 by + wave.  cp holds
 // the six coordinate planes of the points with plane stride cps (>= 6 n_el;
 // larger when the mesh is one crack of a batch).
-template <typename Store>
+template <bool NU, typename Store>
 __device__ __forceinline__ void assemble_wave(const SrcElem<double>* __restrict__ elem,
                                               const double* __restrict__ cp, int64_t cps, int64_t n_el,
                                               const int32_t* __restrict__ dof, double shear_m, double nu,
@@ -150,7 +165,7 @@ __device__ __forceinline__ void assemble_wave(const SrcElem<double>* __restrict_
   } else {
     for (int i = 0; i < 3; ++i) row[i] = 3 * icp + i;
   }
-  pair_influence<double>(E, y0, y1, y2, shear_m, nu, [&](int j, const double (&S)[6][3]) {
+  pair_influence<double, NU>(E, y0, y1, y2, shear_m, nu, [&](int j, const double (&S)[6][3]) {
     // traction in the source frame for each local DD component k
     double tl[3][3];
     for (int k = 0; k < 3; ++k) {
@@ -179,12 +194,12 @@ __device__ __forceinline__ void assemble_wave(const SrcElem<double>* __restrict_
   });
 }
 
-template <typename Store>
+template <bool NU, typename Store>
 __global__ __launch_bounds__(256) void k_assemble(const SrcElem<double>* __restrict__ elem,
                                                   const double* __restrict__ cp, int64_t n_el,
                                                   const int32_t* __restrict__ dof, double shear_m,
                                                   double nu, bool dd_local, Store store) {
-  assemble_wave(elem, cp, 6 * n_el, n_el, dof, shear_m, nu, dd_local, store, blockIdx.x, blockIdx.y);
+  assemble_wave<NU>(elem, cp, 6 * n_el, n_el, dof, shear_m, nu, dd_local, store, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------
@@ -216,7 +231,7 @@ __global__ __launch_bounds__(256) void k_bassemble(const SrcElem<double>* __rest
   const int64_t gx = (6 * c.n_el + 63) / 64;
   const int64_t local = w - c.wg_off;
   StoreBoth st{a64 + c.a_off, c.n, c.a32_off >= 0 ? a32 + c.a32_off : nullptr, c.n};
-  assemble_wave(elem + c.el_off, cp + 6 * c.el_off, cps, c.n_el, nullptr, c.G, c.nu, false, st, local % gx,
+  assemble_wave<false>(elem + c.el_off, cp + 6 * c.el_off, cps, c.n_el, nullptr, c.G, c.nu, false, st, local % gx,
                 local / gx);
 }
 
@@ -264,7 +279,7 @@ __global__ void k_rhs(const SrcElem<double>* __restrict__ elem, int64_t n_el,
 int launch_geometry(hfpx_ctx* ctx, hfpx_mesh* m) {
   const int bs = 128;
   const unsigned gs = (unsigned)((m->n_el + bs - 1) / bs);
-  hipLaunchKernelGGL(k_geometry, dim3(gs), dim3(bs), 0, ctx->stream, m->nods, m->conn, m->n_el,
+  hipLaunchKernelGGL(k_geometry, dim3(gs), dim3(bs), 0, ctx->stream, m->nods, m->conn, m->wts, m->n_el,
                      m->beta, m->elem, m->cp);
   HFPX_TRY_HIP(ctx, hipGetLastError());
   return HFPX_OK;
@@ -278,8 +293,12 @@ int launch_assemble(hfpx_ctx* ctx, const hfpx_mesh* m, double shear_m, double po
   if (gy > 65535u) return set_error(ctx, HFPX_ERR_INVALID, "too many elements for one launch");
   StoreBoth st{a64, lda64, a32, lda32};
   const int tok = prof_begin(ctx, HFPX_KC_ASSEMBLE);
-  hipLaunchKernelGGL(k_assemble<StoreBoth>, dim3(gx, gy), dim3(256), 0, ctx->stream, m->elem,
-                     m->cp, m->n_el, m->identity_dof ? nullptr : m->dof, shear_m, poiss_r, m->dd_local, st);
+  if (m->wts)
+    hipLaunchKernelGGL((k_assemble<true, StoreBoth>), dim3(gx, gy), dim3(256), 0, ctx->stream, m->elem,
+                       m->cp, m->n_el, m->identity_dof ? nullptr : m->dof, shear_m, poiss_r, m->dd_local, st);
+  else
+    hipLaunchKernelGGL((k_assemble<false, StoreBoth>), dim3(gx, gy), dim3(256), 0, ctx->stream, m->elem,
+                       m->cp, m->n_el, m->identity_dof ? nullptr : m->dof, shear_m, poiss_r, m->dd_local, st);
   // algorithmic HBM bytes: every matrix entry written once (FP64 and/or FP32)
   const double entries = (double)m->n_dof * (double)m->n_dof;
   prof_end(ctx, tok, HFPX_KC_ASSEMBLE, 0.0, entries * ((a64 ? 8.0 : 0.0) + (a32 ? 4.0 : 0.0)));

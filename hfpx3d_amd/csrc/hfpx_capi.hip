@@ -253,6 +253,7 @@ int hfpx_mesh_destroy(hfpx_mesh* m) {
   if (m->dof) (void)hipFree(m->dof);
   if (m->elem) (void)hipFree(m->elem);
   if (m->cp) (void)hipFree(m->cp);
+  if (m->wts) (void)hipFree(m->wts);
   delete m;
   return HFPX_OK;
 }
@@ -262,6 +263,31 @@ int64_t hfpx_mesh_n_dof(const hfpx_mesh* m) { return m ? m->n_dof : -1; }
 int hfpx_mesh_set_dd_local(hfpx_mesh* m, int is_dd_local) {
   if (!m || (is_dd_local != 0 && is_dd_local != 1)) return HFPX_ERR_INVALID;
   m->dd_local = is_dd_local != 0;
+  return HFPX_OK;
+}
+
+int hfpx_mesh_set_vertex_weights(hfpx_ctx* ctx, hfpx_mesh* m, const double* wts_host, int64_t n_nodes) {
+  if (!ctx || !m) return HFPX_ERR_INVALID;
+  HFPX_TRY_HIP(ctx, hipSetDevice(ctx->device));
+  if (!wts_host) {
+    if (m->wts) {
+      HFPX_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      HFPX_TRY_HIP(ctx, hipFree(m->wts));
+      m->wts = nullptr;
+    }
+    return HFPX_OK;
+  }
+  if (n_nodes != m->n_nodes)
+    return set_error(ctx, HFPX_ERR_INVALID, "%lld weights for a mesh of %lld nodes", (long long)n_nodes,
+                     (long long)m->n_nodes);
+  for (int64_t j = 0; j < n_nodes; ++j)
+    if (!(wts_host[j] > 0.0) || !std::isfinite(wts_host[j]))
+      return set_error(ctx, HFPX_ERR_INVALID, "vertex weight %lld = %g (need finite and > 0)", (long long)j,
+                       wts_host[j]);
+  if (!m->wts) HFPX_TRY_HIP(ctx, hipMalloc(&m->wts, sizeof(double) * n_nodes));
+  // the copy is synchronous: the caller's buffer may go once this returns
+  HFPX_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  HFPX_TRY_HIP(ctx, hipMemcpy(m->wts, wts_host, sizeof(double) * n_nodes, hipMemcpyHostToDevice));
   return HFPX_OK;
 }
 

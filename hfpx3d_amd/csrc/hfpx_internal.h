@@ -59,6 +59,7 @@ struct hfpx_mesh {
   int32_t* dof = nullptr;      // n_el x 18 (row = element), -1 = fixed
   hfpx::SrcElem<double>* elem = nullptr;  // n_el
   double* cp = nullptr;        // 6 arrays of 6*n_el: cx cy cz nx ny nz
+  double* wts = nullptr;       // n_nodes vertex weights of a non-uniform edge partition, or null
 };
 
 namespace hfpx {

@@ -66,6 +66,13 @@ struct SrcElem {
   T len[3];    // edge lengths
   T l0[3];     // barycentric L_k(w) = l0_k + 2 Re(l1_k w)
   T l1[6];     // (re, im) x 3
+  // Shape functions of a non-uniform edge partition (vertex weights w):
+  //   N_v = L_v^2 - sw[2v] L_v L_{v+1} - sw[2v+1] L_v L_{v+2},
+  //   sw[2v] = w_v / w_{v+1}, sw[2v+1] = w_v / w_{v+2};
+  //   N_{3+v} = sw[6+v] L_{v+1} L_{v+2}, sw[6+v] = (w_a + w_b)^2 / (w_a w_b).
+  // Equal weights give 1 and 4, the uniform element; only the NU variant of
+  // pair_influence reads them.
+  T sw[9];
 };
 
 // Edge moments M[j][l] (only the entries used below), differences of the
@@ -133,7 +140,20 @@ __device__ __forceinline__ T solid_end(T a, T s, T ax3) {
 // x3 = y2).  Calls sink(node, S_local[6][3]) once per node with
 // S_local[c][k] = stress component c (S11 S22 S33 S12 S13 S23) due to unit
 // DD component k (source frame) at the node.
-template <typename T, typename Sink>
+// Adds s L_a L_b to the re-centred density P = p00 + 2 Re(p10 w) +
+// 2 Re(p20 w^2) + p11 |w|^2, with L_k(w) = lam_k + 2 Re(g_k w).
+template <typename T>
+__device__ __forceinline__ void add_bary_product(T s, T la, cx<T> ga, T lb, cx<T> gb, T& p00,
+                                                 cx<T>& p10, cx<T>& p20, T& p11) {
+  p00 += s * la * lb;
+  p10 = p10 + s * (la * gb + lb * ga);
+  p20 = p20 + s * (ga * gb);
+  p11 += T(2) * s * (ga.re * gb.re + ga.im * gb.im);
+}
+
+// NU: the element's shape functions are those of a non-uniform edge
+// partition (SrcElem::sw); the uniform element is the default variant.
+template <typename T, bool NU = false, typename Sink>
 __device__ __forceinline__ void pair_influence(const SrcElem<T>& E, T y0, T y1, T x3,
                                                T shear_m, T nu, Sink&& sink) {
   const T ax3 = fabs(x3);
@@ -173,7 +193,24 @@ __device__ __forceinline__ void pair_influence(const SrcElem<T>& E, T y0, T y1, 
     // density re-centred at z: P = p00 + 2 Re(p10 w) + 2 Re(p20 w^2) + p11 |w|^2
     T p00, p11;
     cx<T> p10, p20;
-    if (j < 3) {
+    if constexpr (NU) {
+      p00 = T(0);
+      p11 = T(0);
+      p10 = mk<T>(0, 0);
+      p20 = mk<T>(0, 0);
+      const int v = j < 3 ? j : j - 3;
+      const int a = (v + 1) % 3, b = (v + 2) % 3;
+      const cx<T> gv = mk<T>(E.l1[2 * v], E.l1[2 * v + 1]);
+      const cx<T> ga = mk<T>(E.l1[2 * a], E.l1[2 * a + 1]);
+      const cx<T> gb = mk<T>(E.l1[2 * b], E.l1[2 * b + 1]);
+      if (j < 3) {
+        add_bary_product<T>(T(1), lam[v], gv, lam[v], gv, p00, p10, p20, p11);
+        add_bary_product<T>(-E.sw[2 * v], lam[v], gv, lam[a], ga, p00, p10, p20, p11);
+        add_bary_product<T>(-E.sw[2 * v + 1], lam[v], gv, lam[b], gb, p00, p10, p20, p11);
+      } else {
+        add_bary_product<T>(E.sw[6 + v], lam[a], ga, lam[b], gb, p00, p10, p20, p11);
+      }
+    } else if (j < 3) {
       const T l = lam[j];
       const cx<T> g = mk<T>(E.l1[2 * j], E.l1[2 * j + 1]);
       p00 = T(2) * l * l - l;

@@ -29,6 +29,7 @@ __global__ void k_dd_local(const SrcElem<double>* __restrict__ elem, const doubl
     dloc[3 * t + k] = already_local ? g[k] : R[3 * k] * g[0] + R[3 * k + 1] * g[1] + R[3 * k + 2] * g[2];
 }
 
+template <bool NU>
 __global__ __launch_bounds__(256) void k_stress(const SrcElem<double>* __restrict__ elem,
                                                 const double* __restrict__ dloc, int64_t n_el,
                                                 const double* __restrict__ pts, int64_t n_pts,
@@ -50,7 +51,7 @@ __global__ __launch_bounds__(256) void k_stress(const SrcElem<double>* __restric
     const double y2 = E.R[6] * d[0] + E.R[7] * d[1] + E.R[8] * d[2];
     double sl[6] = {0, 0, 0, 0, 0, 0};
     const double* de = dloc + 18 * e;
-    pair_influence<double>(E, y0, y1, y2, shear_m, nu, [&](int j, const double (&S)[6][3]) {
+    pair_influence<double, NU>(E, y0, y1, y2, shear_m, nu, [&](int j, const double (&S)[6][3]) {
       const double a = de[3 * j], b = de[3 * j + 1], cc = de[3 * j + 2];
 #pragma unroll
       for (int q = 0; q < 6; ++q) sl[q] += S[q][0] * a + S[q][1] * b + S[q][2] * cc;
@@ -110,8 +111,12 @@ int launch_stress(hfpx_ctx* ctx, hfpx_mesh* m, double shear_m, double nu, const 
   hipLaunchKernelGGL(k_dd_local, dim3((unsigned)((6 * m->n_el + 255) / 256)), dim3(256), 0, ctx->stream,
                      m->elem, dd, ldd, m->n_el, m->dd_local, dloc);
   const int tok = prof_begin(ctx, HFPX_KC_STRESS);
-  hipLaunchKernelGGL(k_stress, dim3((unsigned)pblocks, (unsigned)nchunk), dim3(256), 0, ctx->stream, m->elem,
-                     dloc, m->n_el, pts, n_pts, chunk, shear_m, nu, part);
+  if (m->wts)
+    hipLaunchKernelGGL(k_stress<true>, dim3((unsigned)pblocks, (unsigned)nchunk), dim3(256), 0, ctx->stream,
+                       m->elem, dloc, m->n_el, pts, n_pts, chunk, shear_m, nu, part);
+  else
+    hipLaunchKernelGGL(k_stress<false>, dim3((unsigned)pblocks, (unsigned)nchunk), dim3(256), 0, ctx->stream,
+                       m->elem, dloc, m->n_el, pts, n_pts, chunk, shear_m, nu, part);
   prof_end(ctx, tok, HFPX_KC_STRESS, 0.0, 0.0);
   hipLaunchKernelGGL(k_stress_sum, dim3((unsigned)((6 * n_pts + 255) / 256)), dim3(256), 0, ctx->stream, part,
                      nchunk, n_pts, sinf, out, ldo);

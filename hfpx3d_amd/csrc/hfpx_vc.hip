@@ -26,8 +26,9 @@ namespace hfpx {
 //   A(n, d) = (int N_ns dA) e3_j,   A(d, n) = -e3_j,
 // e3 the element normal (row 2 of its rotation tensor).  The area integrals
 // of the 6-node triangle's shape functions are 0 for the vertex nodes and
-// area / 3 for the mid-edge nodes; the reference evaluates the same integrals
-// as contour integrals of the monomials.  Thread 0 also writes the corner
+// area / 3 for the mid-edge nodes of the uniform element (closed forms for
+// vertex weights below); the reference evaluates the same integrals as
+// contour integrals of the monomials.  Thread 0 also writes the corner
 // A(n, n) = 0 (the reference leaves it unset).  A DoF shared between
 // elements keeps one of their values, as in the reference's OpenMP loop.
 // With DD in element frames (dd_local) the volume row holds the area
@@ -44,7 +45,10 @@ __global__ void k_vc_border(const SrcElem<double>* __restrict__ elem, int64_t n_
   const SrcElem<double>& E = elem[e];
   const double area = 0.5 * ((E.t[2] - E.t[0]) * (E.t[5] - E.t[1]) - (E.t[3] - E.t[1]) * (E.t[4] - E.t[0]));
   for (int ns = 0; ns < 6; ++ns) {
-    const double sf = ns < 3 ? 0.0 : area / 3.0;
+    // int N dA of the quadratic basis: int L_a L_b dA = A (1 + delta_ab) / 12,
+    // so N_v gives A (2 - sw[2v] - sw[2v+1]) / 12 (0 for the uniform
+    // element) and N_{3+v} gives sw[6+v] A / 12 (A / 3 uniform)
+    const double sf = ns < 3 ? (2.0 - E.sw[2 * ns] - E.sw[2 * ns + 1]) * area / 12.0 : E.sw[3 + ns] * area / 12.0;
     for (int j = 0; j < 3; ++j) {
       const int l = 3 * ns + j;
       const int64_t d = dof ? (int64_t)dof[18 * e + l] : 18 * e + l;

@@ -37,6 +37,12 @@ from . import _lib
 class Mesh_Geom_T:
     nods: np.ndarray = None   # (3 or 6) x n_nodes, float64
     conn: np.ndarray = None   # (3 or 5) x n_el, int64, rows 0..2 0-based vertex ids
+    # Optional per-node vertex weights (n_nodes, > 0) of a non-uniform edge
+    # partition: the edge node between vertices a and b sits at
+    # (w_a x_a + w_b x_b) / (w_a + w_b) (make_el_sfm_nonuniform /
+    # el_cp_nonuniform, element_utilities.cpp:240-322, 397-431).  None: the
+    # uniform element (edge nodes mid-edge).
+    vert_wts: np.ndarray = None
 
 
 @dataclass
@@ -176,6 +182,18 @@ class DeviceMesh:
         self.handle = h
         self.n_dof = int(self.ctx.lib.hfpx_mesh_n_dof(h))
         self.set_dd_local(dd_local)
+        self.set_vertex_weights(getattr(mesh, "vert_wts", None))
+
+    def set_vertex_weights(self, wts):
+        """Per-node vertex weights of a non-uniform edge partition (n_nodes,
+        all > 0), or None for the uniform element."""
+        if wts is None:
+            self.ctx.check(self.ctx.lib.hfpx_mesh_set_vertex_weights(self.ctx.handle, self.handle, None, 0))
+        else:
+            w = np.ascontiguousarray(wts, dtype=np.float64).reshape(-1)
+            self.ctx.check(self.ctx.lib.hfpx_mesh_set_vertex_weights(self.ctx.handle, self.handle, _np_ptr(w),
+                                                                     w.size))
+        self.vert_wts = wts
 
     def set_dd_local(self, dd_local):
         """DD in each element's own frame (Num_Param_T::is_dd_local)."""

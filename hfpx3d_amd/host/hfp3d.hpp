@@ -62,6 +62,10 @@ struct Status {
 struct Mesh_Geom_T {
   Array2D<double> nods;   // (3 or 6) x n_nodes
   Array2D<int64_t> conn;  // (3 or 5) x n_el, 0-based vertex ids in rows 0..2
+  // Optional per-node vertex weights (n_nodes, > 0) of a non-uniform edge
+  // partition (make_el_sfm_nonuniform / el_cp_nonuniform,
+  // element_utilities.cpp:240-322, 397-431); empty = uniform element.
+  std::vector<double> vert_wts;
 };
 
 struct DoF_Handle_T {

@@ -94,6 +94,9 @@ struct MeshHandle {
                            mesh.conn.size(0), mesh.conn.size(1), dh, dh ? dof->n_dof : 0, beta, &m),
           "mesh upload");
     check(hfpx_mesh_set_dd_local(m, dd_local ? 1 : 0), "mesh upload");
+    if (!mesh.vert_wts.empty())
+      check(hfpx_mesh_set_vertex_weights(context(), m, mesh.vert_wts.data(), (int64_t)mesh.vert_wts.size()),
+            "vertex weights");
   }
   ~MeshHandle() { hfpx_mesh_destroy(m); }
 };

@@ -74,6 +74,19 @@ int hfpx_mesh_destroy(hfpx_mesh* mesh);
  * (src/Core/model_parameters.h:89; branches at system_assembly.cpp:186-195,
  * 358-387, 529-536). */
 int hfpx_mesh_set_dd_local(hfpx_mesh* mesh, int is_dd_local);
+/* Non-uniform edge partition: per-node vertex weights wts_host (n_nodes,
+ * host, every weight finite and > 0).  Element e takes the weights w of its
+ * three vertex nodes; its edge node opposite vertex v then sits at
+ * (w_a x_a + w_b x_b) / (w_a + w_b), (a, b) = (v+1, v+2), with the quadratic
+ * Lagrange shape functions of those six nodes and collocation points pulled
+ * from them towards the centroid by beta.  Assembly, right-hand sides, the
+ * volume-control border and hfpx_stress follow it.  wts_host = NULL restores
+ * the uniform element.  Replaces the vertex-weight hooks of
+ * hfp3d::make_el_sfm_nonuniform / el_cp_nonuniform
+ * (src/Core/element_utilities.cpp:240-322, 397-431, declared
+ * element_utilities.h:68-71, 87-90; call sites left for them at
+ * system_assembly.cpp:67-81, 92-115). */
+int hfpx_mesh_set_vertex_weights(hfpx_ctx* ctx, hfpx_mesh* mesh, const double* wts_host, int64_t n_nodes);
 int64_t hfpx_mesh_n_dof(const hfpx_mesh* mesh);
 
 /* ---- assembly ---------------------------------------------------------- */

@@ -6,7 +6,8 @@
  * leg may load it.  It is never part of the product path.
  *
  * What it restates (reference paths relative to the reference root):
- *   element geometry        src/Core/element_utilities.cpp:62-238, 339-395
+ *   element geometry        src/Core/element_utilities.cpp:62-431 (uniform and
+ *                           non-uniform edge partitions)
  *   per-pair kernel         src/Elasticity/elasticity_kernel_integration.cpp:34-205
  *                           (values specified by src/Elasticity/h_potential.cpp;
  *                            the integrals are re-derived here, see below)
@@ -128,8 +129,40 @@ static void el_sfm(const cplx* t, cplx* sfm /* 6x6 col-major */) {
   }
 }
 
-void hfpo_el_geom(const double* vert, double beta, double* rt, double* tau,
-                  double* sfm, double* cp) {
+/* Shape functions for a non-uniform edge partition
+ * (make_el_sfm_nonuniform, element_utilities.cpp:240-322): the edge node
+ * opposite vertex v sits at L_a = w_a / (w_a + w_b), L_b = w_b / (w_a + w_b)
+ * on the edge (a, b) = (v+1, v+2).  The quadratic Lagrange basis of the six
+ * nodes, derived here from N_j(node_k) = delta_jk:
+ *   N_{3+v} = (w_a + w_b)^2 / (w_a w_b) L_a L_b,
+ *   N_v     = L_v - sum over the edges (v, u) of L_v(edge node) N_edge
+ *           = L_v^2 - (w_v / w_a) L_v L_a - (w_v / w_b) L_v L_b
+ * (using L_v = L_v (L_0 + L_1 + L_2)).  Equal weights give the uniform basis. */
+static void el_sfm_w(const cplx* t, const double* w, cplx* sfm /* 6x6 col-major */) {
+  double area2 = cimag(conj(t[1] - t[0]) * (t[2] - t[0]));
+  cplx L[3][3];
+  for (int k = 0; k < 3; ++k) {
+    cplx ta = t[(k + 1) % 3], tb = t[(k + 2) % 3];
+    L[k][0] = cimag(conj(ta) * tb) / area2;
+    L[k][1] = (conj(tb) - conj(ta)) / (2.0 * I) / area2;
+    L[k][2] = (ta - tb) / (2.0 * I) / area2;
+  }
+  for (int j = 0; j < 36; ++j) sfm[j] = 0;
+  for (int v = 0; v < 3; ++v) {
+    const int a = (v + 1) % 3, b = (v + 2) % 3;
+    cplx q[6] = {0, 0, 0, 0, 0, 0};
+    lin_mul(L[v], L[v], 1.0, q);
+    lin_mul(L[v], L[a], -w[v] / w[a], q);
+    lin_mul(L[v], L[b], -w[v] / w[b], q);
+    for (int c = 0; c < 6; ++c) sfm[v + 6 * c] = q[c];
+    cplx e[6] = {0, 0, 0, 0, 0, 0};
+    lin_mul(L[a], L[b], (w[a] + w[b]) * (w[a] + w[b]) / (w[a] * w[b]), e);
+    for (int c = 0; c < 6; ++c) sfm[3 + v + 6 * c] = e[c];
+  }
+}
+
+void hfpo_el_geom_w(const double* vert, const double* wts, double beta, double* rt,
+                    double* tau, double* sfm, double* cp) {
   el_rt(vert, rt);
   cplx t[3];
   for (int j = 0; j < 3; ++j) {
@@ -140,21 +173,32 @@ void hfpo_el_geom(const double* vert, double beta, double* rt, double* tau,
     tau[2 * j + 1] = y[1];
   }
   cplx s[36];
-  el_sfm(t, s);
+  if (wts) el_sfm_w(t, wts, s);
+  else el_sfm(t, s);
   for (int j = 0; j < 36; ++j) {
     sfm[2 * j] = creal(s[j]);
     sfm[2 * j + 1] = cimag(s[j]);
   }
-  /* collocation points (element_utilities.cpp:364-395) */
+  /* collocation points (el_cp_uniform / el_cp_nonuniform,
+   * element_utilities.cpp:364-431): the edge point at the weighted node */
   double c[3];
   for (int k = 0; k < 3; ++k) c[k] = (vert[k] + vert[k + 3] + vert[k + 6]) / 3.0;
   for (int v = 0; v < 3; ++v) {
     int m = (v + 1) % 3, l = (m + 1) % 3;
     for (int k = 0; k < 3; ++k) {
       cp[k + 3 * v] = (1.0 - beta) * vert[k + 3 * v] + beta * c[k];
-      cp[k + 3 * (v + 3)] = 0.5 * (1.0 - beta) * (vert[k + 3 * m] + vert[k + 3 * l]) + beta * c[k];
+      if (wts)
+        cp[k + 3 * (v + 3)] = (1.0 - beta) * (wts[m] * vert[k + 3 * m] + wts[l] * vert[k + 3 * l]) /
+                                  (wts[m] + wts[l]) + beta * c[k];
+      else
+        cp[k + 3 * (v + 3)] = 0.5 * (1.0 - beta) * (vert[k + 3 * m] + vert[k + 3 * l]) + beta * c[k];
     }
   }
+}
+
+void hfpo_el_geom(const double* vert, double beta, double* rt, double* tau,
+                  double* sfm, double* cp) {
+  hfpo_el_geom_w(vert, NULL, beta, rt, tau, sfm, cp);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -396,15 +440,23 @@ static void el_vert(const double* nods, int64_t nods_rows, const int64_t* conn,
   }
 }
 
+/* vertex weights of element e from per-node weights (NULL: uniform) */
+static const double* el_wts(const double* node_wts, const int64_t* conn, int64_t conn_rows,
+                            int64_t e, double* w) {
+  if (!node_wts) return NULL;
+  for (int j = 0; j < 3; ++j) w[j] = node_wts[conn[j + conn_rows * e]];
+  return w;
+}
+
 /* 18x18 traction influence of source element es on the 6 collocation
  * points of target element et, global frame (system_assembly.cpp:88-199). */
-static void el2el_block(const double* vs, const double* vt, double shear_m,
-                        double poiss_r, double beta, int dd_local,
+static void el2el_block(const double* vs, const double* ws, const double* vt, const double* wt,
+                        double shear_m, double poiss_r, double beta, int dd_local,
                         double* blk /* 18x18 col-major */) {
   double rs[9], taus[6], sfms[72], cps[18];
-  hfpo_el_geom(vs, beta, rs, taus, sfms, cps);
+  hfpo_el_geom_w(vs, ws, beta, rs, taus, sfms, cps);
   double rt[9], taut[6], sfmt[72], cpt[18];
-  hfpo_el_geom(vt, beta, rt, taut, sfmt, cpt);
+  hfpo_el_geom_w(vt, wt, beta, rt, taut, sfmt, cpt);
   double nglob[3] = {-rt[2], -rt[5], -rt[8]};
   double nloc[3];
   for (int i = 0; i < 3; ++i) nloc[i] = rs[i] * nglob[0] + rs[i + 3] * nglob[1] + rs[i + 6] * nglob[2];
@@ -442,6 +494,15 @@ int64_t hfpo_assemble(const double* nods, int64_t nods_rows, int64_t n_nodes,
                       const int64_t* dof_h, int64_t n_dof, double shear_m,
                       double poiss_r, double beta, int64_t t0, int64_t t1,
                       int is_dd_local, int nthreads, double* A) {
+  return hfpo_assemble_w(nods, nods_rows, n_nodes, conn, conn_rows, n_el, dof_h, n_dof, shear_m,
+                         poiss_r, beta, NULL, t0, t1, is_dd_local, nthreads, A);
+}
+
+int64_t hfpo_assemble_w(const double* nods, int64_t nods_rows, int64_t n_nodes,
+                        const int64_t* conn, int64_t conn_rows, int64_t n_el,
+                        const int64_t* dof_h, int64_t n_dof, double shear_m,
+                        double poiss_r, double beta, const double* node_wts, int64_t t0,
+                        int64_t t1, int is_dd_local, int nthreads, double* A) {
   (void)n_nodes;
   if (t1 <= t0) { t0 = 0; t1 = n_el; }
   memset(A, 0, sizeof(double) * (size_t)n_dof * (size_t)n_dof);
@@ -450,11 +511,13 @@ int64_t hfpo_assemble(const double* nods, int64_t nods_rows, int64_t n_nodes,
 #pragma omp parallel for schedule(dynamic)
 #endif
   for (int64_t es = 0; es < n_el; ++es) {
-    double vs[9], vt[9], blk[324];
+    double vs[9], vt[9], blk[324], wsb[3], wtb[3];
     el_vert(nods, nods_rows, conn, conn_rows, es, vs);
+    const double* ws = el_wts(node_wts, conn, conn_rows, es, wsb);
     for (int64_t et = t0; et < t1; ++et) {
       el_vert(nods, nods_rows, conn, conn_rows, et, vt);
-      el2el_block(vs, vt, shear_m, poiss_r, beta, is_dd_local, blk);
+      const double* wt = el_wts(node_wts, conn, conn_rows, et, wtb);
+      el2el_block(vs, ws, vt, wt, shear_m, poiss_r, beta, is_dd_local, blk);
       for (int c = 0; c < 18; ++c) {
         int64_t gc = dof_h[es + n_el * c];
         if (gc < 0) continue;
@@ -513,14 +576,23 @@ void hfpo_stress(const double* nods, int64_t nods_rows, int64_t n_nodes,
                  double shear_m, double poiss_r, const double* dd,
                  const double* pts, int64_t n_pts, const double* s_inf,
                  int is_dd_local, int nthreads, double* out) {
+  hfpo_stress_w(nods, nods_rows, n_nodes, conn, conn_rows, n_el, shear_m, poiss_r, dd, pts, n_pts,
+                s_inf, NULL, is_dd_local, nthreads, out);
+}
+
+void hfpo_stress_w(const double* nods, int64_t nods_rows, int64_t n_nodes,
+                   const int64_t* conn, int64_t conn_rows, int64_t n_el,
+                   double shear_m, double poiss_r, const double* dd,
+                   const double* pts, int64_t n_pts, const double* s_inf,
+                   const double* node_wts, int is_dd_local, int nthreads, double* out) {
   (void)n_nodes;
   const int64_t nr = 6 * n_el;
   double* geo = (double*)malloc(sizeof(double) * 96 * (size_t)n_el);
   for (int64_t e = 0; e < n_el; ++e) {
-    double v[9], cp[18];
+    double v[9], cp[18], wb[3];
     el_vert(nods, nods_rows, conn, conn_rows, e, v);
     double* g = geo + 96 * e; /* rt 9 | tau 6 | sfm 72 | v0 3 | pad */
-    hfpo_el_geom(v, 0.0, g, g + 9, g + 15, cp);
+    hfpo_el_geom_w(v, el_wts(node_wts, conn, conn_rows, e, wb), 0.0, g, g + 9, g + 15, cp);
     g[87] = v[0]; g[88] = v[1]; g[89] = v[2];
   }
 #ifdef _OPENMP
@@ -711,11 +783,19 @@ void hfpo_sf_integral(const double* tau, const double* sfm, double* out) {
 void hfpo_vc_border(const double* nods, int64_t nods_rows, int64_t n_nodes,
                     const int64_t* conn, int64_t conn_rows, int64_t n_el,
                     const int64_t* dof_h, int is_dd_local, double* row, double* col) {
+  hfpo_vc_border_w(nods, nods_rows, n_nodes, conn, conn_rows, n_el, dof_h, NULL, is_dd_local, row,
+                   col);
+}
+
+void hfpo_vc_border_w(const double* nods, int64_t nods_rows, int64_t n_nodes,
+                      const int64_t* conn, int64_t conn_rows, int64_t n_el,
+                      const int64_t* dof_h, const double* node_wts, int is_dd_local,
+                      double* row, double* col) {
   (void)n_nodes;
   for (int64_t e = 0; e < n_el; ++e) {
-    double v[9], rt[9], tau[6], sfm[72], cp[18], sf[6];
+    double v[9], rt[9], tau[6], sfm[72], cp[18], sf[6], wb[3];
     el_vert(nods, nods_rows, conn, conn_rows, e, v);
-    hfpo_el_geom(v, 0.0, rt, tau, sfm, cp);
+    hfpo_el_geom_w(v, el_wts(node_wts, conn, conn_rows, e, wb), 0.0, rt, tau, sfm, cp);
     hfpo_sf_integral(tau, sfm, sf);
     for (int ns = 0; ns < 6; ++ns)
       for (int j = 0; j < 3; ++j) {

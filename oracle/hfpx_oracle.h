@@ -24,6 +24,13 @@ extern "C" {
  * [1, t, tb, t^2, tb^2, t*tb]), cp (3x6 collocation points for beta). */
 void hfpo_el_geom(const double* vert, double beta, double* rt, double* tau,
                   double* sfm, double* cp);
+/* Same with a non-uniform edge partition: wts (3, all > 0) are the vertex
+ * weights; the edge node opposite vertex v sits at
+ * (w_a v_a + w_b v_b) / (w_a + w_b), (a, b) = (v+1, v+2)
+ * (make_el_sfm_nonuniform / el_cp_nonuniform, element_utilities.cpp:240-322,
+ * 397-431).  wts = NULL is the uniform element. */
+void hfpo_el_geom_w(const double* vert, const double* wts, double beta, double* rt,
+                    double* tau, double* sfm, double* cp);
 
 /* 6x18 stress influence (rows S11 S22 S33 S12 S13 S23, cols 3*node + DD
  * component, all in the source element's local frame) of one element at the
@@ -47,6 +54,13 @@ int64_t hfpo_assemble(const double* nods, int64_t nods_rows, int64_t n_nodes,
                       const int64_t* dof_h, int64_t n_dof, double shear_m,
                       double poiss_r, double beta, int64_t t0, int64_t t1,
                       int is_dd_local, int nthreads, double* A);
+/* Same with per-node vertex weights node_wts (n_nodes, NULL = uniform):
+ * element e takes the weights of its three vertex nodes. */
+int64_t hfpo_assemble_w(const double* nods, int64_t nods_rows, int64_t n_nodes,
+                        const int64_t* conn, int64_t conn_rows, int64_t n_el,
+                        const int64_t* dof_h, int64_t n_dof, double shear_m,
+                        double poiss_r, double beta, const double* node_wts, int64_t t0,
+                        int64_t t1, int is_dd_local, int nthreads, double* A);
 
 /* rhs[dof] -= (n . s_inf)_l for every free DoF (caller zero-fills rhs). */
 void hfpo_rhs(const double* nods, int64_t nods_rows, int64_t n_nodes,
@@ -65,6 +79,11 @@ void hfpo_stress(const double* nods, int64_t nods_rows, int64_t n_nodes,
                  double shear_m, double poiss_r, const double* dd,
                  const double* pts, int64_t n_pts, const double* s_inf,
                  int is_dd_local, int nthreads, double* out);
+void hfpo_stress_w(const double* nods, int64_t nods_rows, int64_t n_nodes,
+                   const int64_t* conn, int64_t conn_rows, int64_t n_el,
+                   double shear_m, double poiss_r, const double* dd,
+                   const double* pts, int64_t n_pts, const double* s_inf,
+                   const double* node_wts, int is_dd_local, int nthreads, double* out);
 
 /* LU with partial pivoting (LAPACK dgetrf semantics): A n x n col-major
  * (lda), ipiv 1-based.  Returns info (0 ok, k > 0: U(k,k) exactly zero). */
@@ -88,6 +107,10 @@ void hfpo_sf_integral(const double* tau, const double* sfm, double* out);
 void hfpo_vc_border(const double* nods, int64_t nods_rows, int64_t n_nodes,
                     const int64_t* conn, int64_t conn_rows, int64_t n_el,
                     const int64_t* dof_h, int is_dd_local, double* row, double* col);
+void hfpo_vc_border_w(const double* nods, int64_t nods_rows, int64_t n_nodes,
+                      const int64_t* conn, int64_t conn_rows, int64_t n_el,
+                      const int64_t* dof_h, const double* node_wts, int is_dd_local,
+                      double* row, double* col);
 /* mod_3dbem_system_vc: M ((used+1)^2 col-major) and rhs (used+1) of the
  * truncated system; dof handles n_el x ndpe col-major.  0 or -1 (invalid). */
 int hfpo_mod_system_vc(const double* A, int64_t lda, int64_t orig_ndof,

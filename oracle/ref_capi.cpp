@@ -11,6 +11,8 @@
 //   hfp3d::make_el_sfm_uniform          src/Core/element_utilities.cpp:170-238
 //   hfp3d::make_el_tau_crd              src/Core/element_utilities.cpp:89-118
 //   hfp3d::el_cp_uniform                src/Core/element_utilities.cpp:364-395
+//   hfp3d::make_el_sfm_nonuniform       src/Core/element_utilities.cpp:240-322
+//   hfp3d::el_cp_nonuniform             src/Core/element_utilities.cpp:397-431
 //   hfp3d::make_dof_h_crack             src/Core/surface_mesh_utilities.cpp:28-112
 //   hfp3d::make_3dbem_matrix_s          src/Solvers/system_assembly.cpp:30-219
 //   hfp3d::add_s_inf_to_3dbem_rhs       src/Solvers/system_assembly.cpp:222-274
@@ -117,6 +119,29 @@ int ref_el_geom(const double* vert, double beta, double* sfm, double* rt,
     tau[2 * k] = t[k].real();
     tau[2 * k + 1] = t[k].imag();
   }
+  for (int n = 0; n < 6; ++n)
+    for (int k = 0; k < 3; ++k) cp[k + 3 * n] = c[n][k];
+  return 0;
+}
+
+// Same for a non-uniform edge partition with vertex weights wts (3).
+int ref_el_geom_nonuniform(const double* vert, const double* wts, double beta, double* sfm,
+                           double* rt, double* cp) {
+  il::StaticArray2D<double, 3, 3> v;
+  for (int j = 0; j < 3; ++j)
+    for (int k = 0; k < 3; ++k) v(k, j) = vert[k + 3 * j];
+  il::StaticArray<double, 3> w;
+  for (int j = 0; j < 3; ++j) w[j] = wts[j];
+  il::StaticArray2D<double, 3, 3> r;
+  il::StaticArray2D<std::complex<double>, 6, 6> s = hfp3d::make_el_sfm_nonuniform(v, w, il::io, r);
+  il::StaticArray<il::StaticArray<double, 3>, 6> c = hfp3d::el_cp_nonuniform(v, w, beta);
+  for (int cc = 0; cc < 6; ++cc)
+    for (int rr = 0; rr < 6; ++rr) {
+      sfm[2 * (rr + 6 * cc)] = s(rr, cc).real();
+      sfm[2 * (rr + 6 * cc) + 1] = s(rr, cc).imag();
+    }
+  for (int j = 0; j < 3; ++j)
+    for (int k = 0; k < 3; ++k) rt[k + 3 * j] = r(k, j);
   for (int n = 0; n < 6; ++n)
     for (int k = 0; k < 3; ++k) cp[k + 3 * n] = c[n][k];
   return 0;

@@ -25,6 +25,15 @@ def _f(a, dt=np.float64):
     return np.asfortranarray(a, dtype=dt)
 
 
+def _pw(w):
+    """Per-node (or per-vertex) weights as a pointer, NULL for uniform."""
+    return None if w is None else _p(w)
+
+
+def _wts(w):
+    return None if w is None else np.ascontiguousarray(w, dtype=np.float64)
+
+
 class _Lib:
     def __init__(self, path, prefix):
         self.lib = ctypes.CDLL(path)
@@ -57,26 +66,32 @@ class Oracle:
     def __init__(self, path=ORACLE):
         L = _Lib(path, "hfpo_")
         self._geom = L.fn("el_geom", None, [_dp, _d, _dp, _dp, _dp, _dp])
+        self._geomw = L.fn("el_geom_w", None, [_dp, _dp, _d, _dp, _dp, _dp, _dp])
         self._sub = L.fn("local_submatrix", None, [_d, _d, _d, _d, _d, _dp, _dp, _dp])
         self._dof = L.fn("dof_h", _i64, [_dp, _i64, _i64, _dp, _i64, _i64, _int, _dp])
-        self._asm = L.fn("assemble", _i64, [_dp, _i64, _i64, _dp, _i64, _i64, _dp, _i64, _d, _d, _d,
-                                            _i64, _i64, _int, _int, _dp])
+        self._asm = L.fn("assemble_w", _i64, [_dp, _i64, _i64, _dp, _i64, _i64, _dp, _i64, _d, _d, _d,
+                                              _dp, _i64, _i64, _int, _int, _dp])
         self._rhs = L.fn("rhs", None, [_dp, _i64, _i64, _dp, _i64, _i64, _dp, _dp, _dp])
         self._wdd = L.fn("write_dd", None, [_dp, _dp, _i64, _dp])
-        self._stress = L.fn("stress", None, [_dp, _i64, _i64, _dp, _i64, _i64, _d, _d, _dp, _dp, _i64,
-                                             _dp, _int, _int, _dp])
+        self._stress = L.fn("stress_w", None, [_dp, _i64, _i64, _dp, _i64, _i64, _d, _d, _dp, _dp, _i64,
+                                               _dp, _dp, _int, _int, _dp])
         self._getrf = L.fn("getrf", _i64, [_i64, _dp, _i64, _dp, _int])
         self._getrs = L.fn("getrs", None, [_i64, _i64, _dp, _i64, _dp, _dp, _i64])
         self._sned = L.fn("sneddon_dd", None, [_d, _d, _d, _d, _dp, _dp])
         self._sfi = L.fn("sf_integral", None, [_dp, _dp, _dp])
-        self._vcb = L.fn("vc_border", None, [_dp, _i64, _i64, _dp, _i64, _i64, _dp, _int, _dp, _dp])
+        self._vcb = L.fn("vc_border_w", None, [_dp, _i64, _i64, _dp, _i64, _i64, _dp, _dp, _int, _dp, _dp])
         self._mvc = L.fn("mod_system_vc", _int, [_dp, _i64, _i64, _dp, _dp, _i64, _i64, _i64, _dp, _i64, _d,
                                                   _dp, _dp])
 
-    def el_geom(self, vert, beta=0.125):
+    def el_geom(self, vert, beta=0.125, wts=None):
+        """wts: the three vertex weights of a non-uniform edge partition."""
         v = _f(vert)
         rt, tau, sfm, cp = np.zeros(9), np.zeros(6), np.zeros(72), np.zeros(18)
-        self._geom(_p(v), beta, _p(rt), _p(tau), _p(sfm), _p(cp))
+        if wts is None:
+            self._geom(_p(v), beta, _p(rt), _p(tau), _p(sfm), _p(cp))
+        else:
+            w = _wts(wts)
+            self._geomw(_p(v), _p(w), beta, _p(rt), _p(tau), _p(sfm), _p(cp))
         return _geom_unpack(sfm, rt, tau, cp)
 
     def local_submatrix(self, G, nu, h, z, tau, sfm):
@@ -92,15 +107,17 @@ class Oracle:
         n = self._dof(_p(nods), nods.shape[0], nods.shape[1], _p(conn), conn.shape[0], n_el, tip_type, _p(dof))
         return int(n), dof.reshape(n_el, 18, order="F")
 
-    def assemble(self, nods, conn, G, nu, beta=0.125, tip_type=0, targets=None, nthreads=8, dd_local=False):
+    def assemble(self, nods, conn, G, nu, beta=0.125, tip_type=0, targets=None, nthreads=8, dd_local=False,
+                 node_wts=None):
         nods, conn = _f(nods), _f(conn, np.int64)
+        w = _wts(node_wts)
         n_el = conn.shape[1]
         n_dof, dof = self.dof_h(nods, conn, tip_type)
         dof_f = _f(dof, np.int64)
         A = np.zeros(n_dof * n_dof)
         t0, t1 = (0, 0) if targets is None else targets
         self._asm(_p(nods), nods.shape[0], nods.shape[1], _p(conn), conn.shape[0], n_el, _p(dof_f), n_dof,
-                  G, nu, beta, t0, t1, int(dd_local), nthreads, _p(A))
+                  G, nu, beta, _pw(w), t0, t1, int(dd_local), nthreads, _p(A))
         return A.reshape(n_dof, n_dof, order="F"), dof
 
     def rhs(self, nods, conn, dof, s_inf):
@@ -121,14 +138,15 @@ class Oracle:
         self._wdd(_p(xx), _p(dof_f), n_el, _p(dd))
         return dd.reshape(6 * n_el, 3, order="F")
 
-    def stress(self, nods, conn, G, nu, dd, pts, s_inf, nthreads=8, dd_local=False):
+    def stress(self, nods, conn, G, nu, dd, pts, s_inf, nthreads=8, dd_local=False, node_wts=None):
         nods, conn = _f(nods), _f(conn, np.int64)
         dd, pts = _f(dd), _f(pts)
+        w = _wts(node_wts)
         n_pts = pts.shape[1]
         out = np.zeros(6 * n_pts)
         s = np.ascontiguousarray(s_inf, dtype=np.float64)
         self._stress(_p(nods), nods.shape[0], nods.shape[1], _p(conn), conn.shape[0], conn.shape[1], G, nu,
-                     _p(dd), _p(pts), n_pts, _p(s), int(dd_local), nthreads, _p(out))
+                     _p(dd), _p(pts), n_pts, _p(s), _pw(w), int(dd_local), nthreads, _p(out))
         return out.reshape(n_pts, 6, order="F")
 
     def getrf(self, A, nthreads=8):
@@ -151,14 +169,15 @@ class Oracle:
         self._sfi(_p(t), _p(s), _p(out))
         return out
 
-    def vc_border(self, nods, conn, dof, dd_local=False):
+    def vc_border(self, nods, conn, dof, dd_local=False, node_wts=None):
         """(row, col) of the volume-control border (n_dof each)."""
         nods, conn = _f(nods), _f(conn, np.int64)
+        w = _wts(node_wts)
         dof_f = _f(dof, np.int64)
         n_dof = int(dof.max()) + 1
         row, col = np.zeros(n_dof), np.zeros(n_dof)
         self._vcb(_p(nods), nods.shape[0], nods.shape[1], _p(conn), conn.shape[0], conn.shape[1], _p(dof_f),
-                  int(dd_local), _p(row), _p(col))
+                  _pw(w), int(dd_local), _p(row), _p(col))
         return row, col
 
     def mod_system_vc(self, A, orig_dof, dof, used_ndof, delta_t, delta_v):
@@ -189,6 +208,7 @@ class Reference:
         L = _Lib(path, "ref_")
         self._sub = L.fn("local_submatrix", _int, [_d, _d, _d, _d, _d, _dp, _dp, _dp])
         self._geom = L.fn("el_geom", _int, [_dp, _d, _dp, _dp, _dp, _dp])
+        self._geomnu = L.fn("el_geom_nonuniform", _int, [_dp, _dp, _d, _dp, _dp, _dp])
         self._dof = L.fn("dof_h", _i64, [_dp, _i64, _i64, _dp, _i64, _i64, _int, _dp])
         self._asm = L.fn("assemble", _i64, [_dp, _i64, _i64, _dp, _i64, _i64, _d, _d, _d, _int, _int, _int, _dp])
         self._rhs = L.fn("rhs", _i64, [_dp, _i64, _i64, _dp, _i64, _i64, _int, _dp, _dp])
@@ -235,6 +255,14 @@ class Reference:
         sfm, rt, tau, cp = np.zeros(72), np.zeros(9), np.zeros(6), np.zeros(18)
         self._geom(_p(v), beta, _p(sfm), _p(rt), _p(tau), _p(cp))
         return _geom_unpack(sfm, rt, tau, cp)
+
+    def el_geom_nonuniform(self, vert, wts, beta=0.125):
+        """(sfm, rt, cp) of make_el_sfm_nonuniform / el_cp_nonuniform."""
+        v, w = _f(vert), _wts(wts)
+        sfm, rt, cp = np.zeros(72), np.zeros(9), np.zeros(18)
+        self._geomnu(_p(v), _p(w), beta, _p(sfm), _p(rt), _p(cp))
+        return ((sfm[0::2] + 1j * sfm[1::2]).reshape(6, 6, order="F"), rt.reshape(3, 3, order="F"),
+                cp.reshape(3, 6, order="F"))
 
     def local_submatrix(self, G, nu, h, z, tau, sfm):
         t, s = _sub_pack(tau, sfm)

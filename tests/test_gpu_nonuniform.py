@@ -1,0 +1,149 @@
+"""Non-uniform edge partition (per-node vertex weights,
+make_el_sfm_nonuniform / el_cp_nonuniform) through the C ABI against the
+oracle (GPU): assembly (FP64 and FP32), right-hand sides, volume-control
+border, stresses, a full solve, equal weights against the uniform element,
+and argument checks."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle_lib
+from oracle_lib import golden
+
+pytestmark = pytest.mark.gpu
+
+O = oracle_lib.oracle()
+
+
+def load_mesh(stem):
+    nods = np.load(golden(f"Nodes_{stem}.npy"))
+    conn = np.load(golden(f"Elems_{stem}.npy")).astype(np.int64)
+    conn[:3] -= 1
+    return nods, conn
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+@pytest.fixture(scope="module")
+def hx():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a device"
+    from hfpx3d_amd import hfp3d
+    return hfp3d
+
+
+def cases():
+    nods, conn = load_mesh("pennymesh24el_32")
+    rng = np.random.default_rng(11)
+    wavy = nods.copy()
+    wavy[2] = 0.05 * np.sin(2.0 * nods[0]) * np.cos(1.5 * nods[1])
+    return {
+        "penny24": (nods, conn, rng.uniform(0.4, 2.5, nods.shape[1]), 1.0, 0.35),
+        "wavy24": (wavy, conn, rng.uniform(0.4, 2.5, nods.shape[1]), 1 / 2.6, 0.3),
+    }
+
+
+@pytest.mark.parametrize("case,tip", [("penny24", 0), ("penny24", 1), ("wavy24", 0)])
+def test_weighted_assembly_matches_oracle(hx, case, tip):
+    import torch
+    print(f"case: weighted assembly {case} tip {tip}", flush=True)
+    nods, conn, wts, G, nu = cases()[case]
+    mesh = hx.Mesh_Geom_T(nods=nods, conn=conn, vert_wts=wts)
+    npar = hx.Num_Param_T(tip_type=tip)
+    A = hx.make_3dbem_matrix_s(G, nu, mesh, npar, hx.DoF_Handle_T()).cpu().numpy()
+    A32 = hx.make_3dbem_matrix_s(G, nu, mesh, npar, hx.DoF_Handle_T(), dtype=torch.float32).cpu().numpy()
+    Ao, _ = O.assemble(nods, conn, G, nu, tip_type=tip, node_wts=wts)
+    Au, _ = O.assemble(nods, conn, G, nu, tip_type=tip)
+    assert rel(A, Ao) < 1e-12
+    assert rel(A32.astype(np.float64), Ao) < 1e-6
+    # the weights matter: the weighted matrix is not the uniform one
+    assert rel(Au, Ao) > 1e-3
+
+
+def test_equal_weights_match_uniform_element(hx):
+    print("case: equal weights vs uniform", flush=True)
+    nods, conn, _, G, nu = cases()["wavy24"]
+    npar = hx.Num_Param_T(tip_type=0)
+    Au = hx.make_3dbem_matrix_s(G, nu, hx.Mesh_Geom_T(nods=nods, conn=conn), npar, hx.DoF_Handle_T())
+    Aw = hx.make_3dbem_matrix_s(G, nu, hx.Mesh_Geom_T(nods=nods, conn=conn, vert_wts=np.full(nods.shape[1], 1.7)),
+                                npar, hx.DoF_Handle_T())
+    assert rel(Aw.cpu().numpy(), Au.cpu().numpy()) < 1e-13
+
+
+def test_weighted_stress_and_volume_border_match_oracle(hx):
+    import torch
+    print("case: weighted stress and volume border wavy24", flush=True)
+    nods, conn, wts, G, nu = cases()["wavy24"]
+    n_el = conn.shape[1]
+    mesh = hx.Mesh_Geom_T(nods=nods, conn=conn, vert_wts=wts)
+    rng = np.random.default_rng(12)
+    dd = rng.normal(size=(6 * n_el, 3))
+    pts = rng.uniform(-1.5, 1.5, size=(3, 300))
+    s_inf = np.array([0.1, -0.2, 1.0, 0.05, 0.0, -0.1])
+    md = hx.Mesh_Data_T(mesh=mesh, dd=dd)
+    S = hx.make_3dbem_stress_f_s(G, nu, md, hx.Load_T(s_inf=s_inf), hx.Num_Param_T(), pts)
+    So = O.stress(nods, conn, G, nu, dd, pts, s_inf, node_wts=wts)
+    assert rel(S, So) < 1e-11
+    # volume-control border: closed-form area integrals of the weighted basis
+    # against the oracle's contour quadrature
+    npar = hx.Num_Param_T(tip_type=0)
+    dof = hx.DoF_Handle_T()
+    Avc = hx.make_3dbem_matrix_vc(G, nu, mesh, npar, dof).cpu().numpy()
+    n = dof.n_dof
+    row, col = O.vc_border(nods, conn, dof.dof_h, node_wts=wts)
+    assert rel(Avc[n, :n], row) < 1e-13
+    # the normals of a curved mesh agree to rounding (frames computed in
+    # different operation orders)
+    assert np.abs(Avc[:n, n] - col).max() < 1e-14
+    Ao, _ = O.assemble(nods, conn, G, nu, tip_type=0, node_wts=wts)
+    assert rel(Avc[:n, :n], Ao) < 1e-12
+    torch.cuda.synchronize()
+
+
+def test_weighted_solve_matches_lapack_on_oracle_matrix(hx):
+    import scipy.linalg as sl
+    print("case: weighted solve penny24 tip 1", flush=True)
+    nods, conn, wts, G, nu = cases()["penny24"]
+    mesh = hx.Mesh_Geom_T(nods=nods, conn=conn, vert_wts=wts)
+    npar = hx.Num_Param_T(tip_type=1)
+    dof = hx.DoF_Handle_T()
+    sae = hx.SAE_T()
+    sae.matrix = hx.make_3dbem_matrix_s(G, nu, mesh, npar, dof)
+    md = hx.Mesh_Data_T(mesh=mesh, dof_h_dd=dof)
+    s_inf = np.array([0.0, 0.0, 1.0, 0.0, 0.0, 0.0])
+    hx.add_s_inf_to_3dbem_rhs(md, hx.Load_T(s_inf=s_inf), sae)
+    Ao, dofo = O.assemble(nods, conn, G, nu, tip_type=1, node_wts=wts)
+    bo = O.rhs(nods, conn, dofo, s_inf)
+    assert np.abs(sae.rhs_v.cpu().numpy() - bo).max() < 1e-15
+    x = hx.LU(sae.matrix).solve(sae.rhs_v).cpu().numpy()
+    xr = sl.solve(Ao, bo)
+    assert rel(x, xr) < 1e-11
+
+
+def test_vertex_weight_arguments_are_checked(hx):
+    from hfpx3d_amd import _lib
+    print("case: vertex weight argument checks", flush=True)
+    nods, conn, wts, G, nu = cases()["penny24"]
+    dm = hx.DeviceMesh(hx.Mesh_Geom_T(nods=nods, conn=conn))
+    lib, ctx = dm.ctx.lib, dm.ctx
+    short = np.ascontiguousarray(wts[:-1])
+    assert lib.hfpx_mesh_set_vertex_weights(ctx.handle, dm.handle, short.ctypes.data_as(ctypes.c_void_p),
+                                            short.size) == _lib.HFPX_ERR_INVALID
+    bad = np.ascontiguousarray(wts.copy())
+    bad[3] = 0.0
+    assert lib.hfpx_mesh_set_vertex_weights(ctx.handle, dm.handle, bad.ctypes.data_as(ctypes.c_void_p),
+                                            bad.size) == _lib.HFPX_ERR_INVALID
+    bad[3] = np.nan
+    assert lib.hfpx_mesh_set_vertex_weights(ctx.handle, dm.handle, bad.ctypes.data_as(ctypes.c_void_p),
+                                            bad.size) == _lib.HFPX_ERR_INVALID
+    # weights on, then off again: back to the uniform matrix
+    dm.set_vertex_weights(wts)
+    dm.set_vertex_weights(None)
+    npar = hx.Num_Param_T(tip_type=0)
+    A = hx.make_3dbem_matrix_s(G, nu, hx.Mesh_Geom_T(nods=nods, conn=conn), npar, hx.DoF_Handle_T(),
+                               device_mesh=dm).cpu().numpy()
+    Au, _ = O.assemble(nods, conn, G, nu, tip_type=0)
+    assert rel(A, Au) < 1e-12
