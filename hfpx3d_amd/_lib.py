@@ -32,6 +32,12 @@ _c_void_p = ctypes.c_void_p
 _c_i64 = ctypes.c_int64
 _c_int = ctypes.c_int
 _c_double = ctypes.c_double
+
+
+class FcParam(ctypes.Structure):
+    """hfpx_fc_param: friction / cohesion parameters of one surface material."""
+    _fields_ = [("cr_open", ctypes.c_double), ("peak_ts", ctypes.c_double), ("cr_slip", ctypes.c_double),
+                ("peak_sc", ctypes.c_double), ("peak_sf", ctypes.c_double), ("res_sf", ctypes.c_double)]
 _pp = ctypes.POINTER(ctypes.c_void_p)
 
 
@@ -60,6 +66,10 @@ SIGNATURES = {
     "hfpx_mesh_n_dof": (_c_i64, [_c_void_p]),
     "hfpx_mesh_set_dd_local": (_c_int, [_c_void_p, _c_int]),
     "hfpx_mesh_set_vertex_weights": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_i64]),
+    "hfpx_match_f_c": (_c_int, [_c_void_p, _c_i64, _c_void_p, _c_i64, _c_void_p, _c_void_p, _c_void_p, _c_i64,
+                                _c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_void_p]),
+    "hfpx_update_mrd": (_c_int, [_c_void_p, _c_i64, _c_void_p, _c_i64, _c_void_p, _c_void_p, _c_i64, _c_void_p,
+                                 _c_void_p]),
     "hfpx_assemble": (_c_int, [_c_void_p, _c_void_p, _c_double, _c_double, _c_void_p, _c_i64]),
     "hfpx_assemble_mixed": (_c_int, [_c_void_p, _c_void_p, _c_double, _c_double, _c_void_p, _c_i64,
                                      _c_void_p, _c_i64]),

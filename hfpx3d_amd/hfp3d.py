@@ -59,10 +59,42 @@ class Num_Param_T:
 
 
 @dataclass
+class F_C_Param_T:
+    """Friction / cohesion parameters of one surface material
+    (src/Core/model_parameters.h:22-30)."""
+    cr_open: float = 1.0    # critical opening
+    peak_ts: float = 0.0    # peak tensile stress
+    cr_slip: float = 1.0    # critical slip
+    peak_sc: float = 0.0    # peak shear cohesion
+    peak_sf: float = 0.0    # peak friction coefficient
+    res_sf: float = 0.0     # residual friction coefficient
+
+
+@dataclass
 class Properties_T:
     n_solid: int = 1
     shear_m: list = field(default_factory=lambda: [1.0])
     poiss_r: list = field(default_factory=lambda: [0.0])
+    n_surface: int = 1
+    # contact model ids per surface material: column 0 cohesion (0 box,
+    # 1 linear softening), column 1 friction (0 constant, 1 linear slip
+    # weakening); n_surface x 2
+    c_model_id: np.ndarray = None
+    f_c_param: list = field(default_factory=list)   # F_C_Param_T per surface material
+
+
+@dataclass
+class Frac_State_T:
+    """Node-wise crack-surface state (src/Core/surface_mesh_utilities.h:51-62)."""
+    mr_open: np.ndarray = None        # largest relative opening reached
+    mr_slip: np.ndarray = None        # largest relative slip reached
+    friction_coef: np.ndarray = None
+    slip_cohesion: np.ndarray = None
+    open_cohesion: np.ndarray = None
+
+    @classmethod
+    def zeros(cls, n_nod):
+        return cls(*(np.zeros(n_nod) for _ in range(5)))
 
 
 @dataclass
@@ -633,3 +665,82 @@ class Batch:
         A = np.zeros((n, n), order="F")
         self.ctx.check(self.lib.hfpx_batch_matrix(self.ctx.handle, self.handle, int(i), _np_ptr(A), None))
         return A
+
+
+# --------------------------------------------------------------------------
+# cohesion / friction state (src/Development/cohesion_friction.cpp)
+# --------------------------------------------------------------------------
+
+def _fc_tables(prop):
+    n_mat = len(prop.f_c_param)
+    if n_mat < 1:
+        raise ValueError("prop.f_c_param is empty")
+    par = (_lib.FcParam * n_mat)(*[_lib.FcParam(p.cr_open, p.peak_ts, p.cr_slip, p.peak_sc, p.peak_sf, p.res_sf)
+                                   for p in prop.f_c_param])
+    cm = None
+    if prop.c_model_id is not None:
+        cm = np.asfortranarray(prop.c_model_id, dtype=np.int32)
+        if cm.shape != (n_mat, 2):
+            raise ValueError("prop.c_model_id must be n_surface x 2 (one row per f_c_param entry)")
+    return par, cm, n_mat
+
+
+def _fc_device(ctx, mat_id, dd, arrays):
+    torch = ctx.torch
+    dd = np.asarray(dd, dtype=np.float64)
+    if dd.ndim != 2 or dd.shape[1] != 3:
+        raise ValueError("dd must be n_nod x 3")
+    n = dd.shape[0]
+    mid = np.ascontiguousarray(mat_id, dtype=np.int32).reshape(-1)
+    if mid.size != n:
+        raise ValueError("mat_id must have one entry per DD row")
+    with torch.cuda.stream(ctx.stream):
+        d_dd = torch.from_numpy(np.ascontiguousarray(dd.T)).to(ctx.device)     # (3, n): col-major n x 3
+        d_mid = torch.from_numpy(mid).to(ctx.device)
+        out = [torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64).reshape(-1)).to(ctx.device)
+               for a in arrays]
+    for a in out:
+        if a.numel() != n:
+            raise ValueError("state arrays must have one entry per DD row")
+    return n, d_dd, d_mid, out
+
+
+def match_f_c(prop, mat_id, dd, is_dd_local, f_state, ctx=None):
+    """Friction coefficients and cohesions of every node from its DD
+    (n_nod x 3: shear 0, 1, opening 2, used as given; the reference's
+    rotation for global DD is an empty branch, so is_dd_local changes
+    nothing), material id and reached opening / slip; writes
+    f_state.friction_coef / slip_cohesion / open_cohesion.  Mirrors
+    hfp3d::match_f_c (src/Development/cohesion_friction.cpp:22-127) on the
+    device."""
+    ctx = ctx or get_context()
+    par, cm, n_mat = _fc_tables(prop)
+    if cm is None:
+        raise ValueError("prop.c_model_id is required")
+    n = np.asarray(dd).shape[0]
+    zeros = np.zeros(n)
+    n, d_dd, d_mid, (mo, ms, fc, sc, oc) = _fc_device(ctx, mat_id, dd, [f_state.mr_open, f_state.mr_slip,
+                                                                          zeros, zeros, zeros])
+    ctx.enter()
+    ctx.check(ctx.lib.hfpx_match_f_c(ctx.handle, n, _ptr(d_dd), n, _ptr(d_mid), _np_ptr(cm), par, n_mat, _ptr(mo),
+                                     _ptr(ms), _ptr(fc), _ptr(sc), _ptr(oc)))
+    ctx.leave()
+    f_state.friction_coef = fc.cpu().numpy()
+    f_state.slip_cohesion = sc.cpu().numpy()
+    f_state.open_cohesion = oc.cpu().numpy()
+    return f_state
+
+
+def update_mrd(prop, mat_id, dd, is_dd_local, f_state, ctx=None):
+    """Largest relative opening and slip reached (f_state.mr_open /
+    mr_slip), updated from the current DD.  Mirrors hfp3d::update_mrd
+    (src/Development/cohesion_friction.cpp:131-189) on the device."""
+    ctx = ctx or get_context()
+    par, _, n_mat = _fc_tables(prop)
+    n, d_dd, d_mid, (mo, ms) = _fc_device(ctx, mat_id, dd, [f_state.mr_open, f_state.mr_slip])
+    ctx.enter()
+    ctx.check(ctx.lib.hfpx_update_mrd(ctx.handle, n, _ptr(d_dd), n, _ptr(d_mid), par, n_mat, _ptr(mo), _ptr(ms)))
+    ctx.leave()
+    f_state.mr_open = mo.cpu().numpy()
+    f_state.mr_slip = ms.cpu().numpy()
+    return f_state

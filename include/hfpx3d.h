@@ -140,6 +140,39 @@ int hfpx_vc_truncate(hfpx_ctx* ctx, const double* A, int64_t lda, int64_t orig_n
                      const int64_t* orig_dof_h_host, const int64_t* dof_h_host, int64_t n_el, int64_t ndpe,
                      int64_t used_ndof, const double* delta_t_host, int64_t tsize, double delta_v, double* M64,
                      int64_t ldm64, float* M32, int64_t ldm32, double* rhs);
+/* ---- cohesion / friction state of the crack surface ---------------------- */
+/* Friction / cohesion parameters of one surface material (F_C_Param_T,
+ * src/Core/model_parameters.h:22-30). */
+typedef struct hfpx_fc_param {
+  double cr_open;  /* critical opening */
+  double peak_ts;  /* peak tensile stress */
+  double cr_slip;  /* critical slip */
+  double peak_sc;  /* peak shear cohesion */
+  double peak_sf;  /* peak friction coefficient */
+  double res_sf;   /* residual friction coefficient */
+} hfpx_fc_param;
+/* Friction coefficient, shear cohesion and opening cohesion of n_nod nodes
+ * (device arrays of n_nod) from their DD (device, n_nod x 3 col-major, ldd;
+ * components used as given: shear 0, 1 and opening 2), material ids mat_id
+ * (device, 0 <= id < n_mat), the largest relative opening / slip reached
+ * (mr_open, mr_slip, device) and, per material, the parameters
+ * f_c_param_host (n_mat) and model ids c_model_id_host (host, n_mat x 2
+ * col-major: cohesion 0 box / 1 linear softening, friction 0 constant /
+ * 1 linear slip weakening; other ids give no cohesion / residual
+ * friction).  Synchronous.  A material id outside [0, n_mat) or a zero
+ * critical opening or slip is HFPX_ERR_INVALID.  Replaces hfp3d::match_f_c
+ * (src/Development/cohesion_friction.cpp:22-127, declared
+ * cohesion_friction.h:22-29). */
+int hfpx_match_f_c(hfpx_ctx* ctx, int64_t n_nod, const double* dd, int64_t ldd, const int32_t* mat_id,
+                   const int32_t* c_model_id_host, const hfpx_fc_param* f_c_param_host, int64_t n_mat,
+                   const double* mr_open, const double* mr_slip, double* friction_coef,
+                   double* slip_cohesion, double* open_cohesion);
+/* Largest relative opening (mr_open) and slip (mr_slip) reached, updated in
+ * place from the DD (same layouts as hfpx_match_f_c).  Synchronous.
+ * Replaces hfp3d::update_mrd (src/Development/cohesion_friction.cpp:131-189,
+ * declared cohesion_friction.h:34-41). */
+int hfpx_update_mrd(hfpx_ctx* ctx, int64_t n_nod, const double* dd, int64_t ldd, const int32_t* mat_id,
+                    const hfpx_fc_param* f_c_param_host, int64_t n_mat, double* mr_open, double* mr_slip);
 /* ---- postprocessing ------------------------------------------------------ */
 /* Stress at n_pts observation points: S (device, n_pts x 6, col-major, ldS,
  * order Sxx Syy Szz Sxy Sxz Syz, tension positive) = sinf_host (6 values) +

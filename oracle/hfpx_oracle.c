@@ -20,6 +20,8 @@
  *   Sneddon opening         src/Solvers/test_radial_crack_static.cpp:316-337
  *   volume control          src/Solvers/system_assembly.cpp:458-629,
  *                           src/Core/element_utilities.cpp:484-595
+ *   cohesion / friction     src/Development/cohesion_friction.cpp:22-189
+ *                           (match_f_c, update_mrd)
  *
  * The per-pair kernel.  The stress at X due to a displacement discontinuity
  * D_i P(xi) on a flat element (plane x3 = 0 of the element frame) follows
@@ -854,4 +856,69 @@ int hfpo_mod_system_vc(const double* A, int64_t lda, int64_t orig_ndof,
         rhs[s] = delta_t[s];
     }
   return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* cohesion / friction state (src/Development/cohesion_friction.cpp)        */
+/* ---------------------------------------------------------------------- */
+
+/* match_f_c (cohesion_friction.cpp:22-127), node by node (the reference
+ * shares its nodal_dd scratch between OpenMP threads; this is its
+ * one-thread behaviour).  dd: n_nod x 3 col-major (ldd), used as given (the
+ * reference's rotation to local coordinates is an empty branch).  par:
+ * 6 doubles per material (cr_open, peak_ts, cr_slip, peak_sc, peak_sf,
+ * res_sf); cmid: n_mat x 2 col-major model ids (cohesion, friction). */
+void hfpo_match_f_c(int64_t n_nod, const double* dd, int64_t ldd, const int32_t* mat_id,
+                    const int32_t* cmid, int64_t n_mat, const double* par,
+                    const double* mr_open, const double* mr_slip, double* friction_coef,
+                    double* slip_cohesion, double* open_cohesion) {
+  for (int64_t n = 0; n < n_nod; ++n) {
+    const int m = mat_id[n];
+    const int c_model = cmid[m], f_model = cmid[m + n_mat];
+    const double* p = par + 6 * m;
+    const double cr_open = p[0], peak_ts = p[1], cr_slip = p[2], peak_sc = p[3], peak_sf = p[4],
+                 res_sf = p[5];
+    const double d0 = dd[n], d1 = dd[n + ldd], d2 = dd[n + 2 * ldd];
+    const double open = d2 / cr_open;                           /* opening / critical */
+    const double slip = sqrt(d0 * d0 + d1 * d1) / cr_slip;      /* slip / critical */
+    /* cohesion factor: Dugdale box (model 0) or linear softening (model 1)
+     * while partly open; 0 otherwise (model 2 and unknown ids) */
+    double f = 0.0;
+    if (open > 0.0 && open < 1.0) f = c_model == 0 ? 1.0 : (c_model == 1 ? 1.0 - open : 0.0);
+    /* unloading below the largest opening reached: scaled linearly */
+    if (open > 0.0 && open < mr_open[n]) f *= open / mr_open[n];
+    open_cohesion[n] = peak_ts * f;
+    slip_cohesion[n] = peak_sc * f;
+    /* friction: none once fully open; residual by default, peak without
+     * weakening (0), linear slip weakening from the largest slip (1) */
+    double mu = 0.0;
+    if (open < 1.0) {
+      mu = res_sf;
+      if (f_model == 0) {
+        mu = peak_sf;
+      } else if (f_model == 1 && slip < 1.0) {
+        const double s = slip < mr_slip[n] ? mr_slip[n] : slip;
+        mu = res_sf + (peak_sf - res_sf) * (1.0 - s);
+      }
+    }
+    friction_coef[n] = mu;
+  }
+}
+
+/* update_mrd (cohesion_friction.cpp:131-189): largest relative opening and
+ * slip reached.  A full opening sets 1 unless the opening itself is larger
+ * than the previous maximum, which then wins (the reference's order). */
+void hfpo_update_mrd(int64_t n_nod, const double* dd, int64_t ldd, const int32_t* mat_id,
+                     int64_t n_mat, const double* par, double* mr_open, double* mr_slip) {
+  (void)n_mat;
+  for (int64_t n = 0; n < n_nod; ++n) {
+    const double* p = par + 6 * mat_id[n];
+    const double d0 = dd[n], d1 = dd[n + ldd], d2 = dd[n + 2 * ldd];
+    const double open = d2 / p[0];
+    const double slip = sqrt(d0 * d0 + d1 * d1) / p[2];
+    const double pr_open = mr_open[n], pr_slip = mr_slip[n];
+    if (slip > pr_slip) mr_slip[n] = slip;
+    if (open >= 1.0) mr_open[n] = 1.0;
+    if (open > pr_open) mr_open[n] = open;
+  }
 }

@@ -119,6 +119,17 @@ int hfpo_mod_system_vc(const double* A, int64_t lda, int64_t orig_ndof,
                        const double* delta_t, int64_t tsize, double delta_v,
                        double* M, double* rhs);
 
+/* Cohesion / friction state per node (src/Development/cohesion_friction.cpp).
+ * dd n_nod x 3 col-major (ldd); mat_id n_nod; cmid n_mat x 2 col-major
+ * (cohesion model, friction model); par 6 per material: cr_open, peak_ts,
+ * cr_slip, peak_sc, peak_sf, res_sf. */
+void hfpo_match_f_c(int64_t n_nod, const double* dd, int64_t ldd, const int32_t* mat_id,
+                    const int32_t* cmid, int64_t n_mat, const double* par,
+                    const double* mr_open, const double* mr_slip, double* friction_coef,
+                    double* slip_cohesion, double* open_cohesion);
+void hfpo_update_mrd(int64_t n_nod, const double* dd, int64_t ldd, const int32_t* mat_id,
+                     int64_t n_mat, const double* par, double* mr_open, double* mr_slip);
+
 #ifdef __cplusplus
 }
 #endif

@@ -20,6 +20,9 @@
 //   hfp3d::el_p2_sf_integral            src/Core/element_utilities.cpp:573-595
 //   hfp3d::make_3dbem_matrix_vc         src/Solvers/system_assembly.cpp:458-557
 //   hfp3d::mod_3dbem_system_vc          src/Solvers/system_assembly.cpp:560-629
+//   hfp3d::match_f_c, hfp3d::update_mrd src/Development/cohesion_friction.cpp:22-189
+//                                       (run on one thread: the reference shares
+//                                       its nodal_dd scratch between threads)
 //
 // All arrays are column-major, like il::Array2D.
 #include <chrono>
@@ -38,6 +41,7 @@
 #include "src/Core/surface_mesh_utilities.h"
 #include "src/Elasticity/elasticity_kernel_integration.h"
 #include "src/Solvers/system_assembly.h"
+#include "src/Development/cohesion_friction.h"
 
 namespace {
 
@@ -286,6 +290,81 @@ int ref_mod_system_vc(const double* A, int64_t orig_ndof, const int64_t* orig_do
   for (il::int_t j = 0; j < nu; ++j)
     for (il::int_t i = 0; i < nu; ++i) M[i + nu * j] = sae.matrix(i, j);
   for (il::int_t i = 0; i < nu; ++i) rhs[i] = sae.rhs_v[i];
+  return 0;
+}
+
+// Cohesion / friction state (layouts as hfpo_match_f_c / hfpo_update_mrd).
+static void fc_inputs(int64_t n_nod, const double* dd, int64_t ldd, const int32_t* mat_id,
+                      const int32_t* cmid, int64_t n_mat, const double* par, hfp3d::Properties_T& prop,
+                      il::Array<int>& mid, il::Array2D<double>& d) {
+  prop.n_surface = n_mat;
+  prop.c_model_id = il::Array2D<int>{n_mat, 2, 0};
+  prop.f_c_param = il::Array<hfp3d::F_C_Param_T>{n_mat};
+  for (int64_t m = 0; m < n_mat; ++m) {
+    prop.c_model_id(m, 0) = cmid ? cmid[m] : 0;
+    prop.c_model_id(m, 1) = cmid ? cmid[m + n_mat] : 0;
+    hfp3d::F_C_Param_T f;
+    f.cr_open = par[6 * m];
+    f.peak_ts = par[6 * m + 1];
+    f.cr_slip = par[6 * m + 2];
+    f.peak_sc = par[6 * m + 3];
+    f.peak_sf = par[6 * m + 4];
+    f.res_sf = par[6 * m + 5];
+    prop.f_c_param[m] = f;
+  }
+  mid = il::Array<int>{n_nod, 0};
+  d = il::Array2D<double>{n_nod, 3, 0.0};
+  for (int64_t n = 0; n < n_nod; ++n) {
+    mid[n] = mat_id[n];
+    for (int i = 0; i < 3; ++i) d(n, i) = dd[n + ldd * i];
+  }
+}
+
+static hfp3d::Frac_State_T fc_state(int64_t n_nod, const double* mr_open, const double* mr_slip) {
+  hfp3d::Frac_State_T fs;
+  fs.mr_open = il::Array<double>{n_nod, 0.0};
+  fs.mr_slip = il::Array<double>{n_nod, 0.0};
+  fs.friction_coef = il::Array<double>{n_nod, 0.0};
+  fs.slip_cohesion = il::Array<double>{n_nod, 0.0};
+  fs.open_cohesion = il::Array<double>{n_nod, 0.0};
+  for (int64_t n = 0; n < n_nod; ++n) {
+    fs.mr_open[n] = mr_open[n];
+    fs.mr_slip[n] = mr_slip[n];
+  }
+  return fs;
+}
+
+int ref_match_f_c(int64_t n_nod, const double* dd, int64_t ldd, const int32_t* mat_id, const int32_t* cmid,
+                  int64_t n_mat, const double* par, const double* mr_open, const double* mr_slip,
+                  double* friction_coef, double* slip_cohesion, double* open_cohesion) {
+  set_threads(1);
+  hfp3d::Properties_T prop;
+  il::Array<int> mid;
+  il::Array2D<double> d;
+  fc_inputs(n_nod, dd, ldd, mat_id, cmid, n_mat, par, prop, mid, d);
+  hfp3d::Frac_State_T fs = fc_state(n_nod, mr_open, mr_slip);
+  hfp3d::match_f_c(prop, mid, d, false, il::io, fs);
+  for (int64_t n = 0; n < n_nod; ++n) {
+    friction_coef[n] = fs.friction_coef[n];
+    slip_cohesion[n] = fs.slip_cohesion[n];
+    open_cohesion[n] = fs.open_cohesion[n];
+  }
+  return 0;
+}
+
+int ref_update_mrd(int64_t n_nod, const double* dd, int64_t ldd, const int32_t* mat_id, int64_t n_mat,
+                   const double* par, double* mr_open, double* mr_slip) {
+  set_threads(1);
+  hfp3d::Properties_T prop;
+  il::Array<int> mid;
+  il::Array2D<double> d;
+  fc_inputs(n_nod, dd, ldd, mat_id, nullptr, n_mat, par, prop, mid, d);
+  hfp3d::Frac_State_T fs = fc_state(n_nod, mr_open, mr_slip);
+  hfp3d::update_mrd(prop, mid, d, false, il::io, fs);
+  for (int64_t n = 0; n < n_nod; ++n) {
+    mr_open[n] = fs.mr_open[n];
+    mr_slip[n] = fs.mr_slip[n];
+  }
   return 0;
 }
 

@@ -60,7 +60,37 @@ def _sub_pack(tau, sfm):
     return t, s
 
 
-class Oracle:
+def _fc_args(dd, mat_id, cmid, par):
+    dd = _f(dd)
+    mid = np.ascontiguousarray(mat_id, dtype=np.int32)
+    cm = _f(cmid, np.int32) if cmid is not None else None
+    pr = np.ascontiguousarray(par, dtype=np.float64).reshape(-1, 6)
+    return dd, mid, cm, pr
+
+
+class _FrictionCohesion:
+    """match_f_c / update_mrd through a library with the hfpo_ / ref_ layout
+    (dd n_nod x 3, mat_id n_nod, cmid n_mat x 2, par n_mat x 6)."""
+
+    def match_f_c(self, dd, mat_id, cmid, par, mr_open, mr_slip):
+        dd, mid, cm, pr = _fc_args(dd, mat_id, cmid, par)
+        n = dd.shape[0]
+        mo = np.ascontiguousarray(mr_open, dtype=np.float64)
+        ms = np.ascontiguousarray(mr_slip, dtype=np.float64)
+        fc, sc, oc = np.zeros(n), np.zeros(n), np.zeros(n)
+        self._mfc(n, _p(dd), n, _p(mid), _p(cm), pr.shape[0], _p(pr), _p(mo), _p(ms), _p(fc), _p(sc), _p(oc))
+        return fc, sc, oc
+
+    def update_mrd(self, dd, mat_id, par, mr_open, mr_slip):
+        dd, mid, _, pr = _fc_args(dd, mat_id, None, par)
+        n = dd.shape[0]
+        mo = np.array(mr_open, dtype=np.float64)
+        ms = np.array(mr_slip, dtype=np.float64)
+        self._mrd(n, _p(dd), n, _p(mid), pr.shape[0], _p(pr), _p(mo), _p(ms))
+        return mo, ms
+
+
+class Oracle(_FrictionCohesion):
     """The C restatement (hfpo_* functions)."""
 
     def __init__(self, path=ORACLE):
@@ -82,6 +112,8 @@ class Oracle:
         self._vcb = L.fn("vc_border_w", None, [_dp, _i64, _i64, _dp, _i64, _i64, _dp, _dp, _int, _dp, _dp])
         self._mvc = L.fn("mod_system_vc", _int, [_dp, _i64, _i64, _dp, _dp, _i64, _i64, _i64, _dp, _i64, _d,
                                                   _dp, _dp])
+        self._mfc = L.fn("match_f_c", None, [_i64, _dp, _i64, _dp, _dp, _i64, _dp, _dp, _dp, _dp, _dp, _dp])
+        self._mrd = L.fn("update_mrd", None, [_i64, _dp, _i64, _dp, _i64, _dp, _dp, _dp])
 
     def el_geom(self, vert, beta=0.125, wts=None):
         """wts: the three vertex weights of a non-uniform edge partition."""
@@ -201,7 +233,7 @@ class Oracle:
         return dd
 
 
-class Reference:
+class Reference(_FrictionCohesion):
     """The reference's own kernel/assembly/stress code (oracle/_ref)."""
 
     def __init__(self, path=REF):
@@ -209,6 +241,8 @@ class Reference:
         self._sub = L.fn("local_submatrix", _int, [_d, _d, _d, _d, _d, _dp, _dp, _dp])
         self._geom = L.fn("el_geom", _int, [_dp, _d, _dp, _dp, _dp, _dp])
         self._geomnu = L.fn("el_geom_nonuniform", _int, [_dp, _dp, _d, _dp, _dp, _dp])
+        self._mfc = L.fn("match_f_c", _int, [_i64, _dp, _i64, _dp, _dp, _i64, _dp, _dp, _dp, _dp, _dp, _dp])
+        self._mrd = L.fn("update_mrd", _int, [_i64, _dp, _i64, _dp, _i64, _dp, _dp, _dp])
         self._dof = L.fn("dof_h", _i64, [_dp, _i64, _i64, _dp, _i64, _i64, _int, _dp])
         self._asm = L.fn("assemble", _i64, [_dp, _i64, _i64, _dp, _i64, _i64, _d, _d, _d, _int, _int, _int, _dp])
         self._rhs = L.fn("rhs", _i64, [_dp, _i64, _i64, _dp, _i64, _i64, _int, _dp, _dp])
