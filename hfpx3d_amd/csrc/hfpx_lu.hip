@@ -703,6 +703,93 @@ __global__ __launch_bounds__(256) void k_trsm_leaf(const T* __restrict__ Lm, int
     if (i < kb) col[i] = x[i];
 }
 
+// FP32 leaf with B staged through LDS: a workgroup of two waves owns
+// TRSM_F32_TC = 128 columns; each wave loads and stores whole 64-row column
+// segments (256 contiguous bytes) instead of one lane per column touching 64
+// cache lines per load.  The unit-lower solve runs by columns: once x_j is
+// final, the updates x_i -= L_ij x_j of the rows below it are independent,
+// so the dependent chain is 63 steps long instead of 2,016 (the row form's
+// running sum), with each x_i still reduced in the same order j = 0, 1, ...
+// and therefore to the same bits.  The triangle sits in LDS transposed
+// (column j contiguous, rows 16-byte aligned) so the reads of one column
+// merge.  The upper solve keeps the row form of k_trsm_leaf.  50 KB of LDS:
+// three workgroups per CU.
+constexpr int TRSM_F32_TC = 128;
+
+template <bool LOWER>
+__global__ __launch_bounds__(TRSM_F32_TC) void k_trsm_leaf_f32(const float* __restrict__ Lm, int64_t ldl,
+                                                               float* __restrict__ B, int64_t ldb, int kb,
+                                                               int64_t ncols) {
+  constexpr int TC = TRSM_F32_TC;
+  constexpr int NW = TC / 64;
+  // LOWER: sM[j][i] = L(i, j); upper: sM[i][j] = U(i, j)
+  __shared__ __align__(16) float sM[LB][LB + 4];
+  __shared__ float sB[TC][LB + 1];
+  constexpr int LQ = LB * LB / TC, BQ = TC / NW;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t c0 = (int64_t)blockIdx.x * TC;
+  const int nc = (int)((ncols - c0) < TC ? (ncols - c0) : TC);
+  // every global load of the triangle and of the tile is issued before the
+  // first is used: one memory latency, not one per loop trip
+  {
+    float lv[LQ];
+#pragma unroll
+    for (int q = 0; q < LQ; ++q) {
+      const int e = tid + TC * q, i = e % LB, j = e / LB;
+      lv[q] = (i < kb && j < kb) ? Lm[i + (int64_t)j * ldl] : 0.0f;
+    }
+    float bv[BQ];
+#pragma unroll
+    for (int q = 0; q < BQ; ++q) {
+      const int c = wave + NW * q;
+      bv[q] = (c < nc && lane < kb) ? B[(c0 + c) * ldb + lane] : 0.0f;
+    }
+#pragma unroll
+    for (int q = 0; q < LQ; ++q) {
+      const int e = tid + TC * q, i = e % LB, j = e / LB;
+      if (LOWER)
+        sM[j][i] = lv[q];
+      else
+        sM[i][j] = lv[q];
+    }
+#pragma unroll
+    for (int q = 0; q < BQ; ++q) sB[wave + NW * q][lane] = bv[q];
+  }
+  __syncthreads();
+  if (tid < nc) {
+    float x[LB];
+#pragma unroll
+    for (int i = 0; i < LB; ++i) x[i] = sB[tid][i];
+    if (LOWER) {
+#pragma unroll
+      for (int j = 0; j < LB - 1; ++j) {
+        // the guard keeps the scheduler from hoisting every LDS read of the
+        // unrolled loop nest at once (which spills)
+        if (j < kb) {
+          const float xj = x[j];
+#pragma unroll
+          for (int i = j + 1; i < LB; ++i) x[i] -= sM[j][i] * xj;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int i = LB - 1; i >= 0; --i) {
+        if (i < kb) {
+          float acc = x[i];
+#pragma unroll
+          for (int j = i + 1; j < LB; ++j) acc -= sM[i][j] * x[j];
+          x[i] = acc / sM[i][i];
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < LB; ++i) sB[tid][i] = x[i];
+  }
+  __syncthreads();
+  for (int c = wave; c < nc; c += NW)
+    if (lane < kb) B[(c0 + c) * ldb + lane] = sB[c][lane];
+}
+
 // ---------------------------------------------------------------------
 // host drivers
 // ---------------------------------------------------------------------
@@ -736,8 +823,12 @@ static int apply_pairs(hfpx_ctx* ctx, T* A, int64_t lda, const PermPairs* pp, in
 template <typename T, bool LOWER>
 static int trsm_leaf(hfpx_ctx* ctx, const T* L, int64_t ldl, T* B, int64_t ldb, int kb, int64_t ncols) {
   const int tok = prof_begin(ctx, HFPX_KC_TRSM);
-  hipLaunchKernelGGL((k_trsm_leaf<T, LOWER>), dim3((unsigned)((ncols + 255) / 256)), dim3(256), 0, ctx->stream,
-                     L, ldl, B, ldb, kb, ncols);
+  if constexpr (sizeof(T) == 4)
+    hipLaunchKernelGGL((k_trsm_leaf_f32<LOWER>), dim3((unsigned)((ncols + TRSM_F32_TC - 1) / TRSM_F32_TC)),
+                       dim3(TRSM_F32_TC), 0, ctx->stream, L, ldl, B, ldb, kb, ncols);
+  else
+    hipLaunchKernelGGL((k_trsm_leaf<T, LOWER>), dim3((unsigned)((ncols + 255) / 256)), dim3(256), 0, ctx->stream,
+                       L, ldl, B, ldb, kb, ncols);
   prof_end(ctx, tok, HFPX_KC_TRSM, (double)kb * kb * ncols, 0.0);
   HFPX_TRY_HIP(ctx, hipGetLastError());
   return HFPX_OK;
