@@ -80,10 +80,33 @@ struct Mesh_Data_T {
   Array2D<double> dd;  // (6 n_el) x 3, global components
 };
 
+// Friction / cohesion parameters of one surface material
+// (src/Core/model_parameters.h:22-30).
+struct F_C_Param_T {
+  double cr_open = 1.0;  // critical opening
+  double peak_ts = 0.0;  // peak tensile stress
+  double cr_slip = 1.0;  // critical slip
+  double peak_sc = 0.0;  // peak shear cohesion
+  double peak_sf = 0.0;  // peak friction coefficient
+  double res_sf = 0.0;   // residual friction coefficient
+};
+
 struct Properties_T {
   int64_t n_solid = 1;
   std::vector<double> shear_m{1.0};
   std::vector<double> poiss_r{0.0};
+  int64_t n_surface = 1;
+  Array2D<int32_t> c_model_id;          // n_surface x 2: cohesion model, friction model
+  std::vector<F_C_Param_T> f_c_param;   // per surface material
+};
+
+// Node-wise crack-surface state (src/Core/surface_mesh_utilities.h:51-62).
+struct Frac_State_T {
+  std::vector<double> mr_open;        // largest relative opening reached
+  std::vector<double> mr_slip;        // largest relative slip reached
+  std::vector<double> friction_coef;
+  std::vector<double> slip_cohesion;
+  std::vector<double> open_cohesion;
 };
 
 struct Load_T {
@@ -155,6 +178,14 @@ void add_s_inf_to_3dbem_rhs(const Mesh_Data_T& mesh_data, const Load_T& load, SA
 Array2D<double> make_3dbem_stress_f_s(double shear_m, double poiss_r, const Mesh_Data_T& m_data,
                                       const Load_T& load, const Num_Param_T& n_par,
                                       const Array2D<double>& m_pts_crd);
+
+// Cohesion / friction laws, node by node, on the device
+// (src/Development/cohesion_friction.cpp:22-189).  dd: n_nod x 3 (shear 0, 1,
+// opening 2, used as given, as in the reference); mat_id: n_nod.
+void match_f_c(const Properties_T& prop, const std::vector<int32_t>& mat_id, const Array2D<double>& dd,
+               bool is_dd_local, Frac_State_T& f_state);
+void update_mrd(const Properties_T& prop, const std::vector<int32_t>& mat_id, const Array2D<double>& dd,
+                bool is_dd_local, Frac_State_T& f_state);
 
 class LU {
  public:

@@ -157,6 +157,84 @@ DeviceMatrix make_3dbem_matrix_s(double shear_m, double poiss_r, const Mesh_Geom
   return A;
 }
 
+// ---- cohesion / friction ---------------------------------------------------
+static std::vector<hfpx_fc_param> fc_params(const Properties_T& prop) {
+  std::vector<hfpx_fc_param> v;
+  for (const F_C_Param_T& p : prop.f_c_param)
+    v.push_back({p.cr_open, p.peak_ts, p.cr_slip, p.peak_sc, p.peak_sf, p.res_sf});
+  return v;
+}
+
+// Device copies of the node arrays of one call; freed on scope exit.
+struct FcArrays {
+  int64_t n;
+  double* dd;
+  int32_t* mid;
+  double* st[5];
+  FcArrays(const std::vector<int32_t>& mat_id, const Array2D<double>& dd_h) : n(dd_h.size(0)) {
+    if (dd_h.size(1) != 3 || (int64_t)mat_id.size() != n) {
+      std::cerr << "hfp3d: cohesion/friction: dd must be n_nod x 3 with one material id per row" << std::endl;
+      std::abort();
+    }
+    dd = dalloc<double>(3 * (size_t)n);
+    mid = dalloc<int32_t>((size_t)n);
+    for (double*& p : st) p = dalloc<double>((size_t)n);
+    h2d(dd, dd_h.data(), sizeof(double) * 3 * n);
+    h2d(mid, mat_id.data(), sizeof(int32_t) * n);
+  }
+  ~FcArrays() {
+    (void)hipFree(dd);
+    (void)hipFree(mid);
+    for (double* p : st) (void)hipFree(p);
+  }
+  void up(int k, const std::vector<double>& v) {
+    if ((int64_t)v.size() != n) {
+      std::cerr << "hfp3d: cohesion/friction: state arrays need one entry per DD row" << std::endl;
+      std::abort();
+    }
+    h2d(st[k], v.data(), sizeof(double) * n);
+  }
+  void down(int k, std::vector<double>& v) const {
+    v.resize(n);
+    d2h(v.data(), st[k], sizeof(double) * n);
+  }
+};
+
+void match_f_c(const Properties_T& prop, const std::vector<int32_t>& mat_id, const Array2D<double>& dd,
+               bool is_dd_local, Frac_State_T& f_state) {
+  (void)is_dd_local;  // the reference rotates nothing either way
+  const std::vector<hfpx_fc_param> par = fc_params(prop);
+  const int64_t n_mat = (int64_t)par.size();
+  if (prop.c_model_id.size(0) != n_mat || prop.c_model_id.size(1) != 2) {
+    std::cerr << "hfp3d: match_f_c: c_model_id must be n_surface x 2" << std::endl;
+    std::abort();
+  }
+  FcArrays a(mat_id, dd);
+  if (a.n == 0) return;
+  a.up(0, f_state.mr_open);
+  a.up(1, f_state.mr_slip);
+  check(hfpx_match_f_c(context(), a.n, a.dd, a.n, a.mid, prop.c_model_id.data(), par.data(), n_mat, a.st[0],
+                       a.st[1], a.st[2], a.st[3], a.st[4]),
+        "match_f_c");
+  a.down(2, f_state.friction_coef);
+  a.down(3, f_state.slip_cohesion);
+  a.down(4, f_state.open_cohesion);
+}
+
+void update_mrd(const Properties_T& prop, const std::vector<int32_t>& mat_id, const Array2D<double>& dd,
+                bool is_dd_local, Frac_State_T& f_state) {
+  (void)is_dd_local;
+  const std::vector<hfpx_fc_param> par = fc_params(prop);
+  FcArrays a(mat_id, dd);
+  if (a.n == 0) return;
+  a.up(0, f_state.mr_open);
+  a.up(1, f_state.mr_slip);
+  check(hfpx_update_mrd(context(), a.n, a.dd, a.n, a.mid, par.data(), (int64_t)par.size(), a.st[0], a.st[1]),
+        "update_mrd");
+  a.down(0, f_state.mr_open);
+  a.down(1, f_state.mr_slip);
+}
+
 DeviceMatrix make_3dbem_matrix_vc(double shear_m, double poiss_r, const Mesh_Geom_T& mesh,
                                   const Num_Param_T& n_par, DoF_Handle_T& dof_hndl) {
   if (dof_hndl.n_dof == 0 || dof_hndl.dof_h.size(0) == 0) dof_hndl = make_dof_h_crack(mesh, 2, n_par.tip_type);

@@ -65,3 +65,27 @@ def test_bad_material_id_and_zero_critical_values_are_rejected(hx):
     fc, _, _ = O.match_f_c(dd, mat, CMID, PAR, fs.mr_open, fs.mr_slip)
     assert np.abs(fs.friction_coef - fc).max() < 1e-15
     assert _lib.HFPX_ERR_INVALID == 1
+
+
+def test_cpp_host_mirror_matches_oracle():
+    """hfp3d::match_f_c / update_mrd of the C++ host mirror, through the
+    hfpx3d_fc_check program, against the oracle on the inputs it prints."""
+    import json
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "hfpx3d_amd", "hfpx3d_fc_check")
+    print("case: C++ host mirror of the friction/cohesion laws, 2000 nodes", flush=True)
+    r = subprocess.run([exe, "2000"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    d = json.loads(r.stdout)
+    n = len(d["mat"])
+    dd = np.array(d["dd"]).reshape(n, 3, order="F")
+    mat = np.array(d["mat"], dtype=np.int32)
+    par = np.array([[1.0, 2.0, 0.5, 3.0, 0.6, 0.2], [0.5, 1.5, 1.0, 0.5, 0.8, 0.1], [2.0, 1.0, 2.0, 1.0, 0.7, 0.3]])
+    cmid = np.array([[0, 0], [1, 1], [2, 1]], dtype=np.int32)
+    fc, sc, oc = O.match_f_c(dd, mat, cmid, par, d["mr_open0"], d["mr_slip0"])
+    mo, ms = O.update_mrd(dd, mat, par, d["mr_open0"], d["mr_slip0"])
+    for got, want in ((d["friction_coef"], fc), (d["slip_cohesion"], sc), (d["open_cohesion"], oc),
+                      (d["mr_open"], mo), (d["mr_slip"], ms)):
+        assert np.abs(np.array(got) - want).max() <= 4e-16 * max(1.0, np.abs(want).max())
+    assert (oc > 0).any() and (fc != fc[0]).any()
