@@ -664,9 +664,20 @@ template <typename T, bool LOWER>
 __global__ __launch_bounds__(256) void k_trsm_leaf(const T* __restrict__ Lm, int64_t ldl, T* __restrict__ B,
                                                    int64_t ldb, int kb, int64_t ncols) {
   __shared__ T sL[LB][LB + 1];
-  for (int e = threadIdx.x; e < LB * LB; e += 256) {
-    const int i = e % LB, j = e / LB;
-    sL[i][j] = (i < kb && j < kb) ? Lm[i + (int64_t)j * ldl] : T(0);
+  {
+    // all loads of the triangle in flight before the first LDS store
+    constexpr int LQ = LB * LB / 256;
+    T lv[LQ];
+#pragma unroll
+    for (int q = 0; q < LQ; ++q) {
+      const int e = threadIdx.x + 256 * q, i = e % LB, j = e / LB;
+      lv[q] = (i < kb && j < kb) ? Lm[i + (int64_t)j * ldl] : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < LQ; ++q) {
+      const int e = threadIdx.x + 256 * q;
+      sL[e % LB][e / LB] = lv[q];
+    }
   }
   __syncthreads();
   const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;

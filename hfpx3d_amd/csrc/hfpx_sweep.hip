@@ -125,11 +125,6 @@ __global__ __launch_bounds__(SWT) void k_sweep_flow(const float* __restrict__ LU
     floatx16 acc;
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[v] = 0.f;
-    // the block's inverse, fetched before the waits (off the critical path)
-    const float* iv = inv + blk * SB * SB + i;  // inverse(i, k) at iv[k SB]
-    float ivv[SB / 2];
-#pragma unroll
-    for (int s = 0; s < SB / 2; ++s) ivv[s] = iv[(2 * s + hk) * SB];
     int64_t ready = 0;  // predecessors 0 .. ready-1 (in sweep order) are known final
     for (int64_t p = 0; p < q; ++p) {
       const int64_t c = LOWER ? p : nblk - 1 - p;
@@ -172,9 +167,22 @@ __global__ __launch_bounds__(SWT) void k_sweep_flow(const float* __restrict__ LU
         ready = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32)) << 32) |
                           (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u));
       }
-      for (int e = tid; e < SB * SWN; e += SWT) {
-        const int k = e % SB, j = e / SB;
-        sX[k][j] = (k < kc && j < ncol) ? B[(k0 + k) + (j0 + j) * ldb] : 0.f;
+      {
+        // every load of the predecessor's solution rows in flight before
+        // the first LDS store: one memory latency per predecessor instead of
+        // one per loop trip
+        constexpr int XQ = SB * SWN / SWT;
+        float xq[XQ];
+#pragma unroll
+        for (int q2 = 0; q2 < XQ; ++q2) {
+          const int e = tid + SWT * q2, k = e % SB, j = e / SB;
+          xq[q2] = (k < kc && j < ncol) ? B[(k0 + k) + (j0 + j) * ldb] : 0.f;
+        }
+#pragma unroll
+        for (int q2 = 0; q2 < XQ; ++q2) {
+          const int e = tid + SWT * q2;
+          sX[e % SB][e / SB] = xq[q2];
+        }
       }
       __syncthreads();
 #pragma unroll
@@ -182,7 +190,13 @@ __global__ __launch_bounds__(SWT) void k_sweep_flow(const float* __restrict__ LU
         acc = __builtin_amdgcn_mfma_f32_32x32x2f32(sX[2 * s + hk][32 * jb + l32], lv[s], acc, 0, 0, 0);
       __syncthreads();  // sX is rewritten by the next predecessor
     }
-    // X_i = inverse(i) (B_i - sum)
+    // X_i = inverse(i) (B_i - sum).  The block's inverse is fetched here,
+    // together with B_i, so both share one memory latency (held across the
+    // predecessor loop it would cost the registers that loop's staging uses)
+    const float* iv = inv + blk * SB * SB + i;  // inverse(i, k) at iv[k SB]
+    float ivv[SB / 2];
+#pragma unroll
+    for (int s = 0; s < SB / 2; ++s) ivv[s] = iv[(2 * s + hk) * SB];
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
       const int j = 32 * jb + 8 * (v >> 2) + 4 * hk + (v & 3);
