@@ -65,6 +65,8 @@ SIGNATURES = {
     "hfpx_mesh_destroy": (_c_int, [_c_void_p]),
     "hfpx_mesh_n_dof": (_c_i64, [_c_void_p]),
     "hfpx_mesh_set_dd_local": (_c_int, [_c_void_p, _c_int]),
+    "hfpx_mesh_create_npy": (_c_int, [_c_void_p, ctypes.c_char_p, ctypes.c_char_p, _c_i64, _c_int, _c_double,
+                                      _c_void_p]),
     "hfpx_mesh_set_vertex_weights": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_i64]),
     "hfpx_match_f_c": (_c_int, [_c_void_p, _c_i64, _c_void_p, _c_i64, _c_void_p, _c_void_p, _c_void_p, _c_i64,
                                 _c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_void_p]),

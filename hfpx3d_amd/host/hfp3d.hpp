@@ -207,6 +207,9 @@ void load_mesh_from_numpy_32(const std::string& src_dir, const std::string& conn
 void load_mesh_from_numpy_64(const std::string& src_dir, const std::string& conn_f_name,
                              const std::string& node_f_name, int64_t origin, Mesh_Geom_T& mesh,
                              Status& status);
+// Same from two file paths (no directory join).
+void load_npy_mesh_files(const std::string& conn_path, const std::string& nods_path, int64_t origin,
+                         Mesh_Geom_T& mesh, Status& status);
 void load_mesh_from_csv(const std::string& src_dir, const std::string& conn_f_name,
                         const std::string& node_f_name, int64_t origin, Mesh_Geom_T& mesh, Status& status);
 Array2D<double> load_crd_from_csv(const std::string& src_dir, const std::string& f_name, Status& status);

@@ -434,6 +434,36 @@ static void load_npy_mesh(const std::string& src_dir, const std::string& cf, con
   shift_origin(mesh, origin);
 }
 
+}  // namespace hfp3d
+
+namespace hfpx {
+int set_error(hfpx_ctx* ctx, int code, const char* fmt, ...);
+}
+
+// npy files straight to a device mesh (C ABI): the reference's npy mesh
+// loader, index-origin shift of the vertex rows, DoF handle and upload.
+extern "C" int hfpx_mesh_create_npy(hfpx_ctx* ctx, const char* nodes_npy, const char* elems_npy,
+                                    int64_t index_origin, int tip_type, double beta, hfpx_mesh** mesh) {
+  if (!ctx || !nodes_npy || !elems_npy || !mesh) return HFPX_ERR_INVALID;
+  if (tip_type < 0 || tip_type > 2) return hfpx::set_error(ctx, HFPX_ERR_INVALID, "tip_type %d not in 0..2", tip_type);
+  hfp3d::Mesh_Geom_T m;
+  hfp3d::Status st;
+  hfp3d::load_npy_mesh_files(elems_npy, nodes_npy, index_origin, m, st);
+  if (!st.ok()) return hfpx::set_error(ctx, HFPX_ERR_INVALID, "%s", st.error.c_str());
+  if (m.nods.size(0) < 3 || m.conn.size(0) < 3 || m.conn.size(1) < 1)
+    return hfpx::set_error(ctx, HFPX_ERR_INVALID, "mesh needs >= 3 coordinate rows and >= 3 connectivity rows");
+  const hfp3d::DoF_Handle_T d = hfp3d::make_dof_h_crack(m, 2, tip_type);
+  return hfpx_mesh_create(ctx, m.nods.data(), m.nods.size(0), m.nods.size(1), m.conn.data(), m.conn.size(0),
+                          m.conn.size(1), d.dof_h.data(), d.n_dof, beta, mesh);
+}
+
+namespace hfp3d {
+
+void load_npy_mesh_files(const std::string& conn_path, const std::string& nods_path, int64_t origin,
+                         Mesh_Geom_T& mesh, Status& status) {
+  load_npy_mesh("", conn_path, nods_path, origin, mesh, status);
+}
+
 void load_mesh_from_numpy_32(const std::string& d, const std::string& cf, const std::string& nf, int64_t origin,
                              Mesh_Geom_T& mesh, Status& status) {
   load_npy_mesh(d, cf, nf, origin, mesh, status);

@@ -68,6 +68,17 @@ int hfpx_mesh_create(hfpx_ctx* ctx, const double* nods_host, int64_t nods_rows,
                      int64_t n_el, const int64_t* dof_h_host, int64_t n_dof,
                      double beta, hfpx_mesh** mesh);
 int hfpx_mesh_destroy(hfpx_mesh* mesh);
+/* A device mesh straight from the reference's npy mesh files: nodes_npy
+ * ((3 or 6) x n_nodes, <f8 or <f4, either memory order) and elems_npy
+ * ((3 or 5) x n_el, <i4 or <i8); index_origin is subtracted from the
+ * vertex rows (1 for the reference's Mesh_Files); the DoF handle is built
+ * for tip_type 0..2 (make_dof_h_crack) and beta is the collocation offset.
+ * hfpx_mesh_n_dof gives the DoF count.  Replaces load_mesh_from_numpy_32 /
+ * _64 (src/IO/mesh_file_io.cpp:186-225, declared mesh_file_io.h:53-70) followed
+ * by make_dof_h_crack and the upload; a missing or malformed file is
+ * HFPX_ERR_INVALID with the reason in hfpx_last_error. */
+int hfpx_mesh_create_npy(hfpx_ctx* ctx, const char* nodes_npy, const char* elems_npy, int64_t index_origin,
+                         int tip_type, double beta, hfpx_mesh** mesh);
 /* DD in each element's own frame (1) or global (0, the default): the
  * assembly's columns, the volume-control row and the DD given to
  * hfpx_stress follow it.  Replaces Num_Param_T::is_dd_local
