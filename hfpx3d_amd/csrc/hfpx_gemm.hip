@@ -421,16 +421,21 @@ __global__ __launch_bounds__(256) void k_dgemm_stream(double* __restrict__ P, co
     for (int cb = 0; cb < NB; ++cb) acc[rb][cb] = doublex4{0, 0, 0, 0};
   const int64_t r_a = row0 + l16, r_b = row0 + 16 + l16;
   const bool ok_a = r_a < M, ok_b = r_b < M;
-  // next chunk's operands, fetched one chunk ahead
-  double an[2][4], xn[XQ];
-  auto fetch = [&](int64_t k0) {
+  // A is fetched two chunks ahead into two static register sets (the loop
+  // is unrolled by two so each set keeps its registers), the X slice one
+  // chunk ahead; X is issued before A so that waiting for it never waits
+  // for the younger A loads
+  double a0[2][4], a1[2][4], xn[XQ];
+  auto fetch_a = [&](double (&dst)[2][4], int64_t k0) {
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       const int64_t k = k0 + 4 * s + lq;
       const bool kin = k < ke;
-      an[0][s] = (ok_a && kin) ? A[r_a + k * lda] : 0.0;
-      an[1][s] = (ok_b && kin) ? A[r_b + k * lda] : 0.0;
+      dst[0][s] = (ok_a && kin) ? A[r_a + k * lda] : 0.0;
+      dst[1][s] = (ok_b && kin) ? A[r_b + k * lda] : 0.0;
     }
+  };
+  auto fetch_x = [&](int64_t k0) {
 #pragma unroll
     for (int q = 0; q < XQ; ++q) {
       const int e = tid + 256 * q, kk = e % KC2, j = e / KC2;
@@ -438,14 +443,16 @@ __global__ __launch_bounds__(256) void k_dgemm_stream(double* __restrict__ P, co
       xn[q] = (k < ke && j < N) ? X[k + j * ldx] : 0.0;
     }
   };
-  fetch(kb);
+  fetch_x(kb);
+  fetch_a(a0, kb);
+  if (kb + KC2 < ke) fetch_a(a1, kb + KC2);
   int buf = 0;
-  for (int64_t k0 = kb; k0 < ke; k0 += KC2, buf ^= 1) {
+  auto step = [&](double (&cur)[2][4], int64_t k0) {
     double av[2][4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-      av[0][s] = an[0][s];
-      av[1][s] = an[1][s];
+      av[0][s] = cur[0][s];
+      av[1][s] = cur[1][s];
     }
 #pragma unroll
     for (int q = 0; q < XQ; ++q) {
@@ -453,7 +460,8 @@ __global__ __launch_bounds__(256) void k_dgemm_stream(double* __restrict__ P, co
       sX[buf][e % KC2][e / KC2] = xn[q];
     }
     __syncthreads();  // one barrier per chunk: sX is double-buffered
-    if (k0 + KC2 < ke) fetch(k0 + KC2);
+    if (k0 + KC2 < ke) fetch_x(k0 + KC2);
+    if (k0 + 2 * KC2 < ke) fetch_a(cur, k0 + 2 * KC2);
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -463,6 +471,11 @@ __global__ __launch_bounds__(256) void k_dgemm_stream(double* __restrict__ P, co
         for (int rb = 0; rb < 2; ++rb)
           acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[rb][s], xv, acc[rb][cb], 0, 0, 0);
       }
+    buf ^= 1;
+  };
+  for (int64_t k0 = kb; k0 < ke; k0 += 2 * KC2) {
+    step(a0, k0);
+    if (k0 + KC2 < ke) step(a1, k0 + KC2);
   }
   double* Pz = P + z * M * N;
 #pragma unroll
