@@ -133,7 +133,7 @@ def cpu_baseline(n_el_sample, G, nu):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+        print(json.dumps({"error": "hfpx3d_amd runs on one GPU; --gpus must be 1 "
                           "(multi-GPU sharding is design-only)"}))
         sys.exit(2)
     if args.workload == "wavy108k":
