@@ -776,3 +776,6 @@ def run_wavy(args):
 
 if __name__ == "__main__":
     main()
+    # free the library's contexts while torch and the HIP runtime are up
+    if "hfpx3d_amd.hfp3d" in sys.modules:
+        sys.modules["hfpx3d_amd.hfp3d"].close_all()

@@ -23,6 +23,8 @@ KNOB_LU_NB = 1
 KNOB_BGEMM_F32 = 2
 KNOB_PANEL = 3
 KNOB_X3_BAND = 4
+KNOB_IR_FALLBACK = 5
+KNOB_TEST_PANEL_TIMEOUT = 6
 GEMM_F32_VARIANTS = 15
 
 KERNEL_CLASSES = {"assemble": 0, "gemm_f64": 1, "gemm_f32": 2, "panel": 3, "laswp": 4, "trsm": 5, "stress": 6,
@@ -105,6 +107,8 @@ SIGNATURES = {
     "hfpx_batch_rhs": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_void_p]),
     "hfpx_batch_matrix": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_void_p, ctypes.POINTER(_c_i64)]),
     "hfpx_tune_get": (_c_int, [_c_void_p, _c_int, ctypes.POINTER(_c_int)]),
+    "hfpx_local_submatrix": (_c_int, [_c_void_p, _c_int, _c_double, _c_double, _c_i64, _c_void_p, _c_void_p,
+                                      _c_void_p, _c_void_p, _c_void_p]),
 }
 
 _lib = None

@@ -28,8 +28,7 @@ CFLAGS = [
 ]
 
 
-EXES = {"hfpx3d_main": "hfpx3d_main.cpp", "hfpx3d_host_check": "hfpx3d_host_check.cpp",
-        "hfpx3d_fc_check": "hfpx3d_fc_check.cpp"}
+EXES = {"hfpx3d_main": "hfpx3d_main.cpp"}
 
 
 def _sources():

@@ -274,6 +274,48 @@ __global__ void k_rhs(const SrcElem<double>* __restrict__ elem, int64_t n_el,
 }
 
 // ---------------------------------------------------------------------
+// the element integral alone, on the reference's own arguments: one thread
+// per case (h, z, tau, sfm) in the source element's frame -> the 6 x 18
+// stress block (rows S11 S22 S33 S12 S13 S23, column 3 node + DD component).
+// tau: vertex coordinates (re, im) x 3; sfm: 6 x 6 complex column-major
+// (re, im interleaved), row = shape function, column = monomial 1, tau,
+// conj(tau), tau^2, conj(tau)^2, tau conj(tau), re-centred at z by the
+// binomial shift of shift_el_sfm (element_utilities.cpp:339-360).
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_local_submatrix(double shear_m, double nu, int64_t n_cases,
+                                                        const double* __restrict__ h, const double* __restrict__ z,
+                                                        const double* __restrict__ tau,
+                                                        const double* __restrict__ sfm, double* __restrict__ out) {
+  const int64_t c = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (c >= n_cases) return;
+  SrcElem<double> E;  // the integral reads the edge data only
+  for (int k = 0; k < 6; ++k) E.t[k] = tau[6 * c + k];
+  for (int m = 0; m < 3; ++m) {
+    const int q = (m + 1) % 3;
+    const double dx = E.t[2 * q] - E.t[2 * m], dy = E.t[2 * q + 1] - E.t[2 * m + 1];
+    const double L = sqrt(dx * dx + dy * dy);
+    E.len[m] = L;
+    E.th[2 * m] = dx / L;
+    E.th[2 * m + 1] = dy / L;
+  }
+  const cx<double> zz = mk(z[2 * c], z[2 * c + 1]), zb = conj(zz);
+  const double* q = sfm + 72 * c;
+  auto dens = [&](int j, double& p00, cx<double>& p10, cx<double>& p20, double& p11) {
+    cx<double> a[6];
+    for (int col = 0; col < 6; ++col) a[col] = mk(q[2 * (j + 6 * col)], q[2 * (j + 6 * col) + 1]);
+    p00 = (a[0] + a[1] * zz + a[2] * zb + a[3] * (zz * zz) + a[4] * (zb * zb) + a[5] * (zz * zb)).re;
+    p10 = a[1] + 2.0 * (a[3] * zz) + a[5] * zb;
+    p20 = a[3];
+    p11 = a[5].re;
+  };
+  double* o = out + 108 * c;
+  pair_influence_dens<double>(E, zz.re, zz.im, -h[c], shear_m, nu, dens, [&](int j, const double (&S)[6][3]) {
+    for (int k = 0; k < 3; ++k)
+      for (int r = 0; r < 6; ++r) o[r + 6 * (3 * j + k)] = S[r][k];
+  });
+}
+
+// ---------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------
 int launch_geometry(hfpx_ctx* ctx, hfpx_mesh* m) {
@@ -302,6 +344,16 @@ int launch_assemble(hfpx_ctx* ctx, const hfpx_mesh* m, double shear_m, double po
   // algorithmic HBM bytes: every matrix entry written once (FP64 and/or FP32)
   const double entries = (double)m->n_dof * (double)m->n_dof;
   prof_end(ctx, tok, HFPX_KC_ASSEMBLE, 0.0, entries * ((a64 ? 8.0 : 0.0) + (a32 ? 4.0 : 0.0)));
+  HFPX_TRY_HIP(ctx, hipGetLastError());
+  return HFPX_OK;
+}
+
+int launch_local_submatrix(hfpx_ctx* ctx, double shear_m, double poiss_r, int64_t n_cases, const double* h,
+                           const double* z, const double* tau, const double* sfm, double* out) {
+  const int64_t nb = (n_cases + 63) / 64;
+  if (nb > 0x7fffffff) return set_error(ctx, HFPX_ERR_INVALID, "local submatrix: too many cases");
+  hipLaunchKernelGGL(k_local_submatrix, dim3((unsigned)nb), dim3(64), 0, ctx->stream, shear_m, poiss_r, n_cases, h,
+                     z, tau, sfm, out);
   HFPX_TRY_HIP(ctx, hipGetLastError());
   return HFPX_OK;
 }

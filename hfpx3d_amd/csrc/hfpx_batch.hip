@@ -399,22 +399,22 @@ __global__ __launch_bounds__(256) void k_bberr(const BCrack* __restrict__ cr, co
   double m[4] = {0, 0, 0, 0};
   for (int64_t i = threadIdx.x; i < c.n; i += 256) {
     const int64_t o = c.x_off + i;
-    m[0] = fmax(m[0], fabs(r[o]));
-    m[1] = fmax(m[1], rs[o]);
-    m[2] = fmax(m[2], fabs(x[o]));
-    m[3] = fmax(m[3], fabs(b[o]));
+    m[0] = nan_max(m[0], fabs(r[o]));
+    m[1] = nan_max(m[1], rs[o]);
+    m[2] = nan_max(m[2], fabs(x[o]));
+    m[3] = nan_max(m[3], fabs(b[o]));
   }
   for (int q = 0; q < 4; ++q)
-    for (int off = 32; off > 0; off >>= 1) m[q] = fmax(m[q], __shfl_xor(m[q], off));
+    for (int off = 32; off > 0; off >>= 1) m[q] = nan_max(m[q], __shfl_xor(m[q], off));
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (lane == 0)
     for (int q = 0; q < 4; ++q) red[wave][q] = m[q];
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < 4; ++w)
-      for (int q = 0; q < 4; ++q) m[q] = fmax(m[q], red[w][q]);
+      for (int q = 0; q < 4; ++q) m[q] = nan_max(m[q], red[w][q]);
     const double den = m[1] * m[2] + m[3];
-    berr[blockIdx.x] = den > 0 ? m[0] / den : 0.0;
+    berr[blockIdx.x] = (den > 0 || den != den) ? m[0] / den : (m[0] == 0.0 ? 0.0 : m[0] / den);
   }
 }
 
@@ -646,6 +646,13 @@ int hfpx_batch_create(hfpx_ctx* ctx, int n_cracks, const int64_t* n_nodes, const
           return set_error(ctx, HFPX_ERR_INVALID, "batch_create: crack %d element %lld: node %lld out of range", i,
                            (long long)(e - el_in[i]), (long long)id);
       }
+  for (int i = 0; i < n_cracks; ++i)
+    for (int64_t e = el_in[i]; e < el_in[i + 1]; ++e) {
+      const double* nb = nods_host + 3 * node_off[i];
+      if (degenerate_triangle(nb + 3 * conn_host[3 * e], nb + 3 * conn_host[3 * e + 1], nb + 3 * conn_host[3 * e + 2]))
+        return set_error(ctx, HFPX_ERR_INVALID, "batch_create: crack %d element %lld is degenerate", i,
+                         (long long)(e - el_in[i]));
+    }
   hfpx_batch* bt = new hfpx_batch();
   bt->ncr = n_cracks;
   std::vector<int> order;

@@ -1,4 +1,5 @@
 // hfpx_capi.hip -- extern "C" entry points: context, mesh, assembly, RHS.
+#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <cstdarg>
@@ -147,7 +148,9 @@ int hfpx_create(hfpx_ctx** out, int device, void* stream) {
 int hfpx_destroy(hfpx_ctx* ctx) {
   if (!ctx) return HFPX_OK;
   (void)hipSetDevice(ctx->device);
-  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  // a borrowed stream belongs to the caller, who finishes its work before
+  // destroying the context; hipFree below waits for the device either way
+  if (ctx->own_stream && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->work) (void)hipFree(ctx->work);
   if (ctx->lu_work) (void)hipFree(ctx->lu_work);
   if (ctx->gemm_work) (void)hipFree(ctx->gemm_work);
@@ -195,6 +198,11 @@ int hfpx_mesh_create(hfpx_ctx* ctx, const double* nods_host, int64_t nods_rows, 
                          (long long)e, (long long)v);
       cn[3 * e + k] = (int32_t)v;
     }
+  // degenerate elements have no frame: the geometry would fill the matrix
+  // with inf / NaN
+  for (int64_t e = 0; e < n_el; ++e)
+    if (degenerate_triangle(&xyz[3 * cn[3 * e]], &xyz[3 * cn[3 * e + 1]], &xyz[3 * cn[3 * e + 2]]))
+      return set_error(ctx, HFPX_ERR_INVALID, "element %lld is degenerate (zero area or a needle)", (long long)e);
   std::vector<int32_t> dh;
   bool identity = (dof_h_host == nullptr);
   if (identity) {
@@ -308,6 +316,20 @@ int hfpx_assemble_mixed(hfpx_ctx* ctx, hfpx_mesh* m, double shear_m, double pois
   int rc = launch_geometry(ctx, m);
   if (rc) return rc;
   return launch_assemble(ctx, m, shear_m, poiss_r, A64, lda64, A32, lda32);
+}
+
+int hfpx_local_submatrix(hfpx_ctx* ctx, int kernel_id, double shear_m, double poiss_r, int64_t n_cases,
+                         const double* h, const double* z, const double* tau, const double* sfm, double* out) {
+  if (!ctx) return HFPX_ERR_INVALID;
+  if (kernel_id != 1)
+    return set_error(ctx, HFPX_ERR_INVALID, "local submatrix: kernel_id %d (only 1, the hypersingular kernel)",
+                     kernel_id);
+  if (n_cases < 1 || !h || !z || !tau || !sfm || !out)
+    return set_error(ctx, HFPX_ERR_INVALID, "local submatrix: need n_cases >= 1 and device arrays");
+  if (!(poiss_r < 1.0) || !(shear_m > 0.0))
+    return set_error(ctx, HFPX_ERR_INVALID, "need shear modulus > 0 and Poisson ratio < 1");
+  HFPX_TRY_HIP(ctx, hipSetDevice(ctx->device));
+  return launch_local_submatrix(ctx, shear_m, poiss_r, n_cases, h, z, tau, sfm, out);
 }
 
 int hfpx_rhs_sinf(hfpx_ctx* ctx, hfpx_mesh* m, const double* sinf_host, int nrhs, double* B,
@@ -436,6 +458,16 @@ int hfpx_tune_set(hfpx_ctx* ctx, int knob, int value) {
   }
   if (knob == HFPX_KNOB_PANEL) {
     if (value < 0 || value > 2) return set_error(ctx, HFPX_ERR_INVALID, "tune: PANEL must be 0, 1 or 2");
+    ctx->knob[knob] = value;
+    return HFPX_OK;
+  }
+  if (knob == HFPX_KNOB_IR_FALLBACK) {
+    if (value < 0 || value > 2) return set_error(ctx, HFPX_ERR_INVALID, "tune: IR_FALLBACK must be 0, 1 or 2");
+    ctx->knob[knob] = value;
+    return HFPX_OK;
+  }
+  if (knob == HFPX_KNOB_TEST_PANEL_TIMEOUT) {
+    if (value < 0 || value > 1) return set_error(ctx, HFPX_ERR_INVALID, "tune: TEST_PANEL_TIMEOUT must be 0 or 1");
     ctx->knob[knob] = value;
     return HFPX_OK;
   }

@@ -158,6 +158,29 @@ int prof_begin(hfpx_ctx* ctx, int cls);
 int launch_vc_border(hfpx_ctx* ctx, const hfpx_mesh* m, double* a64, int64_t lda64, float* a32, int64_t lda32);
 void prof_end(hfpx_ctx* ctx, int token, int cls, double flops, double bytes);
 
+// max that keeps NaN: a NaN entry makes the whole reduction NaN, so a
+// non-finite residual or iterate cannot look converged (fmax drops NaN)
+__host__ __device__ __forceinline__ double nan_max(double a, double b) {
+  return (a != a) ? a : ((b != b || b > a) ? b : a);
+}
+
+// A triangle with no usable frame: twice its area at most 1e-12 of its
+// longest edge squared (zero-area, needle or repeated vertices), or any
+// non-finite value.  Such an element would divide by zero in the geometry.
+inline bool degenerate_triangle(const double* p0, const double* p1, const double* p2) {
+  double a[3], b[3], c[3];
+  for (int k = 0; k < 3; ++k) {
+    a[k] = p1[k] - p0[k];
+    b[k] = p2[k] - p0[k];
+    c[k] = p2[k] - p1[k];
+  }
+  const double cx = a[1] * b[2] - a[2] * b[1], cy = a[2] * b[0] - a[0] * b[2], cz = a[0] * b[1] - a[1] * b[0];
+  const double area2 = __builtin_sqrt(cx * cx + cy * cy + cz * cz);
+  const double l2 = std::max({a[0] * a[0] + a[1] * a[1] + a[2] * a[2], b[0] * b[0] + b[1] * b[1] + b[2] * b[2],
+                              c[0] * c[0] + c[1] * c[1] + c[2] * c[2]});
+  return !(area2 > 1e-12 * l2);
+}
+
 int set_error(hfpx_ctx* ctx, int code, const char* fmt, ...);
 int check_hip(hfpx_ctx* ctx, hipError_t e, const char* what);
 int ensure_work(hfpx_ctx* ctx, size_t bytes);
@@ -169,6 +192,9 @@ int launch_assemble(hfpx_ctx* ctx, const hfpx_mesh* m, double shear_m, double po
                     double* a64, int64_t lda64, float* a32, int64_t lda32);
 int launch_rhs(hfpx_ctx* ctx, const hfpx_mesh* m, const double* sinf_dev, int nrhs,
                double* b, int64_t ldb);
+// the element integral on (h, z, tau, sfm) cases (device arrays)
+int launch_local_submatrix(hfpx_ctx* ctx, double shear_m, double poiss_r, int64_t n_cases, const double* h,
+                           const double* z, const double* tau, const double* sfm, double* out);
 
 // dense linear algebra (hfpx_lu.hip)
 int lu_getrf(hfpx_ctx* ctx, int dtype, void* A, int64_t lda, int64_t n, int32_t* ipiv, int64_t* info);

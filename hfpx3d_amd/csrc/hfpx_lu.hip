@@ -41,8 +41,15 @@ __device__ __forceinline__ void fence_agent_acquire() {
 // then arrives and polls with relaxed atomics and ends with an agent-scope
 // acquire fence, which the closing workgroup barrier extends to the other
 // waves.  The wait is bounded: after ~2^24 polls thread 0 raises *err and
-// leaves (co-residency, guaranteed by the launch, makes this unreachable).
+// leaves (co-residency, guaranteed by the launch, makes this unreachable),
+// and every 256 polls it looks at *err, so that once one wait has failed
+// every other wait of the factorisation gives up at once instead of
+// running to its own bound.
 // ---------------------------------------------------------------------
+__device__ __forceinline__ bool lu_failed(const int32_t* err) {
+  return __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+
 __device__ __forceinline__ void grid_barrier(uint64_t* ctr, uint64_t target, int32_t* err) {
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -51,7 +58,9 @@ __device__ __forceinline__ void grid_barrier(uint64_t* ctr, uint64_t target, int
     uint32_t polls = 0;
     while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
       __builtin_amdgcn_s_sleep(1);
-      if (++polls > (1u << 24)) {
+      ++polls;
+      if ((polls & 255u) == 0 && lu_failed(err)) break;
+      if (polls > (1u << 24)) {
         __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         break;
       }
@@ -59,6 +68,15 @@ __device__ __forceinline__ void grid_barrier(uint64_t* ctr, uint64_t target, int
     fence_agent_acquire();
   }
   __syncthreads();
+}
+
+// A panel launched after a wait of the same factorisation has failed does
+// no work: it records identity interchanges (so later launches stay
+// well-defined) and returns; the host reports the failure.
+__device__ __forceinline__ bool panel_skip(const int32_t* err, int64_t j0, int jb, int32_t* ipiv) {
+  if (!lu_failed(err)) return false;
+  if (blockIdx.x == 0 && (int)threadIdx.x < jb) ipiv[j0 + threadIdx.x] = (int32_t)(j0 + threadIdx.x + 1);
+  return true;
 }
 
 // ---------------------------------------------------------------------
@@ -287,6 +305,7 @@ __global__ __launch_bounds__(PWG) void k_panel(T* __restrict__ A, int64_t lda, i
                                                int32_t* __restrict__ err, PanelScratch<T>* __restrict__ ps,
                                                T* __restrict__ rows /* [2][nwg+1][IB] */, uint64_t* ctr,
                                                uint64_t base) {
+  if (panel_skip(err, j0, jb, ipiv)) return;
   const int nwg = gridDim.x, w = blockIdx.x, t = threadIdx.x;
   const int64_t m = n - j0;
   const int64_t r = (int64_t)w * PWG + t;
@@ -367,7 +386,9 @@ __device__ __forceinline__ uint32_t px_get(uint64_t* p, uint32_t seq, int32_t* e
   uint32_t polls = 0;
   while ((uint32_t)(v >> 32) != seq) {
     __builtin_amdgcn_s_sleep(1);
-    if (++polls > (1u << 22)) {
+    ++polls;
+    if ((polls & 255u) == 0 && lu_failed(err)) break;
+    if (polls > (1u << 22)) {
       __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       break;
     }
@@ -502,6 +523,7 @@ __global__ __launch_bounds__(PWG) void k_panel_x(T* __restrict__ A, int64_t lda,
                                                  int32_t* __restrict__ err, uint64_t* __restrict__ xw) {
   typedef PxFmt<T, IB> F;
   static_assert(PX_MAX_WG * (F::W + 1) <= PX_ROWJ, "polling threads overlap the publishing waves");
+  if (panel_skip(err, j0, jb, ipiv)) return;
   const int nwg = gridDim.x, w = blockIdx.x, t = threadIdx.x;
   const int64_t m = n - j0;
   const int64_t r = (int64_t)w * PWG + t;
@@ -1010,6 +1032,9 @@ static int getrf_impl(hfpx_ctx* ctx, T* A, int64_t lda, int64_t n, int32_t* ipiv
   W.pp = (PermPairs*)(w + off_pp);
   W.xw = (uint64_t*)(w + off_xw);
   HFPX_TRY_HIP(ctx, hipMemsetAsync(W.dflags, 0, 2 * sizeof(int32_t), ctx->stream));
+  // host-forced test of the time-out path: the flag starts raised
+  if (ctx->knob[HFPX_KNOB_TEST_PANEL_TIMEOUT])
+    HFPX_TRY_HIP(ctx, hipMemsetD32Async((hipDeviceptr_t)(W.dflags + 1), 1, 1, ctx->stream));
   HFPX_TRY_HIP(ctx, hipMemsetAsync(W.xw, 0, xw_bytes, ctx->stream));
   HFPX_TRY_HIP(ctx, hipMemsetAsync(ctx->sync, 0, sizeof(uint64_t), ctx->stream));
   ctx->sync_base = 0;
@@ -1031,7 +1056,7 @@ static int getrf_impl(hfpx_ctx* ctx, T* A, int64_t lda, int64_t n, int32_t* ipiv
   int32_t hflags[2] = {0, 0};
   HFPX_TRY_HIP(ctx, hipMemcpyAsync(hflags, W.dflags, sizeof(hflags), hipMemcpyDeviceToHost, ctx->stream));
   HFPX_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  if (hflags[1]) return set_error(ctx, HFPX_ERR_HIP, "getrf: panel grid barrier timed out");
+  if (hflags[1]) return set_error(ctx, HFPX_ERR_HIP, "getrf: a panel wait timed out (factors are invalid)");
   if (info_out) *info_out = hflags[0];
   return hflags[0] ? HFPX_ERR_SINGULAR : HFPX_OK;
 }

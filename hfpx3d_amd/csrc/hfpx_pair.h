@@ -136,10 +136,6 @@ __device__ __forceinline__ T solid_end(T a, T s, T ax3) {
   return atan(a * s * (a * a + s * s) / ((r + ax3) * (s * s * r + ax3 * a * a)));
 }
 
-// Stress influence at a point given in the source frame (z = y0 + i y1,
-// x3 = y2).  Calls sink(node, S_local[6][3]) once per node with
-// S_local[c][k] = stress component c (S11 S22 S33 S12 S13 S23) due to unit
-// DD component k (source frame) at the node.
 // Adds s L_a L_b to the re-centred density P = p00 + 2 Re(p10 w) +
 // 2 Re(p20 w^2) + p11 |w|^2, with L_k(w) = lam_k + 2 Re(g_k w).
 template <typename T>
@@ -151,11 +147,17 @@ __device__ __forceinline__ void add_bary_product(T s, T la, cx<T> ga, T lb, cx<T
   p11 += T(2) * s * (ga.re * gb.re + ga.im * gb.im);
 }
 
-// NU: the element's shape functions are those of a non-uniform edge
-// partition (SrcElem::sw); the uniform element is the default variant.
-template <typename T, bool NU = false, typename Sink>
-__device__ __forceinline__ void pair_influence(const SrcElem<T>& E, T y0, T y1, T x3,
-                                               T shear_m, T nu, Sink&& sink) {
+// Stress influence at a point given in the source frame (z = y0 + i y1,
+// x3 = y2).  Calls sink(node, S_local[6][3]) once per node with
+// S_local[c][k] = stress component c (S11 S22 S33 S12 S13 S23) due to unit
+// DD component k (source frame) at the node.
+// This core takes any quadratic density: dens(j, p00, p10, p20, p11) gives
+// shape function j re-centred at the projected point z, as
+// P = p00 + 2 Re(p10 w) + 2 Re(p20 w^2) + p11 |w|^2 (w = xi - z).  It reads
+// only the edge data of E (t, th, len).
+template <typename T, typename Dens, typename Sink>
+__device__ __forceinline__ void pair_influence_dens(const SrcElem<T>& E, T y0, T y1, T x3, T shear_m, T nu,
+                                                    Dens&& dens, Sink&& sink) {
   const T ax3 = fabs(x3);
   const T sgn3 = x3 > T(0) ? T(1) : (x3 < T(0) ? T(-1) : T(0));
   EdgeMom<T> em[3];
@@ -182,50 +184,13 @@ __device__ __forceinline__ void pair_influence(const SrcElem<T>& E, T y0, T y1, 
   }
   phi1 += x3 * w1;
 
-  // barycentric coordinates at the projected point
-  T lam[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) lam[k] = E.l0[k] + T(2) * (E.l1[2 * k] * y0 - E.l1[2 * k + 1] * y1);
-
   const T K = shear_m / (T(16) * T(3.14159265358979323846) * (T(1) - nu));
 #pragma unroll 1
   for (int j = 0; j < 6; ++j) {
     // density re-centred at z: P = p00 + 2 Re(p10 w) + 2 Re(p20 w^2) + p11 |w|^2
     T p00, p11;
     cx<T> p10, p20;
-    if constexpr (NU) {
-      p00 = T(0);
-      p11 = T(0);
-      p10 = mk<T>(0, 0);
-      p20 = mk<T>(0, 0);
-      const int v = j < 3 ? j : j - 3;
-      const int a = (v + 1) % 3, b = (v + 2) % 3;
-      const cx<T> gv = mk<T>(E.l1[2 * v], E.l1[2 * v + 1]);
-      const cx<T> ga = mk<T>(E.l1[2 * a], E.l1[2 * a + 1]);
-      const cx<T> gb = mk<T>(E.l1[2 * b], E.l1[2 * b + 1]);
-      if (j < 3) {
-        add_bary_product<T>(T(1), lam[v], gv, lam[v], gv, p00, p10, p20, p11);
-        add_bary_product<T>(-E.sw[2 * v], lam[v], gv, lam[a], ga, p00, p10, p20, p11);
-        add_bary_product<T>(-E.sw[2 * v + 1], lam[v], gv, lam[b], gb, p00, p10, p20, p11);
-      } else {
-        add_bary_product<T>(E.sw[6 + v], lam[a], ga, lam[b], gb, p00, p10, p20, p11);
-      }
-    } else if (j < 3) {
-      const T l = lam[j];
-      const cx<T> g = mk<T>(E.l1[2 * j], E.l1[2 * j + 1]);
-      p00 = T(2) * l * l - l;
-      p10 = (T(4) * l - T(1)) * g;
-      p20 = T(2) * (g * g);
-      p11 = T(4) * (g.re * g.re + g.im * g.im);
-    } else {
-      const int a = (j - 3 + 1) % 3, b = (j - 3 + 2) % 3;
-      const cx<T> ga = mk<T>(E.l1[2 * a], E.l1[2 * a + 1]);
-      const cx<T> gb = mk<T>(E.l1[2 * b], E.l1[2 * b + 1]);
-      p00 = T(4) * lam[a] * lam[b];
-      p10 = T(4) * (lam[a] * gb + lam[b] * ga);
-      p20 = T(4) * (ga * gb);
-      p11 = T(8) * (ga.re * gb.re + ga.im * gb.im);
-    }
+    dens(j, p00, p10, p20, p11);
     const cx<T> p01 = conj(p10), p02 = conj(p20);
     T A = T(0), G = T(0);
     cx<T> B = mk<T>(0, 0), C = mk<T>(0, 0), D = mk<T>(0, 0), F = mk<T>(0, 0), H = mk<T>(0, 0);
@@ -318,6 +283,55 @@ __device__ __forceinline__ void pair_influence(const SrcElem<T>& E, T y0, T y1, 
     }
     sink(j, S);
   }
+}
+
+// The 6-node quadratic element of E: uniform (mid-edge nodes) or, with NU,
+// the non-uniform edge partition of SrcElem::sw, through the barycentric
+// coordinates L_k(w) = l0_k + 2 Re(l1_k w) at the projected point.
+template <typename T, bool NU = false, typename Sink>
+__device__ __forceinline__ void pair_influence(const SrcElem<T>& E, T y0, T y1, T x3,
+                                               T shear_m, T nu, Sink&& sink) {
+  auto dens = [&](int j, T& p00, cx<T>& p10, cx<T>& p20, T& p11) {
+    // barycentric coordinates at the projected point (loop-invariant: the
+    // compiler evaluates them once, after the edge terms)
+    T lam[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) lam[k] = E.l0[k] + T(2) * (E.l1[2 * k] * y0 - E.l1[2 * k + 1] * y1);
+    if constexpr (NU) {
+      p00 = T(0);
+      p11 = T(0);
+      p10 = mk<T>(0, 0);
+      p20 = mk<T>(0, 0);
+      const int v = j < 3 ? j : j - 3;
+      const int a = (v + 1) % 3, b = (v + 2) % 3;
+      const cx<T> gv = mk<T>(E.l1[2 * v], E.l1[2 * v + 1]);
+      const cx<T> ga = mk<T>(E.l1[2 * a], E.l1[2 * a + 1]);
+      const cx<T> gb = mk<T>(E.l1[2 * b], E.l1[2 * b + 1]);
+      if (j < 3) {
+        add_bary_product<T>(T(1), lam[v], gv, lam[v], gv, p00, p10, p20, p11);
+        add_bary_product<T>(-E.sw[2 * v], lam[v], gv, lam[a], ga, p00, p10, p20, p11);
+        add_bary_product<T>(-E.sw[2 * v + 1], lam[v], gv, lam[b], gb, p00, p10, p20, p11);
+      } else {
+        add_bary_product<T>(E.sw[6 + v], lam[a], ga, lam[b], gb, p00, p10, p20, p11);
+      }
+    } else if (j < 3) {
+      const T l = lam[j];
+      const cx<T> g = mk<T>(E.l1[2 * j], E.l1[2 * j + 1]);
+      p00 = T(2) * l * l - l;
+      p10 = (T(4) * l - T(1)) * g;
+      p20 = T(2) * (g * g);
+      p11 = T(4) * (g.re * g.re + g.im * g.im);
+    } else {
+      const int a = (j - 3 + 1) % 3, b = (j - 3 + 2) % 3;
+      const cx<T> ga = mk<T>(E.l1[2 * a], E.l1[2 * a + 1]);
+      const cx<T> gb = mk<T>(E.l1[2 * b], E.l1[2 * b + 1]);
+      p00 = T(4) * lam[a] * lam[b];
+      p10 = T(4) * (lam[a] * gb + lam[b] * ga);
+      p20 = T(4) * (ga * gb);
+      p11 = T(8) * (ga.re * gb.re + ga.im * gb.im);
+    }
+  };
+  pair_influence_dens<T>(E, y0, y1, x3, shear_m, nu, dens, sink);
 }
 
 }  // namespace hfpx

@@ -14,6 +14,11 @@
 // All right-hand sides advance together: every matrix product is one
 // n x nrhs block product (the FP64 matrix and the FP32 factors are streamed
 // once per sweep for all columns).
+// Failure: an outer step that does not lower the backward error by 10 %, a
+// non-finite iterate (an FP32 zero pivot, FP32 overflow) or max_outer steps
+// without convergence end the refinement; the solution is then recomputed
+// by an FP64 LU of a copy of A (report fell_back = 1) unless
+// HFPX_KNOB_IR_FALLBACK is 2.  Non-finite A or B is reported as is.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -56,7 +61,7 @@ __global__ __launch_bounds__(RB) void k_colreduce(const double* __restrict__ x, 
   __syncthreads();
   for (int w = RB / 2; w > 0; w >>= 1) {
     if (threadIdx.x < w) s[threadIdx.x] = (op == 0) ? s[threadIdx.x] + s[threadIdx.x + w]
-                                                    : fmax(s[threadIdx.x], s[threadIdx.x + w]);
+                                                    : nan_max(s[threadIdx.x], s[threadIdx.x + w]);
     __syncthreads();
   }
   if (threadIdx.x == 0) part[blockIdx.x + (int64_t)gridDim.x * j] = s[0];
@@ -67,7 +72,7 @@ __global__ void k_colreduce2(const double* __restrict__ part, int64_t nb, int64_
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= ncol) return;
   double v = 0.0;
-  for (int64_t b = 0; b < nb; ++b) v = (op == 0) ? v + part[b + nb * j] : fmax(v, part[b + nb * j]);
+  for (int64_t b = 0; b < nb; ++b) v = (op == 0) ? v + part[b + nb * j] : nan_max(v, part[b + nb * j]);
   out[j] = v;
 }
 // y[:, j] += a[j] * x[:, j]   (a on device)
@@ -161,6 +166,62 @@ static int residual(hfpx_ctx* ctx, const double* A, int64_t lda, int64_t n, cons
   return gemm_sub_any(ctx, HFPX_F64, R, n, A, lda, X, ldx, n, k, n);
 }
 
+// Normwise backward error of every column of X: omega_j =
+// |b - A x|_inf / (|A|_inf |x|_inf + |b|_inf); *worst = max over columns,
+// NaN if any is NaN.  Leaves the residual in w.R.
+static int backward_error(hfpx_ctx* ctx, const Work& w, const double* A, int64_t lda, int64_t n, const double* B,
+                          int64_t ldb, const double* X, int64_t ldx, int64_t k, double anorm,
+                          const std::vector<double>& bnorm, std::vector<double>& rnorm, std::vector<double>& xnorm,
+                          std::vector<double>& omega, double* worst) {
+  int rc;
+  if ((rc = residual(ctx, A, lda, n, B, ldb, X, ldx, w.R, k))) return rc;
+  if ((rc = colred(ctx, w, w.R, n, nullptr, 0, n, k, 1, rnorm.data()))) return rc;
+  if ((rc = colred(ctx, w, X, ldx, nullptr, 0, n, k, 1, xnorm.data()))) return rc;
+  double wv = 0.0;
+  for (int64_t j = 0; j < k; ++j) {
+    const double den = anorm * xnorm[j] + bnorm[j];
+    // 0 / 0 (a zero column of B solved exactly) is 0; a NaN anywhere stays NaN
+    omega[j] = (den > 0 || den != den) ? rnorm[j] / den : (rnorm[j] == 0.0 ? 0.0 : rnorm[j] / den);
+    wv = nan_max(wv, omega[j]);
+  }
+  *worst = wv;
+  return HFPX_OK;
+}
+
+// X := A^-1 B by an FP64 LU of a copy of A (A itself stays untouched: it
+// is the caller's FP64 operator).  The copy is allocated here and freed
+// before returning.
+static int fallback_fp64(hfpx_ctx* ctx, const double* A, int64_t lda, int64_t n, const double* B, int64_t ldb,
+                         double* X, int64_t ldx, int64_t k) {
+  double* F = nullptr;
+  int32_t* piv = nullptr;
+  hipError_t e = hipMalloc((void**)&F, sizeof(double) * (size_t)n * (size_t)n);
+  if (e == hipSuccess) e = hipMalloc((void**)&piv, sizeof(int32_t) * (size_t)n);
+  if (e != hipSuccess) {
+    if (F) (void)hipFree(F);
+    return check_hip(ctx, e, "GMRES-IR fallback: FP64 copy of the matrix");
+  }
+  int rc = HFPX_OK;
+  e = hipMemcpy2DAsync(F, sizeof(double) * n, A, sizeof(double) * lda, sizeof(double) * n, n,
+                       hipMemcpyDeviceToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpy2DAsync(X, sizeof(double) * ldx, B, sizeof(double) * ldb, sizeof(double) * n, k,
+                         hipMemcpyDeviceToDevice, ctx->stream);
+  if (e != hipSuccess) rc = check_hip(ctx, e, "GMRES-IR fallback: copies");
+  int64_t info = 0;
+  if (!rc) {
+    rc = lu_getrf(ctx, HFPX_F64, F, n, n, piv, &info);
+    if (rc == HFPX_ERR_SINGULAR)
+      set_error(ctx, rc, "GMRES-IR fallback: FP64 U(%lld,%lld) is exactly zero", (long long)info, (long long)info);
+  }
+  if (!rc) rc = lu_getrs(ctx, HFPX_F64, F, n, piv, n, X, ldx, k);
+  const hipError_t es = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(F);
+  (void)hipFree(piv);
+  if (!rc && es != hipSuccess) rc = check_hip(ctx, es, "GMRES-IR fallback");
+  return rc;
+}
+
 int solve_ir(hfpx_ctx* ctx, const double* A, int64_t lda, const float* LU, int64_t ldlu,
              const int32_t* ipiv, int64_t n, const double* B, int64_t ldb, double* X, int64_t ldx,
              int64_t k, double tol, int max_outer, int m, hfpx_ir_report* rep) {
@@ -210,22 +271,27 @@ int solve_ir(hfpx_ctx* ctx, const double* A, int64_t lda, const float* LU, int64
 
   std::vector<double> H((size_t)(m + 1) * m * k), cs((size_t)m * k), sn((size_t)m * k), g((size_t)(m + 1) * k);
   int outer = 0, inner_total = 0;
-  double worst = 0.0;
-  bool converged = false;
+  double worst = 0.0, prev_worst = HUGE_VAL;
+  bool converged = false, stalled = false;
+  // non-finite data: nothing to refine, and an FP64 LU of the same data
+  // cannot do better
+  double data_norm = anorm;
+  for (int64_t j = 0; j < k; ++j) data_norm = nan_max(data_norm, bnorm[j]);
+  const bool finite_data = std::isfinite(data_norm);
   for (outer = 0; outer <= max_outer; ++outer) {
-    if ((rc = residual(ctx, A, lda, n, B, ldb, X, ldx, w.R, k))) return rc;
-    if ((rc = colred(ctx, w, w.R, n, nullptr, 0, n, k, 1, rnorm.data()))) return rc;
-    if ((rc = colred(ctx, w, X, ldx, nullptr, 0, n, k, 1, xnorm.data()))) return rc;
-    worst = 0.0;
-    for (int64_t j = 0; j < k; ++j) {
-      const double den = anorm * xnorm[j] + bnorm[j];
-      omega[j] = den > 0 ? rnorm[j] / den : 0.0;
-      worst = std::max(worst, omega[j]);
-    }
+    if ((rc = backward_error(ctx, w, A, lda, n, B, ldb, X, ldx, k, anorm, bnorm, rnorm, xnorm, omega, &worst)))
+      return rc;
     if (worst <= tol) {
       converged = true;
       break;
     }
+    // NaN / inf in the iterate, or no progress: the FP32 factors do not
+    // precondition this matrix well enough (or are singular)
+    if (!finite_data || !std::isfinite(worst) || (outer >= 1 && !(worst < 0.9 * prev_worst))) {
+      stalled = true;
+      break;
+    }
+    prev_worst = worst;
     if (outer == max_outer) break;
     // the correction has to take the backward error from `worst` below tol:
     // the inner iteration stops at a preconditioned-residual reduction of
@@ -314,16 +380,27 @@ int solve_ir(hfpx_ctx* ctx, const double* A, int64_t lda, const float* LU, int64
       HFPX_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));  // host y is reused
     }
   }
+  // FP64 LU of a copy of A when refinement failed (stagnation, non-finite
+  // iterate or no convergence in max_outer steps) and the data are finite
+  bool fell_back = false;
+  if (!converged && finite_data && ctx->knob[HFPX_KNOB_IR_FALLBACK] != 2) {
+    if ((rc = fallback_fp64(ctx, A, lda, n, B, ldb, X, ldx, k))) return rc;
+    fell_back = true;
+    if ((rc = backward_error(ctx, w, A, lda, n, B, ldb, X, ldx, k, anorm, bnorm, rnorm, xnorm, omega, &worst)))
+      return rc;
+    converged = worst <= tol;
+  }
   if (rep) {
     rep->iterations = outer;
     rep->inner_iterations = inner_total;
     rep->backward_error = worst;
     rep->converged = converged ? 1 : 0;
-    rep->fell_back = 0;
+    rep->fell_back = fell_back ? 1 : 0;
   }
   HFPX_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  return converged ? HFPX_OK
-                   : set_error(ctx, HFPX_ERR_NOT_CONVERGED, "GMRES-IR stopped at backward error %.3e", worst);
+  if (converged) return HFPX_OK;
+  return set_error(ctx, HFPX_ERR_NOT_CONVERGED, "GMRES-IR %s at backward error %.3e%s",
+                   stalled ? "stagnated" : "stopped", worst, fell_back ? " after the FP64 LU fallback" : "");
 }
 
 }  // namespace hfpx

@@ -13,6 +13,7 @@ Names, argument meaning and error behaviour follow the reference's namespace
   LU (factor + solve)                           il/linear_algebra/dense/factorization/LU.h:46-138
   write_dd_vector_to_md                         src/Core/surface_mesh_utilities.cpp:256-328
   make_3dbem_stress_f_s                         src/Solvers/system_assembly.cpp:279-455
+  make_local_3dbem_submatrix                    src/Elasticity/elasticity_kernel_integration.cpp:34-205
 
 Matrices and right-hand sides live on the GPU as torch tensors in
 column-major layout (a logical n x n matrix is the transpose view of a
@@ -22,7 +23,9 @@ with IL_EXPECT_FAST raise ValueError; numerical failure (a singular matrix)
 is reported through a Status object like il::Status.
 """
 from dataclasses import dataclass, field
+import atexit
 import ctypes
+import sys
 
 import numpy as np
 
@@ -107,6 +110,10 @@ class Mesh_Data_T:
     mesh: Mesh_Geom_T = None
     dof_h_dd: DoF_Handle_T = field(default_factory=DoF_Handle_T)
     dd: np.ndarray = None     # (6 n_el) x 3
+    dof_h_pp: DoF_Handle_T = field(default_factory=DoF_Handle_T)  # n_el x 6 pressure DoF, -1 = none
+    pp: np.ndarray = None     # fluid pressure per node (6 n_el)
+    ae_set: list = field(default_factory=list)  # elements with a DD DoF ("active")
+    fe_set: list = field(default_factory=list)  # elements with a pressure DoF ("filled")
 
 
 @dataclass
@@ -136,9 +143,20 @@ class Status:
 # context (one per device, on a dedicated default-priority torch stream)
 # --------------------------------------------------------------------------
 
+# Set once the contexts have been closed at interpreter exit: objects that
+# are collected after that point release nothing through HIP (the process's
+# device memory goes with it).
+_shut_down = False
+
+
+def _hip_calls_allowed():
+    return not _shut_down and not sys.is_finalizing()
+
+
 class Context:
     def __init__(self, device=0):
         import torch
+        _register_exit_hook()
         self.torch = torch
         self.device = torch.device("cuda", device)
         self.lib = _lib.load()
@@ -160,8 +178,13 @@ class Context:
         _lib.check(rc, self.handle)
 
     def close(self):
+        """Finish the context's work and free its device memory.  The stream
+        is torch's, so it is synchronised here, while torch is alive; the
+        library only frees what it allocated."""
         if self.handle:
-            self.lib.hfpx_destroy(self.handle)
+            if _hip_calls_allowed():
+                self.stream.synchronize()
+                self.lib.hfpx_destroy(self.handle)
             self.handle = None
 
     def __del__(self):
@@ -172,6 +195,32 @@ class Context:
 
 
 _contexts = {}
+_exit_hook = False
+
+
+def close_all():
+    """Close every context of get_context().  Runs at interpreter exit
+    (registered after torch is imported, so it runs before torch's own exit
+    handlers), and may be called earlier, e.g. at the end of a benchmark."""
+    for ctx in list(_contexts.values()):
+        try:
+            ctx.close()
+        except Exception:
+            pass
+    _contexts.clear()
+
+
+def _at_exit():
+    global _shut_down
+    close_all()
+    _shut_down = True
+
+
+def _register_exit_hook():
+    global _exit_hook
+    if not _exit_hook:
+        atexit.register(_at_exit)
+        _exit_hook = True
 
 
 def get_context(device=0):
@@ -234,7 +283,8 @@ class DeviceMesh:
 
     def close(self):
         if getattr(self, "handle", None):
-            self.ctx.lib.hfpx_mesh_destroy(self.handle)
+            if _hip_calls_allowed():
+                self.ctx.lib.hfpx_mesh_destroy(self.handle)
             self.handle = None
 
     def __del__(self):
@@ -428,17 +478,99 @@ def mod_3dbem_system_vc(orig_matrix, orig_dof_hndl, dof_hndl, delta_t, delta_v, 
     return SAE_T(n_dof=used, matrix=M, rhs_v=rhs, matrix32=M32)
 
 
-def write_dd_vector_to_md(sol_v, dof_h_dd, n_el=None):
-    """Solution vector -> nodal DD array ((6 n_el) x 3); fixed DoF stay 0."""
-    sol = np.asarray(sol_v.cpu() if hasattr(sol_v, "cpu") else sol_v, dtype=np.float64)
+def write_dd_vector_to_md(sol_v, dof_h_dd, include_p=False, dof_h_pp=None, m_data=None):
+    """Solution vector -> nodal DD array ((6 n_el) x 3; fixed DoF stay 0),
+    returned.  With include_p, the pressure unknowns that follow the
+    dof_h_dd.n_dof DD unknowns go to nodal pressures pp[6 el + node] =
+    sol_v[n_dd + dof_h_pp.dof_h[el, node]] (dof_h_pp n_el x 6, -1 = none).
+    With m_data (Mesh_Data_T), its dd (kept where fixed, created when
+    empty), pp, ae_set and fe_set (elements with a DD / pressure DoF) are
+    written as the reference's io argument.  Mirrors
+    hfp3d::write_dd_vector_to_md (src/Core/surface_mesh_utilities.cpp:
+    256-328); pp holds one value per node (the reference sizes it per
+    element and indexes it per node).  Size mismatches raise ValueError
+    (the reference's IL_EXPECT_FAST)."""
+    sol = np.asarray(sol_v.cpu() if hasattr(sol_v, "cpu") else sol_v, dtype=np.float64).reshape(-1)
     dh = np.asarray(dof_h_dd.dof_h)
     n_el = dh.shape[0]
-    dd = np.zeros((6 * n_el, 3))
+    if dh.shape != (n_el, 18):
+        raise ValueError("dof_h_dd must be n_el x 18")
+    n_dd = int(dof_h_dd.n_dof)
+    n_pp = int(dof_h_pp.n_dof) if include_p else 0
+    if sol.size != n_dd + n_pp:
+        raise ValueError(f"sol_v has {sol.size} entries, expected n_dof(dd) + n_dof(pp) = {n_dd + n_pp}")
+    if m_data is not None and m_data.dd is not None and np.asarray(m_data.dd).size:
+        dd = np.array(m_data.dd, dtype=np.float64)
+        if dd.shape != (6 * n_el, 3):
+            raise ValueError("m_data.dd must be (6 n_el) x 3")
+    else:
+        dd = np.zeros((6 * n_el, 3))
     d = dh.reshape(n_el, 6, 3)
+    if d.max(initial=-1) >= n_dd:
+        raise ValueError("DD DoF out of range")
     mask = d >= 0
-    vals = np.where(mask, sol[np.where(mask, d, 0)], 0.0)
-    dd[:] = vals.reshape(6 * n_el, 3)
+    v = dd.reshape(n_el, 6, 3)
+    v[mask] = sol[d[mask]]
+    active = mask.any(axis=(1, 2))
+    pp, filled = None, np.zeros(n_el, dtype=bool)
+    if include_p:
+        ph = np.asarray(dof_h_pp.dof_h)
+        if ph.shape != (n_el, 6):
+            raise ValueError("dof_h_pp must be n_el x 6")
+        if ph.max(initial=-1) >= n_pp:
+            raise ValueError("pressure DoF out of range")
+        pp = None if m_data is None or getattr(m_data, "pp", None) is None else np.array(m_data.pp, dtype=np.float64)
+        if pp is None or pp.size == 0:
+            pp = np.zeros(6 * n_el)
+        if pp.shape != (6 * n_el,):
+            raise ValueError("m_data.pp must hold 6 n_el values")
+        pm = ph >= 0
+        pp.reshape(n_el, 6)[pm] = sol[n_dd + ph[pm]]
+        filled = pm.any(axis=1)
+    if m_data is not None:
+        m_data.dd = dd
+        if include_p:
+            m_data.pp = pp
+        m_data.ae_set = list(getattr(m_data, "ae_set", None) or []) + [int(e) for e in np.flatnonzero(active)]
+        m_data.fe_set = list(getattr(m_data, "fe_set", None) or []) + [int(e) for e in np.flatnonzero(filled)]
     return dd
+
+
+# --------------------------------------------------------------------------
+# the element integral
+# --------------------------------------------------------------------------
+
+def make_local_3dbem_submatrix(kernel_id, shear_m, poiss_r, h, z, tau, sfm, ctx=None):
+    """6 x 18 stress influence (rows S11 S22 S33 S12 S13 S23 in the element
+    frame, column 3 node + DD component) of the element with vertices tau
+    (3 complex) and shape-function coefficients sfm (6 x 6 complex, row =
+    node, column = monomial 1, tau, conj(tau), tau^2, conj(tau)^2,
+    tau conj(tau)) at the point (h, z) of the element's frame, on the GPU.
+    Batched: h (n,), z (n,), tau (n, 3), sfm (n, 6, 6) give (n, 6, 18).
+    kernel_id 1 (hypersingular) only.  Mirrors
+    hfp3d::make_local_3dbem_submatrix
+    (src/Elasticity/elasticity_kernel_integration.cpp:34-205)."""
+    ctx = ctx or get_context()
+    torch = ctx.torch
+    hh = np.atleast_1d(np.asarray(h, dtype=np.float64))
+    single = np.ndim(h) == 0
+    n = hh.size
+    zz = np.asarray(z, dtype=np.complex128).reshape(n)
+    tt = np.asarray(tau, dtype=np.complex128).reshape(n, 3)
+    ss = np.asarray(sfm, dtype=np.complex128).reshape(n, 6, 6)
+    zv = np.stack([zz.real, zz.imag], axis=1).reshape(-1)
+    tv = np.stack([tt.real, tt.imag], axis=2).reshape(-1)
+    sc = np.transpose(ss, (0, 2, 1))  # column-major 6 x 6 per case
+    sv = np.stack([sc.real, sc.imag], axis=3).reshape(-1)
+    with torch.cuda.stream(ctx.stream):
+        d = [torch.from_numpy(np.ascontiguousarray(a)).to(ctx.device) for a in (hh, zv, tv, sv)]
+        out = torch.empty(108 * n, dtype=torch.float64, device=ctx.device)
+    ctx.enter()
+    ctx.check(ctx.lib.hfpx_local_submatrix(ctx.handle, int(kernel_id), float(shear_m), float(poiss_r), n,
+                                           *[_ptr(t) for t in d], _ptr(out)))
+    ctx.leave()
+    M = out.cpu().numpy().reshape(n, 18, 6).transpose(0, 2, 1)
+    return M[0] if single else M
 
 
 # --------------------------------------------------------------------------
@@ -546,7 +678,10 @@ def tune_get(knob, ctx=None):
 def solve_mixed(A64, B, tol=1e-14, max_outer=10, gmres_m=20, ctx=None, LU32=None):
     """Solve A64 X = B: FP32 LU of A64 (or the given FP32 LU object) as the
     preconditioner of FP64 GMRES iterative refinement against A64.
-    Returns (X, report dict)."""
+    Returns (X, report dict: iterations, inner_iterations, backward_error,
+    converged, fell_back (the solution came from an FP64 LU of a copy of
+    A64), status (HFPX_OK or HFPX_ERR_NOT_CONVERGED)).  An FP32 factor
+    with a zero pivot is accepted (the refinement then falls back)."""
     ctx = ctx or get_context()
     torch = ctx.torch
     n, lda = _check_colmajor(A64)
@@ -565,7 +700,8 @@ def solve_mixed(A64, B, tol=1e-14, max_outer=10, gmres_m=20, ctx=None, LU32=None
                                _ptr(X), ldx, nrhs, float(tol), int(max_outer), int(gmres_m), ctypes.byref(rep))
     ctx.leave()
     report = dict(iterations=rep.iterations, inner_iterations=rep.inner_iterations,
-                  backward_error=rep.backward_error, converged=bool(rep.converged))
+                  backward_error=rep.backward_error, converged=bool(rep.converged), fell_back=bool(rep.fell_back),
+                  status=rc)
     if rc not in (_lib.HFPX_OK, _lib.HFPX_ERR_NOT_CONVERGED):
         ctx.check(rc)
     return (X if B.dim() == 2 else X[:, 0]), report
@@ -635,11 +771,17 @@ class Batch:
                                                   _np_ptr(sinf), int(mixed_min_n), ctypes.byref(h)))
         self.handle = h
 
-    def __del__(self):
+    def close(self):
         h = getattr(self, "handle", None)
-        if h is not None and h.value:
+        if h is not None and h.value and _hip_calls_allowed():
             self.lib.hfpx_batch_destroy(h)
-            self.handle = None
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def assemble(self):
         self.ctx.check(self.lib.hfpx_batch_assemble(self.ctx.handle, self.handle))

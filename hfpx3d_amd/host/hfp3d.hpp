@@ -11,6 +11,7 @@
 //   make_3dbem_matrix_s, add_s_inf_to_3dbem_rhs,
 //   make_3dbem_stress_f_s                       src/Solvers/system_assembly.cpp:30-455
 //   make_3dbem_matrix_vc, mod_3dbem_system_vc   src/Solvers/system_assembly.cpp:458-629
+//   make_local_3dbem_submatrix                  src/Elasticity/elasticity_kernel_integration.h:25-29
 //   load_mesh_from_numpy_32/_64, load_mesh_from_csv,
 //   load_crd_from_csv/_numpy                    src/IO/mesh_file_io.cpp:28-471
 //   save_data_to_csv                            src/IO/data_file_io.h:171-204
@@ -21,6 +22,8 @@
 // failures that the reference handles with abort() are reported through
 // Status instead.
 #pragma once
+#include <array>
+#include <complex>
 #include <cstdint>
 #include <string>
 #include <utility>
@@ -76,8 +79,11 @@ struct DoF_Handle_T {
 struct Mesh_Data_T {
   Mesh_Geom_T mesh;
   DoF_Handle_T dof_h_dd;
-  DoF_Handle_T dof_h_pp;
-  Array2D<double> dd;  // (6 n_el) x 3, global components
+  DoF_Handle_T dof_h_pp;   // n_el x 6 pressure DoF (one per node), -1 = none
+  Array2D<double> dd;      // (6 n_el) x 3, global components
+  std::vector<double> pp;  // fluid pressure per node (6 n_el)
+  std::vector<int64_t> ae_set;  // elements with a DD DoF ("active")
+  std::vector<int64_t> fe_set;  // elements with a pressure DoF ("filled")
 };
 
 // Friction / cohesion parameters of one surface material
@@ -163,7 +169,28 @@ struct SAE_T {
 
 // ---- mesh / DoF -----------------------------------------------------------
 DoF_Handle_T make_dof_h_crack(const Mesh_Geom_T& mesh, int ap_order, int tip_type);
+// Solution vector -> nodal DD (m_data.dd, (6 n_el) x 3; fixed DoF keep their
+// value, 0 when dd is created here) and, with include_p, nodal pressures
+// m_data.pp[6 el + node] = sol_v[dof_h_dd.n_dof + dof_h_pp(el, node)];
+// m_data.ae_set / fe_set get the elements with a DD / pressure DoF.  As
+// write_dd_vector_to_md (src/Core/surface_mesh_utilities.cpp:256-328), except
+// that pp holds one value per node (the reference sizes it per element and
+// then indexes it per node).  Size mismatches abort, like IL_EXPECT_FAST.
+void write_dd_vector_to_md(const std::vector<double>& sol_v, const DoF_Handle_T& dof_h_dd, bool include_p,
+                           const DoF_Handle_T& dof_h_pp, Mesh_Data_T& m_data);
+// The DD-only form (include_p = false).
 void write_dd_vector_to_md(const std::vector<double>& sol_v, const DoF_Handle_T& dof_h_dd, Mesh_Data_T& m_data);
+
+// ---- the element integral ---------------------------------------------------
+// 6 x 18 stress influence (rows S11 S22 S33 S12 S13 S23 in the element
+// frame, column 3 node + DD component) at the point (h, z) of the element
+// tau / sfm (6 x 6 complex: row = node, column = monomial 1, tau, conj(tau),
+// tau^2, conj(tau)^2, tau conj(tau)), on the device.  kernel_id 1 (the
+// hypersingular kernel) only.  As hfp3d::make_local_3dbem_submatrix
+// (src/Elasticity/elasticity_kernel_integration.cpp:34-205).
+Array2D<double> make_local_3dbem_submatrix(int kernel_id, double shear_m, double poiss_r, double h,
+                                           std::complex<double> z, const std::array<std::complex<double>, 3>& tau,
+                                           const Array2D<std::complex<double>>& sfm);
 
 // ---- the GPU stages --------------------------------------------------------
 DeviceMatrix make_3dbem_matrix_s(double shear_m, double poiss_r, const Mesh_Geom_T& mesh,
@@ -194,6 +221,9 @@ class LU {
   LU(const LU&) = delete;
   LU& operator=(const LU&) = delete;
   std::vector<double> solve(const std::vector<double>& y) const;
+  // Many right-hand sides at once (n x nrhs, column-major), as
+  // il::LU::solve(il::Array2D<double>) (LU.h:122-138, dgetrs with nrhs).
+  Array2D<double> solve(const Array2D<double>& y) const;
 
  private:
   DeviceMatrix A_;

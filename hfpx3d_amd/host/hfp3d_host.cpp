@@ -1,5 +1,5 @@
 // hfp3d_host.cpp -- implementation of the C++ host mirror (hfp3d.hpp).
-#include <hip/hip_runtime.h>
+#include <hip/hip_runtime_api.h>
 
 #include <cctype>
 #include <cmath>
@@ -135,15 +135,85 @@ DoF_Handle_T make_dof_h_crack(const Mesh_Geom_T& mesh, int ap_order, int tip_typ
   return d;
 }
 
-void write_dd_vector_to_md(const std::vector<double>& sol_v, const DoF_Handle_T& dof_h_dd, Mesh_Data_T& m_data) {
+static void expect(bool cond, const char* what) {
+  if (!cond) {
+    std::cerr << "hfp3d: " << what << std::endl;
+    std::abort();
+  }
+}
+
+void write_dd_vector_to_md(const std::vector<double>& sol_v, const DoF_Handle_T& dof_h_dd, bool include_p,
+                           const DoF_Handle_T& dof_h_pp, Mesh_Data_T& m_data) {
   const int64_t n_el = dof_h_dd.dof_h.size(0);
-  m_data.dd = Array2D<double>(6 * n_el, 3, 0.0);
-  for (int64_t e = 0; e < n_el; ++e)
-    for (int en = 0; en < 6; ++en)
+  const int64_t n_dd = dof_h_dd.n_dof, n_pp = include_p ? dof_h_pp.n_dof : 0;
+  expect(dof_h_dd.dof_h.size(1) == 18, "write_dd_vector_to_md: dof_h_dd must be n_el x 18");
+  if (m_data.dd.size(0) == 0 || m_data.dd.size(1) == 0)
+    m_data.dd = Array2D<double>(6 * n_el, 3, 0.0);
+  else
+    expect(m_data.dd.size(0) == 6 * n_el && m_data.dd.size(1) == 3, "write_dd_vector_to_md: dd must be 6 n_el x 3");
+  if (include_p) {
+    expect(dof_h_pp.dof_h.size(0) == n_el && dof_h_pp.dof_h.size(1) == 6,
+           "write_dd_vector_to_md: dof_h_pp must be n_el x 6");
+    if (m_data.pp.empty()) m_data.pp.assign((size_t)(6 * n_el), 0.0);
+    expect((int64_t)m_data.pp.size() == 6 * n_el, "write_dd_vector_to_md: pp must hold 6 n_el values");
+  }
+  expect((int64_t)sol_v.size() == n_dd + n_pp, "write_dd_vector_to_md: sol_v size != n_dof(dd) + n_dof(pp)");
+  for (int64_t e = 0; e < n_el; ++e) {
+    bool is_a = false, is_f = false;
+    for (int en = 0; en < 6; ++en) {
       for (int i = 0; i < 3; ++i) {
         const int64_t j = dof_h_dd.dof_h(e, 3 * en + i);
-        if (j >= 0) m_data.dd(6 * e + en, i) = sol_v[(size_t)j];
+        if (j >= 0) {
+          expect(j < n_dd, "write_dd_vector_to_md: DD DoF out of range");
+          m_data.dd(6 * e + en, i) = sol_v[(size_t)j];
+          is_a = true;
+        }
       }
+      if (include_p) {
+        const int64_t j = dof_h_pp.dof_h(e, en);
+        if (j >= 0) {
+          expect(j < n_pp, "write_dd_vector_to_md: pressure DoF out of range");
+          m_data.pp[(size_t)(6 * e + en)] = sol_v[(size_t)(n_dd + j)];
+          is_f = true;
+        }
+      }
+    }
+    if (is_a) m_data.ae_set.push_back(e);
+    if (is_f) m_data.fe_set.push_back(e);
+  }
+}
+
+void write_dd_vector_to_md(const std::vector<double>& sol_v, const DoF_Handle_T& dof_h_dd, Mesh_Data_T& m_data) {
+  m_data.dd = Array2D<double>();
+  write_dd_vector_to_md(sol_v, dof_h_dd, false, DoF_Handle_T(), m_data);
+}
+
+Array2D<double> make_local_3dbem_submatrix(int kernel_id, double shear_m, double poiss_r, double h,
+                                           std::complex<double> z, const std::array<std::complex<double>, 3>& tau,
+                                           const Array2D<std::complex<double>>& sfm) {
+  expect(sfm.size(0) == 6 && sfm.size(1) == 6, "make_local_3dbem_submatrix: sfm must be 6 x 6");
+  double hv[1] = {h}, zv[2] = {z.real(), z.imag()}, tv[6], sv[72];
+  for (int k = 0; k < 3; ++k) {
+    tv[2 * k] = tau[(size_t)k].real();
+    tv[2 * k + 1] = tau[(size_t)k].imag();
+  }
+  for (int c = 0; c < 6; ++c)
+    for (int r = 0; r < 6; ++r) {
+      sv[2 * (r + 6 * c)] = sfm(r, c).real();
+      sv[2 * (r + 6 * c) + 1] = sfm(r, c).imag();
+    }
+  double* d = dalloc<double>(1 + 2 + 6 + 72 + 108);
+  h2d(d, hv, sizeof hv);
+  h2d(d + 1, zv, sizeof zv);
+  h2d(d + 3, tv, sizeof tv);
+  h2d(d + 9, sv, sizeof sv);
+  check(hfpx_local_submatrix(context(), kernel_id, shear_m, poiss_r, 1, d, d + 1, d + 3, d + 9, d + 81),
+        "make_local_3dbem_submatrix");
+  check(hfpx_synchronize(context()), "make_local_3dbem_submatrix");
+  Array2D<double> out(6, 18);
+  d2h(out.data(), d + 81, sizeof(double) * 108);
+  (void)hipFree(d);
+  return out;
 }
 
 // ------------------------------------------------------------------ GPU stages
@@ -315,6 +385,19 @@ LU::LU(DeviceMatrix A, Status& status) : A_(std::move(A)) {
 LU::~LU() {
   if (ipiv_) (void)hipFree(ipiv_);
 }
+Array2D<double> LU::solve(const Array2D<double>& y) const {
+  const int64_t n = A_.size(0), nrhs = y.size(1);
+  expect(y.size(0) == n, "LU::solve: right-hand sides must have n rows");
+  Array2D<double> x = y;
+  if (nrhs == 0) return x;
+  double* b = dalloc<double>((size_t)n * nrhs);
+  h2d(b, x.data(), sizeof(double) * n * nrhs);
+  check(hfpx_getrs(context(), HFPX_F64, A_.data(), n, ipiv_, n, b, n, nrhs), "LU solve");
+  check(hfpx_synchronize(context()), "LU solve");
+  d2h(x.data(), b, sizeof(double) * n * nrhs);
+  (void)hipFree(b);
+  return x;
+}
 std::vector<double> LU::solve(const std::vector<double>& y) const {
   const int64_t n = A_.size(0);
   std::vector<double> x(y);
@@ -356,24 +439,36 @@ static bool load_npy(const std::string& path, Array2D<T>& out, std::string& err)
     f.read((char*)b, 4);
     hlen = b[0] | (b[1] << 8) | (b[2] << 16) | ((uint32_t)b[3] << 24);
   }
+  if (!f || hlen > (1u << 20)) {
+    err = path + ": bad npy header";
+    return false;
+  }
   std::string hdr(hlen, '\0');
   f.read(&hdr[0], hlen);
+  if (!f) {
+    err = path + ": truncated npy header";
+    return false;
+  }
+  const std::streamoff data_start = f.tellg();
+  f.seekg(0, std::ios::end);
+  const std::streamoff data_bytes = f.tellg() - data_start;
+  f.seekg(data_start);
+  // value of a header key; npos-safe on malformed headers (empty string)
   auto field = [&](const std::string& key) {
     size_t p = hdr.find("'" + key + "'");
     if (p == std::string::npos) return std::string();
     p = hdr.find(':', p);
+    if (p == std::string::npos) return std::string();
     size_t q = p + 1;
     while (q < hdr.size() && std::isspace((unsigned char)hdr[q])) ++q;
-    if (hdr[q] == '\'') {
-      size_t e = hdr.find('\'', q + 1);
+    if (q >= hdr.size()) return std::string();
+    if (hdr[q] == '\'' || hdr[q] == '(') {
+      const size_t e = hdr.find(hdr[q] == '(' ? ')' : '\'', q + 1);
+      if (e == std::string::npos) return std::string();
       return hdr.substr(q + 1, e - q - 1);
     }
-    if (hdr[q] == '(') {
-      size_t e = hdr.find(')', q);
-      return hdr.substr(q + 1, e - q - 1);
-    }
-    size_t e = hdr.find_first_of(",}", q);
-    return hdr.substr(q, e - q);
+    const size_t e = hdr.find_first_of(",}", q);
+    return e == std::string::npos ? std::string() : hdr.substr(q, e - q);
   };
   const std::string descr = field("descr"), fo = field("fortran_order"), shape = field("shape");
   const bool fortran = fo.find("True") != std::string::npos;
@@ -381,15 +476,30 @@ static bool load_npy(const std::string& path, Array2D<T>& out, std::string& err)
   {
     std::stringstream ss(shape);
     std::string tok;
-    while (std::getline(ss, tok, ','))
-      if (tok.find_first_not_of(" ") != std::string::npos) dims.push_back(std::stoll(tok));
+    while (std::getline(ss, tok, ',')) {
+      const size_t a = tok.find_first_not_of(" \t");
+      if (a == std::string::npos) continue;
+      const size_t b = tok.find_last_not_of(" \t");
+      const std::string t = tok.substr(a, b - a + 1);
+      if (t.empty() || t.size() > 12 || t.find_first_not_of("0123456789") != std::string::npos) {
+        err = path + ": bad shape (" + shape + ")";
+        return false;
+      }
+      dims.push_back(std::strtoll(t.c_str(), nullptr, 10));
+    }
   }
   if (dims.size() == 1) dims.push_back(1);
   if (dims.size() != 2) {
     err = path + ": only 1-D and 2-D arrays are supported";
     return false;
   }
-  const int64_t n0 = dims[0], n1 = dims[1], cnt = n0 * n1;
+  const int64_t n0 = dims[0], n1 = dims[1];
+  const int64_t esz = descr == "<i4" ? 4 : 8;
+  if (n1 > 0 && n0 > data_bytes / esz / n1) {
+    err = path + ": shape (" + shape + ") larger than the file's data";
+    return false;
+  }
+  const int64_t cnt = n0 * n1;
   out.resize(n0, n1);
   auto fill = [&](auto dummy) {
     using S = decltype(dummy);
@@ -445,16 +555,31 @@ int set_error(hfpx_ctx* ctx, int code, const char* fmt, ...);
 extern "C" int hfpx_mesh_create_npy(hfpx_ctx* ctx, const char* nodes_npy, const char* elems_npy,
                                     int64_t index_origin, int tip_type, double beta, hfpx_mesh** mesh) {
   if (!ctx || !nodes_npy || !elems_npy || !mesh) return HFPX_ERR_INVALID;
+  *mesh = nullptr;
   if (tip_type < 0 || tip_type > 2) return hfpx::set_error(ctx, HFPX_ERR_INVALID, "tip_type %d not in 0..2", tip_type);
-  hfp3d::Mesh_Geom_T m;
-  hfp3d::Status st;
-  hfp3d::load_npy_mesh_files(elems_npy, nodes_npy, index_origin, m, st);
-  if (!st.ok()) return hfpx::set_error(ctx, HFPX_ERR_INVALID, "%s", st.error.c_str());
-  if (m.nods.size(0) < 3 || m.conn.size(0) < 3 || m.conn.size(1) < 1)
-    return hfpx::set_error(ctx, HFPX_ERR_INVALID, "mesh needs >= 3 coordinate rows and >= 3 connectivity rows");
-  const hfp3d::DoF_Handle_T d = hfp3d::make_dof_h_crack(m, 2, tip_type);
-  return hfpx_mesh_create(ctx, m.nods.data(), m.nods.size(0), m.nods.size(1), m.conn.data(), m.conn.size(0),
-                          m.conn.size(1), d.dof_h.data(), d.n_dof, beta, mesh);
+  try {
+    hfp3d::Mesh_Geom_T m;
+    hfp3d::Status st;
+    hfp3d::load_npy_mesh_files(elems_npy, nodes_npy, index_origin, m, st);
+    if (!st.ok()) return hfpx::set_error(ctx, HFPX_ERR_INVALID, "%s", st.error.c_str());
+    if (m.nods.size(0) < 3 || m.conn.size(0) < 3 || m.conn.size(1) < 1 || m.nods.size(1) < 3)
+      return hfpx::set_error(ctx, HFPX_ERR_INVALID,
+                             "mesh needs >= 3 coordinate rows, >= 3 nodes, >= 3 connectivity rows, >= 1 element");
+    // vertex ids in range before the DoF handle reads the nodes they name
+    const int64_t n_nodes = m.nods.size(1);
+    for (int64_t e = 0; e < m.conn.size(1); ++e)
+      for (int k = 0; k < 3; ++k)
+        if (m.conn(k, e) < 0 || m.conn(k, e) >= n_nodes)
+          return hfpx::set_error(ctx, HFPX_ERR_INVALID, "element %lld references node %lld out of range",
+                                 (long long)e, (long long)m.conn(k, e));
+    const hfp3d::DoF_Handle_T d = hfp3d::make_dof_h_crack(m, 2, tip_type);
+    return hfpx_mesh_create(ctx, m.nods.data(), m.nods.size(0), m.nods.size(1), m.conn.data(), m.conn.size(0),
+                            m.conn.size(1), d.dof_h.data(), d.n_dof, beta, mesh);
+  } catch (const std::exception& ex) {
+    return hfpx::set_error(ctx, HFPX_ERR_INVALID, "npy mesh: %s", ex.what());
+  } catch (...) {
+    return hfpx::set_error(ctx, HFPX_ERR_INVALID, "npy mesh: unreadable input");
+  }
 }
 
 namespace hfp3d {

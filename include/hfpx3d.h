@@ -39,13 +39,18 @@ typedef struct hfpx_ir_report {
   int inner_iterations;  /* GMRES steps summed over the outer steps       */
   double backward_error; /* max over columns of |b-Ax|/(|A||x|+|b|), inf-norms */
   int converged;         /* 1 if backward_error <= tol                     */
-  int fell_back;         /* reserved (0)                                  */
+  int fell_back;         /* 1 if refinement failed and the solution came
+                            from an FP64 LU of a copy of A64               */
 } hfpx_ir_report;
 
 /* ---- context ---------------------------------------------------------- */
 /* Create a context on `device`; `stream` is a hipStream_t to use (NULL: the
  * library creates its own default-priority stream). */
 int hfpx_create(hfpx_ctx** ctx, int device, void* stream);
+/* Free the context's device memory.  With a caller's stream, the caller
+ * finishes the work it enqueued (the library does not touch a stream it
+ * does not own here); call it while the HIP runtime is still up, not from
+ * exit-time destructors. */
 int hfpx_destroy(hfpx_ctx* ctx);
 const char* hfpx_status_string(int status);
 const char* hfpx_last_error(const hfpx_ctx* ctx);
@@ -112,6 +117,22 @@ int hfpx_assemble(hfpx_ctx* ctx, hfpx_mesh* mesh, double shear_m, double poiss_r
  * the FP32 copy is the factorisation operand of the mixed-precision solve. */
 int hfpx_assemble_mixed(hfpx_ctx* ctx, hfpx_mesh* mesh, double shear_m, double poiss_r,
                         double* A64, int64_t lda64, float* A32, int64_t lda32);
+
+/* The element integral on the reference's own arguments, for n_cases
+ * independent cases (device arrays): h[c] (minus the point's coordinate
+ * along the element normal), z[2c..2c+1] (the point's in-plane coordinates,
+ * re / im), tau[6c..6c+5] (the three vertices' in-plane coordinates, re / im
+ * pairs) and sfm[72c..72c+71] (the 6 x 6 complex shape-function coefficients,
+ * column-major, re / im interleaved; row = node, column = monomial 1, tau,
+ * conj(tau), tau^2, conj(tau)^2, tau conj(tau)), all in the element's own
+ * frame.  out[108c..108c+107] receives the 6 x 18 stress block, column-major:
+ * rows S11 S22 S33 S12 S13 S23, column 3 node + DD component.  kernel_id
+ * must be 1 (the hypersingular traction kernel; the reference's 0 is an
+ * empty branch).  Replaces hfp3d::make_local_3dbem_submatrix
+ * (src/Elasticity/elasticity_kernel_integration.cpp:34-205, declared
+ * elasticity_kernel_integration.h:25-29), one case per thread. */
+int hfpx_local_submatrix(hfpx_ctx* ctx, int kernel_id, double shear_m, double poiss_r, int64_t n_cases,
+                         const double* h, const double* z, const double* tau, const double* sfm, double* out);
 
 /* Right-hand sides for nrhs far-field stress tensors sinf_host (6 x nrhs,
  * order xx yy zz xy xz yz, tension positive): B(:, r) -= traction of
@@ -216,7 +237,10 @@ int hfpx_getrs(hfpx_ctx* ctx, int dtype, const void* LU, int64_t lda, const int3
  * untouched).  X (device, ldx) receives the solution; refinement stops when
  * every column's normwise backward error is <= tol or after max_outer steps
  * (then HFPX_ERR_NOT_CONVERGED, X holding the last iterate and rep the
- * achieved error).  No reference counterpart: it stands in for il::LU::solve
+ * achieved error).  Refinement that stagnates or meets a non-finite iterate
+ * is replaced by an FP64 LU solve of a copy of A64 (HFPX_KNOB_IR_FALLBACK;
+ * rep->fell_back = 1); the backward error is NaN when A64 or B holds a
+ * non-finite value.  No reference counterpart: it stands in for il::LU::solve
  * (LU.h:106-138) on the FP64 matrix. */
 int hfpx_solve_ir(hfpx_ctx* ctx, const double* A64, int64_t lda64, const float* LU32,
                   int64_t ldlu32, const int32_t* ipiv, int64_t n, const double* B, int64_t ldb,
@@ -323,7 +347,18 @@ int hfpx_batch_matrix(hfpx_ctx* ctx, hfpx_batch* batch, int crack, double* A_hos
  * walk: 0 = automatic (the band's split A panel in an XCD's L2),
  * 1 .. 64 = fixed. */
 #define HFPX_KNOB_X3_BAND 4
-#define HFPX_KNOB_COUNT 5
+/* HFPX_KNOB_IR_FALLBACK: what hfpx_solve_ir does when refinement stagnates,
+ * meets a non-finite iterate or runs out of steps with finite data:
+ * 0 = default (1), 1 = solve again by an FP64 LU of a copy of A64 (n^2
+ * doubles allocated for the call; report fell_back = 1), 2 = return
+ * HFPX_ERR_NOT_CONVERGED with the last iterate. */
+#define HFPX_KNOB_IR_FALLBACK 5
+/* HFPX_KNOB_TEST_PANEL_TIMEOUT: 1 = hfpx_getrf starts with its panel
+ * time-out flag already raised, as if a grid barrier had timed out: every
+ * panel returns at entry and getrf reports HFPX_ERR_HIP (a host-forced test
+ * of the error path; every kernel still exits normally).  0 = off. */
+#define HFPX_KNOB_TEST_PANEL_TIMEOUT 6
+#define HFPX_KNOB_COUNT 7
 /* Set / read one knob; HFPX_ERR_INVALID for an unknown knob or value. */
 int hfpx_tune_set(hfpx_ctx* ctx, int knob, int value);
 int hfpx_tune_get(const hfpx_ctx* ctx, int knob, int* value);

@@ -534,6 +534,40 @@ int64_t hfpo_assemble_w(const double* nods, int64_t nods_rows, int64_t n_nodes,
   return n_dof;
 }
 
+/* The rows of target elements [t0, t1) only (the loop of
+ * make_3dbem_matrix_s, system_assembly.cpp:61-213, restricted to those
+ * targets): R is (18 (t1 - t0)) x n_dof column-major, row 18 (et - t0) + r
+ * = the element's local traction slot r (the matrix row dof_h(et, r); left
+ * 0 when that slot is fixed), column = source DoF.  For meshes whose full
+ * matrix does not fit in host memory. */
+void hfpo_assemble_rows(const double* nods, int64_t nods_rows, int64_t n_nodes, const int64_t* conn,
+                        int64_t conn_rows, int64_t n_el, const int64_t* dof_h, int64_t n_dof, double shear_m,
+                        double poiss_r, double beta, int64_t t0, int64_t t1, int nthreads, double* R) {
+  (void)n_nodes;
+  const int64_t nr = 18 * (t1 - t0);
+  memset(R, 0, sizeof(double) * (size_t)nr * (size_t)n_dof);
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic)
+#endif
+  for (int64_t es = 0; es < n_el; ++es) {
+    double vs[9], vt[9], blk[324], wsb[3], wtb[3];
+    el_vert(nods, nods_rows, conn, conn_rows, es, vs);
+    const double* ws = el_wts(NULL, conn, conn_rows, es, wsb);
+    for (int64_t et = t0; et < t1; ++et) {
+      el_vert(nods, nods_rows, conn, conn_rows, et, vt);
+      const double* wt = el_wts(NULL, conn, conn_rows, et, wtb);
+      el2el_block(vs, ws, vt, wt, shear_m, poiss_r, beta, 0, blk);
+      for (int c = 0; c < 18; ++c) {
+        const int64_t gc = dof_h[es + n_el * c];
+        if (gc < 0) continue;
+        for (int r = 0; r < 18; ++r)
+          if (dof_h[et + n_el * r] >= 0) R[18 * (et - t0) + r + nr * gc] += blk[r + 18 * c];
+      }
+    }
+  }
+}
+
 /* ---------------------------------------------------------------------- */
 /* right-hand side (system_assembly.cpp:222-274)                            */
 /* ---------------------------------------------------------------------- */

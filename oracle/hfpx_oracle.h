@@ -61,6 +61,13 @@ int64_t hfpo_assemble_w(const double* nods, int64_t nods_rows, int64_t n_nodes,
                         const int64_t* dof_h, int64_t n_dof, double shear_m,
                         double poiss_r, double beta, const double* node_wts, int64_t t0,
                         int64_t t1, int is_dd_local, int nthreads, double* A);
+/* Rows of target elements [t0, t1) only (global frame, uniform elements):
+ * R (18 (t1 - t0)) x n_dof, row 18 (et - t0) + r = local traction slot r of
+ * target et (0 where fixed), column = source DoF (system_assembly.cpp:61-213
+ * restricted to those targets). */
+void hfpo_assemble_rows(const double* nods, int64_t nods_rows, int64_t n_nodes, const int64_t* conn,
+                        int64_t conn_rows, int64_t n_el, const int64_t* dof_h, int64_t n_dof, double shear_m,
+                        double poiss_r, double beta, int64_t t0, int64_t t1, int nthreads, double* R);
 
 /* rhs[dof] -= (n . s_inf)_l for every free DoF (caller zero-fills rhs). */
 void hfpo_rhs(const double* nods, int64_t nods_rows, int64_t n_nodes,

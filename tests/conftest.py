@@ -14,3 +14,13 @@ if TESTS not in sys.path:
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP path)")
     config.addinivalue_line("markers", "slow: long-running CPU test")
+
+
+@pytest.fixture(scope="session", autouse=True)
+def _close_library_contexts():
+    """Free the HIP library's contexts at the end of the session, while
+    torch and the HIP runtime are still up."""
+    yield
+    mod = sys.modules.get("hfpx3d_amd.hfp3d")
+    if mod is not None:
+        mod.close_all()

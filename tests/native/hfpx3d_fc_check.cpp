@@ -7,7 +7,7 @@
 #include <cstdlib>
 #include <string>
 
-#include "hfp3d.hpp"
+#include "../../hfpx3d_amd/host/hfp3d.hpp"
 
 namespace {
 

@@ -1,14 +1,30 @@
 // hfpx3d_host_check.cpp -- prints what the host layer parses, for the CPU
 // tests (no device call): read_config results, mesh sizes and checksums,
 // and the DoF handle.  Usage: hfpx3d_host_check <config.toml>; with
-// "--npy <dir>" it writes a small array through save_data_to_npy instead.
+// "--npy <dir>" it writes a small array through save_data_to_npy instead;
+// with "--npy-mesh <nodes.npy> <elems.npy>" it runs the npy mesh loader.
+// A test helper, not part of the library.
 #include <cstdio>
 #include <string>
 
-#include "hfp3d.hpp"
+#include "../../hfpx3d_amd/host/hfp3d.hpp"
 
 int main(int argc, char** argv) {
   if (argc < 2) return 2;
+  if (argc == 4 && std::string(argv[1]) == "--npy-mesh") {
+    // loads nodes (argv[2]) and elements (argv[3]) with index origin 1 and
+    // prints the shapes, or the loader's error
+    hfp3d::Mesh_Geom_T m;
+    hfp3d::Status st;
+    hfp3d::load_npy_mesh_files(argv[3], argv[2], 1, m, st);
+    if (!st.ok()) {
+      std::printf("{\"error\": \"%s\"}\n", st.error.c_str());
+      return 1;
+    }
+    std::printf("{\"nods\": [%lld, %lld], \"conn\": [%lld, %lld]}\n", (long long)m.nods.size(0),
+                (long long)m.nods.size(1), (long long)m.conn.size(0), (long long)m.conn.size(1));
+    return 0;
+  }
   if (argc == 3 && std::string(argv[1]) == "--npy") {
     // writes a 3 x 5 array (entry (i, j) = i + 10 j + 0.1) as check64.npy and check32.npy
     hfp3d::Array2D<double> a(3, 5);

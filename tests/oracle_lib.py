@@ -102,6 +102,8 @@ class Oracle(_FrictionCohesion):
         self._asm = L.fn("assemble_w", _i64, [_dp, _i64, _i64, _dp, _i64, _i64, _dp, _i64, _d, _d, _d,
                                               _dp, _i64, _i64, _int, _int, _dp])
         self._rhs = L.fn("rhs", None, [_dp, _i64, _i64, _dp, _i64, _i64, _dp, _dp, _dp])
+        self._rows = L.fn("assemble_rows", None, [_dp, _i64, _i64, _dp, _i64, _i64, _dp, _i64, _d, _d, _d, _i64,
+                                                   _i64, _int, _dp])
         self._wdd = L.fn("write_dd", None, [_dp, _dp, _i64, _dp])
         self._stress = L.fn("stress_w", None, [_dp, _i64, _i64, _dp, _i64, _i64, _d, _d, _dp, _dp, _i64,
                                                _dp, _dp, _int, _int, _dp])
@@ -151,6 +153,19 @@ class Oracle(_FrictionCohesion):
         self._asm(_p(nods), nods.shape[0], nods.shape[1], _p(conn), conn.shape[0], n_el, _p(dof_f), n_dof,
                   G, nu, beta, _pw(w), t0, t1, int(dd_local), nthreads, _p(A))
         return A.reshape(n_dof, n_dof, order="F"), dof
+
+    def assemble_rows(self, nods, conn, G, nu, t0, t1, beta=0.125, tip_type=0, nthreads=8):
+        """Rows 18 t0 .. 18 t1 - 1 of the matrix (local traction slots of the
+        target elements t0 .. t1 - 1, all columns) without forming it."""
+        nods, conn = _f(nods), _f(conn, np.int64)
+        n_el = conn.shape[1]
+        n_dof, dof = self.dof_h(nods, conn, tip_type)
+        dof_f = _f(dof, np.int64)
+        nr = 18 * (t1 - t0)
+        R = np.zeros(nr * n_dof)
+        self._rows(_p(nods), nods.shape[0], nods.shape[1], _p(conn), conn.shape[0], n_el, _p(dof_f), n_dof, G, nu,
+                   beta, t0, t1, nthreads, _p(R))
+        return R.reshape(nr, n_dof, order="F")
 
     def rhs(self, nods, conn, dof, s_inf):
         nods, conn = _f(nods), _f(conn, np.int64)

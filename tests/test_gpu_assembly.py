@@ -89,18 +89,35 @@ def test_penny121_full_matches_oracle(hx):
 
 
 def test_penny1025_sampled_rows_match_oracle_and_reference(hx):
-    """C2 mesh (18,450 DoF, G = 0.4, nu = 0.25): sampled target rows."""
+    """C2 mesh (18,450 DoF, G = 0.4, nu = 0.25), the committed reference rows
+    of three target elements: the GPU rows within 1e-13 of the oracle's, and
+    within 1e-12 of the reference's at every (collocation point, source
+    element) pair except those where the oracle differs from the reference
+    too.  Each of those lies on the extension of a source edge line, where
+    the reference's formulas lose accuracy (the full-matrix adjudication by
+    quadrature and 30-digit evaluation is in test_oracle.py)."""
+    import ref_adjudicate as ra
     print("case: penny1025 rows", flush=True)
     nods, conn = load_mesh("pennymesh1025el_32")
     d = np.load(golden("ref_rows_pennymesh1025el_32.npz"))
     A, _ = gpu_matrix(hx, nods, conn, 0.4, 0.25)
     mx = np.abs(A).max()
+    n_explained = 0
     for i, t in enumerate(d["targets"]):
-        Ao, _ = O.assemble(nods, conn, 0.4, 0.25, targets=(int(t), int(t) + 1))
+        t = int(t)
+        Ao = O.assemble_rows(nods, conn, 0.4, 0.25, t, t + 1)
         rows = slice(18 * t, 18 * t + 18)
-        assert np.abs(A[rows] - Ao[rows]).max() / mx < 1e-13
-        # reference rows: agree up to the reference's own near-edge-line rounding
-        assert np.abs(A[rows] - d["rows"][18 * i:18 * i + 18]).max() / mx < 1e-6
+        assert np.abs(A[rows] - Ao).max() / mx < 1e-13
+        Rr = d["rows"][18 * i:18 * i + 18]
+        pair = lambda M: np.abs(M - Rr).reshape(6, 3, -1, 18).max(axis=(1, 3)) / mx  # noqa: E731
+        bad_gpu, bad_oracle = pair(A[rows]) > 1e-12, pair(Ao) > 1e-12
+        assert not (bad_gpu & ~bad_oracle).any()
+        for k, se in np.argwhere(bad_gpu):
+            _, _, h, z, tau, _ = ra.pair_frame(nods, conn, t, k, se)
+            ok, where = ra.on_edge_extension(tau, h, z)
+            assert ok, (t, k, se, where)
+            n_explained += 1
+    print(f"pairs off the reference by > 1e-12, all on edge-line extensions: {n_explained}")
 
 
 def test_fp32_copy_is_rounded_fp64(hx):

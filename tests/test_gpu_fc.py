@@ -73,7 +73,7 @@ def test_cpp_host_mirror_matches_oracle():
     import json
     import os
     import subprocess
-    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "hfpx3d_amd", "hfpx3d_fc_check")
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests", "native", "hfpx3d_fc_check")
     print("case: C++ host mirror of the friction/cohesion laws, 2000 nodes", flush=True)
     r = subprocess.run([exe, "2000"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr

@@ -1,0 +1,155 @@
+"""The element integral on the device through hfpx_local_submatrix, the C-ABI
+form of the reference's per-pair kernel make_local_3dbem_submatrix
+(src/Elasticity/elasticity_kernel_integration.h:25-29, .cpp:34-205).
+
+Checked against
+  * the MATLAB matrices of the reference's kernel unit tests
+    (src/Elasticity/_test/kernel_integration_test.cpp:28, 118, 208: an
+    in-plane point, a point above a vertex, a general point);
+  * the oracle on the 600 reference-build cases of ref_kernel_cases.npz
+    (six geometry kinds, including points above edge lines and vertices);
+  * the oracle and brute-force Kelvin quadrature (tests/kelvin_quad.py) at
+    degenerate projections: points right above a vertex, above the line of
+    an edge (inside the edge and on its extensions) and in the element's
+    plane on an edge's extension.  The reference evaluates these in its
+    reduced branch (h_potential.cpp:904-1176, selected at
+    elasticity_kernel_integration.cpp:72-104, 156-174); the HIP kernel has
+    no branch for them.
+"""
+import numpy as np
+import pytest
+
+import kelvin_quad
+import oracle_lib
+from oracle_lib import golden
+
+pytestmark = pytest.mark.gpu
+O = oracle_lib.oracle()
+TRI = np.array([[0, 0.1, 0], [1.8, 0, 0], [1.2, 1.8, 0]]).T
+
+
+@pytest.fixture(scope="module")
+def hx():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a device"
+    from hfpx3d_amd import hfp3d
+    return hfp3d
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+def local_point(v, X):
+    """(h, z) of point X in the frame of the element with vertices v."""
+    _, rt, _, _ = O.el_geom(v)
+    y = rt @ (X - v[:, 0])
+    return -y[2], y[0] + 1j * y[1]
+
+
+@pytest.mark.parametrize("k,pt", [(1, [1.0, 1.0, 0.0]), (2, [1.2, 1.8, -1.21]), (3, [1.0, 1.0, -1.21])])
+def test_kernel_matches_matlab_points(hx, k, pt):
+    """G = 1, nu = 0.35; the MATLAB CSVs hold about 9 significant digits."""
+    print(f"case: MATLAB point {k}", flush=True)
+    sfm, rt, tau, cp = O.el_geom(TRI)
+    h, z = local_point(TRI, np.array(pt, dtype=float))
+    M = hx.make_local_3dbem_submatrix(1, 1.0, 0.35, h, z, tau, sfm)
+    F = np.load(golden(f"matlab_integration_pt{k}.npy"))
+    assert rel(M, F) < 1e-8
+    assert rel(M, O.local_submatrix(1.0, 0.35, h, z, tau, sfm)) < 1e-13
+
+
+def test_kernel_matches_oracle_on_reference_cases(hx):
+    """All 600 reference-build cases in one batched launch against the
+    oracle's evaluation of the same (h, z, tau, sfm)."""
+    print("case: 600 reference kernel cases", flush=True)
+    d = np.load(golden("ref_kernel_cases.npz"))
+    hs, zs, taus, sfms, Ms, Gs, nus = [], [], [], [], [], [], []
+    for v, p in zip(d["vert"], d["par"]):
+        sfm, rt, tau, cp = O.el_geom(v)
+        G, nu, h, y0, y1, kind = p
+        hs.append(h)
+        zs.append(y0 + 1j * y1)
+        taus.append(tau)
+        sfms.append(sfm)
+        Ms.append(O.local_submatrix(G, nu, h, y0 + 1j * y1, tau, sfm))
+        Gs.append(G)
+        nus.append(nu)
+    # one launch per material (the C ABI takes one G, nu per call)
+    errs = []
+    for G, nu in sorted(set(zip(Gs, nus))):
+        idx = [i for i in range(len(hs)) if Gs[i] == G and nus[i] == nu]
+        M = hx.make_local_3dbem_submatrix(1, G, nu, np.array([hs[i] for i in idx]), np.array([zs[i] for i in idx]),
+                                          np.array([taus[i] for i in idx]), np.array([sfms[i] for i in idx]))
+        errs += [rel(M[q], Ms[i]) for q, i in enumerate(idx)]
+    errs = np.array(errs)
+    assert len(errs) == len(d["par"])
+    assert np.median(errs) < 1e-14, np.median(errs)
+    assert errs.max() < 1e-11, errs.max()
+
+
+def degenerate_cases(seed=7):
+    """(vertices, point, kind) with the point's projection on a vertex or on
+    an edge line, for three random elements."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(3):
+        v = rng.normal(size=(3, 3))
+        _, rt, tau, _ = O.el_geom(v)
+        L = np.abs(tau).max()
+        e3 = rt[2]
+        for k in range(3):  # above a vertex
+            for hf in (0.3, -1.0):
+                out.append((v, v[:, k] + hf * L * e3, "vertex"))
+        for m in range(3):  # above the line of edge m: inside, before, beyond
+            a, b = v[:, m], v[:, (m + 1) % 3]
+            for t in (0.25, 0.5, -0.5, 1.75):
+                out.append((v, a + t * (b - a) + 0.4 * L * e3, "edge line"))
+            for t in (-0.6, 1.6):  # in the plane, on the edge's extension
+                out.append((v, a + t * (b - a), "in-plane extension"))
+    return out
+
+
+def test_kernel_at_degenerate_projections(hx):
+    """72 points projecting onto vertices and edge lines: the HIP kernel
+    against the oracle (1e-12) and, point by point, against Kelvin
+    quadrature (1e-9 of the block's largest entry; graded towards the
+    element for the in-plane points)."""
+    print("case: degenerate projections", flush=True)
+    G, nu = 0.8, 0.27
+    cases = degenerate_cases()
+    assert len(cases) == 72
+    hs, zs, taus, sfms, vs, Xs, kinds = [], [], [], [], [], [], []
+    for v, X, kind in cases:
+        sfm, rt, tau, _ = O.el_geom(v)
+        h, z = local_point(v, X)
+        hs.append(h)
+        zs.append(z)
+        taus.append(tau)
+        sfms.append(sfm)
+        vs.append(v)
+        Xs.append(X)
+        kinds.append(kind)
+    M = hx.make_local_3dbem_submatrix(1, G, nu, np.array(hs), np.array(zs), np.array(taus), np.array(sfms))
+    n_quad = {"vertex": 0, "edge line": 0, "in-plane extension": 0}
+    for i in range(len(cases)):
+        Mo = O.local_submatrix(G, nu, hs[i], zs[i], taus[i], sfms[i])
+        assert rel(M[i], Mo) < 1e-12, (i, kinds[i], rel(M[i], Mo))
+        assert np.isfinite(M[i]).all()
+        if kinds[i] == "in-plane extension":
+            if n_quad["in-plane extension"] >= 6:
+                continue
+            Q = kelvin_quad.local_submatrix(vs[i], Xs[i], G, nu, n=24, depth=6)
+        else:
+            Q = kelvin_quad.local_submatrix(vs[i], Xs[i], G, nu, n=80)
+        n_quad[kinds[i]] += 1
+        assert rel(M[i], Q) < 1e-9, (i, kinds[i], rel(M[i], Q))
+    assert min(n_quad.values()) >= 6, n_quad
+
+
+def test_kernel_rejects_unknown_kernel_id(hx):
+    from hfpx3d_amd import _lib
+    sfm, rt, tau, cp = O.el_geom(TRI)
+    with pytest.raises(_lib.HfpxError) as e:
+        hx.make_local_3dbem_submatrix(0, 1.0, 0.3, 0.5, 0.3 + 0.2j, tau, sfm)
+    assert e.value.status == _lib.HFPX_ERR_INVALID

@@ -259,6 +259,39 @@ def test_getrf_reports_singular(env):
     assert st.info == 17
 
 
+@pytest.mark.parametrize("dt", ["f64", "f32"])
+def test_getrf_reports_panel_timeout_promptly(env, dt):
+    """The panel's bounded waits, forced from the host: with the time-out
+    flag raised before the factorisation (HFPX_KNOB_TEST_PANEL_TIMEOUT)
+    every panel returns at entry, every kernel exits normally, and getrf
+    reports HFPX_ERR_HIP at once instead of after the waits' bounds.  The
+    next factorisation on the same context is correct again."""
+    import time
+    from hfpx3d_amd import _lib
+    torch, hx = env
+    print(f"case: panel time-out path {dt}", flush=True)
+    n = 3000
+    A = np.random.default_rng(9).normal(size=(n, n))
+    dtype = torch.float64 if dt == "f64" else torch.float32
+    ctx = hx.get_context()
+    hx.tune_set(_lib.KNOB_TEST_PANEL_TIMEOUT, 1, ctx)
+    try:
+        t0 = time.perf_counter()
+        with pytest.raises(_lib.HfpxError) as e:
+            hx.LU(to_dev(torch, hx, A, dtype))
+        dt_s = time.perf_counter() - t0
+    finally:
+        hx.tune_set(_lib.KNOB_TEST_PANEL_TIMEOUT, 0, ctx)
+    assert e.value.status == _lib.HFPX_ERR_HIP
+    assert "timed out" in str(e.value)
+    assert dt_s < 10.0, dt_s
+    lu = hx.LU(to_dev(torch, hx, A, dtype))
+    sl = pytest.importorskip("scipy.linalg")
+    _, piv = sl.lu_factor(A.astype(np.float32) if dt == "f32" else A)
+    if dt == "f64":
+        assert np.array_equal(lu.ipiv.cpu().numpy(), piv + 1)
+
+
 def test_bem_penny121_solution_matches_oracle(env):
     """Assembly + LU + solve on the 121-element penny (tip_type 0, the C1
     configuration E = 1, nu = 0.3, p = 1): DD agree with the oracle's FP64

@@ -60,3 +60,68 @@ def test_mesh_from_npy_reports_missing_file(hx, tmp_path):
     rc = ctx.lib.hfpx_mesh_create_npy(ctx.handle, golden("Nodes_pennymesh24el_32.npy").encode(),
                                       golden("Elems_pennymesh24el_32.npy").encode(), 1, 3, 0.125, ctypes.byref(h))
     assert rc == _lib.HFPX_ERR_INVALID
+
+
+def _write_npy_bytes(path, header, payload=b""):
+    """A .npy v1.0 file with the given header dict text and raw payload."""
+    h = header.encode()
+    pad = 64 - (10 + len(h) + 1) % 64
+    with open(path, "wb") as f:
+        f.write(b"\x93NUMPY\x01\x00" + (len(h) + pad + 1).to_bytes(2, "little") + h + b" " * pad + b"\n")
+        f.write(payload)
+
+
+@pytest.mark.parametrize("header,payload", [
+    ("{'descr': '<f8', 'fortran_order': False, 'shape': (3, 99999999999), }", b"\0" * 72),   # shape beyond the data
+    ("{'descr': '<f8', 'fortran_order': False, 'shape': (3, -5), }", b"\0" * 72),            # negative dimension
+    ("{'descr': '<f8', 'fortran_order': False, 'shape': (3, x), }", b"\0" * 72),             # not a number
+    ("{'descr': '<f8', 'fortran_order': False, 'shape': (3, 3", b"\0" * 72),                 # unterminated
+    ("{'descr': '<f8', 'fortran_order': False, 'shape': (3, 4), }", b"\0" * 40),             # truncated data
+    ("{'descr': '<c16', 'fortran_order': False, 'shape': (3, 3), }", b"\0" * 144),           # unsupported dtype
+])
+def test_mesh_from_npy_rejects_corrupt_nodes(hx, tmp_path, header, payload):
+    from hfpx3d_amd import _lib
+    print(f"case: corrupt npy {header[-24:]}", flush=True)
+    ctx = hx.get_context()
+    bad = os.path.join(str(tmp_path), "bad.npy")
+    _write_npy_bytes(bad, header, payload)
+    h = ctypes.c_void_p()
+    rc = ctx.lib.hfpx_mesh_create_npy(ctx.handle, bad.encode(), golden("Elems_pennymesh24el_32.npy").encode(), 1, 0,
+                                      0.125, ctypes.byref(h))
+    assert rc == _lib.HFPX_ERR_INVALID
+    assert not h.value
+
+
+def test_mesh_from_npy_rejects_out_of_range_vertex(hx, tmp_path):
+    """A vertex id past the node table is refused before the DoF handle
+    reads the node rows it names."""
+    from hfpx3d_amd import _lib
+    print("case: npy vertex id out of range", flush=True)
+    ctx = hx.get_context()
+    conn = np.load(golden("Elems_pennymesh24el_32.npy")).copy()
+    conn[1, 4] = 10 ** 6
+    path = os.path.join(str(tmp_path), "elems.npy")
+    np.save(path, conn)
+    h = ctypes.c_void_p()
+    rc = ctx.lib.hfpx_mesh_create_npy(ctx.handle, golden("Nodes_pennymesh24el_32.npy").encode(), path.encode(), 1, 1,
+                                      0.125, ctypes.byref(h))
+    assert rc == _lib.HFPX_ERR_INVALID
+    assert b"out of range" in ctx.lib.hfpx_last_error(ctx.handle)
+
+
+@pytest.mark.parametrize("kind", ["collinear", "repeated vertex"])
+def test_mesh_rejects_degenerate_element(hx, kind):
+    """A zero-area element has no frame (its geometry would divide by
+    zero): mesh creation refuses it instead of assembling inf / NaN."""
+    from hfpx3d_amd import _lib
+    print(f"case: degenerate element ({kind})", flush=True)
+    nods = np.load(golden("Nodes_pennymesh24el_32.npy"))[:3].copy()
+    conn = np.load(golden("Elems_pennymesh24el_32.npy")).astype(np.int64)[:3] - 1
+    a, b, c = conn[:, 5]
+    if kind == "collinear":
+        nods[:, c] = 0.5 * (nods[:, a] + nods[:, b])
+    else:
+        conn[2, 5] = a
+    with pytest.raises(_lib.HfpxError) as e:
+        hx.DeviceMesh(hx.Mesh_Geom_T(nods=nods, conn=conn), None, 0.125)
+    assert e.value.status == _lib.HFPX_ERR_INVALID and "degenerate" in str(e.value)

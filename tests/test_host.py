@@ -1,5 +1,5 @@
 """C++ host layer (config.toml semantics, mesh loaders, DoF handle) on the
-CPU, through hfpx3d_amd/hfpx3d_host_check (no device call)."""
+CPU, through tests/native/hfpx3d_host_check (no device call)."""
 import json
 import os
 import subprocess
@@ -11,7 +11,7 @@ import oracle_lib
 from oracle_lib import golden
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-CHECK = os.path.join(ROOT, "hfpx3d_amd", "hfpx3d_host_check")
+CHECK = os.path.join(ROOT, "tests", "native", "hfpx3d_host_check")
 
 # the reference's own configuration keys (config.toml), including the ones
 # its reader ignores (injection / fluid / contact)

@@ -263,18 +263,102 @@ def assert_blocks_match_or_reference_wrong(A, Ar, nods, conn, G, nu, tol=1e-9, m
             assert np.abs(M - Q)[tr].max() / np.abs(Q).max() < 1e-9
 
 
+PENNY_CASES = {"pennymesh121el_32": (1 / 2.6, 0.3), "pennymesh1025el_32": (0.4, 0.25)}
+
+
+def adjudicate_against_reference(A, Ar, nods, conn, G, nu, n_quad=24, n_mp=0, seed=0):
+    """Every (collocation point, source element) pair whose 3 x 18 entries
+    differ by more than 1e-12 of max|A| between the oracle's matrix A and the
+    reference's Ar must be explained:
+      * geometrically: the point lies in the thin cone (3e-2 rad) around the
+        extension of a source edge line beyond one of its vertices, the
+        region where the reference's formulas lose accuracy;
+      * numerically, for the n_quad worst pairs and n_quad random others:
+        on the traction rows of the pair's block the oracle agrees with
+        Kelvin quadrature to 1e-11 of max|A| (and 1e-8 of the block's own
+        largest entry: far-field pairs lose digits to cancellation, as the
+        30-digit checks show), while the reference's own per-pair kernel
+        (oracle/_ref), wherever it differs from the oracle by more than
+        1e-10 of max|A|, is at least ten times further from the quadrature;
+      * for the n_mp worst pairs, against a 30-digit evaluation of the
+        integral: the oracle within 1e-8 of the block, the reference off by
+        more than 1e-3 of it.
+    Returns the number of such pairs."""
+    import ref_adjudicate as ra
+    n_el = conn.shape[1]
+    idx, d = ra.suspect_pairs(A, Ar, n_el, 1e-12)
+    for (te, k, se) in idx:
+        _, _, h, z, tau, _ = ra.pair_frame(nods, conn, te, k, se)
+        ok, where = ra.on_edge_extension(tau, h, z)
+        assert ok, (te, k, se, where)
+    rng = np.random.default_rng(seed)
+    pick = list(range(min(n_quad, len(idx))))
+    if len(idx) > n_quad:
+        pick += sorted(rng.choice(np.arange(n_quad, len(idx)), size=min(n_quad, len(idx) - n_quad),
+                                  replace=False).tolist())
+    tr = ra.TRACTION_ROWS
+    mx = np.abs(Ar).max()
+    for q in pick:
+        te, k, se = idx[q]
+        vs, x, h, z, tau, sfm = ra.pair_frame(nods, conn, te, k, se)
+        Mo = O.local_submatrix(G, nu, h, z, tau, sfm)
+        Mr = REF.local_submatrix(G, nu, h, z, tau, sfm)
+        _, _, dv = ra.edge_line_position(tau, h, z)
+        Q = ra.quad_block(vs, x, G, nu, dv)
+        e_o, e_r = np.abs(Mo - Q)[tr].max(), np.abs(Mr - Q)[tr].max()
+        assert e_o < 1e-11 * mx and e_o < 1e-8 * np.abs(Q[tr]).max(), (te, k, se, e_o / mx)
+        if np.abs(Mo - Mr)[tr].max() > 1e-10 * mx:
+            assert e_r > 10 * e_o, (te, k, se, e_o / mx, e_r / mx)
+    for q in range(min(n_mp, len(idx))):
+        te, k, se = idx[q]
+        vs, x, h, z, tau, sfm = ra.pair_frame(nods, conn, te, k, se)
+        _, Mp = ra.mp_traction_rows(vs, x, G, nu)
+        sc = np.abs(Mp).max()
+        assert np.abs(O.local_submatrix(G, nu, h, z, tau, sfm)[tr] - Mp).max() / sc < 1e-8
+        assert np.abs(REF.local_submatrix(G, nu, h, z, tau, sfm)[tr] - Mp).max() / sc > 1e-3
+    return len(idx)
+
+
 @pytest.mark.parametrize("stem", ["pennymesh121el_32", "pennymesh1025el_32"])
-def test_assembly_rows_match_reference_build(stem):
+def test_assembly_rows_match_reference_fixture(stem):
+    """The committed reference rows (a few target elements of each penny
+    mesh): every entry within 1e-12 of max|A|, or its pair explained as in
+    adjudicate_against_reference (geometry; quadrature when the reference
+    build is present)."""
     d = np.load(golden(f"ref_rows_{stem}.npz"))
     nods, conn = load_mesh(stem)
     G, nu = float(d["G"]), float(d["nu"])
-    errs = []
+    assert (G, nu) == PENNY_CASES[stem]
+    import ref_adjudicate as ra
+    mx = np.abs(d["rows"]).max()
     for i, t in enumerate(d["targets"]):
-        A, _ = O.assemble(nods, conn, G, nu, targets=(int(t), int(t) + 1))
-        errs.append(np.abs(A[18 * t:18 * t + 18] - d["rows"][18 * i:18 * i + 18]).max())
-    # norm-wise: the reference's far-field/near-edge-line rounding is ~1e-7 of the
-    # largest entry at worst on these meshes (60-digit checks in DESIGN)
-    assert max(errs) / np.abs(d["rows"]).max() < 1e-6
+        t = int(t)
+        rows = O.assemble_rows(nods, conn, G, nu, t, t + 1)
+        diff = np.abs(rows - d["rows"][18 * i:18 * i + 18]).reshape(6, 3, -1, 18).max(axis=(1, 3)) / mx
+        for k, se in np.argwhere(diff > 1e-12):
+            _, _, h, z, tau, _ = ra.pair_frame(nods, conn, t, k, se)
+            ok, where = ra.on_edge_extension(tau, h, z)
+            assert ok, (t, k, se, where)
+
+
+@needs_ref
+@pytest.mark.parametrize("stem", ["pennymesh121el_32", "pennymesh1025el_32"])
+def test_assembly_matches_reference_build_everywhere(stem):
+    """C1 / C2 meshes, every block of the matrix (14,641 / 1,050,625 blocks)
+    against the reference build's make_3dbem_matrix_s: 1e-12 of max|A|
+    except at pairs whose collocation point lies on the extension of a
+    source edge line, each of which is shown to be the reference's error
+    (adjudicate_against_reference; 30-digit checks of the three worst on
+    penny1025)."""
+    G, nu = PENNY_CASES[stem]
+    nods, conn = load_mesh(stem)
+    A, _ = O.assemble(nods, conn, G, nu, 0.125, 0, nthreads=8)
+    Ar, _ = REF.assemble(nods, conn, G, nu, 0.125, 0, nthreads=8)
+    n_bad = adjudicate_against_reference(A, Ar, nods, conn, G, nu, n_quad=12,
+                                         n_mp=3 if stem == "pennymesh1025el_32" else 0)
+    # the documented scale of the reference defect on these ring meshes
+    # (pairs, of 6 n_el^2): C1 525 of 87,846; C2 44,861 of 6,303,750
+    assert n_bad < 0.01 * 6 * conn.shape[1] ** 2
 
 
 def _rot_blocks(A, Q):
