@@ -61,10 +61,15 @@ def test_kernel_matches_matlab_points(hx, k, pt):
 
 def test_kernel_matches_oracle_on_reference_cases(hx):
     """All 600 reference-build cases in one batched launch against the
-    oracle's evaluation of the same (h, z, tau, sfm)."""
+    oracle's evaluation of the same (h, z, tau, sfm).  The two evaluate the
+    same closed form in a different operation order, so they differ by
+    rounding only: 1e-11 of the block's largest entry where the block is
+    not a small difference of large edge terms, 1e-9 where it is (kinds 1,
+    3, 4: near field, in-plane outside, above an edge line, where the
+    oracle itself is pinned to the reference at 1e-8)."""
     print("case: 600 reference kernel cases", flush=True)
     d = np.load(golden("ref_kernel_cases.npz"))
-    hs, zs, taus, sfms, Ms, Gs, nus = [], [], [], [], [], [], []
+    hs, zs, taus, sfms, Ms, Gs, nus, kinds = [], [], [], [], [], [], [], []
     for v, p in zip(d["vert"], d["par"]):
         sfm, rt, tau, cp = O.el_geom(v)
         G, nu, h, y0, y1, kind = p
@@ -75,17 +80,22 @@ def test_kernel_matches_oracle_on_reference_cases(hx):
         Ms.append(O.local_submatrix(G, nu, h, y0 + 1j * y1, tau, sfm))
         Gs.append(G)
         nus.append(nu)
+        kinds.append(int(kind))
     # one launch per material (the C ABI takes one G, nu per call)
-    errs = []
+    errs = np.full(len(hs), np.nan)
     for G, nu in sorted(set(zip(Gs, nus))):
         idx = [i for i in range(len(hs)) if Gs[i] == G and nus[i] == nu]
         M = hx.make_local_3dbem_submatrix(1, G, nu, np.array([hs[i] for i in idx]), np.array([zs[i] for i in idx]),
                                           np.array([taus[i] for i in idx]), np.array([sfms[i] for i in idx]))
-        errs += [rel(M[q], Ms[i]) for q, i in enumerate(idx)]
-    errs = np.array(errs)
-    assert len(errs) == len(d["par"])
+        for q, i in enumerate(idx):
+            errs[i] = rel(M[q], Ms[i])
+    kinds = np.array(kinds)
+    assert np.isfinite(errs).all() and len(errs) == len(d["par"])
+    worst = {k: float(errs[kinds == k].max()) for k in sorted(set(kinds))}
+    print("max relative difference per kind:", worst)
     assert np.median(errs) < 1e-14, np.median(errs)
-    assert errs.max() < 1e-11, errs.max()
+    for k, e in worst.items():
+        assert e < (1e-9 if k in (1, 3, 4) else 1e-11), (k, e)
 
 
 def degenerate_cases(seed=7):
@@ -112,7 +122,8 @@ def degenerate_cases(seed=7):
 
 def test_kernel_at_degenerate_projections(hx):
     """72 points projecting onto vertices and edge lines: the HIP kernel
-    against the oracle (1e-12) and, point by point, against Kelvin
+    against the oracle (1e-10: rounding of the edge terms, which cancel for
+    points above an edge line's extension) and, point by point, against Kelvin
     quadrature (1e-9 of the block's largest entry; graded towards the
     element for the in-plane points)."""
     print("case: degenerate projections", flush=True)
@@ -134,7 +145,7 @@ def test_kernel_at_degenerate_projections(hx):
     n_quad = {"vertex": 0, "edge line": 0, "in-plane extension": 0}
     for i in range(len(cases)):
         Mo = O.local_submatrix(G, nu, hs[i], zs[i], taus[i], sfms[i])
-        assert rel(M[i], Mo) < 1e-12, (i, kinds[i], rel(M[i], Mo))
+        assert rel(M[i], Mo) < 1e-10, (i, kinds[i], rel(M[i], Mo))
         assert np.isfinite(M[i]).all()
         if kinds[i] == "in-plane extension":
             if n_quad["in-plane extension"] >= 6:
