@@ -109,6 +109,8 @@ SIGNATURES = {
     "hfpx_tune_get": (_c_int, [_c_void_p, _c_int, ctypes.POINTER(_c_int)]),
     "hfpx_local_submatrix": (_c_int, [_c_void_p, _c_int, _c_double, _c_double, _c_i64, _c_void_p, _c_void_p,
                                       _c_void_p, _c_void_p, _c_void_p]),
+    "hfpx_stress_noalloc": (_c_int, [_c_void_p, _c_void_p, _c_double, _c_double, _c_void_p, _c_i64, _c_void_p,
+                                     _c_i64, _c_void_p, _c_void_p, _c_i64]),
 }
 
 _lib = None

@@ -141,6 +141,9 @@ int hfpx_create(hfpx_ctx** out, int device, void* stream) {
     delete ctx;
     return HFPX_ERR_HIP;
   }
+  // the allocation-free stress workspace's device address (once per device);
+  // a failure here only disables hfpx_stress_noalloc
+  if (stress_scratch_address(ctx, &ctx->stress_scratch) != HFPX_OK) ctx->stress_scratch = nullptr;
   *out = ctx;
   return HFPX_OK;
 }
@@ -483,6 +486,17 @@ int hfpx_tune_get(const hfpx_ctx* ctx, int knob, int* value) {
   if (!ctx || !value || knob < 0 || knob >= HFPX_KNOB_COUNT) return HFPX_ERR_INVALID;
   *value = ctx->knob[knob];
   return HFPX_OK;
+}
+
+int hfpx_stress_noalloc(hfpx_ctx* ctx, hfpx_mesh* m, double shear_m, double poiss_r, const double* dd, int64_t ldd,
+                        const double* pts, int64_t n_pts, const double* sinf_host, double* S, int64_t ldS) {
+  if (!ctx || !m) return HFPX_ERR_INVALID;
+  if (!dd || !pts || !sinf_host || !S || n_pts < 1 || ldd < 6 * m->n_el || ldS < n_pts)
+    return set_error(ctx, HFPX_ERR_INVALID, "stress_noalloc: invalid arguments");
+  if (!(poiss_r < 1.0) || !(shear_m > 0.0))
+    return set_error(ctx, HFPX_ERR_INVALID, "need shear modulus > 0 and Poisson ratio < 1");
+  HFPX_TRY_HIP(ctx, hipSetDevice(ctx->device));
+  return launch_stress_noalloc(ctx, m, shear_m, poiss_r, dd, ldd, pts, n_pts, sinf_host, S, ldS);
 }
 
 int hfpx_stress(hfpx_ctx* ctx, hfpx_mesh* m, double shear_m, double poiss_r, const double* dd, int64_t ldd,

@@ -43,6 +43,9 @@ struct hfpx_ctx {
   };
   std::vector<ProfRec> prof_recs;
   std::vector<hipEvent_t> prof_pool;
+  // device address of the allocation-free stress workspace on this
+  // context's device (hfpx_stress.hip), set at creation
+  double* stress_scratch = nullptr;
 };
 
 // Device-resident mesh: raw nodes/connectivity/DoF map plus the element
@@ -245,6 +248,13 @@ size_t bgemm_x3_split_bytes(int64_t rtot, int64_t ctot);
 // stress postprocessing (hfpx_stress.hip)
 int launch_stress(hfpx_ctx* ctx, hfpx_mesh* m, double shear_m, double nu, const double* dd, int64_t ldd,
                   const double* pts, int64_t n_pts, const double* sinf_host, double* out, int64_t ldo);
+// the same on the library's file-scope device workspace: no allocation, no
+// synchronisation; stress_scratch_address resolves that workspace's address
+// on ctx's device (once per device), stress_scratch_doubles is its size
+int launch_stress_noalloc(hfpx_ctx* ctx, hfpx_mesh* m, double shear_m, double nu, const double* dd, int64_t ldd,
+                          const double* pts, int64_t n_pts, const double* sinf_host, double* out, int64_t ldo);
+int stress_scratch_address(hfpx_ctx* ctx, double** out);
+size_t stress_scratch_doubles();
 
 // mixed-precision solve (hfpx_solve.hip)
 int solve_ir(hfpx_ctx* ctx, const double* A, int64_t lda, const float* LU, int64_t ldlu,

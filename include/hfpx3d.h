@@ -217,6 +217,22 @@ int hfpx_stress(hfpx_ctx* ctx, hfpx_mesh* mesh, double shear_m, double poiss_r, 
                 int64_t ldd, const double* pts, int64_t n_pts, const double* sinf_host, double* S,
                 int64_t ldS);
 
+/* hfpx_stress without any device allocation (same arguments and result).
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory -- code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc.  Its workspace (the DD in element frames and per-chunk
+ * partial sums) is a 64 MiB device array of the library, one per GPU, held
+ * until the process ends: 18 n_el + 6 n_pts must not exceed 8,388,608
+ * doubles (else HFPX_ERR_INVALID; fewer element chunks are used where the
+ * allocating form would need more).  The far-field stress is passed by
+ * value.  Enqueues on the context's stream and returns without
+ * synchronising.  Calls on one GPU share the array: calls on one stream are
+ * ordered, calls that overlap in time (other streams or contexts on the same
+ * GPU) race. */
+int hfpx_stress_noalloc(hfpx_ctx* ctx, hfpx_mesh* mesh, double shear_m, double poiss_r, const double* dd,
+                        int64_t ldd, const double* pts, int64_t n_pts, const double* sinf_host, double* S,
+                        int64_t ldS);
+
 /* ---- dense solve -------------------------------------------------------- */
 /* In-place LU with partial pivoting of the n x n matrix A (device, lda),
  * dtype HFPX_F64 or HFPX_F32: P A = L U, ipiv (device, n int32) 1-based
