@@ -464,6 +464,11 @@ int hfpx_tune_set(hfpx_ctx* ctx, int knob, int value) {
     ctx->knob[knob] = value;
     return HFPX_OK;
   }
+  if (knob == HFPX_KNOB_GEMM_F64) {
+    if (value < 0 || value > 5) return set_error(ctx, HFPX_ERR_INVALID, "tune: GEMM_F64 must be 0 .. 5");
+    ctx->knob[knob] = value;
+    return HFPX_OK;
+  }
   if (knob == HFPX_KNOB_IR_FALLBACK) {
     if (value < 0 || value > 2) return set_error(ctx, HFPX_ERR_INVALID, "tune: IR_FALLBACK must be 0, 1 or 2");
     ctx->knob[knob] = value;

@@ -393,6 +393,149 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 }
 
 // ---------------------------------------------------------------------
+// FP64 C (M x N) -= A (M x K) * B (K x N) for the large Schur-complement
+// updates of the FP64 LU, on v_mfma_f64_16x16x4f64.
+//
+// Workgroup tile 128 x 128 (4 waves of 64 x 64, sixteen 16 x 16 MFMA tiles
+// each), K in chunks of 16 through double-buffered LDS, one barrier per
+// chunk.  The kernel is bounded to two waves per SIMD (accumulators in
+// AGPRs, at most 256 registers per lane), so two workgroups share a CU and
+// one's loads, LDS stores and barrier overlap the other's MFMAs: with one
+// wave per SIMD every barrier idles the MFMA pipe.  As in gemm_tile the
+// MFMA forms the transposed product, so 16 lanes write 16 consecutive rows
+// of one column of C.  LDS rows are padded by 16 doubles (half of the
+// banks), so the two half-waves' operand reads of rows k and k + 1 fall in
+// distinct banks.  Options as k_sgemm32: SG_INT (interior tiles, no
+// row / column guards), SG_SWZ (XCD-grouped walk in bands of `band` row
+// tiles), SG_CPRE (C preloaded into the accumulators, B staged negated,
+// plain store at the end).
+// ---------------------------------------------------------------------
+constexpr int DG_T = 128, DG_KC = 16, DG_P = DG_T + 16;
+
+template <int OPT>
+__device__ __forceinline__ void dgemm128_tile(double* __restrict__ sA, double* __restrict__ sB, double* __restrict__ C,
+                                              int64_t ldc, const double* __restrict__ A, int64_t lda,
+                                              const double* __restrict__ B, int64_t ldb, int64_t M, int64_t N,
+                                              int64_t K, int64_t row0, int64_t col0) {
+  // sA[buf][k][i] = A(row0 + i, k0 + k), sB[buf][k][j] = (+-) B(k0 + k, col0 + j)
+  constexpr int L = DG_KC * DG_P;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave & 1) * 64, wc = (wave >> 1) * 64;
+  const int l16 = lane & 15, lq = lane >> 4;
+  const int mrem = (int)((M - row0) < DG_T ? (M - row0) : DG_T);
+  const int nrem = (int)((N - col0) < DG_T ? (N - col0) : DG_T);
+  constexpr bool INT = (OPT & SG_INT) != 0, CPRE = (OPT & SG_CPRE) != 0;
+
+  // acc[tc][tr][v]: row wr + 16 tr + l16, column wc + 16 tc + 4 v + lq.
+  // C is addressed by 32-bit offsets from the tile's corner (the host keeps
+  // 128 ldc below 2^31), so the 64 addresses cost no 64-bit registers.
+  double* __restrict__ Ct = C + row0 + col0 * ldc;
+  const int ld = (int)ldc;
+  const int off0 = wr + l16 + (wc + lq) * ld;
+  doublex4 acc[4][4];
+#pragma unroll
+  for (int tc = 0; tc < 4; ++tc)
+#pragma unroll
+    for (int tr = 0; tr < 4; ++tr)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = wr + 16 * tr + l16, j = wc + 16 * tc + 4 * v + lq;
+        if constexpr (CPRE)
+          acc[tc][tr][v] = (INT || (i < mrem && j < nrem)) ? Ct[off0 + 16 * tr + (16 * tc + 4 * v) * ld] : 0.0;
+        else
+          acc[tc][tr][v] = 0.0;
+      }
+
+  // loaders: A row tid % 128, k = tid / 128 + 2 q; B k = tid % 16, column
+  // tid / 16 + 16 q (consecutive lanes: consecutive addresses)
+  const int ai = tid & 127, ak = tid >> 7, bk = tid & 15, bj = tid >> 4;
+  const double* pa = A + (row0 + ai) + (int64_t)ak * lda;
+  const double* pb = B + bk + (col0 + bj) * ldb;
+  double ra[8], rb[8];
+  auto load_global = [&](int64_t k0) {
+    const bool kfull = k0 + DG_KC <= K;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const bool ok = (INT || ai < mrem) && (kfull || k0 + ak + 2 * q < K);
+      ra[q] = ok ? pa[(k0 + 2 * q) * lda] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const bool ok = (INT || bj + 16 * q < nrem) && (kfull || k0 + bk < K);
+      rb[q] = ok ? pb[k0 + (int64_t)(16 * q) * ldb] : 0.0;
+    }
+  };
+  auto store_lds = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) sA[buf * L + (ak + 2 * q) * DG_P + ai] = ra[q];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) sB[buf * L + bk * DG_P + bj + 16 * q] = CPRE ? -rb[q] : rb[q];
+  };
+
+  const int nchunks = (int)((K + DG_KC - 1) / DG_KC);
+  load_global(0);
+  store_lds(0);
+  __syncthreads();
+  for (int c = 0; c < nchunks; ++c) {
+    const int buf = c & 1;
+    if (c + 1 < nchunks) load_global((int64_t)(c + 1) * DG_KC);
+    const double* qa = sA + buf * L + wr + l16;
+    const double* qb = sB + buf * L + wc + l16;
+#pragma unroll
+    for (int k4 = 0; k4 < DG_KC; k4 += 4) {
+      const int kk = (k4 + lq) * DG_P;
+      double fu[4], fl[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) fu[t] = qb[kk + 16 * t];  // B^T rows = C columns
+#pragma unroll
+      for (int t = 0; t < 4; ++t) fl[t] = qa[kk + 16 * t];  // A^T columns = C rows
+#pragma unroll
+      for (int tc = 0; tc < 4; ++tc)
+#pragma unroll
+        for (int tr = 0; tr < 4; ++tr)
+          acc[tc][tr] = __builtin_amdgcn_mfma_f64_16x16x4f64(fu[tc], fl[tr], acc[tc][tr], 0, 0, 0);
+    }
+    if (c + 1 < nchunks) store_lds(buf ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int tc = 0; tc < 4; ++tc)
+#pragma unroll
+    for (int tr = 0; tr < 4; ++tr) {
+      const int i = wr + 16 * tr + l16;
+      if (!INT && i >= mrem) continue;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int j = wc + 16 * tc + 4 * v + lq;
+        if (INT || j < nrem) {
+          double* pc = Ct + off0 + 16 * tr + (16 * tc + 4 * v) * ld;
+          if constexpr (CPRE)
+            *pc = acc[tc][tr][v];
+          else
+            *pc -= acc[tc][tr][v];
+        }
+      }
+    }
+}
+
+template <int OPT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_dgemm128(
+    double* __restrict__ C, int64_t ldc, const double* __restrict__ A, int64_t lda, const double* __restrict__ B,
+    int64_t ldb, int64_t M, int64_t N, int64_t K, int band) {
+  __shared__ double sA[2 * DG_KC * DG_P];
+  __shared__ double sB[2 * DG_KC * DG_P];
+  const int64_t tm = (M + DG_T - 1) / DG_T, tn = (N + DG_T - 1) / DG_T;
+  int64_t tile_r, tile_c;
+  if constexpr ((OPT & SG_SWZ) != 0) {
+    sg_band_tile(sg_xcd_order(blockIdx.x, tm * tn), tm, tn, tile_r, tile_c, band);
+  } else {
+    tile_r = blockIdx.x % tm;
+    tile_c = blockIdx.x / tm;
+  }
+  dgemm128_tile<OPT>(sA, sB, C, ldc, A, lda, B, ldb, M, N, K, tile_r * DG_T, tile_c * DG_T);
+}
+
+// ---------------------------------------------------------------------
 // FP64 P_z = -A(:, slice z) X(slice z, :) for N <= 64 and long K (the
 // residuals b - A x of the mixed-precision solve): A is streamed once, so
 // every lane loads its MFMA operands straight from HBM into registers (no
@@ -558,6 +701,58 @@ static void launch_sgemm32(hfpx_ctx* ctx, float* C, int64_t ldc, const float* A,
 
 int gemm_f32_variants() { return 15; }
 
+// FP64 128 x 128 product: interior tiles by the unguarded kernel, then the
+// bottom band (all columns) and the right band (interior rows) guarded.
+// variant (HFPX_KNOB_GEMM_F64): 0 = default (3), 2 = plain walk, C read
+// and written at the end; 3 = XCD-grouped walk, C preloaded; 4 = 3 with the
+// plain walk; 5 = 3 with bands of 4 row tiles.
+static int launch_dgemm128(hfpx_ctx* ctx, double* C, int64_t ldc, const double* A, int64_t lda, const double* B,
+                           int64_t ldb, int64_t M, int64_t N, int64_t K, int variant) {
+  if (variant == 0) variant = 3;
+  const int band = variant == 5 ? 4 : SG_GROUP;
+  const int64_t Mi = M / DG_T * DG_T, Ni = N / DG_T * DG_T;
+  auto grid = [](int64_t m, int64_t n) { return ((m + DG_T - 1) / DG_T) * ((n + DG_T - 1) / DG_T); };
+  struct Part {
+    double* c;
+    const double* a;
+    const double* b;
+    int64_t m, n;
+    bool interior;
+  } parts[3] = {{C, A, B, Mi, Ni, true},
+                {C + Mi, A + Mi, B, M - Mi, N, false},
+                {C + Ni * ldc, A, B + Ni * ldb, Mi, N - Ni, false}};
+  for (const Part& p : parts) {
+    if (p.m <= 0 || p.n <= 0) continue;
+    const int64_t nt = grid(p.m, p.n);
+    if (nt > 0x7fffffff) return set_error(ctx, HFPX_ERR_INVALID, "gemm: too many tiles");
+    const dim3 g((unsigned)nt);
+    if (variant == 2) {
+      if (p.interior)
+        hipLaunchKernelGGL(k_dgemm128<SG_INT>, g, dim3(256), 0, ctx->stream, p.c, ldc, p.a, lda, p.b, ldb, p.m, p.n,
+                           K, band);
+      else
+        hipLaunchKernelGGL(k_dgemm128<0>, g, dim3(256), 0, ctx->stream, p.c, ldc, p.a, lda, p.b, ldb, p.m, p.n, K,
+                           band);
+    } else if (variant == 4) {
+      if (p.interior)
+        hipLaunchKernelGGL(k_dgemm128<SG_INT | SG_CPRE>, g, dim3(256), 0, ctx->stream, p.c, ldc, p.a, lda, p.b, ldb,
+                           p.m, p.n, K, band);
+      else
+        hipLaunchKernelGGL(k_dgemm128<SG_CPRE>, g, dim3(256), 0, ctx->stream, p.c, ldc, p.a, lda, p.b, ldb, p.m, p.n,
+                           K, band);
+    } else {
+      if (p.interior)
+        hipLaunchKernelGGL(k_dgemm128<SG_INT | SG_SWZ | SG_CPRE>, g, dim3(256), 0, ctx->stream, p.c, ldc, p.a, lda,
+                           p.b, ldb, p.m, p.n, K, band);
+      else
+        hipLaunchKernelGGL(k_dgemm128<SG_SWZ | SG_CPRE>, g, dim3(256), 0, ctx->stream, p.c, ldc, p.a, lda, p.b, ldb,
+                           p.m, p.n, K, band);
+    }
+  }
+  HFPX_TRY_HIP(ctx, hipGetLastError());
+  return HFPX_OK;
+}
+
 template <typename T>
 int gemm_sub(hfpx_ctx* ctx, T* C, int64_t ldc, const T* A, int64_t lda, const T* B,
                     int64_t ldb, int64_t M, int64_t N, int64_t K) {
@@ -646,6 +841,16 @@ int gemm_sub(hfpx_ctx* ctx, T* C, int64_t ldc, const T* A, int64_t lda, const T*
     hipLaunchKernelGGL(k_splitk_sum<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream, C, ldc, P,
                        M, N, (int)ks);
     done = true;
+  }
+  if constexpr (sizeof(T) == 8) {
+    // the LU's Schur-complement updates: interior tiles unguarded, the edge
+    // bands guarded; HFPX_KNOB_GEMM_F64 selects the walk / epilogue
+    const int v64 = ctx->knob[HFPX_KNOB_GEMM_F64];
+    if (!done && !skinny && v64 != 1 && M >= 256 && N >= 128) {
+      int rc = launch_dgemm128(ctx, (double*)C, ldc, (const double*)A, lda, (const double*)B, ldb, M, N, K, v64);
+      if (rc) return rc;
+      done = true;
+    }
   }
   if (!done) {
     if (skinny)

@@ -374,7 +374,14 @@ int hfpx_batch_matrix(hfpx_ctx* ctx, hfpx_batch* batch, int crack, double* A_hos
  * panel returns at entry and getrf reports HFPX_ERR_HIP (a host-forced test
  * of the error path; every kernel still exits normally).  0 = off. */
 #define HFPX_KNOB_TEST_PANEL_TIMEOUT 6
-#define HFPX_KNOB_COUNT 7
+/* HFPX_KNOB_GEMM_F64: FP64 Schur-complement products with M >= 256 and
+ * N >= 128: 0 = default (3); 1 = the generic 16x16x4 kernel (one wave per
+ * SIMD); 2 = 128 x 128 tiles at two waves per SIMD, unguarded interior,
+ * C read and written at the end; 3 = 2 with an XCD-grouped tile walk and C
+ * preloaded into the accumulators; 4 = 3 without the XCD-grouped walk;
+ * 5 = 3 with bands of 4 row tiles. */
+#define HFPX_KNOB_GEMM_F64 7
+#define HFPX_KNOB_COUNT 8
 /* Set / read one knob; HFPX_ERR_INVALID for an unknown knob or value. */
 int hfpx_tune_set(hfpx_ctx* ctx, int knob, int value);
 int hfpx_tune_get(const hfpx_ctx* ctx, int knob, int* value);

@@ -101,6 +101,29 @@ def test_sgemm_bf16_split_is_fp32_accurate(env, scaled):
     assert err[14] < 3 * err[2] + 1e-7, err
 
 
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("M,N,K", [(1000, 900, 300), (384, 256, 1024), (2304, 1280, 77)])
+def test_dgemm_variants(env, variant, M, N, K):
+    """FP64 Schur-complement products through every HFPX_KNOB_GEMM_F64
+    variant (interior tiles unguarded, edge bands guarded, K tails) against
+    numpy, 1e-13 of the largest entry of C."""
+    from hfpx3d_amd import _lib
+    torch, hx = env
+    print(f"case: dgemm variant {variant} {M}x{N}x{K}", flush=True)
+    rng = np.random.default_rng(M + N + K + variant)
+    A, B, C = rng.normal(size=(M, K)), rng.normal(size=(K, N)), rng.normal(size=(M, N))
+    ctx = hx.get_context()
+    hx.tune_set(_lib.KNOB_GEMM_F64, variant, ctx)
+    try:
+        Cd = to_dev(torch, hx, C, torch.float64)
+        hx.gemm_sub(Cd, to_dev(torch, hx, A, torch.float64), to_dev(torch, hx, B, torch.float64), ctx)
+        got = Cd.cpu().numpy()
+    finally:
+        hx.tune_set(_lib.KNOB_GEMM_F64, 0, ctx)
+    ref = C - A @ B
+    assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-13
+
+
 def test_tune_rejects_unknown_variant(env):
     torch, hx = env
     from hfpx3d_amd import _lib
