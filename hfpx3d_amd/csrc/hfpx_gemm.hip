@@ -405,7 +405,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 // MFMA forms the transposed product, so 16 lanes write 16 consecutive rows
 // of one column of C.  LDS rows are padded by 16 doubles (half of the
 // banks), so the two half-waves' operand reads of rows k and k + 1 fall in
-// distinct banks.  Options as k_sgemm32: SG_INT (interior tiles, no
+// distinct banks, and the B tile's columns are XOR-swizzled within groups
+// of 16 by (k & 14): its transposing stores (16 k rows per half-wave) then
+// hit distinct banks too, where unswizzled rows k and k + 2 share them
+// (measured: a third of the cycles stalled on bank conflicts).  Options as
+// k_sgemm32: SG_INT (interior tiles, no
 // row / column guards), SG_SWZ (XCD-grouped walk in bands of `band` row
 // tiles), SG_CPRE (C preloaded into the accumulators, B staged negated,
 // plain store at the end).
@@ -468,8 +472,9 @@ __device__ __forceinline__ void dgemm128_tile(double* __restrict__ sA, double* _
   auto store_lds = [&](int buf) {
 #pragma unroll
     for (int q = 0; q < 8; ++q) sA[buf * L + (ak + 2 * q) * DG_P + ai] = ra[q];
+    const int swz = bk & 14;  // column swizzle of row bk (bj < 16)
 #pragma unroll
-    for (int q = 0; q < 8; ++q) sB[buf * L + bk * DG_P + bj + 16 * q] = CPRE ? -rb[q] : rb[q];
+    for (int q = 0; q < 8; ++q) sB[buf * L + bk * DG_P + 16 * q + (bj ^ swz)] = CPRE ? -rb[q] : rb[q];
   };
 
   const int nchunks = (int)((K + DG_KC - 1) / DG_KC);
@@ -480,13 +485,14 @@ __device__ __forceinline__ void dgemm128_tile(double* __restrict__ sA, double* _
     const int buf = c & 1;
     if (c + 1 < nchunks) load_global((int64_t)(c + 1) * DG_KC);
     const double* qa = sA + buf * L + wr + l16;
-    const double* qb = sB + buf * L + wc + l16;
+    const double* qb = sB + buf * L + wc;
 #pragma unroll
     for (int k4 = 0; k4 < DG_KC; k4 += 4) {
       const int kk = (k4 + lq) * DG_P;
+      const int cb = l16 ^ ((k4 + lq) & 14);  // the store's swizzle of row k4 + lq
       double fu[4], fl[4];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) fu[t] = qb[kk + 16 * t];  // B^T rows = C columns
+      for (int t = 0; t < 4; ++t) fu[t] = qb[kk + 16 * t + cb];  // B^T rows = C columns
 #pragma unroll
       for (int t = 0; t < 4; ++t) fl[t] = qa[kk + 16 * t];  // A^T columns = C rows
 #pragma unroll
