@@ -26,7 +26,15 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAKS = {"hbm_GBs": 8000.0, "fp64_mfma_TFs": 78.6, "fp32_mfma_TFs": 157.3, "bf16_mfma_TFs": 2500.0}
+PEAKS = {"hbm_GBs": 8000.0, "fp64_mfma_TFs": 78.6, "fp64_valu_TFs": 78.6, "fp32_mfma_TFs": 157.3,
+         "bf16_mfma_TFs": 2500.0}
+# FP64 VALU flops per (point, element) pair evaluation of the element
+# integral: the kernel's SQ_INSTS_VALU_FLOPS_FP64 + SQ_INSTS_VALU_FLOPS_FP64_TRANS
+# counts x 64 (the counters count per wave instruction; calibrated on a kernel
+# of known flop count) / pairs, from rocprofv3 --pmc of
+# `bench.py --workload assemble72k` (k_assemble: 9.6e7 pairs) and
+# `--workload stress1m` (k_stress: 6e9 pairs); profiles/valu_pmc_assemble72k_stress1m.md
+VALU_FLOPS_PER_PAIR = {"assemble72k": 6992.5, "stress1m": 7140.2}
 # The FP32 trailing updates run as six bf16 MFMA products of three-way operand
 # splits: their FP32-equivalent ceiling is the dense bf16 peak / 6.
 X3_PRODUCTS = 6
@@ -48,8 +56,9 @@ def parse():
     ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-el", type=int, default=1400)
+    ap.add_argument("--gemm-dtype", default="f32", choices=["f32", "f64"], help="gemm72k: FP32 or FP64 product")
     ap.add_argument("--workload", default="ellipse72k",
-                    choices=["ellipse72k", "wavy108k", "batch256", "gemm72k", "vc72k"],
+                    choices=["ellipse72k", "wavy108k", "batch256", "gemm72k", "vc72k", "assemble72k", "stress1m"],
                     help="ellipse72k: the headline configuration (default); wavy108k: the non-planar "
                          "6,000-element crack with stresses at 10^6 points; batch256: 256 independent cracks; "
                          "vc72k: volume-control steps (truncation + FP32 LU + FP64 refinement) on the 72k ellipse")
@@ -90,14 +99,34 @@ def c3_problem(n_el, nrhs, seed=1):
     return nods, conn, S
 
 
-def cpu_baseline(n_el_sample, G, nu):
-    """Reference (oracle/_ref) assembly on the same geometry with fewer
-    elements; falls back to the oracle restatement.  Returns a dict."""
+def host_cpu():
+    """(model name, cores this process may run on, cores used): the GPU box
+    gives one GPU's share of its cores (16), whatever the machine has."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    visible = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return model, visible, max(1, min(16, visible))
+
+
+def cpu_baseline(n_el_sample, G, nu, n_target, nrhs):
+    """The reference's own assembly (oracle/_ref, else the oracle
+    restatement) on the same geometry with fewer elements, and LAPACK
+    dgetrf + dgetrs (the reference's il::LU, LU.h:79-138) through scipy at
+    two sizes, extrapolated to n_target by n^3 from the larger one.  Every
+    leg runs on the same number of host threads (OpenMP for the assembly,
+    the BLAS thread pool pinned with threadpoolctl).  Returns a dict."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
     from hfpx3d_amd import meshgen
-    threads = max(1, min(16, os.cpu_count() or 1))
+    model, visible, threads = host_cpu()
     R = meshgen.rotation([1.0, 0.0, 0.0], np.pi / 6)
     nods, conn = meshgen.ellipse(n_el_sample, 1.0, 0.5, R)
     ref = oracle_lib.reference_or_none()
@@ -111,20 +140,29 @@ def cpu_baseline(n_el_sample, G, nu):
     n = A.shape[0]
     del A
     out = {"value": n * n / dt / 1e9, "unit": "Gentries/s", "cores": threads, "kind": kind,
+           "cpu_model": model, "host_cores_visible": visible,
            "sample": f"{'reference make_3dbem_matrix_s (oracle/_ref)' if ref else 'oracle C restatement'} "
                      f"on the same rotated ellipse meshed with {n_el_sample} elements ({n} DoF), "
                      f"{threads} OpenMP threads, {dt:.1f} s; entries/s is the per-entry cost"}
-    # the reference's LU is LAPACK dgetrf (not buildable here without LAPACKE):
-    # time LAPACK dgetrf+dgetrs through scipy at a smaller n and scale by n^3
     try:
         import scipy.linalg as sl
-        m = 4000
-        M = np.random.default_rng(0).normal(size=(m, m)) + m ** 0.5 * np.eye(m)
-        t = time.perf_counter()
-        lu = sl.lu_factor(M, overwrite_a=True, check_finite=False)
-        sl.lu_solve(lu, np.ones((m, 64)), check_finite=False)
-        ts = time.perf_counter() - t
-        out["solve_s_lapack_n4000"] = ts
+        from threadpoolctl import threadpool_limits
+        lapack = {}
+        with threadpool_limits(limits=threads, user_api="blas"):
+            for m in (8000, 16000):
+                M = np.random.default_rng(0).normal(size=(m, m)) + m ** 0.5 * np.eye(m)
+                Bm = np.ones((m, nrhs))
+                t = time.perf_counter()
+                lu = sl.lu_factor(M, overwrite_a=True, check_finite=False)
+                sl.lu_solve(lu, Bm, check_finite=False)
+                lapack[m] = time.perf_counter() - t
+                del M, lu
+        out["lapack_getrf_getrs_s"] = {str(m): round(v, 3) for m, v in lapack.items()}
+        out["lapack_threads"] = threads
+        out["lapack_n3_ratio_16k_8k"] = round(lapack[16000] / lapack[8000], 2)
+        out["solve_s_extrapolated"] = round(lapack[16000] * (n_target / 16000.0) ** 3, 1)
+        out["solve_extrapolation"] = (f"LAPACK dgetrf + dgetrs ({nrhs} RHS) through scipy at n = 16,000, "
+                                      f"scaled by (n / 16,000)^3 to n = {n_target}")
     except Exception as ex:  # pragma: no cover
         out["solve_error"] = str(ex)
     return out
@@ -144,6 +182,8 @@ def main():
         return run_gemm(args)
     if args.workload == "vc72k":
         return run_vc(args)
+    if args.workload in ("assemble72k", "stress1m"):
+        return run_valu(args)
     import numpy as np
     import torch
     from hfpx3d_amd import hfp3d as hx, _lib
@@ -259,10 +299,15 @@ def main():
                                "FETCH_SIZE x2.0 + WRITE_SIZE x1.0, scales from a 2 GiB device copy",
                 "launches_per_step": g["launches"], "avg_launch_ms": g["ms"] / max(1, g["launches"]),
                 "kernel_ms_per_step": g["ms"]}
-    assembly_roofline = {"kernel": "k_assemble (FP64 + FP32 matrix write)", "bound": "hbm",
-                         "achieved": round(asm_gbs, 2), "peak": PEAKS["hbm_GBs"], "unit": "GB/s",
-                         "frac": round(asm_gbs / PEAKS["hbm_GBs"], 4), "kernel_ms": a["ms"],
-                         "note": "compute-bound (VALU FP64); HBM write floor shown"}
+    pairs = 6.0 * n_el * n_el
+    asm_tf = VALU_FLOPS_PER_PAIR["assemble72k"] * pairs / (a["ms"] / 1e3) / 1e12 if a["ms"] > 0 else 0.0
+    assembly_roofline = {"kernel": "k_assemble (FP64 element integrals, FP64 + FP32 matrix write)", "bound": "valu",
+                         "achieved": round(asm_tf, 2), "peak": PEAKS["fp64_valu_TFs"], "unit": "TFLOP/s",
+                         "frac": round(asm_tf / PEAKS["fp64_valu_TFs"], 4), "kernel_ms": a["ms"],
+                         "pairs": pairs, "flops_per_pair": VALU_FLOPS_PER_PAIR["assemble72k"],
+                         "hbm_write_GBs": round(asm_gbs, 2), "hbm_frac": round(asm_gbs / PEAKS["hbm_GBs"], 4),
+                         "note": "FP64 VALU flops per pair from hardware counters (VALU_FLOPS_PER_PAIR); the "
+                                 "matrix write is the secondary HBM bound"}
 
     result = {
         "metric": "assembly Gentries/s; solve s at 72k DOF, FP32-IR vs FP64 (rel err<=1e-10)",
@@ -296,28 +341,43 @@ def main():
             A2.copy_(A64)
             X64.copy_(B)
         e0, e1 = ev(), ev()
+        # HIP events around the FP64 trailing-update GEMM launches only
+        ctx.check(lib.hfpx_profile_classes(ctx.handle, 1 << _lib.KERNEL_CLASSES["gemm_f64"]))
+        ctx.check(lib.hfpx_profile(ctx.handle, 0 if args.no_lib_prof else 1))
         e0.record(ctx.stream)
         ctx.check(lib.hfpx_getrf(ctx.handle, _lib.HFPX_F64, ptr(A2), n, n, ptr(ipiv), __import__("ctypes").byref(info)))
         ctx.check(lib.hfpx_getrs(ctx.handle, _lib.HFPX_F64, ptr(A2), n, ptr(ipiv), n, ptr(X64), n, args.nrhs))
         e1.record(ctx.stream)
         ctx.stream.synchronize()
         t64 = e0.elapsed_time(e1) / 1e3
+        g64 = read_profile(ctx, 1)["gemm_f64"]
+        ctx.check(lib.hfpx_profile(ctx.handle, 0))
         with torch.cuda.stream(ctx.stream):
             err = ((X - X64).abs().amax(dim=0) / X64.abs().amax(dim=0)).max().item()
         ctx.stream.synchronize()
         result["fp64_path"] = {"solve_s": round(t64, 4), "assembly_plus_solve_s": round(t64 + t_asm, 4),
                                "max_rel_err_mixed_vs_fp64": err,
-                               "speedup_mixed_vs_fp64": round((t64 + t_asm) / (t_solve + t_asm), 3)}
+                               "speedup_mixed_vs_fp64": round((t64 + t_asm) / (t_solve + t_asm), 3),
+                               "getrf_TFs": round(2.0 / 3.0 * n ** 3 / t64 / 1e12, 2)}
+        if g64["ms"] > 0:
+            tf64 = g64["flops"] / (g64["ms"] / 1e3) / 1e12
+            result["fp64_path"]["gemm_f64"] = {
+                "kernel": "k_dgemm128 (v_mfma_f64_16x16x4f64, 128 x 128 tiles, two waves per SIMD) + small "
+                          "products on k_gemm_sub<double>",
+                "launches": g64["launches"], "ms": g64["ms"], "achieved_TFs": round(tf64, 2),
+                "peak_TFs": PEAKS["fp64_mfma_TFs"], "frac": round(tf64 / PEAKS["fp64_mfma_TFs"], 4)}
         log(f"bench: FP64 path solve {t64:.3f} s; mixed vs FP64 max rel err {err:.2e}")
         del A2, X64
 
     if not args.no_cpu_baseline:
         log("bench: CPU baseline")
         try:
-            cb = cpu_baseline(args.cpu_sample_el, G, nu)
+            cb = cpu_baseline(args.cpu_sample_el, G, nu, n, args.nrhs)
             result["cpu_baseline"] = cb
-            if "solve_s_lapack_n4000" in cb:
-                cb["solve_s_extrapolated_72k"] = round(cb["solve_s_lapack_n4000"] * (n / 4000.0) ** 3, 1)
+            if "solve_s_extrapolated" in cb:
+                cpu_total = n * n / (cb["value"] * 1e9) + cb["solve_s_extrapolated"]
+                cb["assembly_plus_solve_s_extrapolated"] = round(cpu_total, 1)
+                result["speedup_vs_cpu_baseline"] = round(cpu_total / (t_asm + t_solve), 1)
         except Exception as ex:
             result["cpu_baseline"] = {"error": str(ex)}
     print(json.dumps(result))
@@ -346,8 +406,9 @@ def run_gemm(args):
     K = 1024
     ctx = hx.get_context(0)
     lib = ctx.lib
+    f64 = args.gemm_dtype == "f64"
     if args.gemm_variant:
-        hx.tune_set(_lib.KNOB_GEMM_F32, args.gemm_variant, ctx)
+        hx.tune_set(_lib.KNOB_GEMM_F64 if f64 else _lib.KNOB_GEMM_F32, args.gemm_variant, ctx)
     if args.calib_copy_gib > 0:
         cnt = int(args.calib_copy_gib * 2 ** 30) // 4
         with torch.cuda.stream(ctx.stream):
@@ -357,13 +418,15 @@ def run_gemm(args):
         ctx.stream.synchronize()
         log(f"bench: calibration copy of {cnt * 4} bytes done")
         del src, dst
+    dt = torch.float64 if f64 else torch.float32
     with torch.cuda.stream(ctx.stream):
-        C = torch.randn(M * N, dtype=torch.float32, device=ctx.device)
-        A = torch.randn(M * K, dtype=torch.float32, device=ctx.device)
-        B = torch.randn(K * N, dtype=torch.float32, device=ctx.device)
+        C = torch.randn(M * N, dtype=dt, device=ctx.device)
+        A = torch.randn(M * K, dtype=dt, device=ctx.device)
+        B = torch.randn(K * N, dtype=dt, device=ctx.device)
     ctx.stream.synchronize()
     ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
-    call = lambda: ctx.check(lib.hfpx_gemm_sub(ctx.handle, _lib.HFPX_F32, hx._ptr(C), M, hx._ptr(A), M,  # noqa: E731
+    code = _lib.HFPX_F64 if f64 else _lib.HFPX_F32
+    call = lambda: ctx.check(lib.hfpx_gemm_sub(ctx.handle, code, hx._ptr(C), M, hx._ptr(A), M,  # noqa: E731
                                                hx._ptr(B), K, M, N, K))
     for _ in range(args.warmup):
         call()
@@ -378,20 +441,105 @@ def run_gemm(args):
         times.append(e0.elapsed_time(e1) / 1e3)
     t = float(np.mean(times))
     flops = 2.0 * M * N * K
-    alg_bytes = 4.0 * (2.0 * M * N + M * K + K * N)
-    if not args.gemm_variant or args.gemm_variant == 13:
+    alg_bytes = (8.0 if f64 else 4.0) * (2.0 * M * N + M * K + K * N)
+    if f64:
+        peak, basis = PEAKS["fp64_mfma_TFs"], "FP64 MFMA"
+    elif not args.gemm_variant or args.gemm_variant in (13, 14):
         peak, basis = X3_PEAK, "FP32-equivalent: dense bf16 MFMA 2500 TFLOP/s / 6 products"
     else:
         peak, basis = PEAKS["fp32_mfma_TFs"], "FP32 MFMA"
-    result = {"metric": "FP32 trailing-update GEMM TFLOP/s (72k LU, first update)", "value": round(flops / t / 1e12, 3),
+    name = "FP64" if f64 else "FP32"
+    result = {"metric": f"{name} trailing-update GEMM TFLOP/s (72k LU, first update)", "value": round(flops / t / 1e12, 3),
               "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
               "ms_per_step": round(t * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-              "dtype": "f32 (bf16 MFMA on three-way splits)" if not args.gemm_variant or args.gemm_variant == 13
-              else "f32", "data": "synthetic",
-              "config": {"workload": f"C ({M} x {N}) -= A ({M} x {K}) B ({K} x {N}), FP32"},
+              "dtype": "f64" if f64 else ("f32 (bf16 MFMA on three-way splits)"
+                                          if not args.gemm_variant or args.gemm_variant in (13, 14) else "f32"),
+              "data": "synthetic",
+              "config": {"workload": f"C ({M} x {N}) -= A ({M} x {K}) B ({K} x {N}), {name}"},
               "algorithmic_bytes": alg_bytes,
               "roofline": {"bound": "mfma", "achieved": round(flops / t / 1e12, 3), "peak": round(peak, 1),
                            "unit": "TFLOP/s", "frac": round(flops / t / 1e12 / peak, 4), "peak_basis": basis}}
+    print(json.dumps(result))
+
+
+def run_valu(args):
+    """The FP64-VALU-bound kernels alone (no LU, for hardware counters):
+    assemble72k = the mixed assembly (FP64 + FP32 matrix) and the 64
+    right-hand sides of the headline ellipse; stress1m = stresses of the
+    6,000-element wavy crack at 10^6 points of [-2, 2]^3 (seed 3) from a
+    smooth synthetic opening (the solve is not timed here)."""
+    import numpy as np
+    import torch
+    from hfpx3d_amd import hfp3d as hx, meshgen
+    ctx = hx.get_context(0)
+    lib = ctx.lib
+    ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+    G, nu = 1.0 / 2.6, 0.3
+    if args.workload == "assemble72k":
+        nods, conn, S = c3_problem(args.n_el, args.nrhs)
+        dm = hx.DeviceMesh(hx.Mesh_Geom_T(nods=nods, conn=conn), None, 0.125, ctx)
+        n = dm.n_dof
+        A64, lda = hx.alloc_colmajor(n, n, torch.float64, ctx)
+        A32, lda32 = hx.alloc_colmajor(n, n, torch.float32, ctx)
+        B, ldb = hx.alloc_colmajor(n, args.nrhs, torch.float64, ctx)
+        Sf = np.asfortranarray(S)
+
+        def call():
+            ctx.check(lib.hfpx_assemble_mixed(ctx.handle, dm.handle, G, nu, hx._ptr(A64), lda, hx._ptr(A32), lda32))
+            with torch.cuda.stream(ctx.stream):
+                B.zero_()
+            ctx.check(lib.hfpx_rhs_sinf(ctx.handle, dm.handle, hx._np_ptr(Sf), args.nrhs, hx._ptr(B), ldb))
+        units, unit, pairs = n * n / 1e9, "Gentries/s", 6.0 * conn.shape[1] ** 2
+        wl = (f"assembly of the rotated elliptical crack 1x0.5, {conn.shape[1]} quadratic triangles, {n} DoF "
+              f"(FP64 + FP32 matrices) and {args.nrhs} right-hand sides")
+    else:
+        nods, conn = meshgen.wavy_disk(6000)
+        n_el = conn.shape[1]
+        dm = hx.DeviceMesh(hx.Mesh_Geom_T(nods=nods, conn=conn), None, 0.0, ctx)
+        rng = np.random.default_rng(3)
+        pts = rng.uniform(-2, 2, size=(3, args.n_pts))
+        xyz = np.zeros((6 * n_el, 3))
+        for q in range(3):  # nodal positions of the DD rows (vertices, then mid-edge nodes)
+            v = nods[:3, conn[q]].T
+            xyz[q::6] = v
+            xyz[3 + q::6] = 0.5 * (nods[:3, conn[(q + 1) % 3]].T + nods[:3, conn[(q + 2) % 3]].T)
+        dd = np.zeros((6 * n_el, 3))
+        dd[:, 2] = np.sqrt(np.clip(1 - (xyz[:, 0] ** 2 + xyz[:, 1] ** 2), 0, None))
+        with torch.cuda.stream(ctx.stream):
+            d_dd = torch.from_numpy(np.ascontiguousarray(dd.T)).to(ctx.device)
+            d_pts = torch.from_numpy(np.ascontiguousarray(pts.T)).to(ctx.device)
+            out = torch.empty((6, args.n_pts), dtype=torch.float64, device=ctx.device)
+        s_inf = np.array([1.0, 1.0, 0.5, 0, 0, 0])
+
+        def call():
+            ctx.check(lib.hfpx_stress(ctx.handle, dm.handle, G, nu, hx._ptr(d_dd), 6 * n_el, hx._ptr(d_pts),
+                                      args.n_pts, hx._np_ptr(s_inf), hx._ptr(out), args.n_pts))
+        units, unit, pairs = args.n_pts * n_el / 1e9, "G pair evaluations/s", float(args.n_pts) * n_el
+        wl = (f"stresses of the wavy crack z=0.1 sin(2 pi x) sin(2 pi y), {n_el} quadratic triangles, at "
+              f"{args.n_pts} points uniform in [-2, 2]^3 (seed 3)")
+    for _ in range(args.warmup):
+        call()
+    ctx.stream.synchronize()
+    times = []
+    for _ in range(args.steps):
+        e0, e1 = ev(), ev()
+        e0.record(ctx.stream)
+        call()
+        e1.record(ctx.stream)
+        ctx.stream.synchronize()
+        times.append(e0.elapsed_time(e1) / 1e3)
+    t = float(np.mean(times))
+    result = {"metric": f"{args.workload} {unit}", "value": round(units / t, 3), "unit": unit, "n_gpus": 1,
+              "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t * 1e3, 3),
+              "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+              "data": "synthetic", "config": {"workload": wl}, "pair_evaluations": pairs}
+    if VALU_FLOPS_PER_PAIR.get(args.workload):
+        fl = VALU_FLOPS_PER_PAIR[args.workload] * pairs
+        result["roofline"] = {"bound": "valu", "achieved": round(fl / t / 1e12, 3), "peak": PEAKS["fp64_valu_TFs"],
+                              "unit": "TFLOP/s", "frac": round(fl / t / 1e12 / PEAKS["fp64_valu_TFs"], 4),
+                              "flops_per_pair": VALU_FLOPS_PER_PAIR[args.workload],
+                              "basis": "FP64 VALU flops per pair evaluation from SQ_INSTS_VALU_FLOPS_FP64 "
+                                       "(+ _TRANS) of the kernel, the step's time (includes geometry / RHS)"}
     print(json.dumps(result))
 
 

@@ -158,6 +158,13 @@ int hfpx_destroy(hfpx_ctx* ctx) {
   if (ctx->lu_work) (void)hipFree(ctx->lu_work);
   if (ctx->gemm_work) (void)hipFree(ctx->gemm_work);
   if (ctx->x3_work) (void)hipFree(ctx->x3_work);
+  if (ctx->x3_work2) (void)hipFree(ctx->x3_work2);
+  if (ctx->stream2) {
+    (void)hipStreamSynchronize(ctx->stream2);
+    (void)hipStreamDestroy(ctx->stream2);
+  }
+  for (hipEvent_t e : ctx->la_ev)
+    if (e) (void)hipEventDestroy(e);
   if (ctx->sync) (void)hipFree(ctx->sync);
   if (ctx->hwork) (void)hipHostFree(ctx->hwork);
   prof_clear(ctx);
@@ -461,6 +468,11 @@ int hfpx_tune_set(hfpx_ctx* ctx, int knob, int value) {
   }
   if (knob == HFPX_KNOB_PANEL) {
     if (value < 0 || value > 2) return set_error(ctx, HFPX_ERR_INVALID, "tune: PANEL must be 0, 1 or 2");
+    ctx->knob[knob] = value;
+    return HFPX_OK;
+  }
+  if (knob == HFPX_KNOB_LU_LOOKAHEAD) {
+    if (value < 0 || value > 3) return set_error(ctx, HFPX_ERR_INVALID, "tune: LU_LOOKAHEAD must be 0 .. 3");
     ctx->knob[knob] = value;
     return HFPX_OK;
   }

@@ -705,7 +705,7 @@ static void launch_sgemm32(hfpx_ctx* ctx, float* C, int64_t ldc, const float* A,
   }
 }
 
-int gemm_f32_variants() { return 15; }
+int gemm_f32_variants() { return 16; }
 
 // FP64 128 x 128 product: interior tiles by the unguarded kernel, then the
 // bottom band (all columns) and the right band (interior rows) guarded.
@@ -773,16 +773,17 @@ int gemm_sub(hfpx_ctx* ctx, T* C, int64_t ldc, const T* A, int64_t lda, const T*
   if (var == 0) {
     // automatic choice: the bf16-split product wherever it has tiles to
     // fill (faster on every LU shape measured, 1.5x on the large trailing
-    // updates), with 128 x 256 tiles on the large square updates (+5 to 7 %
-    // there, slower on narrow products); small products on FP32 MFMA, C
-    // preloaded for short K
-    var = (M >= 256 && N >= 128 && K >= 64) ? ((M >= 4096 && N >= 4096 && K >= 512) ? 14 : 13)
+    // updates), with 128 x 256 tiles and the pipelined loop on the large
+    // square updates (+5 to 7 % for the tiles, slower on narrow products;
+    // +5 % for the pipelining); small products on FP32 MFMA, C preloaded
+    // for short K
+    var = (M >= 256 && N >= 128 && K >= 64) ? ((M >= 4096 && N >= 4096 && K >= 512) ? 15 : 13)
                                              : (K <= 384 ? 7 : 2);
   }
   bool done = false;
   if constexpr (sizeof(T) == 4) {
-    if (!skinny && (var == 13 || var == 14)) {
-      int rc = gemm_x3(ctx, C, ldc, A, lda, B, ldb, M, N, K, var == 14 ? 1 : 0);
+    if (!skinny && (var == 13 || var == 14 || var == 15)) {
+      int rc = gemm_x3(ctx, C, ldc, A, lda, B, ldb, M, N, K, var - 13);
       if (rc) return rc;
       done = true;
     } else if (!skinny && var > 1) {

@@ -111,7 +111,12 @@ __global__ __launch_bounds__(256) void k_split_b(const float* __restrict__ B, in
 // per LDS buffer (double buffered, one barrier per chunk; loading two chunks
 // ahead measured slower).  WCB = 2 reads 0.5 LDS operands per MFMA, WCB = 4
 // 0.375.  ROW is the LDS pitch of a 16-k row (bf16): 24 (48 B) or 16
-// (unpadded, less LDS per workgroup).  As the FP32 kernels, the MFMA forms
+// (unpadded, less LDS per workgroup; the two 16-byte halves of row r are
+// then swapped when bit 2 xor bit 3 of r is set, so that the 16-byte
+// operand reads of 16 consecutive lanes, rows r .. r + 15, cover the 64
+// banks once, and those of 8 lanes 32 banks, where the plain 32-byte pitch
+// puts rows r and r + 8 on the same banks: measured, 2.8e9 conflict cycles
+// in a 70,976 x 70,976 x 1,024 product).  As the FP32 kernels, the MFMA forms
 // the transposed product: operand a is a block of Bs^T (C columns), operand
 // b a block of As^T (C rows), so lane l % 32 runs down 32 consecutive rows
 // of C.  v_mfma_f32_32x32x16_bf16: lane l supplies a[l%32][8(l/32) + e] and
@@ -132,7 +137,14 @@ struct X3Lds {
 // One tile of C at (row0, col0): pa = the row tile's A blocks (nkc of them),
 // pb = the column tile's B blocks (NBB x nkc, consecutive), lds = the
 // caller's LDS.
-template <int ROW, int WCB>
+// PIPE: the software-pipelined loop (one scheduling region per chunk, the
+// last chunk loaded and staged again into the idle buffer instead of a
+// branch): the B operands of column block tc + 1 are read from LDS, into a
+// second register set, while the twelve products of block tc issue, and
+// the next chunk's LDS stores go between the last block's products, so
+// that a wave waits for LDS at most once per chunk (after the barrier)
+// where the plain loop waits before every column block.
+template <int ROW, int WCB, bool PIPE = false>
 __device__ __forceinline__ void x3_tile(X3Lds<ROW, WCB>& lds, float* __restrict__ C, int64_t ldc,
                                         const __bf16* __restrict__ pa, const __bf16* __restrict__ pb, int64_t M,
                                         int64_t N, int nkc, int64_t row0, int64_t col0) {
@@ -177,7 +189,18 @@ __device__ __forceinline__ void x3_tile(X3Lds<ROW, WCB>& lds, float* __restrict_
 #pragma unroll
       for (int q = 0; q < 3; ++q) rb[h][q] = pb4[((size_t)h * nkc + kc) * X3_PIECES + t + 256 * q];
   };
-  const int soff = (t >> 1) * ROW + 8 * (t & 1);
+  // half h of row r sits at 8 (h ^ sw(r)), sw(r) = bit 2 ^ bit 3 of r; the
+  // operand rows of a lane are l32 plus multiples of 32
+  constexpr int SWZ = ROW == 16 ? 1 : 0;
+  const int soff = (t >> 1) * ROW + 8 * ((t & 1) ^ (SWZ & ((t >> 3) ^ (t >> 4))));
+  const int rh = 8 * (hk ^ (SWZ & ((l32 >> 2) ^ (l32 >> 3))));  // this lane's operand half
+  // piece q of the staged chunk: A parts q < 3, then B block h part q % 3
+  auto lstore_piece = [&](int buf, int q) {
+    if (q < 3)
+      *(u32x4*)&sA[buf][q * X3_T * ROW + soff] = ra[q];
+    else
+      *(u32x4*)&sB[buf][(q - 3) / 3][((q - 3) % 3) * X3_T * ROW + soff] = rb[(q - 3) / 3][(q - 3) % 3];
+  };
   auto lstore = [&](int buf) {
 #pragma unroll
     for (int q = 0; q < 3; ++q) *(u32x4*)&sA[buf][q * X3_T * ROW + soff] = ra[q];
@@ -190,6 +213,51 @@ __device__ __forceinline__ void x3_tile(X3Lds<ROW, WCB>& lds, float* __restrict_
   gload(0);
   lstore(0);
   __syncthreads();
+  if constexpr (PIPE) {
+    // operand (B part, A part) of product q of a pair, small terms first
+    constexpr int PB[6] = {1, 2, 0, 1, 0, 0}, PA[6] = {1, 0, 2, 0, 1, 0};
+    constexpr int NLD = 3 * (1 + X3Lds<ROW, WCB>::NBB);  // staged pieces per chunk
+    for (int kc = 0; kc < nkc; ++kc) {
+      const int buf = kc & 1;
+      gload(kc + 1 < nkc ? kc + 1 : kc);
+      __builtin_amdgcn_sched_barrier(0);
+      bf16x8 fb[2][3], fa[2][3];
+#pragma unroll
+      for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+          fb[u][p] = *(const bf16x8*)&sA[buf][(p * X3_T + wr + 32 * u + l32) * ROW + rh];
+      auto read_fa = [&](int tc, int p, bf16x8& f) {
+        const int jb = wc + 32 * tc, h = jb / X3_T, jl = jb % X3_T;
+        f = *(const bf16x8*)&sB[buf][h][(p * X3_T + jl + l32) * ROW + rh];
+      };
+#pragma unroll
+      for (int p = 0; p < 3; ++p) read_fa(0, p, fa[0][p]);
+      __builtin_amdgcn_sched_barrier(0);
+      // program order is issue order: product i of block tc (the two row
+      // blocks alternate), then block tc + 1's operand read i (i < 3), or
+      // in the last block the LDS store i - 3 of the next chunk
+#pragma unroll
+      for (int tc = 0; tc < WCB; ++tc) {
+#pragma unroll
+        for (int i = 0; i < 12; ++i) {
+          const int tr = i & 1, q = i >> 1;
+          acc[tc][tr] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[tc & 1][PB[q]], fb[tr][PA[q]], acc[tc][tr], 0,
+                                                                0, 0);
+          __builtin_amdgcn_sched_barrier(0);
+          if (tc + 1 < WCB && i < 3) {
+            read_fa(tc + 1, i, fa[(tc + 1) & 1][i]);
+            __builtin_amdgcn_sched_barrier(0);
+          }
+          if (tc + 1 == WCB && i >= 12 - NLD) {
+            lstore_piece(buf ^ 1, i - (12 - NLD));
+            __builtin_amdgcn_sched_barrier(0);
+          }
+        }
+      }
+      __syncthreads();
+    }
+  } else
   for (int kc = 0; kc < nkc; ++kc) {
     const int buf = kc & 1;
     if (kc + 1 < nkc) gload(kc + 1);
@@ -198,14 +266,14 @@ __device__ __forceinline__ void x3_tile(X3Lds<ROW, WCB>& lds, float* __restrict_
     for (int p = 0; p < 3; ++p)
 #pragma unroll
       for (int u = 0; u < 2; ++u)
-        fb[u][p] = *(const bf16x8*)&sA[buf][(p * X3_T + wr + 32 * u + l32) * ROW + 8 * hk];
+        fb[u][p] = *(const bf16x8*)&sA[buf][(p * X3_T + wr + 32 * u + l32) * ROW + rh];
 #pragma unroll
     for (int tc = 0; tc < WCB; ++tc) {
       // the B operands of one 32-column block at a time (register pressure)
       const int jb = wc + 32 * tc, h = jb / X3_T, jl = jb % X3_T;
       bf16x8 fa[3];
 #pragma unroll
-      for (int p = 0; p < 3; ++p) fa[p] = *(const bf16x8*)&sB[buf][h][(p * X3_T + jl + l32) * ROW + 8 * hk];
+      for (int p = 0; p < 3; ++p) fa[p] = *(const bf16x8*)&sB[buf][h][(p * X3_T + jl + l32) * ROW + rh];
 #pragma unroll
       for (int tr = 0; tr < 2; ++tr) {
         floatx16_x3 c = acc[tc][tr];
@@ -237,7 +305,7 @@ __device__ __forceinline__ void x3_tile(X3Lds<ROW, WCB>& lds, float* __restrict_
     }
 }
 
-template <int ROW, int WCB>
+template <int ROW, int WCB, bool PIPE = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_x3gemm(
     float* __restrict__ C, int64_t ldc, const __bf16* __restrict__ As, const __bf16* __restrict__ Bs, int64_t M,
     int64_t N, int nkc, int64_t tm, int64_t tn, int64_t band) {
@@ -245,7 +313,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   int64_t tile_r, tile_c;
   sg_band_tile(sg_xcd_order(blockIdx.x, tm * tn), tm, tn, tile_r, tile_c, band);
   constexpr int NBB = X3Lds<ROW, WCB>::NBB;
-  x3_tile<ROW, WCB>(lds, C, ldc, As + (size_t)tile_r * nkc * X3_BLK, Bs + (size_t)tile_c * NBB * nkc * X3_BLK, M,
+  x3_tile<ROW, WCB, PIPE>(lds, C, ldc, As + (size_t)tile_r * nkc * X3_BLK, Bs + (size_t)tile_c * NBB * nkc * X3_BLK, M,
                     N, nkc, tile_r * X3_T, tile_c * 64 * WCB);
 }
 
@@ -337,7 +405,7 @@ int launch_bgemm_x3(hfpx_ctx* ctx, float* base, int which, const BCrack* cr, con
 }
 
 // Grow-only workspace of the split operands.
-static int ensure_x3_work(hfpx_ctx* ctx, size_t bytes) {
+int ensure_x3_work(hfpx_ctx* ctx, size_t bytes) {
   if (ctx->x3_work_bytes >= bytes) return HFPX_OK;
   if (ctx->x3_work) {
     HFPX_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -377,7 +445,10 @@ int gemm_x3(hfpx_ctx* ctx, float* C, int64_t ldc, const float* A, int64_t lda, c
   // the 4 MB L2 of an XCD, between 1 and 16 row tiles
   int64_t band = ctx->knob[HFPX_KNOB_X3_BAND];
   if (band <= 0) band = std::max<int64_t>(1, std::min<int64_t>(16, (4 << 20) / ((int64_t)X3_T * nkc * X3_KC * 6)));
-  if (wide)
+  if (wide == 2)
+    hipLaunchKernelGGL((k_x3gemm<16, 4, true>), dim3((unsigned)(tm * tn)), dim3(256), 0, ctx->stream, C, ldc, As, Bs,
+                       M, N, (int)nkc, tm, tn, band);
+  else if (wide)
     hipLaunchKernelGGL((k_x3gemm<16, 4>), dim3((unsigned)(tm * tn)), dim3(256), 0, ctx->stream, C, ldc, As, Bs, M,
                        N, (int)nkc, tm, tn, band);
   else

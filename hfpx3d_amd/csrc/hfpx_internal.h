@@ -27,6 +27,14 @@ struct hfpx_ctx {
   size_t gemm_work_bytes = 0;
   void* x3_work = nullptr;    // grow-only workspace of the bf16-split GEMM operands
   size_t x3_work_bytes = 0;
+  // LU look-ahead: a second default-priority stream for the trailing
+  // update, its own split-operand workspace, and the two events that order
+  // it against the main stream (created at the first factorisation that
+  // uses them)
+  hipStream_t stream2 = nullptr;
+  hipEvent_t la_ev[2] = {nullptr, nullptr};
+  void* x3_work2 = nullptr;
+  size_t x3_work2_bytes = 0;
   uint64_t* sync = nullptr;  // grid-barrier arrival counter (device)
   uint64_t sync_base = 0;    // its value once every launched barrier completes
   int num_cus = 0;
@@ -209,6 +217,7 @@ int gemm_f32_variants();
 int gemm_x3(hfpx_ctx* ctx, float* C, int64_t ldc, const float* A, int64_t lda, const float* B, int64_t ldb,
             int64_t M, int64_t N, int64_t K, int wide);
 size_t gemm_x3_work_bytes(int64_t M, int64_t N, int64_t K);
+int ensure_x3_work(hfpx_ctx* ctx, size_t bytes);
 // C (M x N) -= A (M x K) B (K x N), column-major, on the context stream
 // (hfpx_gemm.hip; instantiated for float and double).
 template <typename T>

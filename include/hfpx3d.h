@@ -333,7 +333,7 @@ int hfpx_batch_matrix(hfpx_ctx* ctx, hfpx_batch* batch, int crack, double* A_hos
 /* ---- tuning -------------------------------------------------------------- */
 /* Kernel-variant knobs of a context (no reference counterpart).
  * HFPX_KNOB_GEMM_F32: FP32 trailing-update GEMM (products with more than 64
- * columns): 0 = automatic (14 when M, N >= 4096 and K >= 512; 13 when
+ * columns): 0 = automatic (15 when M, N >= 4096 and K >= 512; 13 when
  * M >= 256, N >= 128 and K >= 64; else 7 for K <= 384, 2 for longer K); 1 = 16x16x4 FP32 MFMA, 128 x 128 tile;
  * 2..12 = 32x32x2 FP32 MFMA, K chunk 16, with 2 = 128 x 128 tile, no
  * options, 3 = unguarded interior tiles (edge bands in separate launches),
@@ -344,7 +344,8 @@ int hfpx_batch_matrix(hfpx_ctx* ctx, hfpx_batch* batch, int crack, double* A_hos
  * (32x32x16) on three-way bf16 splits of both operands, six products per
  * pair, FP32 accumulation: FP32-accurate, not bitwise equal to the FP32
  * MFMA kernels; 14 = 13 with 128 x 256 tiles (waves of 64 x 128, unpadded
- * LDS rows). */
+ * LDS rows); 15 = 14 with a software-pipelined loop (operand reads and LDS
+ * stores between the products; bitwise the result of 14). */
 #define HFPX_KNOB_GEMM_F32 0
 /* HFPX_KNOB_LU_NB: outer block of the LU (columns per trailing update):
  * 0 = default (1024), or one of 256, 512, 1024, 2048, 4096. */
@@ -381,7 +382,15 @@ int hfpx_batch_matrix(hfpx_ctx* ctx, hfpx_batch* batch, int crack, double* A_hos
  * preloaded into the accumulators; 4 = 3 without the XCD-grouped walk;
  * 5 = 3 with bands of 4 row tiles. */
 #define HFPX_KNOB_GEMM_F64 7
-#define HFPX_KNOB_COUNT 8
+/* HFPX_KNOB_LU_LOOKAHEAD: 0 = default (2 for FP32, 1 for FP64, where the
+ * overlapped FP64 updates measured slower); 1 = off: each outer block's
+ * trailing update follows its factorisation; 2 = look-ahead: the next
+ * block's columns are updated first and that block is factorised on the
+ * context's stream while the rest of the trailing update runs on a second
+ * default-priority stream (the factors are bitwise those of 1); 3 = 2 with
+ * the panel kernel launched without the cooperative launch. */
+#define HFPX_KNOB_LU_LOOKAHEAD 8
+#define HFPX_KNOB_COUNT 9
 /* Set / read one knob; HFPX_ERR_INVALID for an unknown knob or value. */
 int hfpx_tune_set(hfpx_ctx* ctx, int knob, int value);
 int hfpx_tune_get(const hfpx_ctx* ctx, int knob, int* value);

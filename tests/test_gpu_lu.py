@@ -42,7 +42,7 @@ def test_gemm_sub(env, dt, M, N, K):
     assert np.abs(got - ref).max() / (np.abs(A).max() * np.abs(B).max() * K) < tol
 
 
-@pytest.mark.parametrize("variant", range(1, 15))
+@pytest.mark.parametrize("variant", range(1, 16))
 @pytest.mark.parametrize("M,N,K", [(300, 257, 33), (513, 700, 100), (1000, 300, 256), (130, 129, 17),
                                    (256, 256, 32), (77, 200, 20), (512, 384, 256)])
 def test_sgemm_variants(env, variant, M, N, K):
@@ -64,6 +64,29 @@ def test_sgemm_variants(env, variant, M, N, K):
     ref = C.astype(np.float32).astype(np.float64) - A.astype(np.float32).astype(np.float64) @ \
         B.astype(np.float32).astype(np.float64)
     assert np.abs(got - ref).max() / (np.abs(A).max() * np.abs(B).max() * K) < 2e-6
+
+
+@pytest.mark.parametrize("M,N,K", [(4500, 4200, 1024), (700, 650, 40), (4096, 4096, 512)])
+def test_sgemm_pipelined_split_is_bitwise_variant_14(env, M, N, K):
+    """Variant 15 issues the products of variant 14 in another order across
+    accumulators but the same order within each: bitwise the same C."""
+    torch, hx = env
+    from hfpx3d_amd import _lib
+    print(f"case: sgemm 15 vs 14 {M}x{N}x{K}", flush=True)
+    rng = np.random.default_rng(M + 3 * N + K)
+    A, B, C = rng.normal(size=(M, K)), rng.normal(size=(K, N)), rng.normal(size=(M, N))
+    Ad, Bd = to_dev(torch, hx, A, torch.float32), to_dev(torch, hx, B, torch.float32)
+    out = {}
+    old = hx.tune_get(_lib.KNOB_GEMM_F32)
+    try:
+        for v in (14, 15):
+            hx.tune_set(_lib.KNOB_GEMM_F32, v)
+            Cd = to_dev(torch, hx, C, torch.float32)
+            hx.gemm_sub(Cd, Ad, Bd)
+            out[v] = Cd.cpu().numpy()
+    finally:
+        hx.tune_set(_lib.KNOB_GEMM_F32, old)
+    assert np.array_equal(out[14], out[15])
 
 
 @pytest.mark.parametrize("scaled", [False, True])
@@ -89,7 +112,7 @@ def test_sgemm_bf16_split_is_fp32_accurate(env, scaled):
     err = {}
     old = hx.tune_get(_lib.KNOB_GEMM_F32)
     try:
-        for v in (2, 13, 14):
+        for v in (2, 13, 14, 15):
             hx.tune_set(_lib.KNOB_GEMM_F32, v)
             Cd = to_dev(torch, hx, C32, torch.float32)
             hx.gemm_sub(Cd, to_dev(torch, hx, A32, torch.float32), to_dev(torch, hx, B32, torch.float32))
@@ -99,6 +122,7 @@ def test_sgemm_bf16_split_is_fp32_accurate(env, scaled):
     assert err[2] < 1e-6
     assert err[13] < 3 * err[2] + 1e-7, err
     assert err[14] < 3 * err[2] + 1e-7, err
+    assert err[15] < 3 * err[2] + 1e-7, err
 
 
 @pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
@@ -122,6 +146,38 @@ def test_dgemm_variants(env, variant, M, N, K):
         hx.tune_set(_lib.KNOB_GEMM_F64, 0, ctx)
     ref = C - A @ B
     assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-13
+
+
+@pytest.mark.parametrize("mode", [2, 3])
+@pytest.mark.parametrize("dt,n,nb", [("f32", 3000, 1024), ("f64", 3000, 1024), ("f32", 4500, 1024),
+                                     ("f64", 1500, 256)])
+def test_getrf_lookahead_is_bitwise_the_plain_loop(env, mode, dt, n, nb):
+    """The look-ahead factorisation (next block factorised on the main
+    stream while the rest of the trailing update runs on a second stream)
+    against the plain blocked loop on the same matrix: identical pivots and
+    bitwise identical factors; and the pivots are LAPACK's for FP64."""
+    from hfpx3d_amd import _lib
+    torch, hx = env
+    print(f"case: look-ahead mode {mode} {dt} n={n} nb={nb}", flush=True)
+    A = np.random.default_rng(n + nb).normal(size=(n, n))
+    dtype = torch.float64 if dt == "f64" else torch.float32
+    ctx = hx.get_context()
+    out = {}
+    hx.tune_set(_lib.KNOB_LU_NB, nb, ctx)
+    try:
+        for la in (1, mode):
+            hx.tune_set(_lib.KNOB_LU_LOOKAHEAD, la, ctx)
+            lu = hx.LU(to_dev(torch, hx, A, dtype))
+            out[la] = (lu.A.cpu().numpy(), lu.ipiv.cpu().numpy())
+    finally:
+        hx.tune_set(_lib.KNOB_LU_LOOKAHEAD, 0, ctx)
+        hx.tune_set(_lib.KNOB_LU_NB, 0, ctx)
+    assert np.array_equal(out[1][1], out[mode][1])
+    assert np.array_equal(out[1][0], out[mode][0])
+    if dt == "f64":
+        sl = pytest.importorskip("scipy.linalg")
+        _, piv = sl.lu_factor(A)
+        assert np.array_equal(out[mode][1], piv + 1)
 
 
 def test_tune_rejects_unknown_variant(env):
