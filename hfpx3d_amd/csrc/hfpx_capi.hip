@@ -486,6 +486,11 @@ int hfpx_tune_set(hfpx_ctx* ctx, int knob, int value) {
     ctx->knob[knob] = value;
     return HFPX_OK;
   }
+  if (knob == HFPX_KNOB_IR_MODE) {
+    if (value < 0 || value > 2) return set_error(ctx, HFPX_ERR_INVALID, "tune: IR_MODE must be 0, 1 or 2");
+    ctx->knob[knob] = value;
+    return HFPX_OK;
+  }
   if (knob == HFPX_KNOB_TEST_PANEL_TIMEOUT) {
     if (value < 0 || value > 1) return set_error(ctx, HFPX_ERR_INVALID, "tune: TEST_PANEL_TIMEOUT must be 0 or 1");
     ctx->knob[knob] = value;

@@ -677,7 +677,8 @@ def tune_get(knob, ctx=None):
 
 def solve_mixed(A64, B, tol=1e-14, max_outer=10, gmres_m=20, ctx=None, LU32=None):
     """Solve A64 X = B: FP32 LU of A64 (or the given FP32 LU object) as the
-    preconditioner of FP64 GMRES iterative refinement against A64.
+    preconditioner of FP64 iterative refinement against A64 (classic steps
+    while they contract the backward error 100-fold, then GMRES(gmres_m)).
     Returns (X, report dict: iterations, inner_iterations, backward_error,
     converged, fell_back (the solution came from an FP64 LU of a copy of
     A64), status (HFPX_OK or HFPX_ERR_NOT_CONVERGED)).  An FP32 factor

@@ -35,8 +35,8 @@ typedef struct hfpx_mesh hfpx_mesh;
 
 /* Outcome of the mixed-precision solve. */
 typedef struct hfpx_ir_report {
-  int iterations;        /* outer refinement steps taken                 */
-  int inner_iterations;  /* GMRES steps summed over the outer steps       */
+  int iterations;        /* outer refinement steps taken (classic + GMRES) */
+  int inner_iterations;  /* GMRES steps summed over the GMRES outer steps */
   double backward_error; /* max over columns of |b-Ax|/(|A||x|+|b|), inf-norms */
   int converged;         /* 1 if backward_error <= tol                     */
   int fell_back;         /* 1 if refinement failed and the solution came
@@ -248,10 +248,12 @@ int hfpx_getrf(hfpx_ctx* ctx, int dtype, void* A, int64_t lda, int64_t n, int32_
 int hfpx_getrs(hfpx_ctx* ctx, int dtype, const void* LU, int64_t lda, const int32_t* ipiv,
                int64_t n, void* B, int64_t ldb, int64_t nrhs);
 /* Mixed-precision solve of A X = B for nrhs columns: FP32 LU factors of A
- * (LU32, ipiv from hfpx_getrf with HFPX_F32) precondition FP64 GMRES(m)
- * iterative refinement whose residuals use the FP64 matrix A64 (left
- * untouched).  X (device, ldx) receives the solution; refinement stops when
- * every column's normwise backward error is <= tol or after max_outer steps
+ * (LU32, ipiv from hfpx_getrf with HFPX_F32) precondition FP64 iterative
+ * refinement whose residuals use the FP64 matrix A64 (left untouched):
+ * classic steps x += (LU)^-1 r while each lowers the backward error at least
+ * 100-fold, GMRES(m) steps from the first that does not (HFPX_KNOB_IR_MODE).
+ * X (device, ldx) receives the solution; refinement stops when every
+ * column's normwise backward error is <= tol or after max_outer steps
  * (then HFPX_ERR_NOT_CONVERGED, X holding the last iterate and rep the
  * achieved error).  Refinement that stagnates or meets a non-finite iterate
  * is replaced by an FP64 LU solve of a copy of A64 (HFPX_KNOB_IR_FALLBACK;
@@ -370,6 +372,11 @@ int hfpx_batch_matrix(hfpx_ctx* ctx, hfpx_batch* batch, int crack, double* A_hos
  * doubles allocated for the call; report fell_back = 1), 2 = return
  * HFPX_ERR_NOT_CONVERGED with the last iterate. */
 #define HFPX_KNOB_IR_FALLBACK 5
+/* HFPX_KNOB_IR_MODE: correction steps of hfpx_solve_ir: 0 = default
+ * (classic steps while each contracts the backward error 100-fold, then
+ * GMRES), 1 = GMRES steps only, 2 = classic steps only (a step that
+ * lowers the error less than 10 % counts as stagnation). */
+#define HFPX_KNOB_IR_MODE 9
 /* HFPX_KNOB_TEST_PANEL_TIMEOUT: 1 = hfpx_getrf starts with its panel
  * time-out flag already raised, as if a grid barrier had timed out: every
  * panel returns at entry and getrf reports HFPX_ERR_HIP (a host-forced test
@@ -390,7 +397,7 @@ int hfpx_batch_matrix(hfpx_ctx* ctx, hfpx_batch* batch, int crack, double* A_hos
  * default-priority stream (the factors are bitwise those of 1); 3 = 2 with
  * the panel kernel launched without the cooperative launch. */
 #define HFPX_KNOB_LU_LOOKAHEAD 8
-#define HFPX_KNOB_COUNT 9
+#define HFPX_KNOB_COUNT 10
 /* Set / read one knob; HFPX_ERR_INVALID for an unknown knob or value. */
 int hfpx_tune_set(hfpx_ctx* ctx, int knob, int value);
 int hfpx_tune_get(const hfpx_ctx* ctx, int knob, int* value);

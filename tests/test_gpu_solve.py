@@ -1,5 +1,6 @@
-"""Mixed-precision solve (FP32 LU + FP64 GMRES-IR): on BEM matrices with
-many load cases against LAPACK on the oracle's matrix (and against the FP64
+"""Mixed-precision solve (FP32 LU + FP64 iterative refinement, classic and
+GMRES steps): on BEM matrices with many load cases against LAPACK on the
+oracle's matrix (and against the FP64
 GPU LU, itself pinned to LAPACK in test_gpu_configs.py); and its failure
 handling: refinement that cannot converge falls back to an FP64 LU of a
 copy of the matrix (fell_back = 1), a non-finite right-hand side is
@@ -31,13 +32,17 @@ def sinf_cases(nrhs, seed=1):
     return S
 
 
-def test_gmres_ir_matches_lapack_on_oracle_matrix(env):
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_gmres_ir_matches_lapack_on_oracle_matrix(env, mode):
     """penny121 (2,178 DoF), nu = 0.3, 64 load cases: the mixed solution
     against scipy's LAPACK solve of the oracle's matrix and right-hand
-    sides, 1e-10 per load case."""
+    sides, 1e-10 per load case, for every refinement mode (0: classic steps
+    while they contract 100-fold, then GMRES; 1: GMRES only; 2: classic
+    only)."""
     torch, hx = env
+    from hfpx3d_amd import _lib
     sl = pytest.importorskip("scipy.linalg")
-    print("case: gmres-ir penny121 vs LAPACK on the oracle matrix", flush=True)
+    print(f"case: gmres-ir penny121 vs LAPACK on the oracle matrix, IR mode {mode}", flush=True)
     O = oracle_lib.oracle()
     nods = np.load(golden("Nodes_pennymesh121el_32.npy"))
     conn = np.load(golden("Elems_pennymesh121el_32.npy")).astype(np.int64)
@@ -48,8 +53,17 @@ def test_gmres_ir_matches_lapack_on_oracle_matrix(env):
     md = hx.Mesh_Data_T(mesh=mesh)
     A = hx.make_3dbem_matrix_s(G, nu, mesh, hx.Num_Param_T(beta=0.125, tip_type=0), md.dof_h_dd)
     B = hx.add_s_inf_to_3dbem_rhs(md, hx.Load_T(s_inf=S), hx.SAE_T())
-    X, rep = hx.solve_mixed(A, B, tol=1e-15)
+    ctx = hx.get_context()
+    hx.tune_set(_lib.KNOB_IR_MODE, mode, ctx)
+    try:
+        X, rep = hx.solve_mixed(A, B, tol=1e-15)
+    finally:
+        hx.tune_set(_lib.KNOB_IR_MODE, 0, ctx)
     assert rep["converged"] and not rep["fell_back"], rep
+    if mode == 2:
+        assert rep["inner_iterations"] == 0, rep
+    if mode == 1:
+        assert rep["inner_iterations"] >= rep["iterations"], rep
     Ao, dof = O.assemble(nods, conn, G, nu, 0.125, 0)
     Bo = np.column_stack([O.rhs(nods, conn, dof, S[:, r]) for r in range(S.shape[1])])
     Xr = sl.solve(Ao, Bo)
@@ -78,6 +92,25 @@ def _ill_conditioned(n, cond, seed):
 def _berr(A, X, B):
     r = np.abs(B - A @ X).max(axis=0)
     return (r / (np.abs(A).sum(axis=1).max() * np.abs(X).max(axis=0) + np.abs(B).max(axis=0))).max()
+
+
+def test_ir_switches_to_gmres_when_classic_steps_contract_weakly(env):
+    """kappa(A) = 1e7: kappa eps_fp32 is near 1, so a classic step x += M^-1 r
+    contracts the backward error far less than 100-fold; the default mode
+    then continues with GMRES steps (inner_iterations > 0) and converges
+    without the FP64 fallback, to LAPACK's solution within kappa eps_64."""
+    torch, hx = env
+    print("case: IR classic -> GMRES switch, kappa 1e7", flush=True)
+    n = 600
+    A = _ill_conditioned(n, 1e7, 13)
+    B = np.random.default_rng(14).normal(size=(n, 4))
+    X, rep = hx.solve_mixed(_device(torch, hx, A), _device(torch, hx, B), tol=1e-15, max_outer=10, gmres_m=30)
+    assert rep["converged"] and not rep["fell_back"], rep
+    assert rep["inner_iterations"] > 0 and rep["iterations"] >= 2, rep
+    x = X.cpu().numpy()
+    assert _berr(A, x, B) < 1e-15
+    xr = np.linalg.solve(A, B)
+    assert np.abs(x - xr).max() / np.abs(xr).max() < 1e-7
 
 
 def test_gmres_ir_falls_back_to_fp64_lu_when_fp32_cannot_precondition(env):
