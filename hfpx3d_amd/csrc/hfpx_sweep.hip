@@ -20,7 +20,11 @@ namespace hfpx {
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 constexpr int SB = 128;  // rows per block
-constexpr int SWN = 64;  // right-hand sides per workgroup
+// right-hand sides per sweep workgroup: 32 (four waves) up to 64 columns,
+// which halves the products on a block's critical path (72,000 DoF, 64
+// columns: 23.0 -> 19.2 ms per forward + backward pair, the same bits), 64
+// (eight waves) above, where every column block reads the factor once more
+constexpr int SWN_MAX = 64, SWN_MIN = 32;
 
 // Inverse of diagonal block blockIdx.x (rows/columns [128 b, 128 b + kb)),
 // padded with the identity to 128 x 128, column-major into inv + b SB^2.
@@ -78,18 +82,17 @@ __global__ __launch_bounds__(SB) void k_block_inv(const float* __restrict__ LU, 
 // other wait of the sweep gives up too and the sweep completes with garbage,
 // reported by the host) and ends with an agent-scope
 // acquire fence, which the workgroup barrier extends to the other threads.
-// Eight waves, one 32 x 32 block of the 128 x 64 tile each.  Products use
+// 4 SWN / 32 waves, one 32 x 32 block of the 128 x SWN tile each.  Products use
 // the transposed MFMA form (v_mfma_f32_32x32x2f32): lane l holds row
 // 32 (wave % 4) + l % 32 of the tile and column 8 (v/4) + 4 (l/32) + v%4 of
 // the wave's 32-column block, so tile loads and stores are column segments.
-constexpr int SWT = 512;  // threads per sweep workgroup
-
-template <bool LOWER>
-__global__ __launch_bounds__(SWT) void k_sweep_flow(const float* __restrict__ LU, int64_t lda, int64_t n,
+template <bool LOWER, int SWN>
+__global__ __launch_bounds__(8 * SWN) void k_sweep_flow(const float* __restrict__ LU, int64_t lda, int64_t n,
                                                     int64_t nblk, const float* __restrict__ inv,
                                                     float* __restrict__ B, int64_t ldb, int64_t nrhs, int64_t gy,
                                                     uint64_t* __restrict__ flags, uint64_t want,
                                                     uint64_t* __restrict__ ticket, int32_t* __restrict__ err) {
+  constexpr int SWT = 8 * SWN;  // threads
   __shared__ float sX[SB][SWN + 1];  // solution rows of block c: sX[k][j]
   __shared__ float sC[SB][SWN + 1];  // B_i - sum, then X_i: sC[i][j]
   __shared__ uint64_t s_tk;
@@ -230,8 +233,29 @@ __global__ void k_fill_u64(uint64_t* __restrict__ p, int64_t cnt) {
 // work: the block inverses of L and U, then the flags (one per row block
 // and right-hand-side block, per sweep), two ticket counters and two error
 // flags
+static int sweep_width(int64_t nrhs) { return nrhs <= SWN_MAX ? SWN_MIN : SWN_MAX; }
+
+template <int SWN>
+static int launch_sweeps(hfpx_ctx* ctx, const float* LU, int64_t lda, int64_t n, int64_t nblk, const float* invL,
+                         const float* invU, float* B, int64_t ldb, int64_t nrhs, uint64_t* flags, uint64_t* tickets,
+                         int32_t* errs) {
+  const int64_t gy = (nrhs + SWN - 1) / SWN;
+  // as many workgroups as are resident at once (any grid size would make
+  // progress: a workgroup takes tickets only once it runs)
+  int occ = 0;
+  HFPX_TRY_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_sweep_flow<true, SWN>, 8 * SWN, 0));
+  const unsigned nwg = (unsigned)std::min<int64_t>(nblk * gy, std::max(1, occ) * std::max(1, ctx->num_cus));
+  hipLaunchKernelGGL((k_sweep_flow<true, SWN>), dim3(nwg), dim3(8 * SWN), 0, ctx->stream, LU, lda, n, nblk, invL, B,
+                     ldb, nrhs, gy, flags, (uint64_t)1, tickets, errs);
+  hipLaunchKernelGGL((k_sweep_flow<false, SWN>), dim3(nwg), dim3(8 * SWN), 0, ctx->stream, LU, lda, n, nblk, invU, B,
+                     ldb, nrhs, gy, flags + nblk * gy, (uint64_t)1, tickets + 1, errs + 1);
+  HFPX_TRY_HIP(ctx, hipGetLastError());
+  return HFPX_OK;
+}
+
 size_t sweep_f32_work_bytes(int64_t n, int64_t nrhs) {
-  const int64_t nblk = (n + SB - 1) / SB, gy = (nrhs + SWN - 1) / SWN;
+  const int swn = sweep_width(nrhs);
+  const int64_t nblk = (n + SB - 1) / SB, gy = (nrhs + swn - 1) / swn;
   return sizeof(float) * 2 * (size_t)nblk * SB * SB + sizeof(uint64_t) * (2 * (size_t)nblk * gy + 4);
 }
 
@@ -240,7 +264,8 @@ size_t sweep_f32_work_bytes(int64_t n, int64_t nrhs) {
 int sweep_f32(hfpx_ctx* ctx, const float* LU, int64_t lda, int64_t n, float* B, int64_t ldb, int64_t nrhs,
               void* work) {
   if (n <= 0 || nrhs <= 0) return HFPX_OK;
-  const int64_t nblk = (n + SB - 1) / SB, gy = (nrhs + SWN - 1) / SWN;
+  const int swn = sweep_width(nrhs);
+  const int64_t nblk = (n + SB - 1) / SB, gy = (nrhs + swn - 1) / swn;
   float* invL = (float*)work;
   float* invU = invL + (size_t)nblk * SB * SB;
   uint64_t* flags = (uint64_t*)(invU + (size_t)nblk * SB * SB);  // [2][gy][nblk]
@@ -254,18 +279,12 @@ int sweep_f32(hfpx_ctx* ctx, const float* LU, int64_t lda, int64_t n, float* B, 
   prof_end(ctx, tok, HFPX_KC_TRSM, 2.0 * nblk * SB * SB * SB / 3.0, 0.0);
   HFPX_TRY_HIP(ctx, hipGetLastError());
   tok = prof_begin(ctx, HFPX_KC_TRSM);
-  // as many workgroups as are resident at once (any grid size would make
-  // progress: a workgroup takes tickets only once it runs)
-  int occ = 0;
-  HFPX_TRY_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_sweep_flow<true>, SWT, 0));
-  const unsigned nwg = (unsigned)std::min<int64_t>(nblk * gy, std::max(1, occ) * std::max(1, ctx->num_cus));
-  hipLaunchKernelGGL(k_sweep_flow<true>, dim3(nwg), dim3(SWT), 0, ctx->stream, LU, lda, n, nblk, invL, B, ldb,
-                     nrhs, gy, flags, (uint64_t)1, tickets, errs);
-  hipLaunchKernelGGL(k_sweep_flow<false>, dim3(nwg), dim3(SWT), 0, ctx->stream, LU, lda, n, nblk, invU, B, ldb,
-                     nrhs, gy, flags + nblk * gy, (uint64_t)1, tickets + 1, errs + 1);
+  const int rc = swn == SWN_MIN
+                     ? launch_sweeps<SWN_MIN>(ctx, LU, lda, n, nblk, invL, invU, B, ldb, nrhs, flags, tickets, errs)
+                     : launch_sweeps<SWN_MAX>(ctx, LU, lda, n, nblk, invL, invU, B, ldb, nrhs, flags, tickets, errs);
   // algorithmic: 2 n^2 nrhs flops, the factor read once (4 n^2 bytes)
   prof_end(ctx, tok, HFPX_KC_TRSM, 2.0 * n * n * nrhs, 4.0 * n * n);
-  HFPX_TRY_HIP(ctx, hipGetLastError());
+  if (rc) return rc;
   int32_t herr[2] = {0, 0};
   HFPX_TRY_HIP(ctx, hipMemcpyAsync(herr, errs, sizeof herr, hipMemcpyDeviceToHost, ctx->stream));
   HFPX_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));

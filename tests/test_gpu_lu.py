@@ -150,7 +150,7 @@ def test_dgemm_variants(env, variant, M, N, K):
 
 @pytest.mark.parametrize("mode", [2, 3])
 @pytest.mark.parametrize("dt,n,nb", [("f32", 3000, 1024), ("f64", 3000, 1024), ("f32", 4500, 1024),
-                                     ("f64", 1500, 256)])
+                                     ("f64", 1500, 256), ("f32", 2000, 256)])
 def test_getrf_lookahead_is_bitwise_the_plain_loop(env, mode, dt, n, nb):
     """The look-ahead factorisation (next block factorised on the main
     stream while the rest of the trailing update runs on a second stream)
@@ -212,7 +212,7 @@ def test_getrf_f64_matches_lapack(env, n):
     assert np.abs(A @ x - b).max() / (np.abs(A).max() * np.abs(x).max() * n) < 1e-15 * 10
 
 
-@pytest.mark.parametrize("n,nrhs", [(1, 1), (127, 3), (128, 64), (129, 65), (1000, 64), (2049, 130)])
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (127, 3), (128, 64), (129, 65), (1000, 64), (2049, 130), (1500, 33)])
 def test_getrs_f32_block_sweeps(env, n, nrhs):
     """FP32 solve with the GPU's own FP32 factors: the block-inverse sweeps
     against the same factors applied by LAPACK in float64 (lu_solve), within
@@ -302,8 +302,10 @@ def test_getrf_pivot_ties_take_first_row(env, dt, panel):
     assert int(lu.ipiv.cpu().numpy()[0]) == 1501
 
 
-@pytest.mark.parametrize("n", [300, 2049])
+@pytest.mark.parametrize("n", [300, 2049, 5000])
 def test_getrf_f32_backward_error(env, n):
+    """FP32 factors of a random matrix: |PA - LU| within n FP32 roundings
+    and |L| <= 1."""
     torch, hx = env
     print(f"case: getrf f32 n={n}", flush=True)
     rng = np.random.default_rng(n + 1)
