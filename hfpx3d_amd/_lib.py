@@ -57,6 +57,17 @@ class IRReport(ctypes.Structure):
     ]
 
 
+class CfReport(ctypes.Structure):
+    """Mirror of struct hfpx_cf_report."""
+    _fields_ = [
+        ("n_dof", ctypes.c_int64),
+        ("refactored", ctypes.c_int),
+        ("pressure", ctypes.c_double),
+        ("residual", ctypes.c_double),
+        ("ir", IRReport),
+    ]
+
+
 # name -> (restype, argtypes); must match include/hfpx3d.h exactly
 SIGNATURES = {
     "hfpx_create": (_c_int, [_pp, _c_int, _c_void_p]),
@@ -114,6 +125,12 @@ SIGNATURES = {
                                       _c_void_p, _c_void_p, _c_void_p]),
     "hfpx_stress_noalloc": (_c_int, [_c_void_p, _c_void_p, _c_double, _c_double, _c_void_p, _c_i64, _c_void_p,
                                      _c_i64, _c_void_p, _c_void_p, _c_i64]),
+    "hfpx_cf_create": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_i64, ctypes.POINTER(_c_void_p)]),
+    "hfpx_cf_destroy": (_c_int, [_c_void_p]),
+    "hfpx_vc_cf_iteration": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_i64, _c_void_p,
+                                      _c_i64, _c_i64, _c_double, _c_void_p, _c_i64, _c_void_p, _c_void_p,
+                                      _c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_void_p,
+                                      ctypes.POINTER(_c_i64), _c_double, _c_int, ctypes.POINTER(CfReport)]),
 }
 
 _lib = None

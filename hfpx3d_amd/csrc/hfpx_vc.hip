@@ -137,6 +137,24 @@ __global__ __launch_bounds__(256) void k_vc_gather1(const double* __restrict__ A
   }
 }
 
+// M64 / M32 (either may be null) from A by the device list idx (nu
+// entries, the last the border index), one slot per DoF
+int launch_vc_gather_idx(hfpx_ctx* ctx, const double* A, int64_t lda, const int32_t* idx, int64_t nu, double* M64,
+                         int64_t ldm64, float* M32, int64_t ldm32) {
+  const dim3 grid((unsigned)((nu + 255) / 256), (unsigned)std::min<int64_t>(nu, 32768));
+  if (M64 && M32)
+    hipLaunchKernelGGL((k_vc_gather1<true, true>), grid, dim3(256), 0, ctx->stream, A, lda, idx, nu, M64, ldm64, M32,
+                       ldm32);
+  else if (M64)
+    hipLaunchKernelGGL((k_vc_gather1<true, false>), grid, dim3(256), 0, ctx->stream, A, lda, idx, nu, M64, ldm64,
+                       M32, ldm32);
+  else
+    hipLaunchKernelGGL((k_vc_gather1<false, true>), grid, dim3(256), 0, ctx->stream, A, lda, idx, nu, M64, ldm64,
+                       M32, ldm32);
+  HFPX_TRY_HIP(ctx, hipGetLastError());
+  return HFPX_OK;
+}
+
 int launch_vc_border(hfpx_ctx* ctx, const hfpx_mesh* m, double* a64, int64_t lda64, float* a32,
                      int64_t lda32) {
   const int bs = 128;

@@ -103,6 +103,9 @@ class Frac_State_T:
 @dataclass
 class Load_T:
     s_inf: np.ndarray = field(default_factory=lambda: np.zeros(6))
+    # injection locations: rows (element, local node); the first is the
+    # injection point of the volume-control iteration (model_parameters.h:64)
+    inj_loc: np.ndarray = None
 
 
 @dataclass
@@ -114,6 +117,8 @@ class Mesh_Data_T:
     pp: np.ndarray = None     # fluid pressure per node (6 n_el)
     ae_set: list = field(default_factory=list)  # elements with a DD DoF ("active")
     fe_set: list = field(default_factory=list)  # elements with a pressure DoF ("filled")
+    mat_id: np.ndarray = None  # surface material id per node (6 n_el)
+    frac_state: "Frac_State_T" = None
 
 
 @dataclass
@@ -887,3 +892,107 @@ def update_mrd(prop, mat_id, dd, is_dd_local, f_state, ctx=None):
     f_state.mr_open = mo.cpu().numpy()
     f_state.mr_slip = ms.cpu().numpy()
     return f_state
+
+
+# --------------------------------------------------------------------------
+# cohesion / friction iteration (src/Development/c_f_iteration.cpp)
+# --------------------------------------------------------------------------
+
+class CfSystem:
+    """Resident state of the volume-control cohesion / friction iteration:
+    the (n_dof+1)^2 FP64 volume-control matrix (device, kept by reference),
+    the device mesh of its original DoF handle, and the truncated system
+    with the FP32 factors of the last active set."""
+
+    def __init__(self, orig_matrix, device_mesh, ctx=None):
+        self.ctx = ctx or get_context()
+        n1, lda = _check_colmajor(orig_matrix)
+        if orig_matrix.dtype != self.ctx.torch.float64 or n1 != device_mesh.n_dof + 1:
+            raise ValueError("orig_matrix must be the FP64 (n_dof+1)^2 volume-control matrix of the device mesh")
+        self.matrix, self.mesh = orig_matrix, device_mesh
+        h = ctypes.c_void_p()
+        self.ctx.check(self.ctx.lib.hfpx_cf_create(self.ctx.handle, device_mesh.handle, _ptr(orig_matrix), lda,
+                                                   ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            if _hip_calls_allowed():
+                self.ctx.lib.hfpx_cf_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def vc_cf_iteration(m_data_p, orig_vc_sys, orig_dof_h, prop, n_par, load, new_vol, m_data, ctx=None, tol=1e-12,
+                    max_outer=10, report=None):
+    """One step of the volume-control scheme with cohesion and friction on a
+    pre-existing mesh; returns the residual |DD and pressure increments|_1 +
+    |traction adjustments|_1 and updates m_data in place: dd, pp (the
+    pressure of the collocation points that opened), dof_h_dd (the new
+    active handle) and frac_state (friction / cohesion of the entry DD).
+    orig_vc_sys.matrix is the resident FP64 volume-control matrix of
+    orig_dof_h (make_3dbem_matrix_vc); the device mesh and the cached FP32
+    factors of the truncated system are kept on orig_vc_sys between steps.
+    load.inj_loc[0] is the injection node (element, local node).  m_data_p
+    (the previous time step) is accepted and unused, as in the reference.
+    Mirrors hfp3d::vc_cf_iteration (src/Development/c_f_iteration.cpp:27-436)
+    on the device, with the traction adjustment entering the system as
+    -R^T dt in all three components and the opening of a collocation point
+    taken in the element frame (see hfpx_vc_cf_iteration)."""
+    ctx = ctx or get_context()
+    torch = ctx.torch
+    cf = getattr(orig_vc_sys, "_cf", None)
+    if cf is None or cf.matrix is not orig_vc_sys.matrix:
+        dm = DeviceMesh(m_data.mesh, orig_dof_h, n_par.beta, ctx, dd_local=n_par.is_dd_local)
+        dm.set_vertex_weights(getattr(m_data.mesh, "vert_wts", None))
+        cf = CfSystem(orig_vc_sys.matrix, dm, ctx)
+        orig_vc_sys._cf = cf
+    cf.mesh.set_dd_local(n_par.is_dd_local)
+    n_el = cf.mesh.n_el
+    n_nod = 6 * n_el
+    par, cm, n_mat = _fc_tables(prop)
+    if cm is None:
+        raise ValueError("prop.c_model_id is required")
+    dd = np.asarray(m_data.dd, dtype=np.float64)
+    if dd.shape != (n_nod, 3):
+        raise ValueError("m_data.dd must be (6 n_el) x 3")
+    fs = m_data.frac_state if m_data.frac_state is not None else Frac_State_T.zeros(n_nod)
+    if load.inj_loc is None:
+        raise ValueError("load.inj_loc (injection element, local node) is required")
+    inj = np.asarray(load.inj_loc, dtype=np.int64).reshape(-1, 2)[0]
+    dh = np.array(np.asfortranarray(m_data.dof_h_dd.dof_h, dtype=np.int64), order="F")
+    if dh.shape != (n_el, 18):
+        raise ValueError("m_data.dof_h_dd.dof_h must be n_el x 18")
+    si = np.ascontiguousarray(load.s_inf, dtype=np.float64).reshape(6)
+    with torch.cuda.stream(ctx.stream):
+        d_dd = torch.from_numpy(np.ascontiguousarray(dd.T)).to(ctx.device)  # (3, n): col-major n x 3
+        d_pp = torch.from_numpy(np.ascontiguousarray(m_data.pp, dtype=np.float64).reshape(-1)).to(ctx.device)
+        d_mid = torch.from_numpy(np.ascontiguousarray(m_data.mat_id, dtype=np.int32).reshape(-1)).to(ctx.device)
+        d_mo = torch.from_numpy(np.ascontiguousarray(fs.mr_open, dtype=np.float64).reshape(-1)).to(ctx.device)
+        d_ms = torch.from_numpy(np.ascontiguousarray(fs.mr_slip, dtype=np.float64).reshape(-1)).to(ctx.device)
+        fc = torch.zeros(n_nod, dtype=torch.float64, device=ctx.device)
+        sc = torch.zeros_like(fc)
+        oc = torch.zeros_like(fc)
+    for t in (d_pp, d_mid, d_mo, d_ms):
+        if t.numel() != n_nod:
+            raise ValueError("pp, mat_id and the fracture state need one entry per node (6 n_el)")
+    nd = ctypes.c_int64(int(m_data.dof_h_dd.n_dof))
+    rep = report if report is not None else _lib.CfReport()
+    ctx.enter()
+    rc = ctx.lib.hfpx_vc_cf_iteration(ctx.handle, cf.handle, _ptr(d_mid), _np_ptr(cm), par, n_mat, _np_ptr(si),
+                                      int(inj[0]), int(inj[1]), float(new_vol), _ptr(d_dd), n_nod, _ptr(d_pp),
+                                      _ptr(d_mo), _ptr(d_ms), _ptr(fc), _ptr(sc), _ptr(oc), _np_ptr(dh),
+                                      ctypes.byref(nd), float(tol), int(max_outer), ctypes.byref(rep))
+    ctx.leave()
+    ctx.check(rc)
+    m_data.dd = d_dd.cpu().numpy().T.copy()
+    m_data.pp = d_pp.cpu().numpy()
+    m_data.dof_h_dd = DoF_Handle_T(dof_h=dh, n_dof=nd.value)
+    fs.friction_coef, fs.slip_cohesion, fs.open_cohesion = fc.cpu().numpy(), sc.cpu().numpy(), oc.cpu().numpy()
+    m_data.frac_state = fs
+    return rep.residual

@@ -205,6 +205,50 @@ int hfpx_match_f_c(hfpx_ctx* ctx, int64_t n_nod, const double* dd, int64_t ldd, 
  * declared cohesion_friction.h:34-41). */
 int hfpx_update_mrd(hfpx_ctx* ctx, int64_t n_nod, const double* dd, int64_t ldd, const int32_t* mat_id,
                     const hfpx_fc_param* f_c_param_host, int64_t n_mat, double* mr_open, double* mr_slip);
+/* ---- cohesion / friction iteration on a resident volume-control matrix ---- */
+typedef struct hfpx_cf_sys hfpx_cf_sys;
+/* Outcome of one step of hfpx_vc_cf_iteration. */
+typedef struct hfpx_cf_report {
+  int64_t n_dof;      /* DoF active after the step (0: nothing moved)          */
+  int refactored;     /* 1 if the truncated system was gathered and factorised
+                         (its active set differs from the cached factors')    */
+  double pressure;    /* pressure after the step                              */
+  double residual;    /* |increments and pressure increment|_1 +
+                         |traction adjustments|_1                             */
+  hfpx_ir_report ir;  /* refinement of the truncated system                   */
+} hfpx_cf_report;
+/* Iteration state on the volume-control matrix A_vc (device, (n_dof+1)^2,
+ * lda >= n_dof + 1, as written by hfpx_assemble_vc for `mesh`, whose DoF
+ * map is the original handle; every DoF on one slot).  A_vc and mesh must
+ * outlive it.  Holds the truncated system (FP64 and FP32, (n_dof+1)^2
+ * each, allocated here) and the FP32 factors of the last active set. */
+int hfpx_cf_create(hfpx_ctx* ctx, hfpx_mesh* mesh, const double* A_vc, int64_t lda, hfpx_cf_sys** out);
+int hfpx_cf_destroy(hfpx_cf_sys* sys);
+/* One step of the volume-control cohesion / friction iteration.  Device
+ * arrays of the 6 n_el nodes: mat_id, dd (6 n_el x 3, ldd; global, or
+ * element frames with hfpx_mesh_set_dd_local) and pp (nodal pressure) are
+ * updated in place, mr_open / mr_slip read, friction_coef / slip_cohesion /
+ * open_cohesion written (hfpx_match_f_c of the entry DD).  Host: the model
+ * ids and parameters as for hfpx_match_f_c, sinf_host (6, xx yy zz xy xz yz,
+ * tension positive), the injection node (element inj_el, local node
+ * inj_node), the injected volume new_vol, and the current DoF handle
+ * dof_h_host (n_el x 18 col-major, a subset of the mesh's original DoF,
+ * -1 = not active) with *n_dof, replaced by the new handle (the active
+ * original DoF renumbered in their original order).  The truncated system
+ * is solved by FP32 LU + FP64 refinement (tol, max_outer as for
+ * hfpx_solve_ir); its FP32 factors are reused while the active set stays
+ * the same.  Synchronous.  Replaces hfp3d::vc_cf_iteration
+ * (src/Development/c_f_iteration.cpp:27-436, declared c_f_iteration.h:24-38)
+ * with the traction adjustment entering the system as -R^T dt in all three
+ * components (the reference adds +R^T dt in two, :279-285) and the opening
+ * of a collocation point taken in the element frame (the reference rotates
+ * it twice, :404-409). */
+int hfpx_vc_cf_iteration(hfpx_ctx* ctx, hfpx_cf_sys* sys, const int32_t* mat_id, const int32_t* c_model_id_host,
+                         const hfpx_fc_param* f_c_param_host, int64_t n_mat, const double* sinf_host,
+                         int64_t inj_el, int64_t inj_node, double new_vol, double* dd, int64_t ldd, double* pp,
+                         const double* mr_open, const double* mr_slip, double* friction_coef,
+                         double* slip_cohesion, double* open_cohesion, int64_t* dof_h_host, int64_t* n_dof,
+                         double tol, int max_outer, hfpx_cf_report* rep);
 /* ---- postprocessing ------------------------------------------------------ */
 /* Stress at n_pts observation points: S (device, n_pts x 6, col-major, ldS,
  * order Sxx Syy Szz Sxy Sxz Syz, tension positive) = sinf_host (6 values) +

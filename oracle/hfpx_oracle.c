@@ -22,6 +22,11 @@
  *                           src/Core/element_utilities.cpp:484-595
  *   cohesion / friction     src/Development/cohesion_friction.cpp:22-189
  *                           (match_f_c, update_mrd)
+ *   cohesion / friction     src/Development/c_f_iteration.cpp:27-436
+ *   iteration step          (vc_cf_iteration; parity pinned by the plain
+ *                           volume-control solve, the Newton fixed point
+ *                           and the friction bound -- the reference file
+ *                           does not build here)
  *
  * The per-pair kernel.  The stress at X due to a displacement discontinuity
  * D_i P(xi) on a flat element (plane x3 = 0 of the element frame) follows
@@ -955,4 +960,214 @@ void hfpo_update_mrd(int64_t n_nod, const double* dd, int64_t ldd, const int32_t
     if (open >= 1.0) mr_open[n] = 1.0;
     if (open > pr_open) mr_open[n] = open;
   }
+}
+
+/* ---------------------------------------------------------------------- */
+/* one step of the volume-control cohesion / friction iteration             */
+/* (vc_cf_iteration, src/Development/c_f_iteration.cpp:27-436)             */
+/* ---------------------------------------------------------------------- */
+
+/* Shape-function values of the uniform 6-node element at its six
+ * collocation points (set_ele_struct's sf_cp): sf[q][j] = N_j(cp_q), cp_q at
+ * barycentric (1 - beta) L(node q) + beta / 3. */
+static void cf_sf_cp(double beta, double sf[6][6]) {
+  for (int q = 0; q < 6; ++q) {
+    double L[3];
+    for (int k = 0; k < 3; ++k) {
+      double ln;
+      if (q < 3)
+        ln = k == q ? 1.0 : 0.0;
+      else
+        ln = k == q - 3 ? 0.0 : 0.5;
+      L[k] = (1.0 - beta) * ln + beta / 3.0;
+    }
+    for (int v = 0; v < 3; ++v) {
+      sf[q][v] = L[v] * (2.0 * L[v] - 1.0);
+      sf[q][3 + v] = 4.0 * L[(v + 1) % 3] * L[(v + 2) % 3];
+    }
+  }
+}
+
+/* y = R x (transpose = 0) or R^T x (transpose = 1); rt col-major, rows e1..e3 */
+static void cf_rot(const double* rt, int transpose, const double* x, double* y) {
+  double t[3];
+  for (int i = 0; i < 3; ++i)
+    t[i] = transpose ? rt[0 + 3 * i] * x[0] + rt[1 + 3 * i] * x[1] + rt[2 + 3 * i] * x[2]
+                     : rt[i] * x[0] + rt[i + 3] * x[1] + rt[i + 6] * x[2];
+  y[0] = t[0]; y[1] = t[1]; y[2] = t[2];
+}
+
+int hfpo_vc_cf_iteration(const double* nods, int64_t nods_rows, int64_t n_nodes, const int64_t* conn,
+                         int64_t conn_rows, int64_t n_el, double beta, int is_dd_local,
+                         const int64_t* orig_dof_h, int64_t orig_ndof, const double* A, int64_t lda,
+                         const int32_t* mat_id, const int32_t* cmid, int64_t n_mat, const double* par,
+                         const double* mr_open, const double* mr_slip, const double* s_inf, int64_t inj_el,
+                         int64_t inj_node, double new_vol, double* dd, double* pp, int64_t* dof_h,
+                         int64_t* n_dof, double* friction_coef, double* slip_cohesion, double* open_cohesion,
+                         double* delta_trac, double* incr, double* pressure, double* residual) {
+  (void)n_nodes;
+  const int64_t n0 = orig_ndof, n_nod = 6 * n_el, full = 18 * n_el;
+  if (n_el < 1 || n0 < 1 || lda < n0 + 1 || inj_el < 0 || inj_el >= n_el || inj_node < 0 || inj_node >= 6)
+    return -1;
+  for (int64_t k = 0; k < full; ++k)
+    if (orig_dof_h[k] < -1 || orig_dof_h[k] >= n0 || (dof_h[k] >= 0 && orig_dof_h[k] < 0)) return -1;
+  for (int64_t n = 0; n < n_nod; ++n)
+    if (mat_id[n] < 0 || mat_id[n] >= n_mat) return -1;
+  /* friction and cohesion of the current DD (c_f_iteration.cpp:88-91) */
+  hfpo_match_f_c(n_nod, dd, n_nod, mat_id, cmid, n_mat, par, mr_open, mr_slip, friction_coef, slip_cohesion,
+                 open_cohesion);
+  /* DD vector of the DoF active in the current handle, at their original
+   * numbers (get_dd_vector_from_md, surface_mesh_utilities.cpp:193-240, read
+   * into the original numbering the matrix has), its tractions and the
+   * current volume (c_f_iteration.cpp:65-77) */
+  double* ddv = (double*)calloc((size_t)n0, sizeof(double));
+  double* et = (double*)calloc((size_t)n0 + 1, sizeof(double));
+  int64_t* act = (int64_t*)malloc(sizeof(int64_t) * (size_t)n0);
+  for (int64_t e = 0; e < n_el; ++e)
+    for (int l = 0; l < 18; ++l) {
+      const int64_t o = orig_dof_h[e + n_el * l];
+      if (o >= 0 && dof_h[e + n_el * l] >= 0) ddv[o] = dd[6 * e + l / 3 + n_nod * (l % 3)];
+    }
+  for (int64_t c = 0; c < n0; ++c) {
+    const double x = ddv[c];
+    if (x == 0.0) continue;
+    for (int64_t r = 0; r <= n0; ++r) et[r] += A[r + lda * c] * x;
+  }
+  const double curr_vol = et[n0], delta_v = new_vol - curr_vol;
+  const double p_old = pp[6 * inj_el + inj_node];
+  double sf[6][6];
+  cf_sf_cp(beta, sf);
+  for (int64_t c = 0; c < n0; ++c) act[c] = 0;
+  /* admissibility of the traction at each collocation node
+   * (c_f_iteration.cpp:96-276) */
+  for (int64_t e = 0; e < n_el; ++e) {
+    double vert[9], rt[9];
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k) vert[k + 3 * j] = nods[k + nods_rows * conn[j + conn_rows * e]];
+    el_rt(vert, rt);
+    /* traction of the far field on the element (normal -e3), local frame */
+    const double nv[3] = {-rt[2], -rt[5], -rt[8]};
+    double ti[3] = {s_inf[0] * nv[0] + s_inf[3] * nv[1] + s_inf[4] * nv[2],
+                    s_inf[3] * nv[0] + s_inf[1] * nv[1] + s_inf[5] * nv[2],
+                    s_inf[4] * nv[0] + s_inf[5] * nv[1] + s_inf[2] * nv[2]};
+    cf_rot(rt, 0, ti, ti);
+    for (int q = 0; q < 6; ++q) {
+      const int64_t g = 6 * e + q;
+      double tr[3];
+      for (int i = 0; i < 3; ++i) {
+        const int64_t o = orig_dof_h[e + n_el * (3 * q + i)];
+        tr[i] = o >= 0 ? et[o] : 0.0;
+      }
+      cf_rot(rt, 0, tr, tr);
+      double pr = 0.0;
+      for (int j = 0; j < 6; ++j) pr += pp[6 * e + j] * sf[q][j];
+      const double nt = pr - ti[2] - tr[2];
+      const double st[2] = {-ti[0] - tr[0], -ti[1] - tr[1]};
+      const double sh = sqrt(st[0] * st[0] + st[1] * st[1]);
+      const double adm_nt = open_cohesion[g];
+      const double adm_st = slip_cohesion[g] - friction_coef[g] * nt;
+      double dir[2] = {0.0, 0.0};
+      if (sh > 0) { dir[0] = st[0] / sh; dir[1] = st[1] / sh; }
+      double dt[3] = {0.0, 0.0, 0.0};
+      int a[3] = {2, 2, 2}; /* 1 active, 0 released, 2 as in the current handle */
+      if (nt > adm_nt) {
+        /* full separation: all traction released */
+        dt[0] = -st[0]; dt[1] = -st[1]; dt[2] = -nt;
+        a[0] = a[1] = a[2] = 1;
+      } else {
+        if (mr_open[g] < 1.0 && nt > 0.0) {
+          /* cohesive zone: normal traction to the cohesion */
+          dt[2] = adm_nt - nt;
+          a[2] = 1;
+        }
+        if (mr_slip[g] > 0 || sh > adm_st) {
+          /* slip: shear traction to the friction bound */
+          dt[0] = adm_st * dir[0] - st[0];
+          dt[1] = adm_st * dir[1] - st[1];
+          a[0] = a[1] = 1;
+        } else if (nt <= 0.0) {
+          /* intact (closed, no slip): node leaves the system */
+          a[0] = a[1] = a[2] = 0;
+        }
+      }
+      /* the adjustment as a right-hand side of the system, global frame:
+       * -R^T dt, all three components (the reference adds +R^T dt for
+       * components 0 and 1 only, c_f_iteration.cpp:279-285) */
+      double gdt[3];
+      cf_rot(rt, 1, dt, gdt);
+      for (int i = 0; i < 3; ++i) {
+        const int64_t l = 3 * q + i, o = orig_dof_h[e + n_el * l];
+        if (o < 0) continue;
+        delta_trac[o] = -gdt[i];
+        act[o] = a[i] == 2 ? (dof_h[e + n_el * l] >= 0) : a[i];
+      }
+    }
+  }
+  /* new handle: the active original DoF renumbered in their original order
+   * (the reference's dof - dd_dof_dec where that is well defined) */
+  int64_t* num = (int64_t*)malloc(sizeof(int64_t) * (size_t)n0);
+  int64_t nu = 0;
+  for (int64_t c = 0; c < n0; ++c) num[c] = act[c] ? nu++ : -1;
+  for (int64_t k = 0; k < full; ++k) dof_h[k] = orig_dof_h[k] >= 0 ? num[orig_dof_h[k]] : -1;
+  *n_dof = nu;
+  int rc = 0;
+  for (int64_t t = 0; t <= nu; ++t) incr[t] = 0.0;
+  if (nu > 0) {
+    /* truncated system (mod_3dbem_system_vc) and its FP64 LU solve */
+    const int64_t ldm = nu + 1;
+    double* M = (double*)malloc(sizeof(double) * (size_t)(ldm * ldm));
+    int32_t* piv = (int32_t*)malloc(sizeof(int32_t) * (size_t)ldm);
+    hfpo_mod_system_vc(A, lda, n0, orig_dof_h, dof_h, n_el, 18, nu, delta_trac, n0, delta_v, M, incr);
+    if (hfpo_getrf(ldm, M, ldm, piv, 1) != 0)
+      rc = 2;
+    else
+      hfpo_getrs(ldm, 1, M, ldm, piv, incr, ldm);
+    free(M);
+    free(piv);
+  } else {
+    rc = 1;
+  }
+  if (rc == 2) {
+    free(ddv); free(et); free(act); free(num);
+    return 2;
+  }
+  const double p_new = p_old + incr[nu];
+  /* DD, pressure update (c_f_iteration.cpp:310-428) */
+  for (int64_t e = 0; e < n_el; ++e) {
+    double vert[9], rt[9], del[6][3];
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k) vert[k + 3 * j] = nods[k + nods_rows * conn[j + conn_rows * e]];
+    el_rt(vert, rt);
+    for (int q = 0; q < 6; ++q) {
+      const int64_t g = 6 * e + q;
+      double dn[3];
+      for (int i = 0; i < 3; ++i) {
+        const int64_t l = 3 * q + i, o = orig_dof_h[e + n_el * l], d = dof_h[e + n_el * l];
+        dn[i] = dd[g + n_nod * i] + ((o >= 0 && d >= 0) ? incr[d] : 0.0);
+      }
+      if (!is_dd_local) cf_rot(rt, 0, dn, dn);
+      if (dn[2] < 0.0) dn[2] = 0.0; /* no overlap */
+      for (int i = 0; i < 3; ++i) del[q][i] = dn[i];
+      if (!is_dd_local) cf_rot(rt, 1, dn, dn);
+      for (int i = 0; i < 3; ++i)
+        if (orig_dof_h[e + n_el * (3 * q + i)] >= 0) dd[g + n_nod * i] = dn[i];
+    }
+    for (int q = 0; q < 6; ++q) {
+      const int64_t g = 6 * e + q;
+      /* opening at the collocation point, element frame (the reference
+       * rotates this already-local value once more, c_f_iteration.cpp:404-409) */
+      double op = 0.0;
+      for (int j = 0; j < 6; ++j) op += del[j][2] * sf[q][j];
+      double st = op / par[6 * mat_id[g]];
+      if (st >= 1.0 || mr_open[g] >= 1.0) st = 1.0;
+      if (st >= 1.0 || (st > 0.0 && mr_open[g] >= 1.0)) pp[g] = p_new;
+    }
+  }
+  double res = 0.0;
+  for (int64_t t = 0; t <= nu; ++t) res += fabs(incr[t]);
+  for (int64_t c = 0; c < n0; ++c) res += fabs(delta_trac[c]);
+  *pressure = p_new;
+  *residual = res;
+  free(ddv); free(et); free(act); free(num);
+  return rc;
 }

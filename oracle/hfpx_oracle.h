@@ -137,6 +137,26 @@ void hfpo_match_f_c(int64_t n_nod, const double* dd, int64_t ldd, const int32_t*
 void hfpo_update_mrd(int64_t n_nod, const double* dd, int64_t ldd, const int32_t* mat_id,
                      int64_t n_mat, const double* par, double* mr_open, double* mr_slip);
 
+/* One step of the volume-control cohesion / friction iteration
+ * (vc_cf_iteration, src/Development/c_f_iteration.cpp:27-436), uniform
+ * elements.  A: the (orig_ndof+1)^2 volume-control matrix of the original
+ * handle orig_dof_h (n_el x 18 col-major, one slot per DoF).  In/out: dd
+ * (6 n_el x 3 col-major, ld 6 n_el), pp (6 n_el), dof_h / *n_dof (the
+ * current handle in, the new one out).  Out: friction_coef, slip_cohesion,
+ * open_cohesion (match_f_c of the entry DD), delta_trac (orig_ndof, the
+ * right-hand side by original DoF), incr (n_dof + 1 after the call: DD
+ * increments by new DoF, then the pressure increment), *pressure, *residual
+ * (|incr|_1 + |delta_trac|_1).  Returns 0; 1 when no DoF stays active (zero
+ * increments); 2 when the truncated system is singular; -1 on bad input. */
+int hfpo_vc_cf_iteration(const double* nods, int64_t nods_rows, int64_t n_nodes, const int64_t* conn,
+                         int64_t conn_rows, int64_t n_el, double beta, int is_dd_local,
+                         const int64_t* orig_dof_h, int64_t orig_ndof, const double* A, int64_t lda,
+                         const int32_t* mat_id, const int32_t* cmid, int64_t n_mat, const double* par,
+                         const double* mr_open, const double* mr_slip, const double* s_inf, int64_t inj_el,
+                         int64_t inj_node, double new_vol, double* dd, double* pp, int64_t* dof_h,
+                         int64_t* n_dof, double* friction_coef, double* slip_cohesion, double* open_cohesion,
+                         double* delta_trac, double* incr, double* pressure, double* residual);
+
 #ifdef __cplusplus
 }
 #endif

@@ -116,6 +116,9 @@ class Oracle(_FrictionCohesion):
                                                   _dp, _dp])
         self._mfc = L.fn("match_f_c", None, [_i64, _dp, _i64, _dp, _dp, _i64, _dp, _dp, _dp, _dp, _dp, _dp])
         self._mrd = L.fn("update_mrd", None, [_i64, _dp, _i64, _dp, _i64, _dp, _dp, _dp])
+        self._cfi = L.fn("vc_cf_iteration", _int,
+                         [_dp, _i64, _i64, _dp, _i64, _i64, _d, _int, _dp, _i64, _dp, _i64, _dp, _dp, _i64, _dp,
+                          _dp, _dp, _dp, _i64, _i64, _d, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp])
 
     def el_geom(self, vert, beta=0.125, wts=None):
         """wts: the three vertex weights of a non-uniform edge partition."""
@@ -241,6 +244,40 @@ class Oracle(_FrictionCohesion):
         if rc != 0:
             raise ValueError("invalid truncation")
         return M.reshape(nu, nu, order="F"), rhs
+
+    def vc_cf_iteration(self, nods, conn, orig_dof, A, mat_id, cmid, par, mr_open, mr_slip, s_inf, inj,
+                        new_vol, dd, pp, dof, beta=0.125, dd_local=False):
+        """One step of vc_cf_iteration.  dd (6 n_el x 3), pp (6 n_el) and dof
+        (n_el x 18, the current handle) are not modified; returns a dict with
+        the new dd, pp, dof, n_dof, friction_coef, slip_cohesion,
+        open_cohesion, delta_trac, incr, pressure, residual and rc."""
+        nods, conn = _f(nods), _f(conn, np.int64)
+        A = _f(A)
+        od = _f(orig_dof, np.int64)
+        n_el = od.shape[0]
+        n0 = A.shape[0] - 1
+        dd = np.array(_f(dd))
+        pp = np.array(pp, dtype=np.float64)
+        dof = np.array(_f(dof, np.int64), order="F")
+        mid = np.ascontiguousarray(mat_id, dtype=np.int32)
+        cm = _f(cmid, np.int32)
+        pr = np.ascontiguousarray(par, dtype=np.float64).reshape(-1, 6)
+        mo = np.ascontiguousarray(mr_open, dtype=np.float64)
+        ms = np.ascontiguousarray(mr_slip, dtype=np.float64)
+        si = np.ascontiguousarray(s_inf, dtype=np.float64)
+        n_nod = 6 * n_el
+        fc, sc, oc = np.zeros(n_nod), np.zeros(n_nod), np.zeros(n_nod)
+        dtr, incr = np.zeros(n0), np.zeros(n0 + 1)
+        nd, p_new, res = _i64(0), _d(0.0), _d(0.0)
+        rc = self._cfi(_p(nods), nods.shape[0], nods.shape[1], _p(conn), conn.shape[0], n_el, float(beta),
+                       int(dd_local), _p(od), n0, _p(A), A.shape[0], _p(mid), _p(cm), pr.shape[0], _p(pr), _p(mo),
+                       _p(ms), _p(si), int(inj[0]), int(inj[1]), float(new_vol), _p(dd), _p(pp), _p(dof),
+                       ctypes.byref(nd), _p(fc), _p(sc), _p(oc), _p(dtr), _p(incr), ctypes.byref(p_new),
+                       ctypes.byref(res))
+        if rc < 0:
+            raise ValueError("invalid vc_cf_iteration input")
+        return dict(dd=dd, pp=pp, dof=dof, n_dof=nd.value, friction_coef=fc, slip_cohesion=sc, open_cohesion=oc,
+                    delta_trac=dtr, incr=incr[:nd.value + 1], pressure=p_new.value, residual=res.value, rc=rc)
 
     def sneddon_dd(self, G, nu, a, p, xyz):
         dd = np.zeros(3)
