@@ -533,6 +533,10 @@ int hfpx_vc_cf_iteration(hfpx_ctx* ctx, hfpx_cf_sys* S, const int32_t* mat_id, c
   hipLaunchKernelGGL(k_cf_fill, dim3(blocks(n0 + 1, 256)), dim3(256), 0, ctx->stream, S->eneg, n0 + 1, 0.0);
   HFPX_TRY_HIP(ctx, hipGetLastError());
   if ((rc = gemm_sub_any(ctx, HFPX_F64, S->eneg, n0 + 1, S->A, S->lda, S->ddv, n0, n0 + 1, 1, n0))) return rc;
+  // algorithmic HBM bytes of the traction product: the n0 + 1 rows of the
+  // first n0 columns of the matrix, read once
+  const double nn = (double)n0 * (double)(n0 + 1);
+  prof_end(ctx, tok, HFPX_KC_VC, 2.0 * nn, 8.0 * nn);
   // 2. admissibility node by node
   CfLoad F;
   for (int k = 0; k < 6; ++k) F.s[k] = sinf_host[k];
@@ -597,11 +601,6 @@ int hfpx_vc_cf_iteration(hfpx_ctx* ctx, hfpx_cf_sys* S, const int32_t* mat_id, c
   hipLaunchKernelGGL(k_cf_l1, dim3((unsigned)nb), dim3(256), 0, ctx->stream, S->x, nu + 1, S->dtr, n0, S->part);
   hipLaunchKernelGGL(k_cf_l1_sum, dim3(1), dim3(64), 0, ctx->stream, S->part, nb, S->scal + 2);
   HFPX_TRY_HIP(ctx, hipGetLastError());
-  // algorithmic HBM bytes of the step's own passes: the product's read of
-  // the matrix, plus the gather when it ran
-  const double nn = (double)n0 * (double)(n0 + 1);
-  const double gat = rep->refactored ? 20.0 * (double)(nu + 1) * (double)(nu + 1) : 0.0;
-  prof_end(ctx, tok, HFPX_KC_VC, 2.0 * nn, 8.0 * nn + gat);
   double hsc[3];
   HFPX_TRY_HIP(ctx, hipMemcpyAsync(hsc, S->scal, sizeof(hsc), hipMemcpyDeviceToHost, ctx->stream));
   HFPX_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));

@@ -25,6 +25,7 @@
 #include <array>
 #include <complex>
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <utility>
 #include <vector>
@@ -76,6 +77,15 @@ struct DoF_Handle_T {
   Array2D<int64_t> dof_h;  // n_el x 18, -1 = fixed
 };
 
+// Node-wise crack-surface state (src/Core/surface_mesh_utilities.h:51-62).
+struct Frac_State_T {
+  std::vector<double> mr_open;        // largest relative opening reached
+  std::vector<double> mr_slip;        // largest relative slip reached
+  std::vector<double> friction_coef;
+  std::vector<double> slip_cohesion;
+  std::vector<double> open_cohesion;
+};
+
 struct Mesh_Data_T {
   Mesh_Geom_T mesh;
   DoF_Handle_T dof_h_dd;
@@ -84,6 +94,8 @@ struct Mesh_Data_T {
   std::vector<double> pp;  // fluid pressure per node (6 n_el)
   std::vector<int64_t> ae_set;  // elements with a DD DoF ("active")
   std::vector<int64_t> fe_set;  // elements with a pressure DoF ("filled")
+  std::vector<int32_t> mat_id;  // surface material id per node (6 n_el)
+  Frac_State_T frac_state;      // node-wise cohesion / friction state
 };
 
 // Friction / cohesion parameters of one surface material
@@ -106,17 +118,11 @@ struct Properties_T {
   std::vector<F_C_Param_T> f_c_param;   // per surface material
 };
 
-// Node-wise crack-surface state (src/Core/surface_mesh_utilities.h:51-62).
-struct Frac_State_T {
-  std::vector<double> mr_open;        // largest relative opening reached
-  std::vector<double> mr_slip;        // largest relative slip reached
-  std::vector<double> friction_coef;
-  std::vector<double> slip_cohesion;
-  std::vector<double> open_cohesion;
-};
-
 struct Load_T {
   double s_inf[6] = {0, 0, 0, 0, 0, 0};  // xx yy zz xy xz yz, tension positive
+  // injection locations, n x 2 (element, local node); row 0 is the injection
+  // point of the volume-control iteration (model_parameters.h:64)
+  Array2D<int64_t> inj_loc;
 };
 
 struct Num_Param_T {
@@ -161,10 +167,17 @@ class DeviceMatrix {
   int64_t n_ = 0;
 };
 
+// Device state of the cohesion / friction iteration on a resident
+// volume-control matrix (the device mesh of the original handle and the
+// cached FP32 factors of the truncated system).
+struct CfState;
+
 struct SAE_T {
   int64_t n_dof = 0;
   DeviceMatrix matrix;
   std::vector<double> rhs_v;
+  // set by vc_cf_iteration on the volume-control system it iterates on
+  mutable std::shared_ptr<CfState> cf;
 };
 
 // ---- mesh / DoF -----------------------------------------------------------
@@ -213,6 +226,23 @@ void match_f_c(const Properties_T& prop, const std::vector<int32_t>& mat_id, con
                bool is_dd_local, Frac_State_T& f_state);
 void update_mrd(const Properties_T& prop, const std::vector<int32_t>& mat_id, const Array2D<double>& dd,
                 bool is_dd_local, Frac_State_T& f_state);
+
+// One step of the volume-control scheme with cohesion and friction on a
+// pre-existing mesh (src/Development/c_f_iteration.cpp:27-436): returns the
+// residual |DD and pressure increments|_1 + |traction adjustments|_1 and
+// updates m_data.dd, m_data.pp (collocation points that opened take the new
+// pressure), m_data.dof_h_dd (the new active handle) and m_data.frac_state
+// (friction / cohesion of the entry DD).  orig_vc_sys.matrix is the
+// volume-control matrix of orig_dof_h (make_3dbem_matrix_vc); the device
+// state is kept on orig_vc_sys between steps, and the FP32 factors of the
+// truncated system are reused while the active set stays the same.
+// m_data_p is unused, as in the reference.  The traction adjustment enters
+// the system as -R^T dt in all three components (the reference adds +R^T dt
+// in two), and the opening of a collocation point is taken in the element
+// frame (the reference rotates it twice).
+double vc_cf_iteration(const Mesh_Data_T& m_data_p, const SAE_T& orig_vc_sys, const DoF_Handle_T& orig_dof_h,
+                       const Properties_T& prop, const Num_Param_T& n_par, const Load_T& load, double new_vol,
+                       Mesh_Data_T& m_data);
 
 class LU {
  public:

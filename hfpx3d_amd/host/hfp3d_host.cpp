@@ -305,6 +305,80 @@ void update_mrd(const Properties_T& prop, const std::vector<int32_t>& mat_id, co
   a.down(1, f_state.mr_slip);
 }
 
+// device mesh of the original handle and the library's iteration state
+struct CfState {
+  MeshHandle mh;
+  hfpx_cf_sys* sys = nullptr;
+  const double* A = nullptr;
+  CfState(const Mesh_Geom_T& mesh, const DoF_Handle_T& orig_dof_h, const Num_Param_T& n_par, const DeviceMatrix& M)
+      : mh(mesh, &orig_dof_h, n_par.beta, n_par.is_dd_local), A(M.data()) {
+    check(hfpx_cf_create(context(), mh.m, M.data(), M.size(0), &sys), "vc_cf_iteration: iteration state");
+  }
+  ~CfState() { hfpx_cf_destroy(sys); }
+  CfState(const CfState&) = delete;
+  CfState& operator=(const CfState&) = delete;
+};
+
+double vc_cf_iteration(const Mesh_Data_T& m_data_p, const SAE_T& orig_vc_sys, const DoF_Handle_T& orig_dof_h,
+                       const Properties_T& prop, const Num_Param_T& n_par, const Load_T& load, double new_vol,
+                       Mesh_Data_T& m_data) {
+  (void)m_data_p;
+  const int64_t n_el = m_data.mesh.conn.size(1), n_nod = 6 * n_el;
+  auto fail = [](const char* what) {
+    std::cerr << "hfp3d: vc_cf_iteration: " << what << std::endl;
+    std::abort();
+  };
+  if (orig_vc_sys.matrix.size(0) != orig_dof_h.n_dof + 1) fail("the matrix is not orig n_dof + 1 square");
+  if (m_data.dd.size(0) != n_nod || m_data.dd.size(1) != 3) fail("m_data.dd must be (6 n_el) x 3");
+  if ((int64_t)m_data.pp.size() != n_nod || (int64_t)m_data.mat_id.size() != n_nod)
+    fail("m_data.pp and m_data.mat_id need one entry per node");
+  if (m_data.dof_h_dd.dof_h.size(0) != n_el || m_data.dof_h_dd.dof_h.size(1) != 18)
+    fail("m_data.dof_h_dd must be n_el x 18");
+  if (load.inj_loc.size(0) < 1 || load.inj_loc.size(1) != 2) fail("load.inj_loc needs a row (element, node)");
+  Frac_State_T& fs = m_data.frac_state;
+  if (fs.mr_open.empty()) fs.mr_open.assign((size_t)n_nod, 0.0);
+  if (fs.mr_slip.empty()) fs.mr_slip.assign((size_t)n_nod, 0.0);
+  if ((int64_t)fs.mr_open.size() != n_nod || (int64_t)fs.mr_slip.size() != n_nod)
+    fail("the fracture state needs one entry per node");
+  const std::vector<hfpx_fc_param> par = fc_params(prop);
+  const int64_t n_mat = (int64_t)par.size();
+  if (prop.c_model_id.size(0) != n_mat || prop.c_model_id.size(1) != 2) fail("c_model_id must be n_surface x 2");
+  if (!orig_vc_sys.cf || orig_vc_sys.cf->A != orig_vc_sys.matrix.data())
+    orig_vc_sys.cf = std::make_shared<CfState>(m_data.mesh, orig_dof_h, n_par, orig_vc_sys.matrix);
+  CfState& cf = *orig_vc_sys.cf;
+  check(hfpx_mesh_set_dd_local(cf.mh.m, n_par.is_dd_local ? 1 : 0), "vc_cf_iteration");
+  double* dd = dalloc<double>(3 * (size_t)n_nod);
+  double* st[6];  // pp, mr_open, mr_slip, friction, slip cohesion, opening cohesion
+  for (double*& p : st) p = dalloc<double>((size_t)n_nod);
+  int32_t* mid = dalloc<int32_t>((size_t)n_nod);
+  h2d(dd, m_data.dd.data(), sizeof(double) * 3 * n_nod);
+  h2d(st[0], m_data.pp.data(), sizeof(double) * n_nod);
+  h2d(st[1], fs.mr_open.data(), sizeof(double) * n_nod);
+  h2d(st[2], fs.mr_slip.data(), sizeof(double) * n_nod);
+  h2d(mid, m_data.mat_id.data(), sizeof(int32_t) * n_nod);
+  Array2D<int64_t> dh = m_data.dof_h_dd.dof_h;
+  int64_t nd = m_data.dof_h_dd.n_dof;
+  hfpx_cf_report rep;
+  const int rc = hfpx_vc_cf_iteration(context(), cf.sys, mid, prop.c_model_id.data(), par.data(), n_mat, load.s_inf,
+                                      load.inj_loc(0, 0), load.inj_loc(0, 1), new_vol, dd, n_nod, st[0], st[1],
+                                      st[2], st[3], st[4], st[5], dh.data(), &nd, 1e-12, 10, &rep);
+  check(rc, "vc_cf_iteration");
+  d2h(m_data.dd.data(), dd, sizeof(double) * 3 * n_nod);
+  d2h(m_data.pp.data(), st[0], sizeof(double) * n_nod);
+  fs.friction_coef.resize((size_t)n_nod);
+  fs.slip_cohesion.resize((size_t)n_nod);
+  fs.open_cohesion.resize((size_t)n_nod);
+  d2h(fs.friction_coef.data(), st[3], sizeof(double) * n_nod);
+  d2h(fs.slip_cohesion.data(), st[4], sizeof(double) * n_nod);
+  d2h(fs.open_cohesion.data(), st[5], sizeof(double) * n_nod);
+  (void)hipFree(dd);
+  (void)hipFree(mid);
+  for (double* p : st) (void)hipFree(p);
+  m_data.dof_h_dd.dof_h = std::move(dh);
+  m_data.dof_h_dd.n_dof = nd;
+  return rep.residual;
+}
+
 DeviceMatrix make_3dbem_matrix_vc(double shear_m, double poiss_r, const Mesh_Geom_T& mesh,
                                   const Num_Param_T& n_par, DoF_Handle_T& dof_hndl) {
   if (dof_hndl.n_dof == 0 || dof_hndl.dof_h.size(0) == 0) dof_hndl = make_dof_h_crack(mesh, 2, n_par.tip_type);

@@ -198,3 +198,48 @@ def test_rejects_handle_outside_the_original(hx):
     with pytest.raises(hx._lib.HfpxError):
         hx.vc_cf_iteration(None, c.sys, c.dof, c.prop(), c.n_par,
                            hx.Load_T(s_inf=np.array([0, 0, 1.0, 0, 0, 0]), inj_loc=np.array([[0, 0]])), 0.1, md)
+
+
+def test_cpp_host_mirror_matches_oracle():
+    """Two steps of hfp3d::vc_cf_iteration of the C++ host mirror (through
+    the hfpx3d_cf_check program) against the oracle's steps: full separation
+    of the 24-element penny under sigma_zz = 1 with twice the volume the far
+    field opens, then the step from that solution."""
+    import json
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "tests", "native", "hfpx3d_cf_check")
+    nods, conn = load_mesh("pennymesh24el_32")
+    A, dof = O.assemble(nods, conn, G, NU, beta=BETA, tip_type=0)
+    row, col = O.vc_border(nods, conn, dof)
+    n = A.shape[0]
+    Avc = np.zeros((n + 1, n + 1), order="F")
+    Avc[:n, :n], Avc[n, :n], Avc[:n, n] = A, row, col
+    vol = 2.0 * row @ np.linalg.solve(A, O.rhs(nods, conn, dof, np.array([0, 0, 1.0, 0, 0, 0])))
+    print("case: C++ host mirror of vc_cf_iteration, 24-element penny", flush=True)
+    r = subprocess.run([exe, golden("Elems_pennymesh24el_32.npy"), golden("Nodes_pennymesh24el_32.npy"), "1.0",
+                        repr(float(vol))], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    d = json.loads(r.stdout)
+    n_nod = 6 * conn.shape[1]
+    par = np.array([[1e-9, 0.0, 1.0, 0.0, 0.0, 0.0]])
+    cmid = np.zeros((1, 2), np.int32)
+    z = np.zeros(n_nod)
+    dd, pp, cur = np.zeros((n_nod, 3)), z.copy(), dof.copy()
+    for k in range(2):
+        o = O.vc_cf_iteration(nods, conn, dof, Avc, np.zeros(n_nod, np.int32), cmid, par, z, z,
+                              np.array([0, 0, 1.0, 0, 0, 0]), (0, 0), vol, dd, pp, cur, beta=BETA)
+        got_dd = np.array(d[f"dd{k}"]).reshape(n_nod, 3, order="F")
+        if k == 0:
+            assert d["n_dof0"] == o["n_dof"] == n
+            assert rel(got_dd, o["dd"]) < 1e-9
+            assert rel(np.array(d["pp0"]), o["pp"]) < 1e-9
+            assert abs(d["residual0"] - o["residual"]) < 1e-9 * o["residual"]
+            scale = o["residual"]
+        else:
+            # from its own solution nothing moves (the active set there is
+            # decided by rounding noise, so it is not compared)
+            assert d["residual1"] < 1e-9 * scale and o["residual"] < 1e-9 * scale
+            assert rel(got_dd, o["dd"]) < 1e-9
+        dd, pp, cur = o["dd"], o["pp"], o["dof"]

@@ -16,6 +16,12 @@ the step is pinned by what the mathematics fixes:
     the step the shear traction of every node equals the friction bound
     mu |sigma_n| along its shear direction, recomputed from the new DD with
     the matrix and the far field;
+  * rotation: with DD in element frames the slip step on the penny rotated
+    out of z = 0 also reaches the friction bound in one step (the next
+    step's residual vanishes); with global DD it does not, because the laws
+    read the opening and slip from the DD components as given (the
+    reference's rotation to element frames is an empty branch,
+    cohesion_friction.cpp:50-63);
   * invalid handles are rejected.
 """
 import numpy as np
@@ -175,3 +181,31 @@ def test_rejects_handle_outside_the_original():
     cur[tuple(fixed)] = 0
     with pytest.raises(ValueError):
         step(nods, conn, Avc, dof, par, cmid, np.array([0, 0, 1.0, 0, 0, 0]), 0.1, cur=cur)
+
+
+def rotated(nods, R):
+    out = nods.copy()
+    out[:3] = R @ nods[:3]
+    return out
+
+
+@pytest.mark.parametrize("dd_local", [True, False])
+def test_slip_on_a_rotated_crack(dd_local):
+    nods, conn = load_mesh("pennymesh121el_32")
+    c, s = np.cos(np.pi / 6), np.sin(np.pi / 6)
+    R = np.array([[1.0, 0, 0], [0, c, -s], [0, s, c]])
+    nods = rotated(nods, R)
+    Avc, dof = vc_system(nods, conn, dd_local=dd_local)
+    par, cmid = material(mu=0.2)
+    nrm, t = R @ np.array([0, 0, 1.0]), R @ np.array([1.0, 0, 0])
+    S = -np.outer(nrm, nrm) + 0.3 * (np.outer(nrm, t) + np.outer(t, nrm))
+    s_inf = np.array([S[0, 0], S[1, 1], S[2, 2], S[0, 1], S[0, 2], S[1, 2]])
+    n_nod = 6 * conn.shape[1]
+    ms = np.full(n_nod, 0.5)
+    r1 = step(nods, conn, Avc, dof, par, cmid, s_inf, 0.0, dd_local=dd_local, mr_slip=ms)
+    r2 = step(nods, conn, Avc, dof, par, cmid, s_inf, 0.0, dd=r1["dd"], pp=r1["pp"], cur=r1["dof"],
+              dd_local=dd_local, mr_slip=ms)
+    if dd_local:
+        assert r2["residual"] < 1e-12 * r1["residual"]
+    else:
+        assert r2["residual"] > 0.1 * r1["residual"]
