@@ -290,7 +290,8 @@ def main():
     a = prof.get("assemble", dict(ms=0.0, bytes=0.0))
     asm_gbs = a["bytes"] / (a["ms"] / 1e3) / 1e9 if a["ms"] > 0 else 0.0
     roofline = {"kernel": "FP32 trailing-update GEMM of the LU: k_split_a + k_split_b + k_x3gemm (bf16 MFMA "
-                          "32x32x16, six products of three-way operand splits, FP32 accumulation); products "
+                          "32x32x16, six products of three-way operand splits, FP32 accumulation) on the large "
+                          "updates, k_x3gemm_f (the split fused into the product) on the mid-size ones; products "
                           "below 256 x 128 x 64 on FP32 MFMA (k_sgemm32 / k_gemm_sub<float>)",
                 "bound": "mfma", "achieved": round(gemm_tf, 3), "peak": round(X3_PEAK, 1), "unit": "TFLOP/s",
                 "frac": round(gemm_tf / X3_PEAK, 4),

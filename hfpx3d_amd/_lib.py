@@ -28,7 +28,7 @@ KNOB_TEST_PANEL_TIMEOUT = 6
 KNOB_GEMM_F64 = 7
 KNOB_LU_LOOKAHEAD = 8
 KNOB_IR_MODE = 9
-GEMM_F32_VARIANTS = 16
+GEMM_F32_VARIANTS = 18
 
 KERNEL_CLASSES = {"assemble": 0, "gemm_f64": 1, "gemm_f32": 2, "panel": 3, "laswp": 4, "trsm": 5, "stress": 6,
                   "vc": 7}

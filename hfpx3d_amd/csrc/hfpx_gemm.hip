@@ -705,7 +705,7 @@ static void launch_sgemm32(hfpx_ctx* ctx, float* C, int64_t ldc, const float* A,
   }
 }
 
-int gemm_f32_variants() { return 16; }
+int gemm_f32_variants() { return 18; }
 
 // FP64 128 x 128 product: interior tiles by the unguarded kernel, then the
 // bottom band (all columns) and the right band (interior rows) guarded.
@@ -775,15 +775,23 @@ int gemm_sub(hfpx_ctx* ctx, T* C, int64_t ldc, const T* A, int64_t lda, const T*
     // fill (faster on every LU shape measured, 1.5x on the large trailing
     // updates), with 128 x 256 tiles and the pipelined loop on the large
     // square updates (+5 to 7 % for the tiles, slower on narrow products;
-    // +5 % for the pipelining); small products on FP32 MFMA, C preloaded
-    // for short K
-    var = (M >= 256 && N >= 128 && K >= 64) ? ((M >= 4096 && N >= 4096 && K >= 512) ? 15 : 13)
+    // +5 % for the pipelining); on the other products the split fused into
+    // the product (the split pass costs more than the split redone per
+    // tile when one side has few tiles; the same bits; 72,000-DoF mixed
+    // solve 1.869 -> 1.852 s), while on the large updates the per-tile
+    // split costs more (70,976^2 x 1,024: 59.4 against 51.3 ms); small
+    // products on FP32 MFMA, C preloaded for short K
+    var = (M >= 256 && N >= 128 && K >= 64) ? ((M >= 4096 && N >= 4096 && K >= 512) ? 15 : 17)
                                              : (K <= 384 ? 7 : 2);
   }
   bool done = false;
   if constexpr (sizeof(T) == 4) {
-    if (!skinny && (var == 13 || var == 14 || var == 15)) {
-      int rc = gemm_x3(ctx, C, ldc, A, lda, B, ldb, M, N, K, var - 13);
+    if (!skinny && (var >= 13 && var <= 17)) {
+      // 13 .. 15: split pass + product (128 x 128, 128 x 256, pipelined
+      // 128 x 256); 16, 17: the split fused into the product (128 x 256,
+      // 128 x 128)
+      const int wide = var == 16 ? 3 : (var == 17 ? 4 : var - 13);
+      int rc = gemm_x3(ctx, C, ldc, A, lda, B, ldb, M, N, K, wide);
       if (rc) return rc;
       done = true;
     } else if (!skinny && var > 1) {

@@ -305,6 +305,180 @@ __device__ __forceinline__ void x3_tile(X3Lds<ROW, WCB>& lds, float* __restrict_
     }
 }
 
+// The same tile with the split fused into the load stage: the kernel reads
+// the FP32 operands themselves (4 bytes per entry instead of the parts'
+// 6, and no split pass), splits each staged piece in registers into its
+// three bf16 parts and stores them to LDS in the layout of the split blocks.
+// The parts, and so the products, are bitwise those of the split kernels.
+// Thread t stages row row0 + t / 2 of A and column col0 + 128 h + t / 2 of
+// B (h < NBB), k 8 (t % 2) .. + 7 of each chunk.  Software-pipelined as
+// PIPE above; each staged piece is split at the LDS store slot of its
+// first part (one piece's parts live at a time).  VECB: B's columns are
+// 16-byte aligned (two 16-byte loads per piece instead of eight 4-byte
+// ones).
+template <int ROW, int WCB, bool VECB>
+__device__ __forceinline__ void x3f_tile(X3Lds<ROW, WCB>& lds, float* __restrict__ C, int64_t ldc,
+                                         const float* __restrict__ A, int64_t lda, const float* __restrict__ B,
+                                         int64_t ldb, int64_t M, int64_t N, int64_t K, int nkc, int64_t row0,
+                                         int64_t col0) {
+  constexpr int GN = 64 * WCB;
+  constexpr int NBB = X3Lds<ROW, WCB>::NBB;
+  auto& sA = lds.a;
+  auto& sB = lds.b;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int l32 = lane & 31, hk = lane >> 5;
+  const int wr = (wave & 1) * 64, wc = (wave >> 1) * 32 * WCB;
+
+  float* __restrict__ Ct = C + row0 + col0 * ldc;
+  const int ld = (int)ldc;
+  const int off0 = wr + l32 + (wc + 4 * hk) * ld;
+  const int mrem = (int)((M - row0) < X3_T ? (M - row0) : X3_T);
+  const int nrem = (int)((N - col0) < GN ? (N - col0) : GN);
+  floatx16_x3 acc[WCB][2];
+#pragma unroll
+  for (int tc = 0; tc < WCB; ++tc)
+#pragma unroll
+    for (int tr = 0; tr < 2; ++tr) {
+      const int i = wr + 32 * tr + l32;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int jc = wc + 32 * tc + 8 * (v >> 2) + 4 * hk + (v & 3);
+        const int off = off0 + 32 * tr + (32 * tc + 8 * (v >> 2) + (v & 3)) * ld;
+        acc[tc][tr][v] = (i < mrem && jc < nrem) ? Ct[off] : 0.f;
+      }
+    }
+
+  // raw FP32 staging of this thread's pieces
+  const int pr = t >> 1, pk = 8 * (t & 1);
+  const int64_t ga = row0 + pr;
+  const bool a_in = ga < M;
+  float ra[8], rb[NBB][8];
+  auto gload = [&](int kc) {
+    const int64_t k0 = (int64_t)kc * X3_KC + pk;
+    const bool kfull = k0 + 8 <= K;
+    const float* p = A + ga + k0 * lda;
+    if (a_in && kfull) {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) ra[e] = p[e * lda];
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) ra[e] = (a_in && k0 + e < K) ? p[e * lda] : 0.f;
+    }
+#pragma unroll
+    for (int h = 0; h < NBB; ++h) {
+      const int64_t gj = col0 + X3_T * h + pr;
+      const float* q = B + k0 + gj * ldb;
+      if (gj < N && kfull) {
+        if constexpr (VECB) {
+          const float4 x0 = *(const float4*)q, x1 = *(const float4*)(q + 4);
+          rb[h][0] = x0.x; rb[h][1] = x0.y; rb[h][2] = x0.z; rb[h][3] = x0.w;
+          rb[h][4] = x1.x; rb[h][5] = x1.y; rb[h][6] = x1.z; rb[h][7] = x1.w;
+        } else {
+#pragma unroll
+          for (int e = 0; e < 8; ++e) rb[h][e] = q[e];
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) rb[h][e] = (gj < N && k0 + e < K) ? q[e] : 0.f;
+      }
+    }
+  };
+  // the three parts of one staged piece at a time (B negated): piece g = 0
+  // is A's, g = 1 + h block h of B's
+  bf16x8 pp[3];
+  auto split_piece = [&](int g) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      __bf16 h, m, l;
+      if (g == 0)
+        split3(ra[e], h, m, l);
+      else
+        split3(-rb[g - 1][e], h, m, l);
+      pp[0][e] = h;
+      pp[1][e] = m;
+      pp[2][e] = l;
+    }
+  };
+  constexpr int SWZ = ROW == 16 ? 1 : 0;
+  const int soff = (t >> 1) * ROW + 8 * ((t & 1) ^ (SWZ & ((t >> 3) ^ (t >> 4))));
+  const int rh = 8 * (hk ^ (SWZ & ((l32 >> 2) ^ (l32 >> 3))));
+  // store q of a chunk: part q % 3 of piece q / 3 (split at its first part)
+  auto lstore_piece = [&](int buf, int q) {
+    if (q % 3 == 0) split_piece(q / 3);
+    if (q < 3)
+      *(bf16x8*)&sA[buf][q * X3_T * ROW + soff] = pp[q];
+    else
+      *(bf16x8*)&sB[buf][(q - 3) / 3][((q - 3) % 3) * X3_T * ROW + soff] = pp[(q - 3) % 3];
+  };
+  constexpr int NLD = 3 * (1 + NBB);  // staged pieces per chunk
+
+  gload(0);
+#pragma unroll
+  for (int q = 0; q < NLD; ++q) lstore_piece(0, q);
+  __syncthreads();
+  constexpr int PB[6] = {1, 2, 0, 1, 0, 0}, PA[6] = {1, 0, 2, 0, 1, 0};
+  for (int kc = 0; kc < nkc; ++kc) {
+    const int buf = kc & 1;
+    gload(kc + 1 < nkc ? kc + 1 : kc);
+    __builtin_amdgcn_sched_barrier(0);
+    bf16x8 fb[2][3], fa[2][3];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+        fb[u][p] = *(const bf16x8*)&sA[buf][(p * X3_T + wr + 32 * u + l32) * ROW + rh];
+    auto read_fa = [&](int tc, int p, bf16x8& f) {
+      const int jb = wc + 32 * tc, h = jb / X3_T, jl = jb % X3_T;
+      f = *(const bf16x8*)&sB[buf][h][(p * X3_T + jl + l32) * ROW + rh];
+    };
+#pragma unroll
+    for (int p = 0; p < 3; ++p) read_fa(0, p, fa[0][p]);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int tc = 0; tc < WCB; ++tc) {
+#pragma unroll
+      for (int i = 0; i < 12; ++i) {
+        const int tr = i & 1, q = i >> 1;
+        acc[tc][tr] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[tc & 1][PB[q]], fb[tr][PA[q]], acc[tc][tr], 0,
+                                                              0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        if (tc + 1 < WCB && i < 3) {
+          read_fa(tc + 1, i, fa[(tc + 1) & 1][i]);
+          __builtin_amdgcn_sched_barrier(0);
+        }
+        if (tc + 1 == WCB && i >= 12 - NLD) {
+          lstore_piece(buf ^ 1, i - (12 - NLD));
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int tc = 0; tc < WCB; ++tc)
+#pragma unroll
+    for (int tr = 0; tr < 2; ++tr) {
+      const int i = wr + 32 * tr + l32;
+      if (i >= mrem) continue;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int jc = wc + 32 * tc + 8 * (v >> 2) + 4 * hk + (v & 3);
+        const int off = off0 + 32 * tr + (32 * tc + 8 * (v >> 2) + (v & 3)) * ld;
+        if (jc < nrem) Ct[off] = acc[tc][tr][v];
+      }
+    }
+}
+
+template <int ROW, int WCB, bool VECB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_x3gemm_f(
+    float* __restrict__ C, int64_t ldc, const float* __restrict__ A, int64_t lda, const float* __restrict__ B,
+    int64_t ldb, int64_t M, int64_t N, int64_t K, int nkc, int64_t tm, int64_t tn, int64_t band) {
+  __shared__ __attribute__((aligned(16))) X3Lds<ROW, WCB> lds;
+  int64_t tile_r, tile_c;
+  sg_band_tile(sg_xcd_order(blockIdx.x, tm * tn), tm, tn, tile_r, tile_c, band);
+  x3f_tile<ROW, WCB, VECB>(lds, C, ldc, A, lda, B, ldb, M, N, K, nkc, tile_r * X3_T, tile_c * 64 * WCB);
+}
+
 template <int ROW, int WCB, bool PIPE = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_x3gemm(
     float* __restrict__ C, int64_t ldc, const __bf16* __restrict__ As, const __bf16* __restrict__ Bs, int64_t M,
@@ -425,10 +599,42 @@ size_t gemm_x3_work_bytes(int64_t M, int64_t N, int64_t K) {
   return sizeof(__bf16) * (size_t)(tm + tnb) * nkc * X3_BLK;
 }
 
+// The fused-split products (wide 3: 128 x 256 tiles, 4: 128 x 128): no
+// split pass and no workspace.
+static int gemm_x3_fused(hfpx_ctx* ctx, float* C, int64_t ldc, const float* A, int64_t lda, const float* B,
+                         int64_t ldb, int64_t M, int64_t N, int64_t K, bool w256) {
+  const int64_t gn = w256 ? 256 : 128;
+  const int64_t tm = (M + X3_T - 1) / X3_T, tn = (N + gn - 1) / gn, nkc = (K + X3_KC - 1) / X3_KC;
+  if (tm * tn > INT32_MAX || nkc > 65535) return set_error(ctx, HFPX_ERR_INVALID, "gemm: too large");
+  // band height: the band's A rows (128 x K FP32) in the 4 MB L2 of an XCD
+  int64_t band = ctx->knob[HFPX_KNOB_X3_BAND];
+  if (band <= 0) band = std::max<int64_t>(1, std::min<int64_t>(16, (4 << 20) / ((int64_t)X3_T * K * 4)));
+  const bool vec = ((uintptr_t)B % 16 == 0) && (ldb % 4 == 0);
+  const dim3 grid((unsigned)(tm * tn));
+  if (w256) {
+    if (vec)
+      hipLaunchKernelGGL((k_x3gemm_f<16, 4, true>), grid, dim3(256), 0, ctx->stream, C, ldc, A, lda, B, ldb, M, N, K,
+                         (int)nkc, tm, tn, band);
+    else
+      hipLaunchKernelGGL((k_x3gemm_f<16, 4, false>), grid, dim3(256), 0, ctx->stream, C, ldc, A, lda, B, ldb, M, N,
+                         K, (int)nkc, tm, tn, band);
+  } else {
+    if (vec)
+      hipLaunchKernelGGL((k_x3gemm_f<24, 2, true>), grid, dim3(256), 0, ctx->stream, C, ldc, A, lda, B, ldb, M, N, K,
+                         (int)nkc, tm, tn, band);
+    else
+      hipLaunchKernelGGL((k_x3gemm_f<24, 2, false>), grid, dim3(256), 0, ctx->stream, C, ldc, A, lda, B, ldb, M, N,
+                         K, (int)nkc, tm, tn, band);
+  }
+  HFPX_TRY_HIP(ctx, hipGetLastError());
+  return HFPX_OK;
+}
+
 int gemm_x3(hfpx_ctx* ctx, float* C, int64_t ldc, const float* A, int64_t lda, const float* B, int64_t ldb,
             int64_t M, int64_t N, int64_t K, int wide) {
   if (M <= 0 || N <= 0 || K <= 0) return HFPX_OK;
   if (ldc > (int64_t)1 << 24) return set_error(ctx, HFPX_ERR_INVALID, "gemm: ldc above 2^24");
+  if (wide == 3 || wide == 4) return gemm_x3_fused(ctx, C, ldc, A, lda, B, ldb, M, N, K, wide == 3);
   const int64_t tm = (M + X3_T - 1) / X3_T, nkc = (K + X3_KC - 1) / X3_KC;
   const int64_t tnb = ((N + 2 * X3_T - 1) / (2 * X3_T)) * 2;  // B blocks, even
   const int64_t tn = wide ? tnb / 2 : (N + X3_T - 1) / X3_T;  // product tiles along N

@@ -379,7 +379,7 @@ int hfpx_batch_matrix(hfpx_ctx* ctx, hfpx_batch* batch, int crack, double* A_hos
 /* ---- tuning -------------------------------------------------------------- */
 /* Kernel-variant knobs of a context (no reference counterpart).
  * HFPX_KNOB_GEMM_F32: FP32 trailing-update GEMM (products with more than 64
- * columns): 0 = automatic (15 when M, N >= 4096 and K >= 512; 13 when
+ * columns): 0 = automatic (15 when M, N >= 4096 and K >= 512; 17 when
  * M >= 256, N >= 128 and K >= 64; else 7 for K <= 384, 2 for longer K); 1 = 16x16x4 FP32 MFMA, 128 x 128 tile;
  * 2..12 = 32x32x2 FP32 MFMA, K chunk 16, with 2 = 128 x 128 tile, no
  * options, 3 = unguarded interior tiles (edge bands in separate launches),
@@ -391,7 +391,10 @@ int hfpx_batch_matrix(hfpx_ctx* ctx, hfpx_batch* batch, int crack, double* A_hos
  * pair, FP32 accumulation: FP32-accurate, not bitwise equal to the FP32
  * MFMA kernels; 14 = 13 with 128 x 256 tiles (waves of 64 x 128, unpadded
  * LDS rows); 15 = 14 with a software-pipelined loop (operand reads and LDS
- * stores between the products; bitwise the result of 14). */
+ * stores between the products; bitwise the result of 14); 16 = 15 with the
+ * split done in the product's load stage (FP32 operands read directly, no
+ * split pass; bitwise the result of 14); 17 = 16 with 128 x 128 tiles
+ * (bitwise the result of 13). */
 #define HFPX_KNOB_GEMM_F32 0
 /* HFPX_KNOB_LU_NB: outer block of the LU (columns per trailing update):
  * 0 = default (1024), or one of 256, 512, 1024, 2048, 4096. */

@@ -42,7 +42,7 @@ def test_gemm_sub(env, dt, M, N, K):
     assert np.abs(got - ref).max() / (np.abs(A).max() * np.abs(B).max() * K) < tol
 
 
-@pytest.mark.parametrize("variant", range(1, 16))
+@pytest.mark.parametrize("variant", range(1, 18))
 @pytest.mark.parametrize("M,N,K", [(300, 257, 33), (513, 700, 100), (1000, 300, 256), (130, 129, 17),
                                    (256, 256, 32), (77, 200, 20), (512, 384, 256)])
 def test_sgemm_variants(env, variant, M, N, K):
@@ -89,6 +89,38 @@ def test_sgemm_pipelined_split_is_bitwise_variant_14(env, M, N, K):
     assert np.array_equal(out[14], out[15])
 
 
+@pytest.mark.parametrize("M,N,K,ld_pad", [(4500, 4200, 1024, 0), (700, 650, 40, 0), (4096, 4096, 512, 0),
+                                          (1000, 777, 203, 1), (300, 129, 64, 3)])
+def test_sgemm_fused_split_is_bitwise_the_split_pass(env, M, N, K, ld_pad):
+    """Variants 16 / 17 split the operands in the product's load stage; the
+    parts and products are those of the split pass, so C is bitwise that of
+    variant 14 (128 x 256 tiles) / 13 (128 x 128), also with leading
+    dimensions that are not a multiple of 4 (B read with 4-byte loads)."""
+    torch, hx = env
+    from hfpx3d_amd import _lib
+    print(f"case: sgemm fused split {M}x{N}x{K} pad {ld_pad}", flush=True)
+    rng = np.random.default_rng(M + 5 * N + K + ld_pad)
+    A, B, C = rng.normal(size=(M, K)), rng.normal(size=(K, N)), rng.normal(size=(M, N))
+    lda, ldb, ldc = M + ld_pad, K + ld_pad, M + ld_pad
+    Ad = to_dev(torch, hx, np.pad(A, ((0, ld_pad), (0, 0))), torch.float32)
+    Bd = to_dev(torch, hx, np.pad(B, ((0, ld_pad), (0, 0))), torch.float32)
+    out = {}
+    old = hx.tune_get(_lib.KNOB_GEMM_F32)
+    try:
+        for v in (13, 14, 16, 17):
+            hx.tune_set(_lib.KNOB_GEMM_F32, v)
+            Cd = to_dev(torch, hx, np.pad(C, ((0, ld_pad), (0, 0))), torch.float32)
+            ctx = hx.get_context()
+            ctx.check(ctx.lib.hfpx_gemm_sub(ctx.handle, _lib.HFPX_F32, hx._ptr(Cd), ldc, hx._ptr(Ad), lda,
+                                             hx._ptr(Bd), ldb, M, N, K))
+            ctx.stream.synchronize()
+            out[v] = Cd.cpu().numpy()[:M]
+    finally:
+        hx.tune_set(_lib.KNOB_GEMM_F32, old)
+    assert np.array_equal(out[16], out[14])
+    assert np.array_equal(out[17], out[13])
+
+
 @pytest.mark.parametrize("scaled", [False, True])
 def test_sgemm_bf16_split_is_fp32_accurate(env, scaled):
     """The three-way bf16 split product (variants 13, 14) against float64 on the
@@ -112,7 +144,7 @@ def test_sgemm_bf16_split_is_fp32_accurate(env, scaled):
     err = {}
     old = hx.tune_get(_lib.KNOB_GEMM_F32)
     try:
-        for v in (2, 13, 14, 15):
+        for v in (2, 13, 14, 15, 16, 17):
             hx.tune_set(_lib.KNOB_GEMM_F32, v)
             Cd = to_dev(torch, hx, C32, torch.float32)
             hx.gemm_sub(Cd, to_dev(torch, hx, A32, torch.float32), to_dev(torch, hx, B32, torch.float32))
@@ -123,6 +155,8 @@ def test_sgemm_bf16_split_is_fp32_accurate(env, scaled):
     assert err[13] < 3 * err[2] + 1e-7, err
     assert err[14] < 3 * err[2] + 1e-7, err
     assert err[15] < 3 * err[2] + 1e-7, err
+    assert err[16] < 3 * err[2] + 1e-7, err
+    assert err[17] < 3 * err[2] + 1e-7, err
 
 
 @pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
