@@ -67,6 +67,14 @@ struct hfpx_batch {
   int32_t* tiles = nullptr;  // tile prefix tables of the batched updates
   __bf16* xsplit = nullptr;  // bf16 parts of the FP32 outer updates (largest launch)
   std::vector<hfpx::BTab> tab32, tab64;  // update launches of each group, in issue order
+  // the mixed group's solves: dependency-flow sweeps over (crack, 128-row
+  // block) tickets (hfpx_sweep.hip)
+  int64_t ntk32 = 0;          // tickets = row blocks of the mixed group
+  int32_t* tk32 = nullptr;    // (table index, level) per ticket
+  int64_t* bofs = nullptr;    // per table index: first block in the block tables
+  float* inv32 = nullptr;     // block inverses: L (ntk32 x 128^2), then U
+  uint64_t* bflags = nullptr; // sweep flags, tickets, error words
+  float* y32 = nullptr;       // permuted right-hand sides being solved
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graph_refine = -1;
@@ -553,6 +561,39 @@ static int batch_lu(hfpx_ctx* ctx, hfpx_batch* bt, T* base, int which, const int
   return HFPX_OK;
 }
 
+// y[x_off + i] = x[x_off + perm[x_off + i]] (inv = false) or back,
+// x[...] = y[...] unpermuted positions (inv = true: a plain copy), for
+// every crack of the list
+__global__ void k_bgather_f(const BCrack* __restrict__ cr, const int32_t* __restrict__ list,
+                            const int32_t* __restrict__ perm, const float* __restrict__ x, float* __restrict__ y) {
+  const BCrack& c = cr[list[blockIdx.y]];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= c.n) return;
+  y[c.x_off + i] = x[c.x_off + perm[c.x_off + i]];
+}
+
+__global__ void k_bcopy_f(const float* __restrict__ a, float* __restrict__ b, int64_t cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < cnt) b[i] = a[i];
+}
+
+// x32 := U^-1 L^-1 P x32 for every crack of the mixed group: the
+// interchanges as one gather, then the persistent dependency-flow sweeps
+// with the block inverses of this factorisation
+static int batch_sweep_vec(hfpx_ctx* ctx, hfpx_batch* bt) {
+  if (bt->n32 <= 0) return HFPX_OK;
+  hipLaunchKernelGGL(k_bgather_f, dim3((unsigned)((bt->nmax32 + 255) / 256), (unsigned)bt->n32), dim3(256), 0,
+                     ctx->stream, bt->cr, bt->list32, bt->perm, bt->x32, bt->y32);
+  HFPX_TRY_HIP(ctx, hipGetLastError());
+  int rc = launch_bsweeps(ctx, bt->A32, bt->cr, bt->tk32, bt->ntk32, bt->bofs, bt->inv32,
+                          bt->inv32 + bt->ntk32 * 128 * 128, bt->y32, bt->bflags);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_bcopy_f, dim3((unsigned)((bt->x32_total + 255) / 256)), dim3(256), 0, ctx->stream, bt->y32,
+                     bt->x32, bt->x32_total);
+  HFPX_TRY_HIP(ctx, hipGetLastError());
+  return HFPX_OK;
+}
+
 template <typename T>
 static int batch_solve_vec(hfpx_ctx* ctx, hfpx_batch* bt, const T* base, int which, const int32_t* list, int cnt,
                            T* vec) {
@@ -588,15 +629,18 @@ static int batch_enqueue(hfpx_ctx* ctx, hfpx_batch* bt, int n_refine) {
   if (bt->n32 > 0)
     hipLaunchKernelGGL(k_bperm, dim3((unsigned)bt->n32), dim3(PT), 0, ctx->stream, bt->cr, bt->list32, bt->ipiv,
                        bt->perm);
+  if ((rc = launch_bblock_inv(ctx, bt->A32, bt->cr, bt->tk32, bt->ntk32, bt->bofs, bt->inv32,
+                              bt->inv32 + bt->ntk32 * 128 * 128)))
+    return rc;
   if ((rc = bvec(ctx, nullptr, bt->x32, bt->b, bt->x32_total, 0))) return rc;
-  if ((rc = batch_solve_vec<float>(ctx, bt, bt->A32, B_A32, bt->list32, bt->n32, bt->x32))) return rc;
+  if ((rc = batch_sweep_vec(ctx, bt))) return rc;
   if ((rc = bvec(ctx, bt->x, bt->x32, nullptr, bt->x32_total, 1))) return rc;
   const unsigned gr32 = (unsigned)((bt->nmax32 + 255) / 256);
   for (int it = 0; it < n_refine && bt->n32 > 0; ++it) {
     hipLaunchKernelGGL(k_bresid, dim3(gr32, (unsigned)bt->n32), dim3(256), 0, ctx->stream, bt->A64, bt->cr,
                        bt->list32, bt->b, bt->x, bt->r, nullptr);
     if ((rc = bvec(ctx, nullptr, bt->x32, bt->r, bt->x32_total, 0))) return rc;
-    if ((rc = batch_solve_vec<float>(ctx, bt, bt->A32, B_A32, bt->list32, bt->n32, bt->x32))) return rc;
+    if ((rc = batch_sweep_vec(ctx, bt))) return rc;
     if ((rc = bvec(ctx, bt->x, bt->x32, nullptr, bt->x32_total, 2))) return rc;
   }
   // backward errors of every crack
@@ -719,6 +763,26 @@ int hfpx_batch_create(hfpx_ctx* ctx, int n_cracks, const int64_t* n_nodes, const
   std::vector<int32_t> tab;
   if (!plan_updates(bt->n32_sorted, tab, bt->tab32) || !plan_updates(bt->n64_sorted, tab, bt->tab64))
     return fail(set_error(ctx, HFPX_ERR_INVALID, "batch: too many tiles in one update"));
+  // the mixed group's sweep tickets: level-major, largest crack first in a
+  // level; block tables by table index
+  std::vector<int32_t> tk;
+  std::vector<int64_t> bofs(n_cracks, 0);
+  {
+    int64_t off = 0, maxb = 0;
+    for (int ti = 0; ti < bt->n32; ++ti) {
+      bofs[ti] = off;
+      const int64_t nb = (bt->host_cr[ti].n + 127) / 128;
+      off += nb;
+      maxb = std::max(maxb, nb);
+    }
+    for (int64_t q = 0; q < maxb; ++q)
+      for (int32_t ti : l32)
+        if ((bt->host_cr[ti].n + 127) / 128 > q) {
+          tk.push_back(ti);
+          tk.push_back((int32_t)q);
+        }
+    bt->ntk32 = off;
+  }
   size_t split_bytes = 0;  // the largest FP32 outer update's bf16 parts
   for (const BTab& t : bt->tab32) split_bytes = std::max(split_bytes, bgemm_x3_split_bytes(t.rtot, t.ctot));
   auto dalloc = [&](void** p, size_t bytes) -> int {
@@ -746,7 +810,12 @@ int hfpx_batch_create(hfpx_ctx* ctx, int n_cracks, const int64_t* n_nodes, const
       (rc = dalloc((void**)&bt->x32, sizeof(float) * std::max<int64_t>(1, bt->x32_total))) ||
       (rc = dalloc((void**)&bt->berr, sizeof(double) * n_cracks)) ||
       (rc = dalloc((void**)&bt->tiles, sizeof(int32_t) * tab.size())) ||
-      (split_bytes && (rc = dalloc((void**)&bt->xsplit, split_bytes))))
+      (split_bytes && (rc = dalloc((void**)&bt->xsplit, split_bytes))) ||
+      (rc = dalloc((void**)&bt->tk32, sizeof(int32_t) * std::max<size_t>(1, tk.size()))) ||
+      (rc = dalloc((void**)&bt->bofs, sizeof(int64_t) * n_cracks)) ||
+      (rc = dalloc((void**)&bt->inv32, sizeof(float) * 2 * (size_t)bt->ntk32 * 128 * 128)) ||
+      (rc = dalloc((void**)&bt->bflags, sizeof(uint64_t) * (2 * (size_t)bt->ntk32 + 3))) ||
+      (rc = dalloc((void**)&bt->y32, sizeof(float) * std::max<int64_t>(1, bt->x32_total))))
     return fail(rc);
   auto up = [&](void* d, const void* h, size_t bytes) -> int {
     if (bytes) HFPX_TRY_HIP(ctx, hipMemcpy(d, h, bytes, hipMemcpyHostToDevice));
@@ -760,7 +829,9 @@ int hfpx_batch_create(hfpx_ctx* ctx, int n_cracks, const int64_t* n_nodes, const
       (rc = up(bt->list_all, lall.data(), sizeof(int32_t) * n_cracks)) ||
       (rc = up(bt->nods, nods3.data(), sizeof(double) * 3 * nn)) ||
       (rc = up(bt->conn, conn_t.data(), sizeof(int32_t) * conn_t.size())) ||
-      (rc = up(bt->tiles, tab.data(), sizeof(int32_t) * tab.size())))
+      (rc = up(bt->tiles, tab.data(), sizeof(int32_t) * tab.size())) ||
+      (rc = up(bt->tk32, tk.data(), sizeof(int32_t) * tk.size())) ||
+      (rc = up(bt->bofs, bofs.data(), sizeof(int64_t) * n_cracks)))
     return fail(rc);
   // element geometry of all cracks at once (the element table is in crack order)
   hfpx_mesh m;
@@ -781,10 +852,21 @@ int hfpx_batch_destroy(hfpx_batch* bt) {
   if (bt->exec) (void)hipGraphExecDestroy(bt->exec);
   if (bt->graph) (void)hipGraphDestroy(bt->graph);
   void* ps[] = {bt->cr, bt->list32, bt->list64, bt->list_all, bt->nods, bt->conn, bt->elem, bt->cp, bt->A64, bt->A32, bt->F64,
-                bt->ipiv, bt->perm, bt->info, bt->b, bt->x, bt->r, bt->rs, bt->x32, bt->berr, bt->tiles, bt->xsplit};
+                bt->ipiv, bt->perm, bt->info, bt->b, bt->x, bt->r, bt->rs, bt->x32, bt->berr, bt->tiles, bt->xsplit,
+                bt->tk32, bt->bofs, bt->inv32, bt->bflags, bt->y32};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   delete bt;
+  return HFPX_OK;
+}
+
+// after a solve: a bounded wait of the last sweeps that gave up (the error
+// words follow the flags and tickets)
+static int batch_sweep_status(hfpx_ctx* ctx, hfpx_batch* bt) {
+  if (bt->ntk32 <= 0) return HFPX_OK;
+  int32_t e[2] = {0, 0};
+  HFPX_TRY_HIP(ctx, hipMemcpy(e, bt->bflags + 2 * bt->ntk32 + 2, sizeof(e), hipMemcpyDeviceToHost));
+  if (e[0] || e[1]) return set_error(ctx, HFPX_ERR_HIP, "batch: a sweep dependency wait timed out");
   return HFPX_OK;
 }
 
@@ -795,7 +877,7 @@ int hfpx_batch_solve(hfpx_ctx* ctx, hfpx_batch* bt, int n_refine, int use_graph)
   if (!use_graph) {
     if ((rc = batch_enqueue(ctx, bt, n_refine))) return rc;
     HFPX_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return HFPX_OK;
+    return batch_sweep_status(ctx, bt);
   }
   if (!bt->exec || bt->graph_refine != n_refine) {
     if (bt->exec) (void)hipGraphExecDestroy(bt->exec);
@@ -821,7 +903,7 @@ int hfpx_batch_solve(hfpx_ctx* ctx, hfpx_batch* bt, int n_refine, int use_graph)
   }
   HFPX_TRY_HIP(ctx, hipGraphLaunch(bt->exec, ctx->stream));
   HFPX_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  return HFPX_OK;
+  return batch_sweep_status(ctx, bt);
 }
 
 int hfpx_batch_assemble(hfpx_ctx* ctx, hfpx_batch* bt) {

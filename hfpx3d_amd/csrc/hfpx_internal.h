@@ -228,6 +228,14 @@ int gemm_sub_any(hfpx_ctx* ctx, int dtype, void* C, int64_t ldc, const void* A, 
 
 // FP32 triangular sweeps with block inverses (hfpx_sweep.hip)
 size_t sweep_f32_work_bytes(int64_t n, int64_t nrhs);
+// batched FP32 sweeps, one right-hand side per crack: tk holds ntk tickets
+// (crack table index, level), level-major and by size within a level; bofs
+// (per table index) the offset of a crack's blocks in the block tables
+// (inverses of 128 x 128 blocks, flags); flags holds 2 ntk + 3 words
+int launch_bblock_inv(hfpx_ctx* ctx, const float* A32, const BCrack* cr, const int32_t* tk, int64_t ntk,
+                      const int64_t* bofs, float* invL, float* invU);
+int launch_bsweeps(hfpx_ctx* ctx, const float* A32, const BCrack* cr, const int32_t* tk, int64_t ntk,
+                   const int64_t* bofs, const float* invL, const float* invU, float* vec, uint64_t* flags);
 int sweep_f32(hfpx_ctx* ctx, const float* LU, int64_t lda, int64_t n, float* B, int64_t ldb, int64_t nrhs,
               void* work);
 

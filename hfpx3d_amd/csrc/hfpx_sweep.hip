@@ -26,16 +26,17 @@ constexpr int SB = 128;  // rows per block
 // (eight waves) above, where every column block reads the factor once more
 constexpr int SWN_MAX = 64, SWN_MIN = 32;
 
-// Inverse of diagonal block blockIdx.x (rows/columns [128 b, 128 b + kb)),
-// padded with the identity to 128 x 128, column-major into inv + b SB^2.
-// Thread t forms column t by column-oriented substitution in LDS.
+// Inverse of diagonal block b (rows/columns [128 b, 128 b + kb)) of the
+// n x n factor LU (lda), padded with the identity to 128 x 128, column-major
+// into out (SB^2 floats).  128 threads: thread t forms column t by
+// column-oriented substitution in LDS.
 template <bool LOWER>
-__global__ __launch_bounds__(SB) void k_block_inv(const float* __restrict__ LU, int64_t lda, int64_t n,
-                                                  float* __restrict__ inv) {
+__device__ __forceinline__ void block_inv_tile(const float* __restrict__ LU, int64_t lda, int64_t n, int64_t b,
+                                               float* __restrict__ out) {
   constexpr int P = SB + 1;
   __shared__ float sT[SB * P];  // sT[i + j P] = T(i, j)
   __shared__ float sX[SB * P];  // sX[i + t P] = inverse(i, t)
-  const int64_t b = blockIdx.x, k = b * SB;
+  const int64_t k = b * SB;
   const int kb = (int)std::min<int64_t>(SB, n - k);
   const int t = threadIdx.x;
   for (int e = t; e < SB * SB; e += SB) {
@@ -65,8 +66,14 @@ __global__ __launch_bounds__(SB) void k_block_inv(const float* __restrict__ LU, 
     }
   }
   __syncthreads();
-  float* out = inv + b * SB * SB;
   for (int e = t; e < SB * SB; e += SB) out[e] = sX[(e % SB) + (e / SB) * P];
+}
+
+// Inverse of diagonal block blockIdx.x into inv + blockIdx.x SB^2.
+template <bool LOWER>
+__global__ __launch_bounds__(SB) void k_block_inv(const float* __restrict__ LU, int64_t lda, int64_t n,
+                                                  float* __restrict__ inv) {
+  block_inv_tile<LOWER>(LU, lda, n, blockIdx.x, inv + (int64_t)blockIdx.x * SB * SB);
 }
 
 // One sweep (see the file comment).  LOWER: forward with unit L, blocks in
@@ -249,6 +256,160 @@ static int launch_sweeps(hfpx_ctx* ctx, const float* LU, int64_t lda, int64_t n,
                      ldb, nrhs, gy, flags, (uint64_t)1, tickets, errs);
   hipLaunchKernelGGL((k_sweep_flow<false, SWN>), dim3(nwg), dim3(8 * SWN), 0, ctx->stream, LU, lda, n, nblk, invU, B,
                      ldb, nrhs, gy, flags + nblk * gy, (uint64_t)1, tickets + 1, errs + 1);
+  HFPX_TRY_HIP(ctx, hipGetLastError());
+  return HFPX_OK;
+}
+
+// ---------------------------------------------------------------------
+// Batched form: one right-hand side per crack, many cracks of uneven size.
+// One launch per sweep for all the cracks of a list, persistent: workgroups
+// take tickets from an atomic counter in a table ordered by level (the
+// position of a row block in sweep order) and, within a level, by crack
+// size, largest first -- a ticket only waits on tickets of lower levels of
+// its own crack, so any number of resident workgroups makes progress, the
+// first tickets start every crack's chain at once and the largest cracks'
+// long chains start first.  A ticket (crack c, level q) accumulates
+// T(i, p) X_p over the crack's earlier blocks p as each one's flag appears
+// (256 threads: row r = t % 128, half t / 128 of the 128 columns), then
+// X_i = inverse(i) (b_i - sum) in place and raises block i's flag.
+// ---------------------------------------------------------------------
+// block inverses of every (crack, block) of the ticket table (the level of
+// a ticket is its block in forward order)
+template <bool LOWER>
+__global__ __launch_bounds__(SB) void k_bblock_inv(const float* __restrict__ A32, const BCrack* __restrict__ cr,
+                                                   const int32_t* __restrict__ tk, const int64_t* __restrict__ bofs,
+                                                   float* __restrict__ inv) {
+  const int c = tk[2 * blockIdx.x], b = tk[2 * blockIdx.x + 1];
+  const BCrack& C = cr[c];
+  block_inv_tile<LOWER>(A32 + C.a32_off, C.n, C.n, b, inv + (bofs[c] + b) * SB * SB);
+}
+
+template <bool LOWER>
+__global__ __launch_bounds__(2 * SB) __attribute__((amdgpu_waves_per_eu(4))) void k_bsweep(
+    const float* __restrict__ A32, const BCrack* __restrict__ cr,
+                                                   const int32_t* __restrict__ tk, int64_t ntk,
+                                                   const int64_t* __restrict__ bofs, const float* __restrict__ inv,
+                                                   float* __restrict__ vec, uint64_t* __restrict__ flags,
+                                                   uint64_t* __restrict__ ticket, int32_t* __restrict__ err) {
+  constexpr int KH = SB / 2;  // columns per half
+  __shared__ float sX[SB];
+  __shared__ float sP[2][SB];
+  __shared__ uint64_t s_tk;
+  const int tid = threadIdx.x, r = tid & (SB - 1), kh = tid >> 7;
+  auto uniform = [](uint64_t u) -> int64_t {
+    return (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32)) << 32) |
+                     (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u));
+  };
+  // the ticket is taken at the end of the loop body (see k_sweep_flow)
+  auto next_ticket = [&]() -> int64_t {
+    if (tid == 0) s_tk = __hip_atomic_fetch_add(ticket, (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const int64_t t = uniform(s_tk);
+    __syncthreads();
+    return t;
+  };
+  for (int64_t tkn = next_ticket(); tkn < ntk; tkn = next_ticket()) {
+    const int c = tk[2 * tkn], q = tk[2 * tkn + 1];
+    const BCrack& C = cr[c];
+    const int64_t n = C.n, nb = (n + SB - 1) / SB;
+    const int64_t blk = LOWER ? q : nb - 1 - q;
+    const int64_t r0 = blk * SB;
+    const int rows = (int)std::min<int64_t>(SB, n - r0);
+    const float* LU = A32 + C.a32_off;
+    float* x = vec + C.x_off;
+    uint64_t* fl = flags + bofs[c];
+    const bool rok = r < rows;
+    float acc = 0.f;
+    int64_t ready = 0;
+    for (int64_t p = 0; p < q; ++p) {
+      const int64_t cb = LOWER ? p : nb - 1 - p;
+      const int64_t k0 = cb * SB;
+      const int kc = (int)std::min<int64_t>(SB, n - k0);
+      // the factor's tile does not depend on the flag: loaded before the
+      // wait (the column pointer steps, so no table of the column offsets
+      // is held in registers)
+      const float* lp = LU + (r0 + r) + (k0 + KH * kh) * n;
+      float lv[KH];
+#pragma unroll
+      for (int k = 0; k < KH; ++k) {
+        lv[k] = (rok && KH * kh + k < kc) ? *lp : 0.f;
+        lp += n;
+      }
+      if (p >= ready) {
+        if (tid == 0) {
+          uint32_t polls = 0;
+          while (__hip_atomic_load(fl + cb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 1) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++polls > (1u << 20) ||
+                ((polls & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+              __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              break;
+            }
+          }
+          int64_t rr = p + 1;
+          while (rr < q && __hip_atomic_load(fl + (LOWER ? rr : nb - 1 - rr), __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT) == 1)
+            ++rr;
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+          s_tk = (uint64_t)rr;
+        }
+        __syncthreads();
+        ready = uniform(s_tk);
+      }
+      if (tid < SB) sX[tid] = tid < kc ? x[k0 + tid] : 0.f;
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < KH; ++k) acc += lv[k] * sX[KH * kh + k];
+      __syncthreads();  // sX is rewritten by the next predecessor
+    }
+    sP[kh][r] = acc;
+    __syncthreads();
+    if (tid < SB) sX[tid] = tid < rows ? x[r0 + tid] - (sP[0][tid] + sP[1][tid]) : 0.f;
+    __syncthreads();
+    // X_i = inverse(i) (b_i - sum): inverse(r, k) at iv[k SB]
+    const float* iv = inv + (bofs[c] + blk) * SB * SB + r + KH * kh * SB;
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < KH; ++k) {
+      s += *iv * sX[KH * kh + k];
+      iv += SB;
+    }
+    sP[kh][r] = s;
+    __syncthreads();
+    if (tid < rows) x[r0 + tid] = sP[0][tid] + sP[1][tid];
+    __syncthreads();  // every store of X_i before the flag
+    if (tid == 0) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __hip_atomic_store(fl + blk, (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+int launch_bblock_inv(hfpx_ctx* ctx, const float* A32, const BCrack* cr, const int32_t* tk, int64_t ntk,
+                      const int64_t* bofs, float* invL, float* invU) {
+  if (ntk <= 0) return HFPX_OK;
+  hipLaunchKernelGGL(k_bblock_inv<true>, dim3((unsigned)ntk), dim3(SB), 0, ctx->stream, A32, cr, tk, bofs, invL);
+  hipLaunchKernelGGL(k_bblock_inv<false>, dim3((unsigned)ntk), dim3(SB), 0, ctx->stream, A32, cr, tk, bofs, invU);
+  HFPX_TRY_HIP(ctx, hipGetLastError());
+  return HFPX_OK;
+}
+
+// flags: 2 ntk, then 2 tickets (uint64), then 2 error flags (int32); cleared
+// here by a kernel (graph-safe)
+int launch_bsweeps(hfpx_ctx* ctx, const float* A32, const BCrack* cr, const int32_t* tk, int64_t ntk,
+                   const int64_t* bofs, const float* invL, const float* invU, float* vec, uint64_t* flags) {
+  if (ntk <= 0) return HFPX_OK;
+  uint64_t* tickets = flags + 2 * ntk;
+  int32_t* errs = (int32_t*)(tickets + 2);
+  const int64_t nclear = 2 * ntk + 3;
+  hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nclear + 255) / 256)), dim3(256), 0, ctx->stream, flags, nclear);
+  int occ = 0;
+  HFPX_TRY_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_bsweep<true>, 2 * SB, 0));
+  const unsigned nwg = (unsigned)std::min<int64_t>(ntk, std::max(1, occ) * std::max(1, ctx->num_cus));
+  hipLaunchKernelGGL(k_bsweep<true>, dim3(nwg), dim3(2 * SB), 0, ctx->stream, A32, cr, tk, ntk, bofs, invL, vec,
+                     flags, tickets, errs);
+  hipLaunchKernelGGL(k_bsweep<false>, dim3(nwg), dim3(2 * SB), 0, ctx->stream, A32, cr, tk, ntk, bofs, invU, vec,
+                     flags + ntk, tickets + 1, errs + 1);
   HFPX_TRY_HIP(ctx, hipGetLastError());
   return HFPX_OK;
 }
