@@ -39,11 +39,14 @@ VALU_FLOPS_PER_PAIR = {"assemble72k": 6992.5, "stress1m": 7140.2}
 # splits: their FP32-equivalent ceiling is the dense bf16 peak / 6.
 X3_PRODUCTS = 6
 X3_PEAK = PEAKS["bf16_mfma_TFs"] / X3_PRODUCTS
-# bytes per call of the representative trailing update, from rocprofv3 --pmc (profiles/gemm72k_pmc.md)
 # bytes per call of the first trailing update (bench.py --workload gemm72k),
 # from the FETCH_SIZE / WRITE_SIZE passes summarised in profiles/
 GEMM72K_TRAFFIC = 1.378e+11
 GEMM72K_ALG_BYTES = 4.0882e+10
+# PMC bytes of one truncation gather of the default volume-control workload
+# (72,001^2 resident, 58,285^2 truncated; k_vc_gather1): FETCH_SIZE x 1.99995
+# + WRITE_SIZE x 1.0, the factors from a 2 GiB device copy in the same runs
+VC72K_TRAFFIC = 7.249e+10
 
 
 def parse():
@@ -58,14 +61,16 @@ def parse():
     ap.add_argument("--cpu-sample-el", type=int, default=1400)
     ap.add_argument("--gemm-dtype", default="f32", choices=["f32", "f64"], help="gemm72k: FP32 or FP64 product")
     ap.add_argument("--workload", default="ellipse72k",
-                    choices=["ellipse72k", "wavy108k", "batch256", "gemm72k", "vc72k", "cf72k", "assemble72k", "stress1m"],
+                    choices=["ellipse72k", "wavy108k", "batch256", "gemm72k", "vc72k", "vctrunc72k", "cf72k", "assemble72k",
+                             "stress1m"],
                     help="ellipse72k: the headline configuration (default); wavy108k: the non-planar "
                          "6,000-element crack with stresses at 10^6 points; batch256: 256 independent cracks; "
                          "vc72k: volume-control steps (truncation + FP32 LU + FP64 refinement) on the 72k ellipse; "
+                         "vctrunc72k: the truncation (mod_3dbem_system_vc) alone, same system; "
                          "cf72k: steps of the cohesion/friction volume-control iteration on the 72k ellipse")
     ap.add_argument("--vc-active", type=float, default=0.9,
                     help="vc72k: DoF of elements whose centroid lies inside this fraction of the ellipse stay active")
-    ap.add_argument("--mixed-min-n", type=int, default=2048,
+    ap.add_argument("--mixed-min-n", type=int, default=1024,
                     help="batch256: cracks with at least this many DoF use FP32 LU + FP64 refinement")
     ap.add_argument("--n-pts", type=int, default=1_000_000)
     ap.add_argument("--fp64-compare", action="store_true", help="wavy108k: also run the plain FP64 path")
@@ -74,7 +79,9 @@ def parse():
     ap.add_argument("--lu-nb", type=int, default=0, help="outer block of the LU (0: library default)")
     ap.add_argument("--gemm-variant", type=int, default=0, help="FP32 GEMM variant (0: library default)")
     ap.add_argument("--autotune", default="", help="apply the knob settings of an autotune table "
-                                                   "(python -m hfpx3d_amd.autotune) for the LU's n")
+                                                   "(python -m hfpx3d_amd.autotune) for the LU's n; "
+                                                   "batch256: the table's batch row (update kernel and "
+                                                   "FP32 threshold, overriding --mixed-min-n)")
     ap.add_argument("--calib-copy-gib", type=float, default=0.0,
                     help="before the workload, copy this many GiB device-to-device (a transfer of known size "
                          "for scaling hardware byte counters)")
@@ -181,7 +188,7 @@ def main():
         return run_batch(args)
     if args.workload == "gemm72k":
         return run_gemm(args)
-    if args.workload == "vc72k":
+    if args.workload in ("vc72k", "vctrunc72k"):
         return run_vc(args)
     if args.workload == "cf72k":
         return run_cf(args)
@@ -205,15 +212,7 @@ def main():
         hx.tune_set(_lib.KNOB_LU_NB, args.lu_nb, ctx)
     if args.gemm_variant:
         hx.tune_set(_lib.KNOB_GEMM_F32, args.gemm_variant, ctx)
-    if args.calib_copy_gib > 0:
-        cnt = int(args.calib_copy_gib * 2 ** 30) // 4
-        with torch.cuda.stream(ctx.stream):
-            src = torch.ones(cnt, dtype=torch.float32, device=ctx.device)
-            dst = torch.empty_like(src)
-            dst.copy_(src)
-        ctx.stream.synchronize()
-        log(f"bench: calibration copy of {cnt * 4} bytes done")
-        del src, dst
+    calib_copy(args, ctx)
     log(f"bench: n_el={n_el} n={n} nrhs={args.nrhs}")
 
     mesh = hx.Mesh_Geom_T(nods=nods, conn=conn)
@@ -413,15 +412,7 @@ def run_gemm(args):
     f64 = args.gemm_dtype == "f64"
     if args.gemm_variant:
         hx.tune_set(_lib.KNOB_GEMM_F64 if f64 else _lib.KNOB_GEMM_F32, args.gemm_variant, ctx)
-    if args.calib_copy_gib > 0:
-        cnt = int(args.calib_copy_gib * 2 ** 30) // 4
-        with torch.cuda.stream(ctx.stream):
-            src = torch.ones(cnt, dtype=torch.float32, device=ctx.device)
-            dst = torch.empty_like(src)
-            dst.copy_(src)
-        ctx.stream.synchronize()
-        log(f"bench: calibration copy of {cnt * 4} bytes done")
-        del src, dst
+    calib_copy(args, ctx)
     dt = torch.float64 if f64 else torch.float32
     with torch.cuda.stream(ctx.stream):
         C = torch.randn(M * N, dtype=dt, device=ctx.device)
@@ -564,10 +555,18 @@ def run_batch(args):
                            s_inf=np.array([p, p, p, 0.0, 0.0, 0.0])))
     ns = np.array([18 * c["conn"].shape[1] for c in cracks], dtype=np.float64)
     entries, flops = float((ns ** 2).sum()), float((2.0 / 3.0 * ns ** 3).sum())
+    ctx = hx.get_context(0)
+    if args.autotune:
+        from hfpx3d_amd import autotune
+        m = autotune.apply_batch(autotune.load(args.autotune), ctx)
+        if m is None:
+            sys.exit(f"bench: {args.autotune} has no batch row (python -m hfpx3d_amd.autotune --batch 256)")
+        args.mixed_min_n = m
+        log(f"bench: autotune batch row: BGEMM_F32 {hx.tune_get(autotune.BATCH_KNOBS['BGEMM_F32'], ctx)}, "
+            f"mixed_min_n {m}")
     n_mixed = int((ns >= args.mixed_min_n).sum())
     log(f"bench: batch of {len(cracks)} cracks, n {int(ns.min())}..{int(ns.max())}, {entries:.3e} entries, "
         f"{n_mixed} mixed")
-    ctx = hx.get_context(0)
     bt = hx.Batch(cracks, mixed_min_n=args.mixed_min_n, ctx=ctx)
     for w in range(max(1, args.warmup)):
         bt.solve(n_refine=3, use_graph=True)  # the first call also captures the graph
@@ -660,13 +659,32 @@ def vc_handles(nods, conn, R, active):
     return orig, hx.DoF_Handle_T(n_dof=18 * k, dof_h=dh)
 
 
+def calib_copy(args, ctx):
+    """--calib-copy-gib: one device-to-device copy of that size on the
+    library stream, the known-size kernel that scales the PMC byte
+    counters of the same run."""
+    if args.calib_copy_gib <= 0:
+        return
+    import torch
+    cnt = int(args.calib_copy_gib * 2 ** 30) // 4
+    with torch.cuda.stream(ctx.stream):
+        src = torch.ones(cnt, dtype=torch.float32, device=ctx.device)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+    ctx.stream.synchronize()
+    log(f"bench: calibration copy of {cnt * 4} bytes done")
+    del src, dst
+
+
 def run_vc(args):
     """Volume-control steps on the 72k ellipse (4,000 elements, E = 1,
     nu = 0.3): the (72,001)^2 volume-control matrix is assembled once and
     stays resident; one step gathers the truncated system of the active DoF
     (FP64 + FP32 in one pass), factorises it in FP32 and solves for a volume
     increment with FP64 GMRES-IR.  The gather is HBM-bound: per entry of the
-    truncated system 8 bytes read, 8 + 4 bytes written."""
+    truncated system 8 bytes read, 8 + 4 bytes written.  vctrunc72k: one step
+    is the truncation call alone (hfpx_vc_truncate, the reference's
+    mod_3dbem_system_vc) on the same system."""
     import ctypes
     import numpy as np
     import torch
@@ -681,6 +699,7 @@ def run_vc(args):
     ctx = hx.get_context(0)
     lib, ptr = ctx.lib, hx._ptr
     log(f"bench: vc n_el={n_el} orig n={n0} active n={n1}")
+    calib_copy(args, ctx)
     mesh = hx.Mesh_Geom_T(nods=nods, conn=conn)
     dm = hx.DeviceMesh(mesh, None, 0.125, ctx)
     A, lda = hx.alloc_colmajor(n0 + 1, n0 + 1, torch.float64, ctx)
@@ -705,12 +724,19 @@ def run_vc(args):
     t_asm = e0.elapsed_time(e1) / 1e3
     log(f"bench: vc matrix assembled in {t_asm * 1e3:.1f} ms")
 
+    solve = args.workload == "vc72k"
+
     def step(rec=None):
         e = [ev() for _ in range(3)]
         e[0].record(ctx.stream)
         ctx.check(lib.hfpx_vc_truncate(ctx.handle, ptr(A), lda, n0, hx._np_ptr(od), hx._np_ptr(ud), n_el, 18, n1,
                                        hx._np_ptr(dt), n1, 1e-3, ptr(M), ldm, ptr(M32), ldm32, ptr(rhs)))
         e[1].record(ctx.stream)
+        if not solve:
+            e[2].record(ctx.stream)
+            if rec is not None:
+                rec.append((e, 0, 0, None, None))
+            return
         ctx.check(lib.hfpx_getrf(ctx.handle, _lib.HFPX_F32, ptr(M32), ldm32, nu1, ptr(ipiv), ctypes.byref(info)))
         rc = lib.hfpx_solve_ir(ctx.handle, ptr(M), ldm, ptr(M32), ldm32, ptr(ipiv), nu1, ptr(rhs), nu1, ptr(X), nu1,
                                1, args.tol, 10, 20, ctypes.byref(rep))
@@ -739,18 +765,25 @@ def run_vc(args):
     ctx.check(lib.hfpx_profile_read(ctx.handle, _lib.KERNEL_CLASSES["vc"], ctypes.byref(launches), ctypes.byref(ms),
                                     ctypes.byref(_), ctypes.byref(nbytes)))
     ctx.check(lib.hfpx_profile(ctx.handle, 0))
-    t_gather = ms.value / max(1, launches.value) / 1e3
-    gbps = nbytes.value / max(1, launches.value) / t_gather / 1e9
     t_trunc = np.mean([r[0][0].elapsed_time(r[0][1]) for r in recs]) / 1e3
+    if launches.value > 0:
+        t_gather = ms.value / launches.value / 1e3
+        gbps = nbytes.value / launches.value / t_gather / 1e9
+    else:  # --no-lib-prof: the whole truncation call, algorithmic bytes
+        t_gather = t_trunc
+        gbps = 20.0 * (n1 + 1) ** 2 / t_gather / 1e9
     t_solve = np.mean([r[0][1].elapsed_time(r[0][2]) for r in recs]) / 1e3
-    with torch.cuda.stream(ctx.stream):
-        vol = torch.dot(M[n1, :n1], X[:n1]).item()
+    vol = None
+    if solve:
+        with torch.cuda.stream(ctx.stream):
+            vol = torch.dot(M[n1, :n1], X[:n1]).item()
     last = recs[-1]
     log(f"bench: gather {t_gather * 1e3:.2f} ms ({gbps:.0f} GB/s), truncate call {t_trunc * 1e3:.1f} ms, "
-        f"solve {t_solve:.3f} s, volume {vol:.6g}")
+        f"solve {t_solve:.3f} s, volume {vol}")
     entries = float(nu1) * nu1
     result = {
-        "metric": "vc 72k: truncated-system gather Gentries/s; step s",
+        "metric": ("vc 72k: truncated-system gather Gentries/s; step s" if solve else
+                   "vc 72k truncation (mod_3dbem_system_vc): gather Gentries/s"),
         "value": round(entries / t_gather / 1e9, 3), "unit": "Gentries/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -760,11 +793,14 @@ def run_vc(args):
                                f"(elements inside {args.vc_active} of the ellipse active), dV = 1e-3",
                    "n_dof": n0, "n_active": n1},
         "vc_assembly_s": round(t_asm, 5), "truncate_call_s": round(t_trunc, 5), "gather_kernel_s": round(t_gather, 5),
-        "solve_s": round(t_solve, 4), "step_s": round(t_trunc + t_solve, 4),
-        "ir": {"outer": last[1], "inner": last[2], "backward_error": last[3], "converged": last[4]},
-        "volume_check": {"v_dot_d": vol, "prescribed": 1e-3},
+        "solve_s": round(t_solve, 4) if solve else None, "step_s": round(t_trunc + t_solve, 4),
+        "ir": {"outer": last[1], "inner": last[2], "backward_error": last[3], "converged": last[4]} if solve else None,
+        "volume_check": {"v_dot_d": vol, "prescribed": 1e-3} if solve else None,
         "roofline": {"bound": "hbm", "achieved": round(gbps, 1), "peak": 8000.0, "unit": "GB/s",
-                     "frac": round(gbps / 8000.0, 4), "traffic": None,
+                     "frac": round(gbps / 8000.0, 4),
+                     "traffic": VC72K_TRAFFIC if (args.n_el, args.vc_active) == (4000, 0.9) else None,
+                     "traffic_for": "one gather of the default workload (--n-el 4000 --vc-active 0.9): "
+                                    "FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, scales from a 2 GiB device copy",
                      "kernel": "k_vc_gather (FP64 + FP32 outputs)",
                      "achieved_basis": "algorithmic 20 bytes per truncated-system entry (8 read, 8 + 4 written) "
                                        "/ gather duration from HIP events on the library stream"},

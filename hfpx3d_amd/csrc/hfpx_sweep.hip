@@ -28,45 +28,87 @@ constexpr int SWN_MAX = 64, SWN_MIN = 32;
 
 // Inverse of diagonal block b (rows/columns [128 b, 128 b + kb)) of the
 // n x n factor LU (lda), padded with the identity to 128 x 128, column-major
-// into out (SB^2 floats).  128 threads: thread t forms column t by
-// column-oriented substitution in LDS.
+// into out (SB^2 floats).  128 threads: thread t forms column t of the
+// inverse by substitution in 32-row chunks.  A chunk's partial sums stay in
+// registers (fully unrolled over the chunk, factor entries read from LDS as
+// broadcasts); the finished chunks of every column sit in sX for the later
+// chunks and the final coalesced store.  The terms are added in the order
+// of plain column-oriented substitution (unit lower: columns of L ascending;
+// upper: columns of U descending, each entry divided by its diagonal when it
+// is reached; entries below the diagonal of column t of an upper inverse are
+// never divided), so the result is that of the one-column-at-a-time form.
 template <bool LOWER>
 __device__ __forceinline__ void block_inv_tile(const float* __restrict__ LU, int64_t lda, int64_t n, int64_t b,
                                                float* __restrict__ out) {
-  constexpr int P = SB + 1;
-  __shared__ float sT[SB * P];  // sT[i + j P] = T(i, j)
-  __shared__ float sX[SB * P];  // sX[i + t P] = inverse(i, t)
+  constexpr int BC = 32;       // rows per chunk
+  constexpr int NC = SB / BC;  // chunks
+  constexpr int PT = SB + 4;   // sT pitch: 16-byte aligned column segments
+  constexpr int PX = SB + 1;   // sX pitch
+  __shared__ __align__(16) float sT[SB * PT];  // sT[i + j PT] = T(i, j)
+  __shared__ float sX[SB * PX];                // sX[i PX + c] = inverse(i, c)
   const int64_t k = b * SB;
   const int kb = (int)std::min<int64_t>(SB, n - k);
   const int t = threadIdx.x;
-  for (int e = t; e < SB * SB; e += SB) {
-    const int i = e % SB, j = e / SB;
-    float v;
-    if (i < kb && j < kb)
-      v = LU[(k + i) + (k + j) * lda];
-    else
-      v = (i == j) ? 1.f : 0.f;
-    sT[i + j * P] = v;
-    sX[i + j * P] = (i == j) ? 1.f : 0.f;
+  // stage T: thread t holds row t of every column; 16 loads in flight
+  for (int j0 = 0; j0 < SB; j0 += 16) {
+    float v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int j = j0 + u;
+      v[u] = (t < kb && j < kb) ? LU[(k + t) + (k + j) * lda] : (t == j ? 1.f : 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) sT[t + (j0 + u) * PT] = v[u];
   }
   __syncthreads();
-  float* x = sX + t * P;
-  if (LOWER) {
-    // unit lower: column t is zero above row t
-    for (int j = t; j < SB; ++j) {
-      const float xj = x[j];
-      for (int i = j + 1; i < SB; ++i) x[i] -= sT[i + j * P] * xj;
+  for (int q = 0; q < NC; ++q) {
+    const int I = LOWER ? q : NC - 1 - q;  // chunk in substitution order
+    float acc[BC];
+#pragma unroll
+    for (int i = 0; i < BC; ++i) acc[i] = (I * BC + i == t) ? 1.f : 0.f;
+    // finished chunks: acc -= T(chunk I, chunk K) x(chunk K)
+    for (int r = 0; r < q; ++r) {
+      const int K = LOWER ? r : NC - 1 - r;
+#pragma unroll
+      for (int u = 0; u < BC; ++u) {
+        const int kk = LOWER ? u : BC - 1 - u;
+        const float xk = sX[(K * BC + kk) * PX + t];
+        const float* col = sT + I * BC + (K * BC + kk) * PT;
+#pragma unroll
+        for (int i = 0; i < BC; i += 4) {
+          const float4 tv = *reinterpret_cast<const float4*>(col + i);
+          acc[i] = fmaf(-tv.x, xk, acc[i]);
+          acc[i + 1] = fmaf(-tv.y, xk, acc[i + 1]);
+          acc[i + 2] = fmaf(-tv.z, xk, acc[i + 2]);
+          acc[i + 3] = fmaf(-tv.w, xk, acc[i + 3]);
+        }
+      }
     }
-  } else {
-    // upper: column t is zero below row t
-    for (int j = t; j >= 0; --j) {
-      const float xj = x[j] / sT[j + j * P];
-      x[j] = xj;
-      for (int i = 0; i < j; ++i) x[i] -= sT[i + j * P] * xj;
+    // the chunk's own triangle
+    if (LOWER) {
+#pragma unroll
+      for (int kk = 0; kk < BC; ++kk) {
+        const float xk = acc[kk];
+        const float* col = sT + I * BC + (I * BC + kk) * PT;
+#pragma unroll
+        for (int i = kk + 1; i < BC; ++i) acc[i] = fmaf(-col[i], xk, acc[i]);
+      }
+    } else {
+#pragma unroll
+      for (int kk = BC - 1; kk >= 0; --kk) {
+        const int j = I * BC + kk;
+        const float* col = sT + I * BC + j * PT;
+        const float xk = (j > t) ? 0.f : acc[kk] / col[kk];
+        acc[kk] = xk;
+#pragma unroll
+        for (int i = 0; i < kk; ++i) acc[i] = fmaf(-col[i], xk, acc[i]);
+      }
     }
+#pragma unroll
+    for (int i = 0; i < BC; ++i) sX[(I * BC + i) * PX + t] = acc[i];
   }
   __syncthreads();
-  for (int e = t; e < SB * SB; e += SB) out[e] = sX[(e % SB) + (e / SB) * P];
+  for (int e = t; e < SB * SB; e += SB) out[e] = sX[(e % SB) * PX + e / SB];
 }
 
 // Inverse of diagonal block blockIdx.x into inv + blockIdx.x SB^2.

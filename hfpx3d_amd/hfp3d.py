@@ -759,7 +759,7 @@ class Batch:
     Cracks with n >= mixed_min_n are solved by FP32 LU + FP64 refinement,
     the others by FP64 LU."""
 
-    def __init__(self, cracks, mixed_min_n=2048, beta=0.125, ctx=None):
+    def __init__(self, cracks, mixed_min_n=1024, beta=0.125, ctx=None):
         self.ctx = ctx or get_context()
         self.lib = self.ctx.lib
         self.n_cracks = len(cracks)

@@ -74,3 +74,74 @@ def test_sweep_and_apply():
         ctx.autotune = None
         for k, v in before.items():
             hx.tune_set(autotune.KNOBS[k], v, ctx)
+
+
+class _FakeCtx:
+    """Records hfpx_tune_set calls; check() passes return codes through."""
+
+    def __init__(self):
+        self.set = {}
+        self.handle = None
+        ctx = self
+
+        class Lib:
+            @staticmethod
+            def hfpx_tune_set(handle, knob, value):
+                ctx.set[knob] = value
+                return 0
+
+        self.lib = Lib()
+
+    def check(self, rc):
+        assert rc == 0
+
+
+def test_batch_row_apply_and_load(tmp_path):
+    from hfpx3d_amd import _lib
+    table = {"entries": [], "batch": {"knobs": {"BGEMM_F32": 2}, "mixed_min_n": 4096, "ms": 1.0, "tried": []}}
+    ctx = _FakeCtx()
+    assert autotune.apply_batch(table, ctx) == 4096
+    assert ctx.set == {_lib.KNOB_BGEMM_F32: 2}
+    ctx2 = _FakeCtx()
+    assert autotune.apply_batch({"entries": []}, ctx2) is None and ctx2.set == {}
+    p = str(tmp_path / "b.json")
+    autotune.save({"batch": table["batch"]}, p)  # a table with only a batch row loads
+    assert autotune.load(p)["batch"]["mixed_min_n"] == 4096
+
+
+@pytest.mark.gpu
+def test_batch_sweep_and_apply():
+    """A batch sweep over 10 small cracks (720 to 2,340 DoF) with four
+    candidates: the batch row is the fastest trial, the knob is left as it
+    was, and a batch built with the row's settings solves every crack to a
+    backward error below 1e-14."""
+    import numpy as np
+    import torch
+    from hfpx3d_amd import hfp3d as hx, meshgen
+    assert torch.cuda.is_available(), "GPU tests need a device"
+    print("case: autotune batch sweep", flush=True)
+    ctx = hx.get_context()
+    cracks = []
+    for c in meshgen.batch_cracks(10, 5, 40, 130):
+        p = c["p"]
+        cracks.append(dict(nods=c["nods"], conn=c["conn"], G=1.0 / (2.0 * (1.0 + c["nu"])), nu=c["nu"],
+                           s_inf=[p, p, p, 0.0, 0.0, 0.0]))
+    k = autotune.BATCH_KNOBS["BGEMM_F32"]
+    before = hx.tune_get(k, ctx)
+    cands = [{"BGEMM_F32": b, "mixed_min_n": m} for b in (2, 3) for m in (1024, 10 ** 6)]
+    table = autotune.sweep_batch(cracks, candidates=cands, reps=1, ctx=ctx)
+    row = table["batch"]
+    assert row["ms"] == min(t["ms"] for t in row["tried"]) and math.isfinite(row["ms"])
+    assert len(row["tried"]) == 4 and row["n_cracks"] == 10
+    assert hx.tune_get(k, ctx) == before
+    try:
+        m = autotune.apply_batch(table, ctx)
+        assert m == row["mixed_min_n"] and hx.tune_get(k, ctx) == row["knobs"]["BGEMM_F32"]
+        bt = hx.Batch(cracks, mixed_min_n=m, ctx=ctx)
+        bt.solve(n_refine=3, use_graph=True)
+        for i in range(len(cracks)):
+            x, berr, info = bt.solution(i)
+            assert info == 0 and berr < 1e-14 and np.isfinite(x).all()
+        bt.close()
+    finally:
+        hx.tune_set(k, before, ctx)
