@@ -78,6 +78,8 @@ def parse():
                     help="do not time kernel launches with HIP events (for runs under an external profiler)")
     ap.add_argument("--lu-nb", type=int, default=0, help="outer block of the LU (0: library default)")
     ap.add_argument("--gemm-variant", type=int, default=0, help="FP32 GEMM variant (0: library default)")
+    ap.add_argument("--panel", type=int, default=0, help="LU panel kernel knob (0: library default)")
+    ap.add_argument("--lookahead", type=int, default=0, help="LU look-ahead knob (0: library default)")
     ap.add_argument("--autotune", default="", help="apply the knob settings of an autotune table "
                                                    "(python -m hfpx3d_amd.autotune) for the LU's n; "
                                                    "batch256: the table's batch row (update kernel and "
@@ -212,6 +214,10 @@ def main():
         hx.tune_set(_lib.KNOB_LU_NB, args.lu_nb, ctx)
     if args.gemm_variant:
         hx.tune_set(_lib.KNOB_GEMM_F32, args.gemm_variant, ctx)
+    if args.panel:
+        hx.tune_set(_lib.KNOB_PANEL, args.panel, ctx)
+    if args.lookahead:
+        hx.tune_set(_lib.KNOB_LU_LOOKAHEAD, args.lookahead, ctx)
     calib_copy(args, ctx)
     log(f"bench: n_el={n_el} n={n} nrhs={args.nrhs}")
 

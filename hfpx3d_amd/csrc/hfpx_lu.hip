@@ -1131,8 +1131,17 @@ static int getrf_impl(hfpx_ctx* ctx, T* A, int64_t lda, int64_t n, int32_t* ipiv
   ctx->sync_base = 0;
 
   // look-ahead by default in FP32 only (72,000 DoF: FP32 2.14 -> 1.81 s,
-  // FP64 5.24 -> 6.28 s with the overlapped FP64 updates)
-  const int la = ctx->knob[HFPX_KNOB_LU_LOOKAHEAD] ? ctx->knob[HFPX_KNOB_LU_LOOKAHEAD] : (sizeof(T) == 4 ? 2 : 1);
+  // FP64 5.24 -> 6.28 s with the overlapped FP64 updates), with the panels
+  // launched on the context's stream rather than through the cooperative
+  // launch (mode 3): the cooperative launches go to a queue of their own,
+  // and each panel then paid two cross-queue waits of about 12 us (72,000
+  // DoF: 14 to 25 ms per factorisation).  Co-residency still holds: the
+  // grid is at most one workgroup per CU, and the only other kernels that
+  // can hold CUs meanwhile are this factorisation's own trailing updates on
+  // the second stream, which never wait on the panel, so every panel
+  // workgroup is placed once they drain; the barrier's bounded wait turns
+  // anything else into an error.
+  const int la = ctx->knob[HFPX_KNOB_LU_LOOKAHEAD] ? ctx->knob[HFPX_KNOB_LU_LOOKAHEAD] : (sizeof(T) == 4 ? 3 : 1);
   const bool lookahead = (la == 2 || la == 3) && n > NB;
   W.coop = la != 3;
   if (!lookahead) {

@@ -10,9 +10,13 @@
 // counter; the workgroup of block i accumulates T(i, c) X_c over the blocks
 // c before it in sweep order as each one's flag appears, then multiplies
 // B_i - sum by block i's inverse,
-// which makes X_i final in place, and raises block i's flag.  The critical
-// path is one tile product plus one 128 x 128 product per block, instead of
-// one launch per block; the factor is read once per sweep.
+// which makes X_i final in place, and raises block i's flag.  The last
+// predecessor p of block i (its neighbour in sweep order) is taken through
+// W_i = inverse(i) T(i, p), formed once per factorisation next to the
+// inverses: X_i = inverse(i) (B_i - sum over the other predecessors) - W_i X_p,
+// where the first term is complete before X_p is, so the critical path is
+// one 128 x 128 product per block instead of two, with no dependent load of
+// the inverse; the factor is read once per sweep.
 #include "hfpx_internal.h"
 
 namespace hfpx {
@@ -118,6 +122,59 @@ __global__ __launch_bounds__(SB) void k_block_inv(const float* __restrict__ LU, 
   block_inv_tile<LOWER>(LU, lda, n, blockIdx.x, inv + (int64_t)blockIdx.x * SB * SB);
 }
 
+// W_b = inverse(b) T(b, p) for the block p before b in sweep order
+// (LOWER: p = b - 1, unit L; UPPER: p = b + 1, U), 128 x 128 column-major
+// at W + b SB^2; nothing for the first block in sweep order.  T(b, p) is
+// staged in LDS (zero outside the matrix; the inverse is identity-padded, so
+// W is zero outside the blocks' rows and columns); 4 waves, wave w forms
+// rows 32 w .. 32 w + 31 in the transposed MFMA form of the sweep.
+template <bool LOWER>
+__global__ __launch_bounds__(256) void k_block_w(const float* __restrict__ LU, int64_t lda, int64_t n, int64_t nblk,
+                                                 const float* __restrict__ inv, float* __restrict__ W) {
+  const int64_t b = blockIdx.x;
+  const int64_t p = LOWER ? b - 1 : b + 1;
+  if (p < 0 || p >= nblk) return;
+  __shared__ float sT[SB][SB + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, l32 = lane & 31, hk = lane >> 5;
+  const int64_t r0 = b * SB, k0 = p * SB;
+  const int rows = (int)std::min<int64_t>(SB, n - r0), kc = (int)std::min<int64_t>(SB, n - k0);
+  {
+    // 64 loads per thread in flight, then the LDS stores: sT[k][j] = T(k, j)
+    float v[SB * SB / 256];
+#pragma unroll
+    for (int q = 0; q < SB * SB / 256; ++q) {
+      const int e = tid + 256 * q, k = e % SB, j = e / SB;
+      v[q] = (k < rows && j < kc) ? LU[(r0 + k) + (k0 + j) * lda] : 0.f;
+    }
+#pragma unroll
+    for (int q = 0; q < SB * SB / 256; ++q) {
+      const int e = tid + 256 * q;
+      sT[e % SB][e / SB] = v[q];
+    }
+  }
+  const int i = 32 * wave + l32;
+  const float* iv = inv + b * SB * SB + i;
+  float ivv[SB / 2];
+#pragma unroll
+  for (int s2 = 0; s2 < SB / 2; ++s2) ivv[s2] = iv[(2 * s2 + hk) * SB];
+  __syncthreads();
+  float* wb = W + b * SB * SB;
+#pragma unroll 1
+  for (int jb = 0; jb < SB / 32; ++jb) {
+    floatx16 acc;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+#pragma unroll
+    for (int s2 = 0; s2 < SB / 2; ++s2)
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(sT[2 * s2 + hk][32 * jb + l32], ivv[s2], acc, 0, 0, 0);
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int j = 32 * jb + 8 * (v >> 2) + 4 * hk + (v & 3);
+      wb[i + j * SB] = acc[v];
+    }
+  }
+}
+
 // One sweep (see the file comment).  LOWER: forward with unit L, blocks in
 // increasing order; UPPER: backward with U, decreasing order.  A ticket t is
 // the pair (q = t / gy, right-hand-side block t % gy) with q the position of
@@ -138,6 +195,7 @@ __global__ __launch_bounds__(SB) void k_block_inv(const float* __restrict__ LU, 
 template <bool LOWER, int SWN>
 __global__ __launch_bounds__(8 * SWN) void k_sweep_flow(const float* __restrict__ LU, int64_t lda, int64_t n,
                                                     int64_t nblk, const float* __restrict__ inv,
+                                                    const float* __restrict__ wpred,
                                                     float* __restrict__ B, int64_t ldb, int64_t nrhs, int64_t gy,
                                                     uint64_t* __restrict__ flags, uint64_t want,
                                                     uint64_t* __restrict__ ticket, int32_t* __restrict__ err) {
@@ -174,11 +232,60 @@ __global__ __launch_bounds__(8 * SWN) void k_sweep_flow(const float* __restrict_
     const bool rok = i < rows;
     uint64_t* fl = flags + rb * nblk;
 
+    // wait until predecessor p (in sweep order) is final, then take every
+    // later one already final, so that one acquire fence covers the run
+    int64_t ready = 0;  // predecessors 0 .. ready-1 (in sweep order) are known final
+    auto wait_for = [&](int64_t p) {
+      if (tid == 0) {
+        const int64_t c = LOWER ? p : nblk - 1 - p;
+        uint32_t polls = 0;
+        while (__hip_atomic_load(fl + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != want) {
+          __builtin_amdgcn_s_sleep(1);
+          // give up past ~2^20 polls, or at once when another wait already has
+          if (++polls > (1u << 20) ||
+              ((polls & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+        }
+        int64_t r = p + 1;
+        while (r < q && __hip_atomic_load(fl + (LOWER ? r : nblk - 1 - r), __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT) == want)
+          ++r;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        s_tk = (uint64_t)r;
+      }
+      __syncthreads();
+      const uint64_t u = s_tk;
+      ready = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32)) << 32) |
+                        (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u));
+    };
+    // the solution rows of block c (kc rows) into sX: every load in flight
+    // before the first LDS store, one memory latency per predecessor
+    auto stage_x = [&](int64_t k0, int kc) {
+      constexpr int XQ = SB * SWN / SWT;
+      float xq[XQ];
+#pragma unroll
+      for (int q2 = 0; q2 < XQ; ++q2) {
+        const int e = tid + SWT * q2, k = e % SB, j = e / SB;
+        xq[q2] = (k < kc && j < ncol) ? B[(k0 + k) + (j0 + j) * ldb] : 0.f;
+      }
+#pragma unroll
+      for (int q2 = 0; q2 < XQ; ++q2) {
+        const int e = tid + SWT * q2;
+        sX[e % SB][e / SB] = xq[q2];
+      }
+      __syncthreads();
+    };
+
+    // predecessors through the factor tiles T(i, c); the last one through
+    // W_i when it is given
+    const bool usew = wpred != nullptr && q > 0;
+    const int64_t pend = usew ? q - 1 : q;
     floatx16 acc;
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[v] = 0.f;
-    int64_t ready = 0;  // predecessors 0 .. ready-1 (in sweep order) are known final
-    for (int64_t p = 0; p < q; ++p) {
+    for (int64_t p = 0; p < pend; ++p) {
       const int64_t c = LOWER ? p : nblk - 1 - p;
       const int64_t k0 = c * SB;
       const int kc = (int)std::min<int64_t>(SB, n - k0);
@@ -193,56 +300,14 @@ __global__ __launch_bounds__(8 * SWN) void k_sweep_flow(const float* __restrict_
         lv[s] = (rok && 2 * s + hk < kc) ? *lp : 0.f;
         lp += 2 * lda;
       }
-      if (p >= ready) {
-        // wait for this predecessor, then take every later one already
-        // final, so that one acquire fence covers the whole run
-        if (tid == 0) {
-          uint32_t polls = 0;
-          while (__hip_atomic_load(fl + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != want) {
-            __builtin_amdgcn_s_sleep(1);
-            // give up past ~2^20 polls, or at once when another wait already has
-            if (++polls > (1u << 20) ||
-                ((polls & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-              __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              break;
-            }
-          }
-          int64_t r = p + 1;
-          while (r < q && __hip_atomic_load(fl + (LOWER ? r : nblk - 1 - r), __ATOMIC_RELAXED,
-                                            __HIP_MEMORY_SCOPE_AGENT) == want)
-            ++r;
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-          s_tk = (uint64_t)r;
-        }
-        __syncthreads();
-        const uint64_t u = s_tk;
-        ready = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32)) << 32) |
-                          (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u));
-      }
-      {
-        // every load of the predecessor's solution rows in flight before
-        // the first LDS store: one memory latency per predecessor instead of
-        // one per loop trip
-        constexpr int XQ = SB * SWN / SWT;
-        float xq[XQ];
-#pragma unroll
-        for (int q2 = 0; q2 < XQ; ++q2) {
-          const int e = tid + SWT * q2, k = e % SB, j = e / SB;
-          xq[q2] = (k < kc && j < ncol) ? B[(k0 + k) + (j0 + j) * ldb] : 0.f;
-        }
-#pragma unroll
-        for (int q2 = 0; q2 < XQ; ++q2) {
-          const int e = tid + SWT * q2;
-          sX[e % SB][e / SB] = xq[q2];
-        }
-      }
-      __syncthreads();
+      if (p >= ready) wait_for(p);
+      stage_x(k0, kc);
 #pragma unroll
       for (int s = 0; s < SB / 2; ++s)
         acc = __builtin_amdgcn_mfma_f32_32x32x2f32(sX[2 * s + hk][32 * jb + l32], lv[s], acc, 0, 0, 0);
       __syncthreads();  // sX is rewritten by the next predecessor
     }
-    // X_i = inverse(i) (B_i - sum).  The block's inverse is fetched here,
+    // inverse(i) (B_i - sum).  The block's inverse is fetched here,
     // together with B_i, so both share one memory latency (held across the
     // predecessor loop it would cost the registers that loop's staging uses)
     const float* iv = inv + blk * SB * SB + i;  // inverse(i, k) at iv[k SB]
@@ -261,6 +326,31 @@ __global__ __launch_bounds__(8 * SWN) void k_sweep_flow(const float* __restrict_
 #pragma unroll
     for (int s = 0; s < SB / 2; ++s)
       y = __builtin_amdgcn_mfma_f32_32x32x2f32(sC[2 * s + hk][32 * jb + l32], ivv[s], y, 0, 0, 0);
+    if (usew) {
+      // X_i = y - W_i X_p for the last predecessor p: W_i is fetched before
+      // the wait, so after X_p appears one product remains.  (The scheduling
+      // barrier keeps the W_i loads after the inverse product: hoisted above
+      // it they would hold the inverse and W_i in registers together.)
+      __builtin_amdgcn_sched_barrier(0);
+      const int64_t p = q - 1;
+      const int64_t c = LOWER ? p : nblk - 1 - p;
+      const int64_t k0 = c * SB;
+      const int kc = (int)std::min<int64_t>(SB, n - k0);
+      const float* wp = wpred + blk * SB * SB + i;  // W_i(i, k) at wp[k SB]
+      float wv[SB / 2];
+#pragma unroll
+      for (int s = 0; s < SB / 2; ++s) wv[s] = wp[(2 * s + hk) * SB];
+      if (p >= ready) wait_for(p);
+      stage_x(k0, kc);
+      floatx16 a2;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) a2[v] = 0.f;
+#pragma unroll
+      for (int s = 0; s < SB / 2; ++s)
+        a2 = __builtin_amdgcn_mfma_f32_32x32x2f32(sX[2 * s + hk][32 * jb + l32], wv[s], a2, 0, 0, 0);
+#pragma unroll
+      for (int v = 0; v < 16; ++v) y[v] -= a2[v];
+    }
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
       const int j = 32 * jb + 8 * (v >> 2) + 4 * hk + (v & 3);
@@ -286,18 +376,18 @@ static int sweep_width(int64_t nrhs) { return nrhs <= SWN_MAX ? SWN_MIN : SWN_MA
 
 template <int SWN>
 static int launch_sweeps(hfpx_ctx* ctx, const float* LU, int64_t lda, int64_t n, int64_t nblk, const float* invL,
-                         const float* invU, float* B, int64_t ldb, int64_t nrhs, uint64_t* flags, uint64_t* tickets,
-                         int32_t* errs) {
+                         const float* invU, const float* wL, const float* wU, float* B, int64_t ldb, int64_t nrhs,
+                         uint64_t* flags, uint64_t* tickets, int32_t* errs) {
   const int64_t gy = (nrhs + SWN - 1) / SWN;
   // as many workgroups as are resident at once (any grid size would make
   // progress: a workgroup takes tickets only once it runs)
   int occ = 0;
   HFPX_TRY_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_sweep_flow<true, SWN>, 8 * SWN, 0));
   const unsigned nwg = (unsigned)std::min<int64_t>(nblk * gy, std::max(1, occ) * std::max(1, ctx->num_cus));
-  hipLaunchKernelGGL((k_sweep_flow<true, SWN>), dim3(nwg), dim3(8 * SWN), 0, ctx->stream, LU, lda, n, nblk, invL, B,
-                     ldb, nrhs, gy, flags, (uint64_t)1, tickets, errs);
-  hipLaunchKernelGGL((k_sweep_flow<false, SWN>), dim3(nwg), dim3(8 * SWN), 0, ctx->stream, LU, lda, n, nblk, invU, B,
-                     ldb, nrhs, gy, flags + nblk * gy, (uint64_t)1, tickets + 1, errs + 1);
+  hipLaunchKernelGGL((k_sweep_flow<true, SWN>), dim3(nwg), dim3(8 * SWN), 0, ctx->stream, LU, lda, n, nblk, invL, wL,
+                     B, ldb, nrhs, gy, flags, (uint64_t)1, tickets, errs);
+  hipLaunchKernelGGL((k_sweep_flow<false, SWN>), dim3(nwg), dim3(8 * SWN), 0, ctx->stream, LU, lda, n, nblk, invU,
+                     wU, B, ldb, nrhs, gy, flags + nblk * gy, (uint64_t)1, tickets + 1, errs + 1);
   HFPX_TRY_HIP(ctx, hipGetLastError());
   return HFPX_OK;
 }
@@ -459,7 +549,7 @@ int launch_bsweeps(hfpx_ctx* ctx, const float* A32, const BCrack* cr, const int3
 size_t sweep_f32_work_bytes(int64_t n, int64_t nrhs) {
   const int swn = sweep_width(nrhs);
   const int64_t nblk = (n + SB - 1) / SB, gy = (nrhs + swn - 1) / swn;
-  return sizeof(float) * 2 * (size_t)nblk * SB * SB + sizeof(uint64_t) * (2 * (size_t)nblk * gy + 4);
+  return sizeof(float) * 4 * (size_t)nblk * SB * SB + sizeof(uint64_t) * (2 * (size_t)nblk * gy + 4);
 }
 
 // B := U^{-1} L^{-1} B for the FP32 factors (row interchanges already
@@ -471,7 +561,9 @@ int sweep_f32(hfpx_ctx* ctx, const float* LU, int64_t lda, int64_t n, float* B, 
   const int64_t nblk = (n + SB - 1) / SB, gy = (nrhs + swn - 1) / swn;
   float* invL = (float*)work;
   float* invU = invL + (size_t)nblk * SB * SB;
-  uint64_t* flags = (uint64_t*)(invU + (size_t)nblk * SB * SB);  // [2][gy][nblk]
+  float* wL = invU + (size_t)nblk * SB * SB;  // W blocks of the forward sweep
+  float* wU = wL + (size_t)nblk * SB * SB;    // and of the backward sweep
+  uint64_t* flags = (uint64_t*)(wU + (size_t)nblk * SB * SB);  // [2][gy][nblk]
   uint64_t* tickets = flags + 2 * nblk * gy;                      // [2]
   int32_t* errs = (int32_t*)(tickets + 2);                       // [2]
   const int64_t nclear = 2 * nblk * gy + 4;
@@ -479,12 +571,16 @@ int sweep_f32(hfpx_ctx* ctx, const float* LU, int64_t lda, int64_t n, float* B, 
   int tok = prof_begin(ctx, HFPX_KC_TRSM);
   hipLaunchKernelGGL(k_block_inv<true>, dim3((unsigned)nblk), dim3(SB), 0, ctx->stream, LU, lda, n, invL);
   hipLaunchKernelGGL(k_block_inv<false>, dim3((unsigned)nblk), dim3(SB), 0, ctx->stream, LU, lda, n, invU);
-  prof_end(ctx, tok, HFPX_KC_TRSM, 2.0 * nblk * SB * SB * SB / 3.0, 0.0);
+  hipLaunchKernelGGL(k_block_w<true>, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, LU, lda, n, nblk, invL, wL);
+  hipLaunchKernelGGL(k_block_w<false>, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, LU, lda, n, nblk, invU, wU);
+  prof_end(ctx, tok, HFPX_KC_TRSM, 2.0 * nblk * SB * SB * SB / 3.0 + 4.0 * nblk * SB * SB * SB, 0.0);
   HFPX_TRY_HIP(ctx, hipGetLastError());
   tok = prof_begin(ctx, HFPX_KC_TRSM);
   const int rc = swn == SWN_MIN
-                     ? launch_sweeps<SWN_MIN>(ctx, LU, lda, n, nblk, invL, invU, B, ldb, nrhs, flags, tickets, errs)
-                     : launch_sweeps<SWN_MAX>(ctx, LU, lda, n, nblk, invL, invU, B, ldb, nrhs, flags, tickets, errs);
+                     ? launch_sweeps<SWN_MIN>(ctx, LU, lda, n, nblk, invL, invU, wL, wU, B, ldb, nrhs, flags,
+                                              tickets, errs)
+                     : launch_sweeps<SWN_MAX>(ctx, LU, lda, n, nblk, invL, invU, wL, wU, B, ldb, nrhs, flags,
+                                              tickets, errs);
   // algorithmic: 2 n^2 nrhs flops, the factor read once (4 n^2 bytes)
   prof_end(ctx, tok, HFPX_KC_TRSM, 2.0 * n * n * nrhs, 4.0 * n * n);
   if (rc) return rc;

@@ -436,7 +436,7 @@ int hfpx_batch_matrix(hfpx_ctx* ctx, hfpx_batch* batch, int crack, double* A_hos
  * preloaded into the accumulators; 4 = 3 without the XCD-grouped walk;
  * 5 = 3 with bands of 4 row tiles. */
 #define HFPX_KNOB_GEMM_F64 7
-/* HFPX_KNOB_LU_LOOKAHEAD: 0 = default (2 for FP32, 1 for FP64, where the
+/* HFPX_KNOB_LU_LOOKAHEAD: 0 = default (3 for FP32, 1 for FP64, where the
  * overlapped FP64 updates measured slower); 1 = off: each outer block's
  * trailing update follows its factorisation; 2 = look-ahead: the next
  * block's columns are updated first and that block is factorised on the
