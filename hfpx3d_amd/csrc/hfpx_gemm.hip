@@ -552,10 +552,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 // Operand layout: lane l supplies A[i = l%16][k = l/16] and X[k = l/16][j =
 // l%16]; result element v of lane l is D[4v + l/16][l%16].
 // ---------------------------------------------------------------------
+// rabs (optional): the row sums of |A| over the slice, rabs[z M + row],
+// from the A values the product streams anyway (the normwise backward
+// error's |A|_inf without a pass of its own over A).
 template <int NB>
 __global__ __launch_bounds__(256) void k_dgemm_stream(double* __restrict__ P, const double* __restrict__ A,
                                                       int64_t lda, const double* __restrict__ X, int64_t ldx,
-                                                      int64_t M, int64_t N, int64_t K, int64_t kslice) {
+                                                      int64_t M, int64_t N, int64_t K, int64_t kslice,
+                                                      double* __restrict__ rabs) {
   constexpr int KC2 = 16;
   constexpr int XQ = KC2 * NB * 16 / 256;  // X elements staged per thread per chunk
   __shared__ double sX[2][KC2][NB * 16 + 1];
@@ -596,12 +600,20 @@ __global__ __launch_bounds__(256) void k_dgemm_stream(double* __restrict__ P, co
   fetch_a(a0, kb);
   if (kb + KC2 < ke) fetch_a(a1, kb + KC2);
   int buf = 0;
+  double sa = 0.0, sb = 0.0;  // |A| row sums of rows r_a, r_b over this lane's k
   auto step = [&](double (&cur)[2][4], int64_t k0) {
     double av[2][4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       av[0][s] = cur[0][s];
       av[1][s] = cur[1][s];
+    }
+    if (rabs) {
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        sa += fabs(av[0][s]);
+        sb += fabs(av[1][s]);
+      }
     }
 #pragma unroll
     for (int q = 0; q < XQ; ++q) {
@@ -626,6 +638,17 @@ __global__ __launch_bounds__(256) void k_dgemm_stream(double* __restrict__ P, co
     step(a0, k0);
     if (k0 + KC2 < ke) step(a1, k0 + KC2);
   }
+  if (rabs) {
+    // the four k groups of a row are lanes l16 + 16 lq
+    sa += __shfl_xor(sa, 16);
+    sa += __shfl_xor(sa, 32);
+    sb += __shfl_xor(sb, 16);
+    sb += __shfl_xor(sb, 32);
+    if (lq == 0) {
+      if (ok_a) rabs[z * M + r_a] = sa;
+      if (ok_b) rabs[z * M + r_b] = sb;
+    }
+  }
   double* Pz = P + z * M * N;
 #pragma unroll
   for (int rb = 0; rb < 2; ++rb)
@@ -637,6 +660,15 @@ __global__ __launch_bounds__(256) void k_dgemm_stream(double* __restrict__ P, co
         const int64_t c = 16 * cb + l16;
         if (r < M && c < N) Pz[r + c * M] = -acc[rb][cb][v];
       }
+}
+
+// out[i] = sum over slices z of part[z M + i], in slice order.
+__global__ void k_slice_rowsum(const double* __restrict__ part, int64_t M, int ks, double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  double s = 0.0;
+  for (int z = 0; z < ks; ++z) s += part[z * M + i];
+  out[i] = s;
 }
 
 // C (M x N, ldc) += sum over slices z = 0 .. ks-1 of P_z (M x N, ld M),
@@ -653,6 +685,8 @@ __global__ __launch_bounds__(256) void k_splitk_sum(T* __restrict__ C, int64_t l
 }
 
 constexpr int64_t SPLITK_MIN_K = 8192;
+
+
 constexpr int64_t SPLITK_MAX = 32;
 
 // Grow-only workspace of the split-K products (not ctx->work, which callers
@@ -667,6 +701,40 @@ static int ensure_gemm_work(hfpx_ctx* ctx, size_t bytes) {
   }
   HFPX_TRY_HIP(ctx, hipMalloc(&ctx->gemm_work, bytes));
   ctx->gemm_work_bytes = bytes;
+  return HFPX_OK;
+}
+
+bool dgemm_stream_applies(int64_t N, int64_t K) { return N <= 64 && K >= SPLITK_MIN_K; }
+
+// C -= A B for few columns (N <= 64) and long K, FP64: A streamed once in
+// 128-row tiles x K slices, the slices' partial products summed in order;
+// with rowabs, also rowabs[i] = sum_k |A(i, k)| (summed over the slices in
+// order).
+int dgemm_stream_sub(hfpx_ctx* ctx, double* C, int64_t ldc, const double* A, int64_t lda, const double* B,
+                     int64_t ldb, int64_t M, int64_t N, int64_t K, double* rowabs) {
+  const int64_t nrt = (M + 127) / 128;
+  int64_t ks = std::min<int64_t>(SPLITK_MAX, std::max<int64_t>(1, (8 * ctx->num_cus + nrt - 1) / nrt));
+  const int64_t kslice = ((K + ks - 1) / ks + 15) / 16 * 16;
+  ks = (K + kslice - 1) / kslice;
+  const size_t slice = (size_t)M * N;
+  int rc = ensure_gemm_work(ctx, sizeof(double) * (slice * ks + (rowabs ? (size_t)M * ks : 0)));
+  if (rc) return rc;
+  double* P = (double*)ctx->gemm_work;
+  double* RA = rowabs ? P + slice * ks : nullptr;
+  const dim3 g((unsigned)nrt, (unsigned)ks);
+  if (N <= 16)
+    hipLaunchKernelGGL(k_dgemm_stream<1>, g, dim3(256), 0, ctx->stream, P, A, lda, B, ldb, M, N, K, kslice, RA);
+  else if (N <= 32)
+    hipLaunchKernelGGL(k_dgemm_stream<2>, g, dim3(256), 0, ctx->stream, P, A, lda, B, ldb, M, N, K, kslice, RA);
+  else
+    hipLaunchKernelGGL(k_dgemm_stream<4>, g, dim3(256), 0, ctx->stream, P, A, lda, B, ldb, M, N, K, kslice, RA);
+  const int64_t tot = M * N;
+  hipLaunchKernelGGL(k_splitk_sum<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream, C, ldc,
+                     P, M, N, (int)ks);
+  if (rowabs)
+    hipLaunchKernelGGL(k_slice_rowsum, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, ctx->stream, RA, M,
+                       (int)ks, rowabs);
+  HFPX_TRY_HIP(ctx, hipGetLastError());
   return HFPX_OK;
 }
 
@@ -813,28 +881,9 @@ int gemm_sub(hfpx_ctx* ctx, T* C, int64_t ldc, const T* A, int64_t lda, const T*
   }
   if constexpr (sizeof(T) == 8) {
     if (!done && skinny && K >= SPLITK_MIN_K) {
-      // stream A once: 128-row tiles x K slices, partials summed in order
-      const int64_t nrt = (M + 127) / 128;
-      int64_t ks = std::min<int64_t>(SPLITK_MAX, std::max<int64_t>(1, (8 * ctx->num_cus + nrt - 1) / nrt));
-      const int64_t kslice = ((K + ks - 1) / ks + 15) / 16 * 16;
-      ks = (K + kslice - 1) / kslice;
-      const size_t slice = (size_t)M * N;
-      int rc = ensure_gemm_work(ctx, sizeof(T) * slice * ks);
+      int rc = dgemm_stream_sub(ctx, (double*)C, ldc, (const double*)A, lda, (const double*)B, ldb, M, N, K,
+                                nullptr);
       if (rc) return rc;
-      double* P = (double*)ctx->gemm_work;
-      const dim3 g((unsigned)nrt, (unsigned)ks);
-      if (N <= 16)
-        hipLaunchKernelGGL(k_dgemm_stream<1>, g, dim3(256), 0, ctx->stream, P, (const double*)A, lda,
-                           (const double*)B, ldb, M, N, K, kslice);
-      else if (N <= 32)
-        hipLaunchKernelGGL(k_dgemm_stream<2>, g, dim3(256), 0, ctx->stream, P, (const double*)A, lda,
-                           (const double*)B, ldb, M, N, K, kslice);
-      else
-        hipLaunchKernelGGL(k_dgemm_stream<4>, g, dim3(256), 0, ctx->stream, P, (const double*)A, lda,
-                           (const double*)B, ldb, M, N, K, kslice);
-      const int64_t tot = M * N;
-      hipLaunchKernelGGL(k_splitk_sum<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream,
-                         (double*)C, ldc, P, M, N, (int)ks);
       done = true;
     }
   }

@@ -223,6 +223,12 @@ int ensure_x3_work(hfpx_ctx* ctx, size_t bytes);
 template <typename T>
 int gemm_sub(hfpx_ctx* ctx, T* C, int64_t ldc, const T* A, int64_t lda, const T* B, int64_t ldb,
              int64_t M, int64_t N, int64_t K);
+// C -= A B (FP64, N <= 64, K >= 8192) streaming A once, and with rowabs
+// also rowabs[i] = sum_k |A(i, k)|
+int dgemm_stream_sub(hfpx_ctx* ctx, double* C, int64_t ldc, const double* A, int64_t lda, const double* B,
+                     int64_t ldb, int64_t M, int64_t N, int64_t K, double* rowabs);
+// the FP64 products that gemm_sub routes to dgemm_stream_sub
+bool dgemm_stream_applies(int64_t N, int64_t K);
 int gemm_sub_any(hfpx_ctx* ctx, int dtype, void* C, int64_t ldc, const void* A, int64_t lda,
                  const void* B, int64_t ldb, int64_t M, int64_t N, int64_t K);
 

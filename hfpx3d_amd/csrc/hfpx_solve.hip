@@ -133,6 +133,7 @@ struct Work {
   double* part;   // reduction partials
   double* col;    // k column scalars (device)
   double* col2;   // k
+  int64_t m1;     // blocks in V (m + 1)
 };
 
 inline dim3 grid2(int64_t n, int64_t k, int bs) { return dim3((unsigned)((n + bs - 1) / bs), (unsigned)k); }
@@ -172,11 +173,18 @@ static int precond(hfpx_ctx* ctx, const Work& w, const float* LU, int64_t ldlu, 
   return HFPX_OK;
 }
 
-// R := B - A X
+// R := B - A X; with rowabs (the first residual of a solve), also the row
+// sums of |A| from the same pass over A
 static int residual(hfpx_ctx* ctx, const double* A, int64_t lda, int64_t n, const double* B,
-                    int64_t ldb, const double* X, int64_t ldx, double* R, int64_t k) {
+                    int64_t ldb, const double* X, int64_t ldx, double* R, int64_t k, double* rowabs = nullptr) {
   hipLaunchKernelGGL(k_copy, grid2(n, k, 256), dim3(256), 0, ctx->stream, B, ldb, R, n, n, k);
   HFPX_TRY_HIP(ctx, hipGetLastError());
+  if (rowabs) {
+    const int tok = prof_begin(ctx, HFPX_KC_GEMM_F64);
+    const int rc = dgemm_stream_sub(ctx, R, n, A, lda, X, ldx, n, k, n, rowabs);
+    prof_end(ctx, tok, HFPX_KC_GEMM_F64, 2.0 * n * k * n, 8.0 * ((double)n * n + 2.0 * n * k));
+    return rc;
+  }
   return gemm_sub_any(ctx, HFPX_F64, R, n, A, lda, X, ldx, n, k, n);
 }
 
@@ -184,11 +192,31 @@ static int residual(hfpx_ctx* ctx, const double* A, int64_t lda, int64_t n, cons
 // |b - A x|_inf / (|A|_inf |x|_inf + |b|_inf); *worst = max over columns,
 // NaN if any is NaN.  Leaves the residual in w.R.
 static int backward_error(hfpx_ctx* ctx, const Work& w, const double* A, int64_t lda, int64_t n, const double* B,
-                          int64_t ldb, const double* X, int64_t ldx, int64_t k, double anorm,
+                          int64_t ldb, const double* X, int64_t ldx, int64_t k, double& anorm,
                           const std::vector<double>& bnorm, std::vector<double>& rnorm, std::vector<double>& xnorm,
                           std::vector<double>& omega, double* worst) {
   int rc;
-  if ((rc = residual(ctx, A, lda, n, B, ldb, X, ldx, w.R, k))) return rc;
+  // anorm < 0: not known yet.  Where the residual streams A (many rows,
+  // few right-hand sides) that pass also gives the row sums of |A|;
+  // otherwise they take a pass of their own.  V is free until the first
+  // correction.
+  const bool first = anorm < 0;
+  const bool fused = first && dgemm_stream_applies(k, n);
+  if (first && !fused) {
+    const int ns = (int)std::max<int64_t>(1, std::min<int64_t>(32, (int64_t)w.m1 * k - 1));
+    const int64_t cw = (n + ns - 1) / ns;
+    hipLaunchKernelGGL(k_rowabs, dim3((unsigned)((n + 255) / 256), (unsigned)ns), dim3(256), 0, ctx->stream, A,
+                       lda, n, cw, w.V + n);
+    hipLaunchKernelGGL(k_rowabs_sum, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, w.V + n, n, ns,
+                       w.V);
+    HFPX_TRY_HIP(ctx, hipGetLastError());
+  }
+  if ((rc = residual(ctx, A, lda, n, B, ldb, X, ldx, w.R, k, fused ? w.V : nullptr))) return rc;
+  if (first) {
+    double h;
+    if ((rc = colred(ctx, w, w.V, n, nullptr, 0, n, 1, 1, &h))) return rc;
+    anorm = h;
+  }
   if ((rc = colred(ctx, w, w.R, n, nullptr, 0, n, k, 1, rnorm.data()))) return rc;
   if ((rc = colred(ctx, w, X, ldx, nullptr, 0, n, k, 1, xnorm.data()))) return rc;
   double wv = 0.0;
@@ -257,26 +285,14 @@ int solve_ir(hfpx_ctx* ctx, const double* A, int64_t lda, const float* LU, int64
   w.R = (double*)take(sizeof(double) * nk);
   w.Wv = (double*)take(sizeof(double) * nk);
   w.V = (double*)take(sizeof(double) * nk * (m + 1));
+  w.m1 = m + 1;
   w.W32 = (float*)take(sizeof(float) * nk);
   w.part = (double*)take(sizeof(double) * nb * k);
   w.col = (double*)take(sizeof(double) * k);
   w.col2 = (double*)take(sizeof(double) * k);
 
-  // |A|_inf (rows sums) -> part (n doubles fit: nb*k >= n/256*k ... use V as scratch)
-  // V holds (m + 1) k n doubles: ns slices of partial row sums + the sums
-  const int ns = (int)std::max<int64_t>(1, std::min<int64_t>(32, (int64_t)(m + 1) * k - 1));
-  const int64_t cw = (n + ns - 1) / ns;
-  hipLaunchKernelGGL(k_rowabs, dim3((unsigned)((n + 255) / 256), (unsigned)ns), dim3(256), 0, ctx->stream, A, lda,
-                     n, cw, w.V);
-  hipLaunchKernelGGL(k_rowabs_sum, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, w.V, n, ns,
-                     w.V + (size_t)ns * n);
-  HFPX_TRY_HIP(ctx, hipGetLastError());
-  double anorm = 0.0;
-  {
-    double h;
-    if ((rc = colred(ctx, w, w.V + (size_t)ns * n, n, nullptr, 0, n, 1, 1, &h))) return rc;
-    anorm = h;
-  }
+  // |A|_inf comes with the first residual (backward_error, anorm < 0)
+  double anorm = -1.0;
   std::vector<double> bnorm(k), xnorm(k), rnorm(k), omega(k), beta(k), tmp(k);
   if ((rc = colred(ctx, w, B, ldb, nullptr, 0, n, k, 1, bnorm.data()))) return rc;
 
@@ -289,9 +305,7 @@ int solve_ir(hfpx_ctx* ctx, const double* A, int64_t lda, const float* LU, int64
   bool converged = false, stalled = false;
   // non-finite data: nothing to refine, and an FP64 LU of the same data
   // cannot do better
-  double data_norm = anorm;
-  for (int64_t j = 0; j < k; ++j) data_norm = nan_max(data_norm, bnorm[j]);
-  const bool finite_data = std::isfinite(data_norm);
+  bool finite_data = true;  // set once |A|_inf is known (first backward error)
   // 0: classic steps while they contract 100-fold, then GMRES; 1: GMRES
   // only; 2: classic only
   const int ir_mode = ctx->knob[HFPX_KNOB_IR_MODE];
@@ -299,7 +313,12 @@ int solve_ir(hfpx_ctx* ctx, const double* A, int64_t lda, const float* LU, int64
   for (outer = 0; outer <= max_outer; ++outer) {
     if ((rc = backward_error(ctx, w, A, lda, n, B, ldb, X, ldx, k, anorm, bnorm, rnorm, xnorm, omega, &worst)))
       return rc;
-    if (worst <= tol) {
+    if (outer == 0) {
+      double data_norm = anorm;
+      for (int64_t j = 0; j < k; ++j) data_norm = nan_max(data_norm, bnorm[j]);
+      finite_data = std::isfinite(data_norm);
+    }
+    if (finite_data && worst <= tol) {
       converged = true;
       break;
     }
