@@ -18,10 +18,13 @@ def env():
 
 
 def to_dev(torch, hx, a, dtype):
-    """numpy (n x m) -> column-major device tensor."""
+    """numpy (n x m) -> column-major device tensor, copied on the context's
+    stream (the tests that call the C ABI directly launch on that stream;
+    a copy on torch's current stream would race with them)."""
     ctx = hx.get_context()
     t, _ = hx.alloc_colmajor(a.shape[0], a.shape[1], dtype, ctx)
-    t.copy_(torch.from_numpy(np.ascontiguousarray(a)))
+    with torch.cuda.stream(ctx.stream):
+        t.copy_(torch.from_numpy(np.ascontiguousarray(a)))
     return t
 
 

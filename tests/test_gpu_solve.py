@@ -77,7 +77,8 @@ def _device(torch, hx, a):
     if a.ndim == 1:
         a = a[:, None]
     d, _ = hx.alloc_colmajor(a.shape[0], a.shape[1], torch.float64, ctx)
-    d.copy_(torch.from_numpy(a))
+    with torch.cuda.stream(ctx.stream):
+        d.copy_(torch.from_numpy(a))
     return d
 
 
