@@ -565,15 +565,23 @@ struct PermPairs {
   int32_t src[PAIR_MAX];
 };
 
-__global__ __launch_bounds__(256) void k_perm_pairs(const int32_t* __restrict__ ipiv, int64_t k0, int64_t k1,
-                                                    int64_t n, PermPairs* __restrict__ out) {
-  __shared__ int32_t piv[SWAP_BLK], borig[SWAP_BLK];
-  __shared__ int32_t hkey[1024], horig[1024];  // rows outside the block (open addressing)
-  __shared__ int32_t olist[SWAP_BLK];
+// The pairs of the interchanges ipiv[s0, s0 + nb) (nb <= SWAP_BLK) into
+// dst / src / *count, by the whole workgroup (256 threads); the tables are
+// the caller's LDS.
+struct PairLds {
+  int32_t piv[SWAP_BLK], borig[SWAP_BLK];
+  int32_t hkey[1024], horig[1024];  // rows outside the block (open addressing)
+  int32_t olist[SWAP_BLK];
+};
+
+__device__ __forceinline__ void build_pairs(const int32_t* __restrict__ ipiv, int64_t s0, int nb, int64_t n,
+                                            PairLds& L, int32_t* dst, int32_t* src, int32_t* count) {
+  int32_t* piv = L.piv;
+  int32_t* borig = L.borig;
+  int32_t* hkey = L.hkey;
+  int32_t* horig = L.horig;
+  int32_t* olist = L.olist;
   const int t = threadIdx.x;
-  const int64_t s0 = k0 + (int64_t)blockIdx.x * SWAP_BLK;
-  const int nb = (int)std::min<int64_t>(SWAP_BLK, k1 - s0);
-  PermPairs* pp = out + blockIdx.x;
   if (t < nb) {
     piv[t] = ipiv[s0 + t] - 1;
     borig[t] = (int32_t)(s0 + t);
@@ -606,19 +614,68 @@ __global__ __launch_bounds__(256) void k_perm_pairs(const int32_t* __restrict__ 
   int cnt = 0;
   for (int i = 0; i < nb; ++i)
     if (borig[i] != (int32_t)(s0 + i)) {
-      pp->dst[cnt] = (int32_t)(s0 + i);
-      pp->src[cnt] = borig[i];
+      dst[cnt] = (int32_t)(s0 + i);
+      src[cnt] = borig[i];
       ++cnt;
     }
   for (int q = 0; q < nout; ++q) {
     const int h = olist[q];
     if (horig[h] != hkey[h]) {
-      pp->dst[cnt] = hkey[h];
-      pp->src[cnt] = horig[h];
+      dst[cnt] = hkey[h];
+      src[cnt] = horig[h];
       ++cnt;
     }
   }
-  pp->count = cnt;
+  *count = cnt;
+}
+
+__global__ __launch_bounds__(256) void k_perm_pairs(const int32_t* __restrict__ ipiv, int64_t k0, int64_t k1,
+                                                    int64_t n, PermPairs* __restrict__ out) {
+  __shared__ PairLds L;
+  const int64_t s0 = k0 + (int64_t)blockIdx.x * SWAP_BLK;
+  const int nb = (int)std::min<int64_t>(SWAP_BLK, k1 - s0);
+  PermPairs* pp = out + blockIdx.x;
+  build_pairs(ipiv, s0, nb, n, L, pp->dst, pp->src, &pp->count);
+}
+
+// Small interchange blocks (k1 - k0 <= SWAP_SMALL) in one launch: every
+// workgroup builds the pairs in its LDS (thread 0 walks at most SWAP_SMALL
+// interchanges) and applies them to its four columns as k_apply_pairs does.
+constexpr int SWAP_SMALL = 64;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_swap_small(T* __restrict__ A, int64_t lda, const int32_t* __restrict__ ipiv,
+                                                    int64_t k0, int64_t k1, int64_t n, int64_t c0, int64_t c1,
+                                                    int64_t c2, int64_t c3) {
+  __shared__ PairLds L;
+  __shared__ int32_t sdst[2 * SWAP_SMALL], ssrc[2 * SWAP_SMALL], scnt;
+  build_pairs(ipiv, k0, (int)(k1 - k0), n, L, sdst, ssrc, &scnt);
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int64_t ci = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t n1 = c1 - c0;
+  if (ci >= n1 + (c3 - c2)) return;
+  const int64_t col = ci < n1 ? c0 + ci : c2 + (ci - n1);
+  const int cnt = scnt;
+  T v[2 * SWAP_SMALL / 64];
+  int32_t d[2 * SWAP_SMALL / 64];
+#pragma unroll
+  for (int q = 0; q < 2 * SWAP_SMALL / 64; ++q) {
+    const int i = lane + 64 * q;
+    if (i < cnt) {
+      d[q] = sdst[i];
+      v[q] = A[ssrc[i] + col * lda];
+    }
+  }
+  // every load of this wave has returned before any store (one wave per
+  // column: the wave's own loads and stores are the only ones on it)
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int q = 0; q < 2 * SWAP_SMALL / 64; ++q) {
+    const int i = lane + 64 * q;
+    if (i < cnt) A[d[q] + col * lda] = v[q];
+  }
 }
 
 // Columns [c0, c1) and [c2, c3), one wave each.
@@ -982,6 +1039,17 @@ static int swap_cols(hfpx_ctx* ctx, const LuWork<T>& W, T* A, int64_t lda, const
                      int64_t k0, int64_t k1, int64_t c0, int64_t c1, int64_t c2, int64_t c3) {
   if (k1 <= k0 || (c1 <= c0 && c3 <= c2)) return HFPX_OK;
   int rc;
+  if (k1 - k0 <= SWAP_SMALL) {
+    const int64_t ncol = std::max<int64_t>(0, c1 - c0) + std::max<int64_t>(0, c3 - c2);
+    if (c1 < c0) c1 = c0;
+    if (c3 < c2) c3 = c2;
+    const int tok = prof_begin(ctx, HFPX_KC_LASWP);
+    hipLaunchKernelGGL(k_swap_small<T>, dim3((unsigned)((ncol + 3) / 4)), dim3(256), 0, ctx->stream, A, lda, ipiv,
+                       k0, k1, n, c0, c1, c2, c3);
+    prof_end(ctx, tok, HFPX_KC_LASWP, 0.0, 2.0 * sizeof(T) * ncol * 2 * (k1 - k0));
+    HFPX_TRY_HIP(ctx, hipGetLastError());
+    return HFPX_OK;
+  }
   if ((rc = perm_pairs(ctx, ipiv, k0, k1, n, W.pp))) return rc;
   const int64_t nblk = (k1 - k0 + SWAP_BLK - 1) / SWAP_BLK;
   for (int64_t b = 0; b < nblk; ++b)
