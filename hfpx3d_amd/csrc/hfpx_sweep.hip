@@ -203,9 +203,7 @@ __global__ __launch_bounds__(8 * SWN) void k_sweep_flow(const float* __restrict_
   __shared__ float sX[SB][SWN + 1];  // solution rows of block c: sX[k][j]
   __shared__ float sC[SB][SWN + 1];  // B_i - sum, then X_i: sC[i][j]
   __shared__ uint64_t s_tk;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, l32 = lane & 31, hk = lane >> 5;
-  const int i = 32 * (wave & 3) + l32;  // tile row of this lane
-  const int jb = wave >> 2;             // 32-column block of this wave
+  const int tid = threadIdx.x, wave = tid >> 6, jb = wave >> 2;  // jb: 32-column block of this wave
   const int64_t total = nblk * gy;
   // Thread 0 takes the tickets; every thread reads them as wave-uniform
   // (scalar) values.  The ticket is taken at the end of the loop body, not at
@@ -223,6 +221,15 @@ __global__ __launch_bounds__(8 * SWN) void k_sweep_flow(const float* __restrict_
     return (int64_t)t;
   };
   for (int64_t tk = next_ticket(); tk < total; tk = next_ticket()) {
+    // The lane's coordinates are made opaque per ticket: otherwise the
+    // compiler hoists every value derived from them (the 64 column indices
+    // and their comparisons, the LDS addresses of each product) out of the
+    // ticket loop and holds them in registers for the whole kernel, which
+    // leaves too few for the double-buffered factor tiles below.
+    int lane = tid & 63;
+    asm volatile("" : "+v"(lane));
+    const int l32 = lane & 31, hk = lane >> 5;
+    const int i = 32 * (wave & 3) + l32;  // tile row of this lane
     const int64_t q = tk / gy, rb = tk % gy;
     const int64_t blk = LOWER ? q : nblk - 1 - q;
     const int64_t r0 = blk * SB;
@@ -279,33 +286,79 @@ __global__ __launch_bounds__(8 * SWN) void k_sweep_flow(const float* __restrict_
     };
 
     // predecessors through the factor tiles T(i, c); the last one through
-    // W_i when it is given
+    // W_i when it is given.  The loop is software-pipelined over two static
+    // register sets (unrolled by two): the next predecessor's factor tile,
+    // and its solution rows when it is already known final, are loaded while
+    // the current one's products run, so an iteration costs its products
+    // rather than two memory latencies (the sweep is bound by this loop:
+    // every workgroup walks all of its block's predecessors).
     const bool usew = wpred != nullptr && q > 0;
     const int64_t pend = usew ? q - 1 : q;
     floatx16 acc;
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[v] = 0.f;
-    for (int64_t p = 0; p < pend; ++p) {
+    // factor tile T(i, c) of predecessor p into lv: a wave-uniform column
+    // base stepped by two columns per load plus one 32-bit lane offset, so
+    // the addresses live in scalar registers (a 64-bit pointer per load would
+    // cost two vector registers each)
+    // (the step is made opaque per call so that the 64 column offsets are
+    // not hoisted out of the ticket loop into registers).  A tile inside the
+    // matrix loads without masks, back to back; only tiles of the last row
+    // or column block take the masked path.
+    const uint32_t loff = (uint32_t)i + (uint32_t)hk * (uint32_t)lda;
+    auto load_t = [&](int64_t p, float* lv) {
       const int64_t c = LOWER ? p : nblk - 1 - p;
       const int64_t k0 = c * SB;
       const int kc = (int)std::min<int64_t>(SB, n - k0);
-      // the factor tile does not depend on the flag: its loads are issued
-      // before the wait.  The column pointer steps by two columns (a table
-      // of the 64 offsets would be hoisted out of the loop and held in
-      // registers).
-      const float* lp = LU + (r0 + i) + (k0 + hk) * lda;
-      float lv[SB / 2];
+      const float* col = LU + r0 + k0 * lda;
+      int64_t step = 2 * lda;
+      asm volatile("" : "+s"(step));
+      if (kc == SB && rows == SB) {
 #pragma unroll
-      for (int s = 0; s < SB / 2; ++s) {
-        lv[s] = (rok && 2 * s + hk < kc) ? *lp : 0.f;
-        lp += 2 * lda;
+        for (int s = 0; s < SB / 2; ++s) {
+          lv[s] = col[loff];
+          col += step;
+        }
+      } else {
+        const float* lp = LU + (r0 + i) + (k0 + hk) * lda;
+#pragma unroll
+        for (int s = 0; s < SB / 2; ++s) {
+          lv[s] = (rok && 2 * s + hk < kc) ? *lp : 0.f;
+          lp += step;
+        }
       }
+    };
+    auto stage_p = [&](int64_t p) {
       if (p >= ready) wait_for(p);
-      stage_x(k0, kc);
+      const int64_t c = LOWER ? p : nblk - 1 - p;
+      stage_x(c * SB, (int)std::min<int64_t>(SB, n - c * SB));
+    };
+    auto mma = [&](const float* lv) {
 #pragma unroll
       for (int s = 0; s < SB / 2; ++s)
         acc = __builtin_amdgcn_mfma_f32_32x32x2f32(sX[2 * s + hk][32 * jb + l32], lv[s], acc, 0, 0, 0);
       __syncthreads();  // sX is rewritten by the next predecessor
+    };
+    if constexpr (SWN == SWN_MIN) {
+      float lvA[SB / 2], lvB[SB / 2];
+      if (pend > 0) load_t(0, lvA);
+      for (int64_t p = 0; p < pend; p += 2) {
+        stage_p(p);
+        if (p + 1 < pend) load_t(p + 1, lvB);
+        mma(lvA);
+        if (p + 1 >= pend) break;
+        stage_p(p + 1);
+        if (p + 2 < pend) load_t(p + 2, lvA);
+        mma(lvB);
+      }
+    } else {
+      // two waves per SIMD leave no registers for a second tile
+      for (int64_t p = 0; p < pend; ++p) {
+        float lv[SB / 2];
+        load_t(p, lv);
+        stage_p(p);
+        mma(lv);
+      }
     }
     // inverse(i) (B_i - sum).  The block's inverse is fetched here,
     // together with B_i, so both share one memory latency (held across the
