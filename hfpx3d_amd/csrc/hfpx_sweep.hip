@@ -182,12 +182,18 @@ __global__ __launch_bounds__(256) void k_block_w(const float* __restrict__ LU, i
 // of right-hand-side block rb is final.  Tickets are taken in order and a
 // workgroup only waits on lower tickets, which were handed to workgroups
 // already running: any grid size makes progress.
-// Memory ordering: the producer's workgroup barrier, then an agent-scope
-// release fence and a relaxed flag store by thread 0; the consumer's thread 0
+// Memory ordering: the solution rows that cross workgroups are written and
+// read only as agent-scope relaxed atomics (device-coherent stores and loads
+// that do not stop in an XCD's L2), so no fence has to write back or
+// invalidate a whole L2: an agent-scope fence costs a write-back of the
+// producer's L2 and an invalidation of the consumer's, on every chain step
+// of the sweep and under every other workgroup's streaming loads.  The
+// producer waits until its own stores are complete (vmcnt 0) before the
+// workgroup barrier, then thread 0 stores the flag; the consumer's thread 0
 // polls relaxed (bounded: past ~2^20 polls *err is raised, after which every
 // other wait of the sweep gives up too and the sweep completes with garbage,
-// reported by the host) and ends with an agent-scope
-// acquire fence, which the workgroup barrier extends to the other threads.
+// reported by the host) and the workgroup barrier orders the other threads'
+// loads of the rows after the flag was seen.
 // 4 SWN / 32 waves, one 32 x 32 block of the 128 x SWN tile each.  Products use
 // the transposed MFMA form (v_mfma_f32_32x32x2f32): lane l holds row
 // 32 (wave % 4) + l % 32 of the tile and column 8 (v/4) + 4 (l/32) + v%4 of
@@ -259,7 +265,7 @@ __global__ __launch_bounds__(8 * SWN) void k_sweep_flow(const float* __restrict_
         while (r < q && __hip_atomic_load(fl + (LOWER ? r : nblk - 1 - r), __ATOMIC_RELAXED,
                                           __HIP_MEMORY_SCOPE_AGENT) == want)
           ++r;
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         s_tk = (uint64_t)r;
       }
       __syncthreads();
@@ -275,7 +281,9 @@ __global__ __launch_bounds__(8 * SWN) void k_sweep_flow(const float* __restrict_
 #pragma unroll
       for (int q2 = 0; q2 < XQ; ++q2) {
         const int e = tid + SWT * q2, k = e % SB, j = e / SB;
-        xq[q2] = (k < kc && j < ncol) ? B[(k0 + k) + (j0 + j) * ldb] : 0.f;
+        xq[q2] = (k < kc && j < ncol)
+                     ? __hip_atomic_load(B + (k0 + k) + (j0 + j) * ldb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                     : 0.f;
       }
 #pragma unroll
       for (int q2 = 0; q2 < XQ; ++q2) {
@@ -407,11 +415,13 @@ __global__ __launch_bounds__(8 * SWN) void k_sweep_flow(const float* __restrict_
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
       const int j = 32 * jb + 8 * (v >> 2) + 4 * hk + (v & 3);
-      if (rok && j < ncol) B[(r0 + i) + (j0 + j) * ldb] = y[v];
+      if (rok && j < ncol)
+        __hip_atomic_store(B + (r0 + i) + (j0 + j) * ldb, y[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this thread's stores of X_i are complete
     __syncthreads();  // every store of X_i, and every read of sC, before the flag
     if (tid == 0) {
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       __hip_atomic_store(fl + blk, want, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
