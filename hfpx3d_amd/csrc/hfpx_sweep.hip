@@ -466,7 +466,9 @@ static int launch_sweeps(hfpx_ctx* ctx, const float* LU, int64_t lda, int64_t n,
 // long chains start first.  A ticket (crack c, level q) accumulates
 // T(i, p) X_p over the crack's earlier blocks p as each one's flag appears
 // (256 threads: row r = t % 128, half t / 128 of the 128 columns), then
-// X_i = inverse(i) (b_i - sum) in place and raises block i's flag.
+// X_i = inverse(i) (b_i - sum) in place and raises block i's flag.  The
+// solved rows cross workgroups as device-coherent atomics, with the same
+// ordering as the single-crack sweep (no L2 write-back or invalidation).
 // ---------------------------------------------------------------------
 // block inverses of every (crack, block) of the ticket table (the level of
 // a ticket is its block in forward order)
@@ -545,13 +547,14 @@ __global__ __launch_bounds__(2 * SB) __attribute__((amdgpu_waves_per_eu(4))) voi
           while (rr < q && __hip_atomic_load(fl + (LOWER ? rr : nb - 1 - rr), __ATOMIC_RELAXED,
                                              __HIP_MEMORY_SCOPE_AGENT) == 1)
             ++rr;
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
           s_tk = (uint64_t)rr;
         }
         __syncthreads();
         ready = uniform(s_tk);
       }
-      if (tid < SB) sX[tid] = tid < kc ? x[k0 + tid] : 0.f;
+      if (tid < SB)
+        sX[tid] = tid < kc ? __hip_atomic_load(x + k0 + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.f;
       __syncthreads();
 #pragma unroll
       for (int k = 0; k < KH; ++k) acc += lv[k] * sX[KH * kh + k];
@@ -571,10 +574,12 @@ __global__ __launch_bounds__(2 * SB) __attribute__((amdgpu_waves_per_eu(4))) voi
     }
     sP[kh][r] = s;
     __syncthreads();
-    if (tid < rows) x[r0 + tid] = sP[0][tid] + sP[1][tid];
+    if (tid < rows)
+      __hip_atomic_store(x + r0 + tid, sP[0][tid] + sP[1][tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this thread's store of X_i is complete
     __syncthreads();  // every store of X_i before the flag
     if (tid == 0) {
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       __hip_atomic_store(fl + blk, (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
