@@ -803,8 +803,10 @@ __global__ __launch_bounds__(256) void k_trsm_leaf(const T* __restrict__ Lm, int
 // and therefore to the same bits.  The triangle sits in LDS transposed
 // (column j contiguous, rows 16-byte aligned) so the reads of one column
 // merge.  The upper solve keeps the row form of k_trsm_leaf.  50 KB of LDS:
-// three workgroups per CU.
+// three workgroups per CU.  The unit-lower leaves of the factorisation take
+// the lane form below (k_trsm_leaf_lane), in FP32 and FP64.
 constexpr int TRSM_F32_TC = 128;
+constexpr int64_t TRSM_LANE_NARROW = 1024;  // columns up to which the lane leaf takes one column per wave
 
 template <bool LOWER>
 __global__ __launch_bounds__(TRSM_F32_TC) void k_trsm_leaf_f32(const float* __restrict__ Lm, int64_t ldl,
@@ -880,6 +882,60 @@ __global__ __launch_bounds__(TRSM_F32_TC) void k_trsm_leaf_f32(const float* __re
     if (lane < kb) B[(c0 + c) * ldb + lane] = sB[c][lane];
 }
 
+// Unit-lower leaf in lane form: a wave owns CPW columns of B and lane i row
+// i, holding L(i, 0 .. i-1) in registers; step j broadcasts x_j with
+// v_readlane (scalar registers) and every lane below j applies
+// x_i -= L_ij x_j.  No LDS and no barrier: the dependent chain is 63
+// readlane + FMA steps, so a small solve (the leaves inside an outer block,
+// at most a few hundred columns) is bound by its loads rather than by the
+// per-step LDS latency of the staged form.  Each x_i is reduced in the same
+// order j = 0, 1, ... with fused multiply-adds, as in the staged FP32 form.
+template <typename T>
+__device__ __forceinline__ T lane_bcast(T v, int j) {
+  if constexpr (sizeof(T) == 4) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+  } else {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), j);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), j);
+    return __hiloint2double(hi, lo);
+  }
+}
+
+template <typename T, int CPW>
+__global__ __launch_bounds__(256) void k_trsm_leaf_lane(const T* __restrict__ Lm, int64_t ldl, T* __restrict__ B,
+                                                        int64_t ldb, int kb, int64_t ncols) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t c0 = ((int64_t)blockIdx.x * 4 + wave) * CPW;
+  if (c0 >= ncols) return;  // whole waves only; no barrier below
+  const bool rin = lane < kb;
+  // unconditional loads (a masked lane reads L(0, 0)), then the mask: the 64
+  // loads issue back to back without a branch each
+  const uint32_t ld = (uint32_t)ldl;
+  T l[LB];
+#pragma unroll
+  for (int j = 0; j < LB; ++j) {
+    const bool ok = j < lane && rin;
+    const T v = Lm[ok ? (uint32_t)lane + (uint32_t)j * ld : 0u];
+    l[j] = ok ? v : T(0);
+  }
+  T x[CPW];
+#pragma unroll
+  for (int c = 0; c < CPW; ++c) x[c] = (rin && c0 + c < ncols) ? B[lane + (c0 + c) * ldb] : T(0);
+#pragma unroll
+  for (int j = 0; j < LB - 1; ++j) {
+    if (j + 1 >= kb) break;  // uniform
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+      const T u = std::fma(-l[j], lane_bcast(x[c], j), x[c]);
+      x[c] = lane > j ? u : x[c];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < CPW; ++c)
+    if (rin && c0 + c < ncols) B[lane + (c0 + c) * ldb] = x[c];
+}
+
 // ---------------------------------------------------------------------
 // host drivers
 // ---------------------------------------------------------------------
@@ -913,7 +969,17 @@ static int apply_pairs(hfpx_ctx* ctx, T* A, int64_t lda, const PermPairs* pp, in
 template <typename T, bool LOWER>
 static int trsm_leaf(hfpx_ctx* ctx, const T* L, int64_t ldl, T* B, int64_t ldb, int kb, int64_t ncols) {
   const int tok = prof_begin(ctx, HFPX_KC_TRSM);
-  if constexpr (sizeof(T) == 4)
+  if constexpr (LOWER) {
+    // lane form: one column per wave for the small solves inside an outer
+    // block, eight per wave (L loaded once per eight columns) for the wide
+    // ones
+    if (ncols <= TRSM_LANE_NARROW)
+      hipLaunchKernelGGL((k_trsm_leaf_lane<T, 1>), dim3((unsigned)((ncols + 3) / 4)), dim3(256), 0, ctx->stream,
+                         L, ldl, B, ldb, kb, ncols);
+    else
+      hipLaunchKernelGGL((k_trsm_leaf_lane<T, 8>), dim3((unsigned)((ncols + 31) / 32)), dim3(256), 0,
+                         ctx->stream, L, ldl, B, ldb, kb, ncols);
+  } else if constexpr (sizeof(T) == 4)
     hipLaunchKernelGGL((k_trsm_leaf_f32<LOWER>), dim3((unsigned)((ncols + TRSM_F32_TC - 1) / TRSM_F32_TC)),
                        dim3(TRSM_F32_TC), 0, ctx->stream, L, ldl, B, ldb, kb, ncols);
   else
