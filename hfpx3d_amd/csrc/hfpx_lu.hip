@@ -807,6 +807,7 @@ __global__ __launch_bounds__(256) void k_trsm_leaf(const T* __restrict__ Lm, int
 // the lane form below (k_trsm_leaf_lane), in FP32 and FP64.
 constexpr int TRSM_F32_TC = 128;
 constexpr int64_t TRSM_LANE_NARROW = 1024;  // columns up to which the lane leaf takes one column per wave
+constexpr int64_t TRSM_LANE_WIDE = 24576;   // FP32 columns above which the staged leaf is faster
 
 template <bool LOWER>
 __global__ __launch_bounds__(TRSM_F32_TC) void k_trsm_leaf_f32(const float* __restrict__ Lm, int64_t ldl,
@@ -969,7 +970,13 @@ static int apply_pairs(hfpx_ctx* ctx, T* A, int64_t lda, const PermPairs* pp, in
 template <typename T, bool LOWER>
 static int trsm_leaf(hfpx_ctx* ctx, const T* L, int64_t ldl, T* B, int64_t ldb, int kb, int64_t ncols) {
   const int tok = prof_begin(ctx, HFPX_KC_TRSM);
-  if constexpr (LOWER) {
+  if (LOWER && sizeof(T) == 4 && ncols > TRSM_LANE_WIDE) {
+    // very wide FP32 solves: the lane form is VALU-bound there (three
+    // instructions per column and step for 64 rows), the staged form's
+    // thread per column does one FMA per entry (same bits)
+    hipLaunchKernelGGL((k_trsm_leaf_f32<true>), dim3((unsigned)((ncols + TRSM_F32_TC - 1) / TRSM_F32_TC)),
+                       dim3(TRSM_F32_TC), 0, ctx->stream, (const float*)L, ldl, (float*)B, ldb, kb, ncols);
+  } else if constexpr (LOWER) {
     // lane form: one column per wave for the small solves inside an outer
     // block, eight per wave (L loaded once per eight columns) for the wide
     // ones
